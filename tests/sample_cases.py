@@ -1,0 +1,28 @@
+"""Inputs of the reference's own example runs (test/optIon, test/optDen) and the
+numbers their recorded outputs hold, as data. Lattices in Angstrom (rows are
+lattice vectors as written in the .ion files), fractional positions, 16^3 grids."""
+import numpy as np
+
+OPTION_LATTICE_A = np.array([[3.869104452400000, 0.100000000000000, 0.0],
+                             [0.000000000000000, 3.969104452400000, 0.0],
+                             [0.000000000000000, 0.000000000000000, 4.0]])
+OPTION_FRAC = np.array([[0.1, 0.0, 0.0], [0.5, 0.55, 0.99], [0.0, 0.5, 0.5], [0.505, 0.0, 0.5]])
+# test/optIon/sampleoutput/displaced.out:92-99 (eV) and :119 (ion-ion)
+OPTION_STEP0_EV = -2.199483598873e+02
+OPTION_FINAL_EV = -2.311425327622e+02
+OPTION_EII_EV = -2.986846473280e+02
+OPTION_ITERS = 7
+# standard.out:62-68 (Ha), per NTN iteration
+OPTION_ITER_HA = [-8.4856061397, -8.4936397786, -8.4942793002, -8.4943223357,
+                  -8.4943318670, -8.4943322391, -8.4943322847]
+
+OPTDEN_LATTICE_A = np.eye(3) * 3.9691
+OPTDEN_FRAC = np.array([[0, 0, 0], [0.5, 0.5, 0], [0, 0.5, 0.5], [0.5, 0, 0.5]], dtype=float)
+OPTDEN_ALPHA = 1.2060113295832982827348622781219   # test/optDen/test.inpt:6
+OPTDEN_BETA = 0.46065533708336838393180438854479   # test/optDen/test.inpt:7
+# test/optDen/sample.output:91-97 (eV), :123-135 (final terms)
+OPTDEN_STEP_EV = [-2.226243159654e+02, -2.312025858301e+02, -2.317471544164e+02,
+                  -2.317593951594e+02, -2.317606991483e+02, -2.317607243753e+02,
+                  -2.317607263207e+02]
+OPTDEN_NEWTON_ITERS = [2, 8, 6, 8, 4, 7]
+OPTDEN_EII_EV = -2.994046165635e+02

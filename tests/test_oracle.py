@@ -1,0 +1,262 @@
+"""Pins the CPU oracle (oracle/profess_oracle.py) before it is trusted as the
+checker: closed forms, brute force, finite differences, the reference's own
+Fortran (oracle/_ref) and the reference's recorded sample outputs."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import profess_oracle as O
+from oracle import refbuild as R
+from tests import sample_cases as S
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _al_psp():
+    with open(os.path.join(GOLDEN, "al_HC.lda.recpot")) as fh:
+        return O.make_psp(*O.read_recpot_text(fh.read()))
+
+
+def _tri_cell(n=(6, 5, 4)):
+    lat = np.array([[7.3, 0.2, 0.1], [0.3, 7.5, 0.0], [-0.2, 0.4, 7.6]]).T
+    return O.Cell(lat, *n)
+
+
+def test_fft_matches_brute_force_dft():
+    rng = np.random.default_rng(1)
+    for shape in [(4, 5, 6), (3, 6, 10), (7, 2, 8)]:
+        a = rng.standard_normal(shape)
+        n3, n2, n1 = shape
+        c = O.fft_forward(a)
+        i3, i2, i1 = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+        for k3, k2, k1 in [(0, 0, 0), (1, 1, 1), (n3 - 1, 0, n1 // 2), (2 % n3, 1, 2)]:
+            ph = np.exp(-2j * np.pi * (k1 * i1 / n1 + k2 * i2 / n2 + k3 * i3 / n3))
+            ref = np.sum(a * ph) / a.size
+            assert abs(c[k3, k2, k1] - ref) < 1e-13
+        back = O.fft_backward(c, shape)
+        np.testing.assert_allclose(back, a, rtol=0, atol=1e-12)
+
+
+def test_g_tables_triclinic():
+    cell = _tri_cell((6, 5, 4))
+    qv, qt, mask = O.g_tables(cell)
+    b = cell.recip
+    # recip . lattice = 2 pi I
+    np.testing.assert_allclose(b.T @ cell.lattice, 2 * np.pi * np.eye(3), atol=1e-12)
+    # index (k1=2, i2=3 -> m2=-2 (5//2=2, 3>2), i3=2 -> m3=2 (4//2=2 stays positive))
+    g = 2 * b[:, 0] - 2 * b[:, 1] + 2 * b[:, 2]
+    np.testing.assert_allclose(qv[2, 3, 2], g, atol=1e-13)
+    assert abs(qt[2, 3, 2] - np.linalg.norm(g)) < 1e-13
+    # qMask: every G paired with -G counted once; G = 0 excluded
+    assert not mask[0, 0, 0]
+
+
+def test_lind_g_branches():
+    lam, mu = 1.0, 1.0
+    eta = np.array([0.3, 0.9, 1.5, 3.0])
+    closed = 1.0 / (0.5 + (1 - eta ** 2) / (4 * eta) * np.log(np.abs((1 + eta) / (1 - eta)))) \
+        - 3 * mu * eta ** 2 - lam
+    np.testing.assert_allclose(O.lind_g(eta, lam, mu), closed, rtol=1e-14)
+    # the Taylor branch continues the closed form at the 3.65 switch point
+    lo = O.lind_g(np.array([3.65]), lam, mu)[0]
+    e = 3.65
+    cf = 1.0 / (0.5 + (1 - e * e) / (4 * e) * math.log(abs((1 + e) / (1 - e)))) - 3 * mu * e * e - lam
+    hi = O.lind_g(np.array([3.65 + 1e-12]), lam, mu)[0]
+    assert abs(lo - cf) < 1e-12 and abs(hi - cf) < 1e-9
+    assert O.lind_g(np.array([0.0]), 0.3, 1.0)[0] == pytest.approx(0.7)
+    assert O.lind_g(np.array([1.0]), 1.0, 1.0)[0] == pytest.approx(-2.0)
+
+
+def _cos_field(cell, m, base, amp):
+    n3, n2, n1 = cell.shape
+    i3, i2, i1 = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+    arg = 2 * np.pi * (m[0] * i1 / n1 + m[1] * i2 / n2 + m[2] * i3 / n3)
+    g = cell.recip @ np.array(m, dtype=float)
+    return base + amp * np.cos(arg), np.cos(arg), g @ g
+
+
+def test_hartree_and_vw_single_cosine_mode():
+    cell = _tri_cell((8, 6, 10))
+    _, qt, _ = O.g_tables(cell)
+    rho, cosr, g2 = _cos_field(cell, (1, 2, -1), 0.03, 0.004)
+    vh, eh = O.hartree(rho, qt)
+    np.testing.assert_allclose(vh, 4 * np.pi * 0.004 / g2 * cosr, atol=1e-13)
+    assert eh * cell.dV == pytest.approx(np.pi * 0.004 ** 2 * cell.vol / g2, rel=1e-12)
+    t, evw = O.vw_sqrt(rho, qt, 0.7)
+    np.testing.assert_allclose(t, 0.7 * 0.004 * g2 * cosr, atol=1e-13)
+    # E = 1/2 mu sum(phi t) dV = mu b^2 G^2 Omega / 4 for phi = a + b cos
+    assert evw * cell.dV == pytest.approx(0.7 * 0.004 ** 2 * g2 * cell.vol / 4, rel=1e-12)
+
+
+def test_uniform_density_closed_forms():
+    cell = _tri_cell((6, 6, 6))
+    rho0 = 0.027
+    rho = np.full(cell.shape, rho0)
+    f = O.Functional(kinetic=4).resolve(rho0 * cell.vol, cell.vol)
+    sysm = O.System(cell, f, np.zeros(cell.shape))
+    pot, et = O.calculate_potential_plus(rho, sysm)
+    assert et[6] == pytest.approx(O.CTF * rho0 ** (5 / 3) * cell.vol, rel=1e-13)
+    assert abs(et[7]) < 1e-14 and abs(et[8]) < 1e-14 and abs(et[3]) < 1e-14
+    rs = 0.62035049089940009 / rho0 ** (1 / 3)
+    exc = -0.73855876638202234 * rho0 ** (1 / 3) + (-0.1423) / (1 + 1.0529 * math.sqrt(rs) + 0.3334 * rs)
+    assert et[4] == pytest.approx(exc * rho0 * cell.vol, rel=1e-13)
+
+
+def test_potential_is_energy_derivative():
+    cell = _tri_cell((8, 6, 6))
+    psp = _al_psp()
+    frac = np.array([[0.1, 0.2, 0.3], [0.6, 0.5, 0.8]])
+    vion = O.ion_potential(cell, [(psp, frac)])
+    rng = np.random.default_rng(3)
+    for kin in (4, 5):
+        f = O.Functional(kinetic=kin).resolve(6.0, cell.vol)
+        sysm = O.System(cell, f, vion)
+        if kin == 5:
+            # smooth synthetic 1D table standing in for the ODE output (shape only)
+            eta = O.wgc_eta_knots()
+            w = -1.0 * eta ** 2 / (1 + eta ** 2)
+            sysm.wgc_kernels = O.wgc_kernels_from_table(sysm.qtable, eta, w, np.gradient(w, eta),
+                                                        np.gradient(np.gradient(w, eta), eta), f)
+        rho = 6.0 / cell.vol * (1 + 0.3 * rng.random(cell.shape))
+        d = rng.standard_normal(cell.shape)
+        pot, _ = O.calculate_potential_plus(rho, sysm, opt_sqrt=False)
+        h = 1e-6
+        ep = O.calculate_potential_plus(rho + h * d, sysm, False)[1][0]
+        em = O.calculate_potential_plus(rho - h * d, sysm, False)[1][0]
+        fd = (ep - em) / (2 * h)
+        assert fd == pytest.approx(np.sum(pot * d) * cell.dV, rel=1e-7)
+        # dE/dphi form: optSqrt multiplies by 2 sqrt(rho) before vW is added
+        p2, _ = O.calculate_potential_plus(rho, sysm, opt_sqrt=True)
+        np.testing.assert_allclose(p2, 2 * np.sqrt(rho) * pot, rtol=1e-11, atol=1e-13)
+
+
+def test_ion_potential_spline_converges_to_exact_structure_factor():
+    cell = _tri_cell((12, 12, 12))
+    psp = _al_psp()
+    frac = np.array([[0.1, 0.0, 0.0], [0.5, 0.55, 0.99]])
+    exact = O.ion_energy_exact_recip(cell, [(psp, frac)])
+    errs = []
+    for order in (6, 10, 14):
+        v = O.fft_forward(O.ion_potential(cell, [(psp, frac)], order=order))
+        errs.append(np.max(np.abs(v - exact)) / np.max(np.abs(exact)))
+    assert errs[1] < errs[0] and errs[2] < errs[1] and errs[1] < 1e-3
+
+
+def test_ion_forces_are_minus_energy_gradient():
+    cell = _tri_cell((10, 10, 10))
+    psp = _al_psp()
+    frac = np.array([[0.13, 0.21, 0.37], [0.61, 0.52, 0.83]])
+    rng = np.random.default_rng(5)
+    rho = 6.0 / cell.vol * (1 + 0.2 * rng.random(cell.shape))
+    forces = O.ion_forces(cell, [(psp, frac)], rho)
+    h = 1e-5
+    for ion in range(2):
+        for c in range(3):
+            dr = np.zeros(3)
+            dr[c] = h
+            ds = np.linalg.solve(cell.lattice, dr)
+            fp = frac.copy()
+            fp[ion] += ds
+            fm = frac.copy()
+            fm[ion] -= ds
+            ep = np.sum(rho * O.ion_potential(cell, [(psp, fp)])) * cell.dV
+            em = np.sum(rho * O.ion_potential(cell, [(psp, fm)])) * cell.dV
+            assert -(ep - em) / (2 * h) == pytest.approx(forces[ion, c], rel=1e-6, abs=1e-9)
+
+
+def test_recpot_charge_and_units():
+    psp = _al_psp()
+    assert psp.charge == 3 and psp.numPoints == 15003
+    assert psp.maxG == pytest.approx(56.6993428892377764 * O.BOHR)
+    # synthetic V_loc round trip through the same ReadPseudo semantics
+    maxg, vals = O.synthetic_recpot_values(3, 1.2)
+    sp = O.make_psp(maxg, vals)
+    assert sp.charge == 3
+    q = np.array([0.5, 1.3, 4.0])
+    np.testing.assert_allclose(O.psp_lookup(sp, q), -4 * np.pi * 3 / q ** 2 * np.exp(-q ** 2 * 1.44 / 4),
+                               rtol=1e-7)
+
+
+needs_ref = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built (no reference tree)")
+
+
+@needs_ref
+def test_splines_bit_exact_against_reference_fortran():
+    rng = np.random.default_rng(7)
+    t = np.cumsum(rng.random(40) + 0.1)
+    y = np.sin(t)
+    for ib, ie in [(0, 0), (1, 1), (2, 1)]:
+        a = O.spline_cubic_set(t, y, ib, 0.3, ie, -0.2)
+        b = R.spline_set(t, y, ib, 0.3, ie, -0.2)
+        assert np.array_equal(a, b)
+        tv = np.concatenate([rng.random(50) * (t[-1] + 2) - 1, t[:5]])
+        va = np.array([O.spline_cubic_val(t, y, a, x) for x in tv])
+        vb = R.spline_val(t, y, b, tv)
+        assert np.array_equal(va, vb)
+        np.testing.assert_allclose(O.spline_cubic_val_vec(t, y, a, tv), vb, rtol=1e-13, atol=1e-13)
+
+
+@needs_ref
+def test_dcsrch_step_for_step_against_reference_fortran():
+    # Moré–Thuente test functions (phi(a) = -a/(a^2+b), and a cubic-like one)
+    fns = [lambda a: (-a / (a * a + 2.0), -(2.0 - a * a) / (a * a + 2.0) ** 2),
+           lambda a: ((a + 0.004) ** 5 - 2 * (a + 0.004) ** 4, 5 * (a + 0.004) ** 4 - 8 * (a + 0.004) ** 3)]
+    for fn in fns:
+        for stp0 in (1e-3, 0.1, 1.0, 10.0):
+            ours = O.Dcsrch(1e-3, 0.1, 1e-10, 0.0, 1e10)
+            theirs = R.RefDcsrch(1e-3, 0.1, 1e-10, 0.0, 1e10)
+            f, g = fn(0.0)
+            s1 = s2 = stp0
+            for _ in range(40):
+                s1, t1 = ours.step(s1, f, g)
+                s2, t2 = theirs.step(s2, f, g)
+                assert s1 == s2 and t1[:2] == t2[:2]
+                if not t1.startswith("FG"):
+                    break
+                f, g = fn(s1)
+
+
+def _option_system():
+    psp = _al_psp()
+    cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
+    vion = O.ion_potential(cell, [(psp, S.OPTION_FRAC)])
+    f = O.Functional(kinetic=4).resolve(12.0, cell.vol)
+    return O.System(cell, f, vion, S.OPTION_EII_EV / O.HARTREE_TO_EV)
+
+
+def test_reference_sample_optIon_wt_triclinic():
+    """test/optIon sample: step-0 energy (uniform density) and the converged NTN
+    total energy on a triclinic 16^3 cell (reference outputs agree to about
+    8 significant figures across machines, per the reference's manual FAQ)."""
+    sysm = _option_system()
+    rho = np.full(sysm.cell.shape, 12.0 / sysm.cell.vol)
+    full, pot = O.make_evaluators(sysm)
+    _, et = full(rho)
+    assert et[0] * O.HARTREE_TO_EV == pytest.approx(S.OPTION_STEP0_EV, rel=1e-12)
+    res = O.rho_minimization(rho, full, pot, sysm.cell.dV)
+    assert res.outcome == 0 and res.iters == S.OPTION_ITERS
+    assert abs(res.energy[0] * O.HARTREE_TO_EV - S.OPTION_FINAL_EV) / 4 < 1e-6   # eV/atom
+    for h, ref in zip(res.history[1:], S.OPTION_ITER_HA):
+        assert h["energy"] == pytest.approx(ref, rel=2e-6)
+
+
+@needs_ref
+def test_reference_sample_optDen_wgc():
+    """test/optDen sample (WGC, second order): same Newton inner-iteration counts
+    per step and the same energies within the accuracy of the 1D kernel table
+    (RKSUITE at TOL 5e-5, KEDF_KernelODE.f90:376)."""
+    psp = _al_psp()
+    cell = O.Cell(S.OPTDEN_LATTICE_A.T / O.BOHR, 16, 16, 16)
+    vion = O.ion_potential(cell, [(psp, S.OPTDEN_FRAC)])
+    f = O.Functional(kinetic=5, alpha=S.OPTDEN_ALPHA, beta=S.OPTDEN_BETA).resolve(12.0, cell.vol)
+    sysm = O.System(cell, f, vion, S.OPTDEN_EII_EV / O.HARTREE_TO_EV)
+    eta, w, w1, w2 = R.wgc_eta_table(f.alpha, f.beta, f.gamma)
+    sysm.wgc_kernels = O.wgc_kernels_from_table(sysm.qtable, eta, w, w1, w2, f)
+    rho = np.full(cell.shape, 12.0 / cell.vol)
+    full, pot = O.make_evaluators(sysm)
+    res = O.rho_minimization(rho, full, pot, cell.dV)
+    assert [h["newton_iter"] for h in res.history[1:]] == S.OPTDEN_NEWTON_ITERS
+    for h, ref in zip(res.history, S.OPTDEN_STEP_EV):
+        assert abs(h["energy"] * O.HARTREE_TO_EV - ref) / 4 < 5e-6
