@@ -1,0 +1,171 @@
+/*
+ * ofd.h -- C ABI of the MI355X-native PROFESS energy/potential evaluation and
+ * truncated-Newton (NTN) density optimizer (libprofess_amd.so).
+ *
+ * Conventions (all entry points):
+ *   - Real-space arrays are FP64, n1*n2*n3 values, x fastest: element
+ *     (i1,i2,i3) at i1 + n1*(i2 + n2*i3), exactly the reference's Fortran
+ *     rho(n1,n2,n3) layout (CellInfo.f90:27-36).
+ *   - Energies are in Hartree; etable9 uses the reference's eTable(9) slots
+ *     (CalPotential.f90:610-644), 0-based here:
+ *     [0] total [1] kinetic [2] ion-electron [3] Hartree [4] XC [5] ion-ion
+ *     [6] Thomas-Fermi [7] von Weizsacker [8] nonlocal (WT or WGC).
+ *   - Every call returns 0 on success and a negative status otherwise; the
+ *     message is available from ofd_last_error(). A Fortran caller maps a
+ *     nonzero status to QUIT(message), which keeps the reference's fail-stop
+ *     behaviour (Output.f90:1505-1549).
+ *   - Host pointers unless the name says _device. The caller owns all arrays.
+ *   - One context per host thread; a context is not reentrant. Calls on one
+ *     context are ordered on its own HIP stream and synchronous at return.
+ */
+#ifndef PROFESS_AMD_OFD_H
+#define PROFESS_AMD_OFD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ofd_ctx ofd_ctx;
+
+/* Grid and cell. cell[3*j + i] = Cartesian component i of lattice vector j
+ * (bohr): the columns of the reference's cell%cellReal (CellInfo.f90:247-330). */
+typedef struct {
+  int n1, n2, n3;
+  double cell[9];
+} ofd_grid;
+
+/* Kinetic functional and parameters (SetupKEDF.f90:217-322). kinetic = 4 (WT)
+ * or 5 (WGC, second order, WGCT = 2). Negative alpha/beta/gamma/rho0/rhoS
+ * select the reference's defaults, resolved as KEDFRefresh does
+ * (SetupKEDF.f90:613-683) from num_electrons / volume. */
+typedef struct {
+  int kinetic;
+  double lambda, mu;
+  double alpha, beta, gamma;
+  double rho0, rhoS;
+  int precision;   /* 0 = FP64 everywhere; 1 = FP32 FFT/convolution passes
+                      (FP64 fields, epilogue and reductions) */
+} ofd_functional;
+
+/* Optimizer controls (RhoOptimizers.f90:44-101). conv_check: 1 = ENE,
+ * 2 = POT, 3 = BOTH (ReadInputFile.f90:591-611). Negative pot_tol/tole/max_iter
+ * select the defaults (1.E-5 and 1.E-6 as single-precision literals, 500).
+ * mixed_switch: with precision = 1, potential norm below which the optimizer
+ * switches from FP32 to FP64 passes (<= 0: stay in the context's precision). */
+typedef struct {
+  int max_iter;
+  double pot_tol;
+  double tole;
+  int conv_check;
+  int max_line;
+  double mixed_switch;
+} ofd_opt_params;
+
+typedef struct {
+  int iterations;          /* outer NTN iterations (RhoOptN.f90:553) */
+  int num_evals;           /* CalculatePotentialPlus calls */
+  int newton_iters_total;  /* inner CG iterations summed */
+  int fp32_evals;          /* evaluations done in FP32 passes */
+  double final_pot_norm;
+  double seconds;
+} ofd_opt_stats;
+
+/* ---- context (replaces the module globals listed in CalPotential.f90:41-116:
+ *      grid dims, cell%dV/vol/cellRecip, kinetic, lambda, mu, alpha, beta,
+ *      gamma, rho0, rhoS, qTable, keKernel, ionPotReal, ionIonEnergy) ---- */
+int ofd_create(const ofd_grid* grid, const ofd_functional* func, double num_electrons, ofd_ctx** out);
+void ofd_destroy(ofd_ctx* ctx);
+const char* ofd_last_error(const ofd_ctx* ctx);
+/* resolved functional parameters: alpha, beta, gamma, rho0, rhoS, lambda, mu */
+int ofd_get_params(const ofd_ctx* ctx, double* out7);
+
+/* ionPotReal and ionIonEnergy as RefreshIonTerms leaves them
+ * (RefreshIons.f90:196,221). */
+int ofd_set_ion_terms(ofd_ctx* ctx, const double* vion_real, double e_ionion);
+
+/* Device build of ionPotReal: cardinal B-spline structure factor per species,
+ * V_loc(|G|) from .recpot data, b-spline moduli, conjugate, C2R
+ * (IonElectronPotRecipSpline, IonElectronSpline.f90:83-216; FillQIonTable
+ * :537-638; FillB CBSpline.f90:93-196; ReadPseudo units/spline
+ * ReadIonFile.f90:848-976; PspRecipCubicSpline LocalPseudoPot.f90:136-198).
+ * Ions are grouped by species: species s owns ions
+ * [type_start[s], type_start[s+1]). frac_xyz: 3 fractional coordinates per ion.
+ * Per species s the recpot table is psp_values[psp_offset[s] .. +psp_npts[s])
+ * in the file's units (eV*A^3) with psp_maxg_invA[s] the file's maxG (1/A). */
+int ofd_build_ion_potential(ofd_ctx* ctx, int nspecies, const int* type_start, const double* frac_xyz,
+                            const double* psp_maxg_invA, const int* psp_npts, const int* psp_offset,
+                            const double* psp_values, int spline_order, double e_ionion);
+/* the ionic potential currently held (n1*n2*n3) */
+int ofd_get_ion_potential(ofd_ctx* ctx, double* vion_real);
+
+/* WGC 1D kernel table w(eta), w'(eta), w''(eta) on increasing knots eta, as
+ * makeKernel2 produces it (KEDF_KernelODE.f90:254-478). The 3D kernels
+ * K1..K4 are then filled on the device (FillWGC_ReciprocalSpace,
+ * KEDF_WGCkernel.f90:109-277). */
+int ofd_set_wgc_table(ofd_ctx* ctx, int nknots, const double* eta, const double* w, const double* w1,
+                      const double* w2);
+/* Same table integrated by this library (adaptive RK on the host, on
+ * makeKernel2's knots); returns the knot count in *nknots_out if not NULL. */
+int ofd_build_wgc_table(ofd_ctx* ctx, int* nknots_out);
+
+/* CalculatePotentialPlus(rho, optSqrt, potential, eTable) (CalPotential.f90:41).
+ * etable9 == NULL means the reference's call without eTable (no energies). */
+int ofd_calpot(ofd_ctx* ctx, const double* rho, int opt_sqrt, double* potential, double* etable9);
+/* Same with device-resident rho/potential (pointers from hipMalloc). */
+int ofd_calpot_device(ofd_ctx* ctx, const double* rho_device, int opt_sqrt, double* potential_device,
+                      double* etable9);
+
+/* Asynchronous form for device-resident callers (no reference counterpart:
+ * the reference is synchronous): enqueues the evaluation on the context's
+ * stream and returns at once. sums9_device (device, 9 doubles, may be NULL)
+ * receives the raw energy sums; ofd_sums_to_etable turns them into eTable. */
+int ofd_calpot_enqueue(ofd_ctx* ctx, const double* rho_device, int opt_sqrt, double* potential_device,
+                       double* sums9_device);
+int ofd_sums_to_etable(const ofd_ctx* ctx, const double* sums9, double* etable9);
+/* the context's HIP stream (hipStream_t) and a blocking wait on it */
+void* ofd_get_stream(ofd_ctx* ctx);
+int ofd_sync(ofd_ctx* ctx);
+/* Batched evaluation of nb contexts of one grid shape and functional in one
+ * launch set (device-resident rho/potential per context); etables: nb x 9 on
+ * the host, or NULL. */
+int ofd_calpot_many(ofd_ctx* const* ctxs, int nb, const double* const* rho_device, double* const* potential_device,
+                    int opt_sqrt, double* etables);
+
+/* ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60-850) with
+ * dir_type = 1 (NTN) and numSpin = 1; rhoR in/out (SYS::rhoR), energy9 out,
+ * rho_outcome as RhoOptimizers::rhoOutcome (0 converged, 1 failed). */
+int ofd_rho_opt(ofd_ctx* ctx, double* rhoR, int dir_type, const ofd_opt_params* params, double* energy9,
+                int* rho_outcome, ofd_opt_stats* stats);
+
+/* Ion-electron forces from the B-spline structure factor
+ * (IonElectronForcesSpline, IonElectronSpline.f90:219-355, called with
+ * FFT(rho) as in CalForce.f90:180). forces: 3 per ion (Ha/bohr), ion order as
+ * given to ofd_build_ion_potential. */
+int ofd_ion_forces(ofd_ctx* ctx, const double* rho, double* forces);
+
+/* ---- batched serving (no reference counterpart: many density
+ *      optimizations of one grid shape advanced together) ---- */
+typedef struct ofd_pool ofd_pool;
+/* A pool of up to `capacity` concurrent requests of grid n1*n2*n3. */
+int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic, int precision, ofd_pool** out);
+void ofd_pool_destroy(ofd_pool* pool);
+const char* ofd_pool_last_error(const ofd_pool* pool);
+/* Admit a request: takes ownership of a context (same grid, ion terms set)
+ * and the starting density; returns the slot id (>= 0) or a negative status
+ * when the pool is full. */
+int ofd_pool_admit(ofd_pool* pool, ofd_ctx* ctx, const double* rho_init, const ofd_opt_params* params);
+/* Advance every active request by one evaluation (one batched launch set).
+ * Returns the number of requests still active. */
+int ofd_pool_step(ofd_pool* pool);
+/* Collect a finished request: returns 1 and fills outputs when slot `slot`
+ * has finished, 0 while it is running. The slot is freed on collection and
+ * the context is handed back to the caller. */
+int ofd_pool_collect(ofd_pool* pool, int slot, double* rho_out, double* energy9, int* rho_outcome,
+                     ofd_opt_stats* stats, ofd_ctx** ctx_out);
+/* Device bytes one request of this grid needs (admission by HBM footprint). */
+long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PROFESS_AMD_OFD_H */

@@ -1,0 +1,106 @@
+"""ctypes binding of libprofess_amd.so (the C ABI declared in include/ofd.h).
+
+The library is required: importing this module raises when it is missing, and
+every wrapper raises ``OFDError`` with the library's message on a nonzero status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libprofess_amd.so")
+
+
+class OFDError(RuntimeError):
+    pass
+
+
+class ofd_grid(C.Structure):
+    _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int), ("cell", C.c_double * 9)]
+
+
+class ofd_functional(C.Structure):
+    _fields_ = [("kinetic", C.c_int), ("lambda_", C.c_double), ("mu", C.c_double), ("alpha", C.c_double),
+                ("beta", C.c_double), ("gamma", C.c_double), ("rho0", C.c_double), ("rhoS", C.c_double),
+                ("precision", C.c_int)]
+
+
+class ofd_opt_params(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("pot_tol", C.c_double), ("tole", C.c_double), ("conv_check", C.c_int),
+                ("max_line", C.c_int), ("mixed_switch", C.c_double)]
+
+
+class ofd_opt_stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("num_evals", C.c_int), ("newton_iters_total", C.c_int),
+                ("fp32_evals", C.c_int), ("final_pot_norm", C.c_double), ("seconds", C.c_double)]
+
+
+_lib = None
+
+# every symbol the header declares (checked by the CPU test suite)
+EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd_set_ion_terms",
+           "ofd_build_ion_potential", "ofd_get_ion_potential", "ofd_set_wgc_table", "ofd_build_wgc_table",
+           "ofd_calpot", "ofd_calpot_device", "ofd_rho_opt", "ofd_ion_forces", "ofd_pool_create",
+           "ofd_pool_destroy", "ofd_pool_last_error", "ofd_pool_admit", "ofd_pool_step", "ofd_pool_collect",
+           "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
+           "ofd_calpot_many"]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OFDError("libprofess_amd.so is not built (run `python -m profess_amd.build`)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    L.ofd_create.argtypes = [C.POINTER(ofd_grid), C.POINTER(ofd_functional), C.c_double, C.POINTER(vp)]
+    L.ofd_destroy.argtypes = [vp]
+    L.ofd_destroy.restype = None
+    L.ofd_last_error.argtypes = [vp]
+    L.ofd_last_error.restype = C.c_char_p
+    L.ofd_get_params.argtypes = [vp, dp]
+    L.ofd_set_ion_terms.argtypes = [vp, dp, C.c_double]
+    L.ofd_build_ion_potential.argtypes = [vp, C.c_int, ip, dp, dp, ip, ip, dp, C.c_int, C.c_double]
+    L.ofd_get_ion_potential.argtypes = [vp, dp]
+    L.ofd_set_wgc_table.argtypes = [vp, C.c_int, dp, dp, dp, dp]
+    L.ofd_build_wgc_table.argtypes = [vp, ip]
+    L.ofd_calpot.argtypes = [vp, dp, C.c_int, dp, dp]
+    L.ofd_calpot_device.argtypes = [vp, C.c_void_p, C.c_int, C.c_void_p, dp]
+    L.ofd_rho_opt.argtypes = [vp, dp, C.c_int, C.POINTER(ofd_opt_params), dp, ip, C.POINTER(ofd_opt_stats)]
+    L.ofd_ion_forces.argtypes = [vp, dp, dp]
+    L.ofd_pool_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+    L.ofd_pool_destroy.argtypes = [vp]
+    L.ofd_pool_destroy.restype = None
+    L.ofd_pool_last_error.argtypes = [vp]
+    L.ofd_pool_last_error.restype = C.c_char_p
+    L.ofd_pool_admit.argtypes = [vp, vp, dp, C.POINTER(ofd_opt_params)]
+    L.ofd_pool_step.argtypes = [vp]
+    L.ofd_pool_collect.argtypes = [vp, C.c_int, dp, dp, ip, C.POINTER(ofd_opt_stats), C.POINTER(vp)]
+    L.ofd_request_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.ofd_request_bytes.restype = C.c_longlong
+    L.ofd_calpot_enqueue.argtypes = [vp, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.ofd_sums_to_etable.argtypes = [vp, dp, dp]
+    L.ofd_get_stream.argtypes = [vp]
+    L.ofd_get_stream.restype = C.c_void_p
+    L.ofd_sync.argtypes = [vp]
+    L.ofd_calpot_many.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, dp]
+    _lib = L
+    return L
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_int))

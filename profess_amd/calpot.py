@@ -1,0 +1,169 @@
+"""Python mirror of the reference's call surface for the energy/potential path.
+
+Names follow PROFESS: ``CalculatePotentialPlus`` (CalPotential.f90:41),
+``ProjNormRhoMinimization`` (RhoOptN.f90:60), ``IonElectronForcesSpline``
+(IonElectronSpline.f90:219). Arrays are numpy float64 of shape (n3, n2, n1),
+C order, which is the reference's Fortran rho(n1,n2,n3) memory layout.
+Everything runs through libprofess_amd.so on the GPU; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (OFDError, dptr, iptr, lib, ofd_functional, ofd_grid, ofd_opt_params, ofd_opt_stats)
+
+ETABLE_NAMES = ["total", "kinetic", "ion-electron", "hartree", "xc", "ion-ion", "tf", "vw", "nonlocal"]
+CONV = {"ENE": 1, "POT": 2, "BOTH": 3}
+
+
+class OFDSystem:
+    """One cell + grid + functional: the module state CalculatePotentialPlus reads.
+
+    lattice: 3x3, columns are the lattice vectors in bohr (cell%cellReal).
+    kinetic: 4 = Wang-Teter, 5 = WGC. Negative parameters take the reference's
+    defaults (SetupKEDF.f90:217-322). precision: 0 FP64, 1 FP32 FFT passes.
+    """
+
+    def __init__(self, lattice, grid, num_electrons, kinetic=4, lam=-100.0, mu=-100.0, alpha=-100.0,
+                 beta=-100.0, gamma=-1.0, rho0=-1.0, rhoS=-1.0, precision=0):
+        L = lib()
+        self.n1, self.n2, self.n3 = (int(x) for x in grid)
+        self.lattice = np.asarray(lattice, dtype=np.float64).reshape(3, 3)
+        g = ofd_grid(self.n1, self.n2, self.n3, (C.c_double * 9)(*self.lattice.T.ravel()))
+        f = ofd_functional(kinetic, lam, mu, alpha, beta, gamma, rho0, rhoS, precision)
+        h = C.c_void_p()
+        rc = L.ofd_create(C.byref(g), C.byref(f), float(num_electrons), C.byref(h))
+        self._h = h
+        self.kinetic = kinetic
+        self.num_electrons = float(num_electrons)
+        if rc != 0:
+            msg = L.ofd_last_error(h).decode() if h.value else "ofd_create failed"
+            self.close()
+            raise OFDError(msg)
+        self.vol = abs(np.linalg.det(self.lattice))
+        self.dV = self.vol / (self.n1 * self.n2 * self.n3)
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ofd_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def detach(self):
+        """Hand the native context over (to a pool); this object no longer owns it."""
+        h = self._h
+        self._h = None
+        return h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise OFDError(lib().ofd_last_error(self._h).decode())
+
+    @property
+    def shape(self):
+        return (self.n3, self.n2, self.n1)
+
+    @property
+    def params(self):
+        out = np.zeros(7)
+        self._check(lib().ofd_get_params(self._h, dptr(out)))
+        return dict(zip(["alpha", "beta", "gamma", "rho0", "rhoS", "lambda", "mu"], out.tolist()))
+
+    # -- per-geometry tables
+    def set_ion_terms(self, vion, e_ionion=0.0):
+        v = np.ascontiguousarray(vion, dtype=np.float64)
+        assert v.shape == self.shape
+        self._check(lib().ofd_set_ion_terms(self._h, dptr(v), float(e_ionion)))
+
+    def build_ion_potential(self, species, spline_order=10, e_ionion=0.0):
+        """species: list of (maxg_invA, values_eVA3, frac (nion, 3)) in .recpot file units."""
+        starts = [0]
+        fr, vals, npts, offs, maxg = [], [], [], [], []
+        for mg, v, f in species:
+            f = np.asarray(f, dtype=np.float64).reshape(-1, 3)
+            fr.append(f)
+            starts.append(starts[-1] + f.shape[0])
+            offs.append(sum(len(x) for x in vals))
+            vals.append(np.asarray(v, dtype=np.float64))
+            npts.append(len(v))
+            maxg.append(float(mg))
+        frac = np.ascontiguousarray(np.concatenate(fr), dtype=np.float64)
+        ts = np.array(starts, dtype=np.int32)
+        self._check(lib().ofd_build_ion_potential(
+            self._h, len(species), iptr(ts), dptr(frac), dptr(np.array(maxg)),
+            iptr(np.array(npts, dtype=np.int32)), iptr(np.array(offs, dtype=np.int32)),
+            dptr(np.ascontiguousarray(np.concatenate(vals))), int(spline_order), float(e_ionion)))
+        self.nion = frac.shape[0]
+
+    def ion_potential(self):
+        out = np.empty(self.shape)
+        self._check(lib().ofd_get_ion_potential(self._h, dptr(out)))
+        return out
+
+    def set_wgc_table(self, eta, w, w1, w2):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (eta, w, w1, w2)]
+        self._check(lib().ofd_set_wgc_table(self._h, a[0].size, *[dptr(x) for x in a]))
+
+    def build_wgc_table(self):
+        n = C.c_int(0)
+        self._check(lib().ofd_build_wgc_table(self._h, C.byref(n)))
+        return n.value
+
+
+def CalculatePotentialPlus(system: OFDSystem, rho, optSqrt=True, eTable=True):
+    """CalculatePotentialPlus(rho, optSqrt, potential[, eTable]) (CalPotential.f90:41).
+    Returns (potential, eTable(9) in Hartree or None)."""
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    if r.shape != system.shape:
+        raise ValueError("rho has shape %r, grid is %r" % (r.shape, system.shape))
+    pot = np.empty_like(r)
+    et = np.zeros(9) if eTable else None
+    system._check(lib().ofd_calpot(system._h, dptr(r), 1 if optSqrt else 0, dptr(pot), dptr(et)))
+    return pot, et
+
+
+def CalculatePotentialPlusDevice(system: OFDSystem, rho_ptr, pot_ptr, optSqrt=True, eTable=True):
+    """Device-resident form: rho_ptr/pot_ptr are device addresses (e.g. tensor.data_ptr())."""
+    et = np.zeros(9) if eTable else None
+    system._check(lib().ofd_calpot_device(system._h, C.c_void_p(rho_ptr), 1 if optSqrt else 0,
+                                          C.c_void_p(pot_ptr), dptr(et)))
+    return et
+
+
+def opt_params(max_iter=-1, pot_tol=-1.0, tole=-1.0, conv_check="ENE", max_line=-1, mixed_switch=0.0):
+    return ofd_opt_params(int(max_iter), float(pot_tol), float(tole), CONV[conv_check.upper()], int(max_line),
+                          float(mixed_switch))
+
+
+def ProjNormRhoMinimization(system: OFDSystem, rhoR, dir_type=1, **kw):
+    """ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60) with NTN.
+    Returns (rho, energy(9), rhoOutcome, stats dict)."""
+    r = np.array(rhoR, dtype=np.float64, order="C", copy=True)
+    et = np.zeros(9)
+    outc = C.c_int(-1)
+    st = ofd_opt_stats()
+    p = opt_params(**kw)
+    system._check(lib().ofd_rho_opt(system._h, dptr(r), int(dir_type), C.byref(p), dptr(et), C.byref(outc),
+                                    C.byref(st)))
+    stats = {k: getattr(st, k) for k, _ in st._fields_}
+    return r, et, outc.value, stats
+
+
+def IonElectronForcesSpline(system: OFDSystem, rho):
+    """Ion-electron forces (Ha/bohr), ions in the order given to build_ion_potential."""
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    out = np.zeros((system.nion, 3))
+    system._check(lib().ofd_ion_forces(system._h, dptr(r), dptr(out)))
+    return out
+
+
+def request_bytes(grid, kinetic=4, precision=0):
+    return int(lib().ofd_request_bytes(int(grid[0]), int(grid[1]), int(grid[2]), int(kinetic), int(precision)))

@@ -1,0 +1,1342 @@
+// C ABI of libprofess_amd.so: contexts, CalculatePotentialPlus, the NTN
+// density optimizer and the batched request pool.
+//
+// The optimizer is a resumable state machine (NtnSolver) that stops every
+// time it needs one evaluation of the potential; a single-request call
+// (ofd_rho_opt) drives one solver to completion, and the pool drives many,
+// batching the evaluations of all requests that share a grid shape. Vector
+// state lives on the device; the host keeps only scalars (CG coefficients,
+// the Moré–Thuente line-search state, convergence bookkeeping), exactly the
+// data the reference keeps in scalars (RhoOptN.f90, RhoDirNew.f90,
+// RhoLineSearch.f90).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ofd.h"
+#include "ofd_internal.h"
+
+using namespace ofd;
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643;
+constexpr double kCTF = 2.87123400018819;
+
+#define HIPCHK(x)                                                          \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess) {                                                \
+      throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    }                                                                      \
+  } while (0)
+
+template <typename T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  return (T*)p;
+}
+
+// ------------------------------------------------------------------ Moré–Thuente
+// Restates MINPACK-2 dcstep/dcsrch (MathDCstep.f, MathDCsrch.f) as used by
+// LineSearchTN (RhoLineSearch.f90:369-419).
+struct LineSearch {
+  double ftol, gtol, xtol, stpmin, stpmax;
+  bool started = false;
+  bool brackt = false;
+  int stage = 1;
+  double ginit = 0, gtest = 0, gx = 0, gy = 0, finit = 0, fx = 0, fy = 0, stx = 0, sty = 0, stmin = 0, stmax = 0,
+         width = 0, width1 = 0;
+
+  static void step(double& stx, double& fx, double& dx, double& sty, double& fy, double& dy, double& stp, double fp,
+                   double dp, bool& brackt, double stpmin, double stpmax) {
+    const double sgnd = dp * (dx / std::fabs(dx));
+    double stpf, stpc, stpq, theta, s, gamma, p, q, r;
+    if (fp > fx) {
+      theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+      s = std::fmax(std::fabs(theta), std::fmax(std::fabs(dx), std::fabs(dp)));
+      gamma = s * std::sqrt((theta / s) * (theta / s) - (dx / s) * (dp / s));
+      if (stp < stx) gamma = -gamma;
+      p = (gamma - dx) + theta;
+      q = ((gamma - dx) + gamma) + dp;
+      r = p / q;
+      stpc = stx + r * (stp - stx);
+      stpq = stx + ((dx / ((fx - fp) / (stp - stx) + dx)) / 2.0) * (stp - stx);
+      stpf = std::fabs(stpc - stx) < std::fabs(stpq - stx) ? stpc : stpc + (stpq - stpc) / 2.0;
+      brackt = true;
+    } else if (sgnd < 0.0) {
+      theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+      s = std::fmax(std::fabs(theta), std::fmax(std::fabs(dx), std::fabs(dp)));
+      gamma = s * std::sqrt((theta / s) * (theta / s) - (dx / s) * (dp / s));
+      if (stp > stx) gamma = -gamma;
+      p = (gamma - dp) + theta;
+      q = ((gamma - dp) + gamma) + dx;
+      r = p / q;
+      stpc = stp + r * (stx - stp);
+      stpq = stp + (dp / (dp - dx)) * (stx - stp);
+      stpf = std::fabs(stpc - stp) > std::fabs(stpq - stp) ? stpc : stpq;
+      brackt = true;
+    } else if (std::fabs(dp) < std::fabs(dx)) {
+      theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+      s = std::fmax(std::fabs(theta), std::fmax(std::fabs(dx), std::fabs(dp)));
+      gamma = s * std::sqrt(std::fmax(0.0, (theta / s) * (theta / s) - (dx / s) * (dp / s)));
+      if (stp > stx) gamma = -gamma;
+      p = (gamma - dp) + theta;
+      q = (gamma + (dx - dp)) + gamma;
+      r = p / q;
+      if (r < 0.0 && gamma != 0.0) stpc = stp + r * (stx - stp);
+      else if (stp > stx) stpc = stpmax;
+      else stpc = stpmin;
+      stpq = stp + (dp / (dp - dx)) * (stx - stp);
+      if (brackt) {
+        stpf = std::fabs(stpc - stp) < std::fabs(stpq - stp) ? stpc : stpq;
+        if (stp > stx) stpf = std::fmin(stp + 0.66 * (sty - stp), stpf);
+        else stpf = std::fmax(stp + 0.66 * (sty - stp), stpf);
+      } else {
+        stpf = std::fabs(stpc - stp) > std::fabs(stpq - stp) ? stpc : stpq;
+        stpf = std::fmin(stpmax, stpf);
+        stpf = std::fmax(stpmin, stpf);
+      }
+    } else {
+      if (brackt) {
+        theta = 3.0 * (fp - fy) / (sty - stp) + dy + dp;
+        s = std::fmax(std::fabs(theta), std::fmax(std::fabs(dy), std::fabs(dp)));
+        gamma = s * std::sqrt((theta / s) * (theta / s) - (dy / s) * (dp / s));
+        if (stp > sty) gamma = -gamma;
+        p = (gamma - dp) + theta;
+        q = ((gamma - dp) + gamma) + dy;
+        r = p / q;
+        stpc = stp + r * (sty - stp);
+        stpf = stpc;
+      } else if (stp > stx) {
+        stpf = stpmax;
+      } else {
+        stpf = stpmin;
+      }
+    }
+    if (fp > fx) {
+      sty = stp; fy = fp; dy = dp;
+    } else {
+      if (sgnd < 0.0) { sty = stx; fy = fx; dy = dx; }
+      stx = stp; fx = fp; dx = dp;
+    }
+    stp = stpf;
+  }
+
+  // returns 1 = FG (evaluate at stp), 2 = convergence, 3 = warning, 4 = error
+  int next(double& stp, double f, double g) {
+    if (!started) {
+      started = true;
+      if (stp < stpmin || stp > stpmax || g >= 0.0 || ftol < 0.0 || gtol < 0.0 || xtol < 0.0 || stpmin < 0.0 ||
+          stpmax < stpmin)
+        return 4;
+      brackt = false;
+      stage = 1;
+      finit = f;
+      ginit = g;
+      gtest = ftol * ginit;
+      width = stpmax - stpmin;
+      width1 = width / 0.5;
+      stx = 0.0; fx = finit; gx = ginit;
+      sty = 0.0; fy = finit; gy = ginit;
+      stmin = 0.0;
+      stmax = stp + 4.0 * stp;
+      return 1;
+    }
+    const double ftest = finit + stp * gtest;
+    if (stage == 1 && f <= ftest && g >= 0.0) stage = 2;
+    int task = 1;
+    if (brackt && (stp <= stmin || stp >= stmax)) task = 3;
+    if (brackt && stmax - stmin <= xtol * stmax) task = 3;
+    if (stp == stpmax && f <= ftest && g <= gtest) task = 3;
+    if (stp == stpmin && (f > ftest || g >= gtest)) task = 3;
+    if (f <= ftest && std::fabs(g) <= gtol * (-ginit)) task = 2;
+    if (task != 1) return task;
+    if (stage == 1 && f <= fx && f > ftest) {
+      double fm = f - stp * gtest, fxm = fx - stx * gtest, fym = fy - sty * gtest;
+      double gm = g - gtest, gxm = gx - gtest, gym = gy - gtest;
+      step(stx, fxm, gxm, sty, fym, gym, stp, fm, gm, brackt, stmin, stmax);
+      fx = fxm + stx * gtest;
+      fy = fym + sty * gtest;
+      gx = gxm + gtest;
+      gy = gym + gtest;
+    } else {
+      step(stx, fx, gx, sty, fy, gy, stp, f, g, brackt, stmin, stmax);
+    }
+    if (brackt) {
+      if (std::fabs(sty - stx) >= 0.66 * width1) stp = stx + 0.5 * (sty - stx);
+      width1 = width;
+      width = std::fabs(sty - stx);
+    }
+    if (brackt) {
+      stmin = std::fmin(stx, sty);
+      stmax = std::fmax(stx, sty);
+    } else {
+      stmin = stp + 1.1 * (stp - stx);
+      stmax = stp + 4.0 * (stp - stx);
+    }
+    stp = std::fmax(stp, stpmin);
+    stp = std::fmin(stp, stpmax);
+    if ((brackt && (stp <= stmin || stp >= stmax)) || (brackt && stmax - stmin <= xtol * stmax)) stp = stx;
+    return 1;
+  }
+};
+
+}  // namespace
+
+// ====================================================================== context
+struct ofd_ctx {
+  std::string err;
+  ShapePlan sp;
+  int kind = kWT3;
+  int kinetic = 4;
+  int precision = 0;
+  double lat[9];      // columns = lattice vectors (bohr)
+  double rb[9];       // G_c = sum_j rb[3c+j] m_j
+  double vol = 0, dV = 0, num_ele = 0;
+  double lam = 1, mu = 1, alpha = -100, beta = -100, gamma = -1, rho0 = -1, rhoS = -1, coef = 0, tkf = 0;
+  double e_ionion = 0;
+  bool have_vion = false, have_wgc = false;
+  hipStream_t st = nullptr;
+  // device buffers
+  double* vion = nullptr;
+  double* rho = nullptr;     // staging of a host density
+  double* pot = nullptr;     // staging of a host potential
+  void* work = nullptr;      // NF x nc complex (FP64 size)
+  double* sums = nullptr;    // partials
+  double* fin = nullptr;     // kNSum finalized sums
+  double* wgck = nullptr;    // 4 x nc
+  ReqDesc* desc = nullptr;   // one descriptor
+  ReqDesc last_desc{};       // what `desc` holds after an enqueue
+  ReqDesc* many_desc = nullptr;
+  double* many_sums = nullptr;
+  int many_cap = 0;
+  std::vector<ReqDesc> many_last;
+  // optimizer vectors
+  double *phi = nullptr, *dir = nullptr, *dedphi = nullptr, *dldphi = nullptr, *r = nullptr, *p = nullptr,
+         *ap = nullptr, *tmp = nullptr;
+  double* dot_part = nullptr;
+  double* dot_out = nullptr;
+  // ion data kept for forces
+  std::vector<int> type_start;
+  std::vector<double> frac;
+  std::vector<double> psp_vqs, psp_dd, psp_maxg, psp_v0;
+  std::vector<int> psp_off, psp_npts, psp_charge;
+  int spline_order = 10;
+  double* d_frac = nullptr;
+  double* d_psp_vqs = nullptr;
+  double* d_psp_dd = nullptr;
+
+  ReqDesc base_desc() const {
+    ReqDesc D;
+    std::memset(&D, 0, sizeof(D));
+    D.vion = vion;
+    D.sums = sums;
+    D.wgck = wgck;
+    D.work = work;
+    for (int i = 0; i < 9; ++i) D.rb[i] = rb[i];
+    D.lam = lam; D.mu = mu; D.alpha = alpha; D.beta = beta; D.rhoS = rhoS; D.coef = coef; D.tkf = tkf;
+    D.opt_sqrt = 1;
+    return D;
+  }
+  long long nr() const { return sp.d.nr; }
+};
+
+static int fail(ofd_ctx* c, const std::string& m) {
+  if (c) c->err = m;
+  return -1;
+}
+
+// Inverse-transpose of the lattice (long double), G vectors rb = 2 pi A^{-T}
+// (CellInfo.f90:281), volume |det A| (MathFunctions Volume).
+static void cell_setup(const double* cell, double* rb, double* vol, double* inv_t) {
+  long double a[3][3];
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) a[i][j] = cell[3 * j + i];
+  long double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                    a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  long double inv[3][3];
+  inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+  inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+  inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+  inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+  inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+  inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+  inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+  inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+  inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+  const long double tp = 6.283185307179586476925286766559005768L;
+  // recip = inv^T * 2 pi ; rb[3c + j] = recip(c, j) = inv(j, c) * 2pi
+  for (int c = 0; c < 3; ++c)
+    for (int j = 0; j < 3; ++j) {
+      rb[3 * c + j] = (double)(inv[j][c] * tp);
+      if (inv_t) inv_t[3 * c + j] = (double)inv[j][c];
+    }
+  *vol = (double)fabsl(det);
+}
+
+extern "C" int ofd_create(const ofd_grid* g, const ofd_functional* f, double num_electrons, ofd_ctx** out) {
+  if (!g || !f || !out) return -1;
+  *out = nullptr;
+  ofd_ctx* c = new ofd_ctx();
+  try {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+      c->err = "no HIP device available";
+      *out = c;
+      return -2;
+    }
+    if (shape_plan_init(&c->sp, g->n1, g->n2, g->n3, &c->err)) {
+      *out = c;
+      return -1;
+    }
+    std::memcpy(c->lat, g->cell, sizeof(c->lat));
+    cell_setup(g->cell, c->rb, &c->vol, nullptr);
+    if (!(c->vol > 0.0)) {
+      c->err = "cell volume is zero";
+      *out = c;
+      return -1;
+    }
+    c->dV = c->vol / (double)c->sp.d.nr;    // CellInfo.f90:288
+    c->num_ele = num_electrons;
+    c->kinetic = f->kinetic;
+    c->precision = f->precision;
+    c->lam = f->lambda < -99.0 ? 1.0 : f->lambda;
+    c->mu = f->mu < -99.0 ? 1.0 : f->mu;
+    const double tmp_rho = num_electrons / c->vol;
+    if (f->kinetic == 4) {   // SetupKEDF.f90:217-251, KEDFRefresh :654-660
+      c->alpha = f->alpha < 0.0 ? 5.0 / 6.0 : f->alpha;
+      c->beta = f->beta < 0.0 ? c->alpha : f->beta;
+      c->rho0 = f->rho0 > 0.0 ? f->rho0 : tmp_rho;
+      c->coef = 5.0 / (9.0 * c->alpha * c->beta * std::pow(c->rho0, c->alpha + c->beta - 5.0 / 3.0));
+      c->tkf = 2.0 * std::pow(3.0 * c->rho0 * kPi * kPi, 1.0 / 3.0);
+      c->kind = c->alpha == c->beta ? kWT3 : kWT4;
+    } else if (f->kinetic == 5) {   // SetupKEDF.f90:253-322, KEDFRefresh :662-665
+      c->alpha = f->alpha < 0.0 ? (5.0 + std::sqrt(5.0)) / 6.0 : f->alpha;
+      c->beta = f->beta < 0.0 ? (5.0 - std::sqrt(5.0)) / 6.0 : f->beta;
+      c->gamma = f->gamma < 0.0 ? 2.7 : f->gamma;
+      if (c->alpha < c->beta) std::swap(c->alpha, c->beta);
+      c->rho0 = f->rho0 > 0.0 ? f->rho0 : tmp_rho;
+      c->rhoS = f->rhoS > 0.0 ? f->rhoS : tmp_rho;
+      c->kind = kWGC8;
+    } else {
+      c->err = "kinetic must be 4 (WT) or 5 (WGC): other KEDFs are outside this library";
+      *out = c;
+      return -1;
+    }
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    const long long nr = c->sp.d.nr, nc = c->sp.d.nc;
+    const int nf = func_nf(c->kind);
+    c->vion = dalloc<double>(nr);
+    HIPCHK(hipMemset(c->vion, 0, nr * sizeof(double)));
+    c->rho = dalloc<double>(nr);
+    c->pot = dalloc<double>(nr);
+    c->work = dalloc<double2>((size_t)nf * nc);
+    const long long nrows = (long long)g->n2 * g->n3;
+    c->sums = dalloc<double>((size_t)nrows * kNSum);
+    c->fin = dalloc<double>(kNSum);
+    if (c->kind == kWGC8) c->wgck = dalloc<double>(4 * nc);
+    c->desc = dalloc<ReqDesc>(1);
+    c->dot_part = dalloc<double>(512 * 4);
+    c->dot_out = dalloc<double>(4);
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    *out = c;
+    return -1;
+  }
+  *out = c;
+  return 0;
+}
+
+static void free_opt(ofd_ctx* c) {
+  double** v[] = {&c->phi, &c->dir, &c->dedphi, &c->dldphi, &c->r, &c->p, &c->ap, &c->tmp};
+  for (auto pp : v) {
+    hipFree(*pp);
+    *pp = nullptr;
+  }
+}
+
+extern "C" void ofd_destroy(ofd_ctx* c) {
+  if (!c) return;
+  if (c->st) hipStreamSynchronize(c->st);
+  free_opt(c);
+  hipFree(c->vion); hipFree(c->rho); hipFree(c->pot); hipFree(c->work); hipFree(c->sums); hipFree(c->fin);
+  hipFree(c->wgck); hipFree(c->desc); hipFree(c->dot_part); hipFree(c->dot_out);
+  hipFree(c->many_desc); hipFree(c->many_sums);
+  hipFree(c->d_frac); hipFree(c->d_psp_vqs); hipFree(c->d_psp_dd);
+  if (c->sp.tw_x) shape_plan_free(&c->sp);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+}
+
+extern "C" const char* ofd_last_error(const ofd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int ofd_get_params(const ofd_ctx* c, double* o) {
+  if (!c || !o) return -1;
+  o[0] = c->alpha; o[1] = c->beta; o[2] = c->gamma; o[3] = c->rho0; o[4] = c->rhoS; o[5] = c->lam; o[6] = c->mu;
+  return 0;
+}
+
+extern "C" int ofd_set_ion_terms(ofd_ctx* c, const double* vion, double eii) {
+  if (!c || !vion) return fail(c, "null argument");
+  if (hipMemcpy(c->vion, vion, c->nr() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(c, "copy of the ion potential failed");
+  c->e_ionion = eii;
+  c->have_vion = true;
+  return 0;
+}
+
+extern "C" int ofd_get_ion_potential(ofd_ctx* c, double* vion) {
+  if (!c || !vion) return fail(c, "null argument");
+  if (hipMemcpy(vion, c->vion, c->nr() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(c, "copy of the ion potential failed");
+  return 0;
+}
+
+// eTable from the finalized sums (CalculateEnergy, CalPotential.f90:610-644)
+static void etable_from_sums(const ofd_ctx* c, const double* s, double* et) {
+  double loc[9] = {0};
+  loc[2] = s[0];
+  loc[3] = s[1] / 2.0;
+  loc[4] = s[2];
+  loc[6] = c->lam * kCTF * s[3];
+  loc[7] = 0.5 * s[4] * c->mu;
+  loc[8] = kCTF * s[5];
+  loc[1] = loc[6] + loc[7] + loc[8];
+  for (int i = 0; i < 9; ++i) et[i] = loc[i] * c->dV;
+  et[5] = c->e_ionion;
+  et[0] = et[1] + et[2] + et[3] + et[4] + et[5];
+}
+
+// One evaluation on the context's own descriptor; sums (kNSum) copied back.
+static int eval_one(ofd_ctx* c, const ReqDesc& D, int inmode, int outmode, int fp32, double* h_sums) {
+  if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set (ofd_set_wgc_table)");
+  if (hipMemcpy(c->desc, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(c, "descriptor upload failed");
+  c->last_desc = D;
+  int rc = run_eval(c->sp, c->kind, inmode, outmode, fp32, c->desc, 1, c->st);
+  if (rc) return fail(c, "kernel launch failed");
+  if (run_finalize(c->sp, c->kind, fp32, c->desc, 1, c->fin, c->st)) return fail(c, "finalize launch failed");
+  if (h_sums) {
+    if (hipMemcpyAsync(h_sums, c->fin, kNSum * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess)
+      return fail(c, "sum copy failed");
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess) return fail(c, "stream synchronize failed");
+  return 0;
+}
+
+extern "C" int ofd_calpot_device(ofd_ctx* c, const double* rho, int opt_sqrt, double* pot, double* et) {
+  if (!c || !rho || !pot) return fail(c, "null argument");
+  ReqDesc D = c->base_desc();
+  D.u = rho;
+  D.out = pot;
+  D.opt_sqrt = opt_sqrt ? 1 : 0;
+  double s[kNSum];
+  if (eval_one(c, D, kInRho, kOutPot, c->precision == 1, s)) return -1;
+  if (et) etable_from_sums(c, s, et);
+  return 0;
+}
+
+extern "C" int ofd_calpot_enqueue(ofd_ctx* c, const double* rho, int opt_sqrt, double* pot, double* sums_dev) {
+  if (!c || !rho || !pot) return fail(c, "null argument");
+  if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set (ofd_set_wgc_table)");
+  ReqDesc D = c->base_desc();
+  D.u = rho;
+  D.out = pot;
+  D.opt_sqrt = opt_sqrt ? 1 : 0;
+  // the descriptor is uploaded only when it changes (blocking copy, so the
+  // host copy may go out of scope at once)
+  if (std::memcmp(&D, &c->last_desc, sizeof(D)) != 0) {
+    if (hipStreamSynchronize(c->st) != hipSuccess) return fail(c, "stream synchronize failed");
+    if (hipMemcpy(c->desc, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(c, "descriptor upload failed");
+    c->last_desc = D;
+  }
+  const int fp32 = c->precision == 1;
+  if (run_eval(c->sp, c->kind, kInRho, kOutPot, fp32, c->desc, 1, c->st)) return fail(c, "kernel launch failed");
+  if (run_finalize(c->sp, c->kind, fp32, c->desc, 1, sums_dev ? sums_dev : c->fin, c->st))
+    return fail(c, "finalize launch failed");
+  return 0;
+}
+
+extern "C" int ofd_sums_to_etable(const ofd_ctx* c, const double* s, double* et) {
+  if (!c || !s || !et) return -1;
+  etable_from_sums(c, s, et);
+  return 0;
+}
+
+extern "C" void* ofd_get_stream(ofd_ctx* c) { return c ? (void*)c->st : nullptr; }
+
+extern "C" int ofd_sync(ofd_ctx* c) {
+  if (!c) return -1;
+  return hipStreamSynchronize(c->st) == hipSuccess ? 0 : fail(c, "stream synchronize failed");
+}
+
+extern "C" int ofd_calpot_many(ofd_ctx* const* cs, int nb, const double* const* rho, double* const* pot, int opt_sqrt,
+                               double* ets) {
+  if (!cs || nb < 1 || !rho || !pot) return -1;
+  ofd_ctx* c0 = cs[0];
+  for (int i = 1; i < nb; ++i) {
+    if (!cs[i] || cs[i]->sp.d.n1 != c0->sp.d.n1 || cs[i]->sp.d.n2 != c0->sp.d.n2 || cs[i]->sp.d.n3 != c0->sp.d.n3 ||
+        cs[i]->kind != c0->kind || cs[i]->precision != c0->precision)
+      return fail(c0, "ofd_calpot_many: contexts differ in grid, functional or precision");
+  }
+  try {
+    if ((int)c0->many_cap < nb) {
+      hipFree(c0->many_desc);
+      hipFree(c0->many_sums);
+      c0->many_desc = dalloc<ReqDesc>(nb);
+      c0->many_sums = dalloc<double>((size_t)nb * kNSum);
+      c0->many_cap = nb;
+    }
+    std::vector<ReqDesc> h(nb);
+    for (int i = 0; i < nb; ++i) {
+      h[i] = cs[i]->base_desc();
+      h[i].u = rho[i];
+      h[i].out = pot[i];
+      h[i].opt_sqrt = opt_sqrt ? 1 : 0;
+    }
+    if (std::memcmp(h.data(), c0->many_last.data(), 0) != 0 || c0->many_last.size() != (size_t)nb ||
+        std::memcmp(h.data(), c0->many_last.data(), sizeof(ReqDesc) * nb) != 0) {
+      HIPCHK(hipStreamSynchronize(c0->st));
+      HIPCHK(hipMemcpy(c0->many_desc, h.data(), sizeof(ReqDesc) * nb, hipMemcpyHostToDevice));
+      c0->many_last = h;
+    }
+    const int fp32 = c0->precision == 1;
+    if (run_eval(c0->sp, c0->kind, kInRho, kOutPot, fp32, c0->many_desc, nb, c0->st))
+      return fail(c0, "kernel launch failed");
+    if (run_finalize(c0->sp, c0->kind, fp32, c0->many_desc, nb, c0->many_sums, c0->st))
+      return fail(c0, "finalize failed");
+    if (ets) {
+      std::vector<double> s((size_t)nb * kNSum);
+      HIPCHK(hipMemcpyAsync(s.data(), c0->many_sums, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c0->st));
+      HIPCHK(hipStreamSynchronize(c0->st));
+      for (int i = 0; i < nb; ++i) etable_from_sums(cs[i], &s[(size_t)i * kNSum], ets + 9 * (size_t)i);
+    }
+  } catch (const std::exception& e) {
+    return fail(c0, e.what());
+  }
+  return 0;
+}
+
+extern "C" int ofd_calpot(ofd_ctx* c, const double* rho, int opt_sqrt, double* pot, double* et) {
+  if (!c || !rho || !pot) return fail(c, "null argument");
+  const size_t b = c->nr() * sizeof(double);
+  if (hipMemcpyAsync(c->rho, rho, b, hipMemcpyHostToDevice, c->st) != hipSuccess) return fail(c, "copy failed");
+  if (ofd_calpot_device(c, c->rho, opt_sqrt, c->pot, et)) return -1;
+  if (hipMemcpy(pot, c->pot, b, hipMemcpyDeviceToHost) != hipSuccess) return fail(c, "copy failed");
+  return 0;
+}
+
+// ====================================================================== NTN solver
+// ProjNormRhoMinimization with dir_type = 1 (NTN), numSpin = 1.
+namespace {
+
+struct EvalReq {
+  int outmode;
+  const double *u, *v, *bvec;
+  double* out;
+  double ca, cb, s1, s2, mu0, eps;
+  bool energy;
+};
+
+enum SState { S_INIT, S_AFTER_INIT, S_NEWTON_ITER, S_AFTER_NEWTON, S_LINE_EVAL_DONE, S_DONE };
+
+struct NtnSolver {
+  ofd_ctx* c = nullptr;
+  ofd_opt_params prm;
+  // scalars
+  double tot_ele = 0, mu0 = 0, theta = 0.2, energy[9] = {0};
+  double e_hist[3] = {0, 0, 0};
+  int it = 1, outcome = -1, success = 1;
+  // Newton
+  int nit = 0, ntflag = 0;
+  double rr = 0, rr_last = 0, criteria = 0, ry = 0, ry_last = 0, ss1 = 0, eps = 0;
+  // line search
+  LineSearch ls;
+  int nline = 0;
+  double f_cur = 0, g_cur = 0;
+  double last_sums[kNSum];
+  double pot_norm = 0;
+  bool fp32 = false;
+  SState state = S_INIT;
+  ofd_opt_stats stats{};
+  std::vector<double> hist_energy, hist_potnorm;
+  double mach = 2.220446049250313e-16;
+
+  void dots(int np, const double* a0, const double* b0, const double* a1, const double* b1, const double* a2,
+            const double* b2, double* out) {
+    const double* A[4] = {a0, a1, a2, a0};
+    const double* B[4] = {b0, b1, b2, b0};
+    vec_dots(A, B, np, c->nr(), c->dot_part, c->dot_out, c->st);
+    HIPCHK(hipMemcpyAsync(out, c->dot_out, sizeof(double) * np, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+
+  void set_precision_for(double pnorm) {
+    fp32 = (c->precision == 1) && !(prm.mixed_switch > 0.0 && pnorm < prm.mixed_switch);
+    mach = fp32 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+  }
+
+  bool need_eval(EvalReq* q) {
+    switch (state) {
+      case S_INIT:
+        *q = EvalReq{kOutSigned, c->phi, nullptr, nullptr, c->dedphi, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, true};
+        return true;
+      case S_NEWTON_ITER: {
+        *q = EvalReq{kOutNewton, c->phi, c->p, c->dldphi, c->ap, 1.0, eps, 0.0, 0.0, mu0, eps, false};
+        return true;
+      }
+      case S_LINE_EVAL_DONE: {
+        const double ct = std::cos(theta), st = std::sin(theta);
+        *q = EvalReq{kOutSigned, c->phi, c->dir, nullptr, c->dedphi, ct, st, -st, ct, mu0, 0.0, true};
+        return true;
+      }
+      default:
+        return false;
+    }
+  }
+
+  double dV() const { return c->dV; }
+
+  // begin a Newton direction (RhoDirNew.f90:95-115)
+  void newton_begin() {
+    const long long n = c->nr();
+    vec_lincomb(c->r, c->dldphi, c->dldphi, -1.0, 0.0, n, c->st);   // r = -b
+    HIPCHK(hipMemsetAsync(c->dir, 0, n * sizeof(double), c->st));
+    double d2[2];
+    dots(2, c->r, c->r, c->phi, c->phi, nullptr, nullptr, d2);
+    rr = d2[0];
+    ss1 = d2[1];
+    criteria = rr;
+    rr_last = rr * 2.0;
+    nit = 1;
+    ry = 0.0;
+    ntflag = -1;
+    newton_prep_p();
+  }
+
+  // y = r; p = y (+ ry/ryLast p); eps (RhoDirNew.f90:116-147)
+  void newton_prep_p() {
+    const long long n = c->nr();
+    ry_last = ry;
+    ry = rr;   // sum(r*y) with y = r
+    if (nit == 1) vec_lincomb(c->p, c->r, c->r, 1.0, 0.0, n, c->st);
+    else vec_axpby(c->p, c->r, 1.0, ry / ry_last, n, c->st);   // p = r + (ry/ryLast) p
+    double s2;
+    dots(1, c->p, c->p, nullptr, nullptr, nullptr, nullptr, &s2);
+    eps = 2.0 * std::sqrt(mach) * (1.0 + std::sqrt(ss1)) / std::sqrt(s2);
+    state = S_NEWTON_ITER;
+  }
+
+  // getNextGradient + checkGradient loop, then start the line search
+  // (RhoOptN.f90:577-591, RhoLineSearch.f90:40-309)
+  bool line_prep() {
+    const long long n = c->nr();
+    bool switched = false;
+    for (;;) {
+      vec_lincomb(c->tmp, c->phi, c->dir, 1.0, 1.0, n, c->st);
+      double d3[3];
+      dots(2, c->tmp, c->tmp, c->phi, c->dir, nullptr, nullptr, d3);
+      const double norm_phi_dir = std::sqrt(d3[0] * c->dV);
+      const double proj = d3[1] * c->dV;
+      vec_axpby(c->dir, c->phi, -(proj / tot_ele), 1.0, n, c->st);   // dir = dir - phi*<phi,dir>/Ne
+      double dd;
+      dots(1, c->dir, c->dir, nullptr, nullptr, nullptr, nullptr, &dd);
+      const double norm_perp = std::sqrt(dd * c->dV);
+      theta = std::fmin(theta, norm_perp / norm_phi_dir);
+      vec_axpby(c->dir, c->dir, 0.0, std::sqrt(tot_ele) / norm_perp, n, c->st);
+      double gd;
+      dots(1, c->dedphi, c->dir, nullptr, nullptr, nullptr, nullptr, &gd);
+      const double gradth = c->dV * gd;
+      if (gradth > 1.0e5) throw std::runtime_error("The gradient is too large");
+      if (gradth >= 0.0) {
+        if (switched) {
+          outcome = 1;
+          return false;
+        }
+        switched = true;
+        vec_lincomb(c->dir, c->dldphi, c->dldphi, -1.0, 0.0, n, c->st);
+        continue;
+      }
+      // LineSearchTN start (RhoLineSearch.f90:369-374)
+      const float ftol32 = 1e-4f, gtol32 = 2e-1f;   // default-real literals (RhoOptN.f90:255-258)
+      ls = LineSearch();
+      ls.ftol = (double)ftol32;
+      ls.gtol = (double)gtol32;
+      ls.xtol = 1e-12;
+      ls.stpmin = 0.0;
+      ls.stpmax = kPi;
+      nline = 0;
+      f_cur = energy[0];
+      g_cur = gradth;
+      return true;
+    }
+  }
+
+  // drives dcsrch until it asks for an evaluation or finishes
+  bool line_advance() {
+    int task = ls.next(theta, f_cur, g_cur);
+    if (task == 1) {
+      state = S_LINE_EVAL_DONE;
+      return true;
+    }
+    success = task == 2 ? 1 : (task == 3 ? -2 : -1);
+    accept();
+    return false;
+  }
+
+  // StepInformation + CheckExit + update (RhoOptN.f90:595-627, :692-977)
+  void accept() {
+    const long long n = c->nr();
+    if (success == -1 && it > prm.max_iter) throw std::runtime_error("Fail rho optimizations");
+    pot_norm = std::sqrt(last_sums[7] / (double)n);
+    hist_energy.push_back(energy[0]);
+    hist_potnorm.push_back(pot_norm);
+    bool exit_flag = false;
+    const bool pconv = pot_norm < prm.pot_tol;
+    e_hist[2] = e_hist[1];
+    e_hist[1] = e_hist[0];
+    e_hist[0] = energy[0];
+    const bool econv =
+        it >= 3 && std::fabs(e_hist[0] - e_hist[1]) < prm.tole && std::fabs(e_hist[0] - e_hist[2]) < prm.tole;
+    if (prm.conv_check == 1 && econv) { outcome = 0; exit_flag = true; }
+    if (prm.conv_check == 2 && pconv) { outcome = 0; exit_flag = true; }
+    if (prm.conv_check == 3 && (pconv || econv)) { outcome = 0; exit_flag = true; }
+    if (outcome == 1) exit_flag = true;
+    if (it >= prm.max_iter) { outcome = 1; exit_flag = true; }
+    // mixed precision: an FP32 phase never ends the run
+    if (exit_flag && fp32 && outcome == 0) {
+      exit_flag = false;
+      outcome = -1;
+    }
+    // phi = tempPhi; mu; dLdphi
+    const double ct = std::cos(theta), st = std::sin(theta);
+    vec_lincomb(c->tmp, c->phi, c->dir, ct, st, n, c->st);
+    std::swap(c->phi, c->tmp);
+    mu0 = c->dV * last_sums[8] / (2.0 * tot_ele);
+    vec_lincomb(c->dldphi, c->dedphi, c->phi, 1.0, -2.0 * mu0, n, c->st);
+    stats.iterations = it;
+    if (exit_flag) {
+      state = S_DONE;
+      return;
+    }
+    const bool was32 = fp32;
+    set_precision_for(pot_norm);
+    if (was32 && !fp32) {
+      // entering FP64 refinement: FP32 energies never count toward ENE
+      e_hist[0] = e_hist[1] = e_hist[2] = 0.0;
+    }
+    ++it;
+    newton_begin();
+  }
+
+  // consume the sums of the evaluation requested by need_eval
+  void resume(const double* s) {
+    std::memcpy(last_sums, s, sizeof(last_sums));
+    ++stats.num_evals;
+    if (fp32) ++stats.fp32_evals;
+    const long long n = c->nr();
+    switch (state) {
+      case S_INIT: {
+        etable_from_sums(c, s, energy);
+        mu0 = c->dV * s[8] / (2.0 * tot_ele);   // ChemicalPotential
+        vec_lincomb(c->dldphi, c->dedphi, c->phi, 1.0, -2.0 * mu0, n, c->st);
+        double pn;
+        dots(1, c->dldphi, c->dldphi, nullptr, nullptr, nullptr, nullptr, &pn);
+        pot_norm = std::sqrt(pn / (double)n);
+        hist_energy.push_back(energy[0]);
+        hist_potnorm.push_back(pot_norm);
+        e_hist[0] = energy[0];
+        set_precision_for(pot_norm);
+        it = 1;
+        newton_begin();
+        return;
+      }
+      case S_NEWTON_ITER: {
+        const double pap = s[6];
+        bool done = false;
+        if (pap < 0.0) {
+          if (nit == 1) vec_axpby(c->dir, c->p, ry / pap, -1.0, n, c->st);   // dir = -(dir - ry/pAp p)
+          ntflag = -2;
+          done = true;
+        } else {
+          const double alpha = ry / pap;
+          vec_axpby(c->dir, c->p, alpha, 1.0, n, c->st);
+          vec_axpby(c->r, c->ap, -alpha, 1.0, n, c->st);
+          rr_last = rr;
+          dots(1, c->r, c->r, nullptr, nullptr, nullptr, nullptr, &rr);
+          if (rr < 0.1 * criteria) { ntflag = 0; done = true; }
+          else if (nit > 50) { ntflag = 1; done = true; }
+          else if (std::fabs(rr - rr_last) / rr < 0.01 && nit > 9) { ntflag = 2; done = true; }
+        }
+        if (!done) {
+          ++nit;
+          newton_prep_p();
+          return;
+        }
+        stats.newton_iters_total += nit;
+        if (!line_prep()) {
+          // checkGradient gave up (rhoOutcome = 1)
+          state = S_DONE;
+          return;
+        }
+        line_advance();
+        return;
+      }
+      case S_LINE_EVAL_DONE: {
+        etable_from_sums(c, s, energy);
+        f_cur = energy[0];
+        g_cur = c->dV * s[6];
+        ++nline;
+        if (nline > prm.max_line) {
+          accept();
+          return;
+        }
+        line_advance();
+        return;
+      }
+      default:
+        return;
+    }
+  }
+};
+
+void fill_params(const ofd_opt_params* in, ofd_opt_params* p) {
+  p->max_iter = 500;
+  p->pot_tol = (double)1.0e-5f;
+  p->tole = (double)1.0e-6f;
+  p->conv_check = 1;
+  p->max_line = 200;
+  p->mixed_switch = 0.0;
+  if (!in) return;
+  if (in->max_iter > 0) p->max_iter = in->max_iter;
+  if (in->pot_tol > 0) p->pot_tol = in->pot_tol;
+  if (in->tole > 0) p->tole = in->tole;
+  if (in->conv_check >= 1 && in->conv_check <= 3) p->conv_check = in->conv_check;
+  if (in->max_line > 0) p->max_line = in->max_line;
+  p->mixed_switch = in->mixed_switch;
+}
+
+void alloc_opt(ofd_ctx* c) {
+  if (c->phi) return;
+  const long long n = c->nr();
+  c->phi = dalloc<double>(n); c->dir = dalloc<double>(n); c->dedphi = dalloc<double>(n);
+  c->dldphi = dalloc<double>(n); c->r = dalloc<double>(n); c->p = dalloc<double>(n);
+  c->ap = dalloc<double>(n); c->tmp = dalloc<double>(n);
+}
+
+ReqDesc desc_for(ofd_ctx* c, const EvalReq& q) {
+  ReqDesc D = c->base_desc();
+  D.u = q.u; D.v = q.v; D.bvec = q.bvec; D.out = q.out;
+  D.ca = q.ca; D.cb = q.cb; D.s1 = q.s1; D.s2 = q.s2; D.mu0 = q.mu0; D.eps = q.eps;
+  D.opt_sqrt = 1;
+  return D;
+}
+
+}  // namespace
+
+// sum of a device vector (deterministic): dot with a ones vector
+static double dsum(ofd_ctx* c, const double* x, double* ones) {
+  const double* A[4] = {x, x, x, x};
+  const double* B[4] = {ones, ones, ones, ones};
+  vec_dots(A, B, 1, c->nr(), c->dot_part, c->dot_out, c->st);
+  double s;
+  HIPCHK(hipMemcpyAsync(&s, c->dot_out, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return s;
+}
+
+extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt_params* prm, double* energy9,
+                           int* rho_outcome, ofd_opt_stats* stats) {
+  if (!c || !rhoR) return fail(c, "null argument");
+  if (dir_type != 1) return fail(c, "only dir_type 1 (NTN) is implemented");
+  if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set");
+  auto t0 = std::chrono::steady_clock::now();
+  try {
+    alloc_opt(c);
+    const long long n = c->nr();
+    HIPCHK(hipMemcpyAsync(c->rho, rhoR, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    NtnSolver S;
+    S.c = c;
+    fill_params(prm, &S.prm);
+    // totEleNum = sum(rhoR) * dV  (RhoOptN.f90:326-329)
+    {
+      std::vector<double> ones(n, 1.0);
+      HIPCHK(hipMemcpyAsync(c->tmp, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
+      S.tot_ele = dsum(c, c->rho, c->tmp) * c->dV;
+    }
+    vec_sqrt_abs(c->phi, c->rho, n, c->st);   // phi = SQRT(ABS(rhoR)) (:352)
+    S.state = S_INIT;
+    EvalReq q;
+    double s[kNSum];
+    while (S.need_eval(&q)) {
+      ReqDesc D = desc_for(c, q);
+      if (eval_one(c, D, kInPhi, q.outmode, S.fp32 ? 1 : 0, s)) return -1;
+      S.resume(s);
+    }
+    // rhoR = phi**2 (:495)
+    vec_square(c->rho, c->phi, n, c->st);
+    HIPCHK(hipMemcpyAsync(rhoR, c->rho, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (energy9) std::memcpy(energy9, S.energy, sizeof(S.energy));
+    if (rho_outcome) *rho_outcome = S.outcome;
+    if (stats) {
+      *stats = S.stats;
+      stats->final_pot_norm = S.pot_norm;
+      stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+  return 0;
+}
+
+// ====================================================================== splines (host)
+// spline_cubic_set (MathSplines.f90:178-330) with its s3_fs tridiagonal solve
+static void spline_set_host(int n, const double* t, const double* y, int ibcbeg, double ybcbeg, int ibcend,
+                            double ybcend, double* ypp) {
+  std::vector<double> diag(n), sub(n), sup(n), b(n);
+  if (ibcbeg == 0) { b[0] = 0.0; diag[0] = 1.0; sup[0] = -1.0; }
+  else if (ibcbeg == 1) { b[0] = (y[1] - y[0]) / (t[1] - t[0]) - ybcbeg; diag[0] = (t[1] - t[0]) / 3.0; sup[0] = (t[1] - t[0]) / 6.0; }
+  else { b[0] = ybcbeg; diag[0] = 1.0; sup[0] = 0.0; }
+  for (int i = 1; i < n - 1; ++i) {
+    b[i] = (y[i + 1] - y[i]) / (t[i + 1] - t[i]) - (y[i] - y[i - 1]) / (t[i] - t[i - 1]);
+    sub[i] = (t[i] - t[i - 1]) / 6.0;
+    diag[i] = (t[i + 1] - t[i - 1]) / 3.0;
+    sup[i] = (t[i + 1] - t[i]) / 6.0;
+  }
+  if (ibcend == 0) { b[n - 1] = 0.0; sub[n - 1] = -1.0; diag[n - 1] = 1.0; }
+  else if (ibcend == 1) { b[n - 1] = ybcend - (y[n - 1] - y[n - 2]) / (t[n - 1] - t[n - 2]); sub[n - 1] = (t[n - 1] - t[n - 2]) / 6.0; diag[n - 1] = (t[n - 1] - t[n - 2]) / 3.0; }
+  else { b[n - 1] = ybcend; sub[n - 1] = 0.0; diag[n - 1] = 1.0; }
+  if (n == 2 && ibcbeg == 0 && ibcend == 0) { ypp[0] = ypp[1] = 0.0; return; }
+  for (int i = 1; i < n - 1; ++i) {
+    double xm = sub[i] / diag[i - 1];
+    diag[i] = diag[i] - xm * sup[i - 1];
+    b[i] = b[i] - xm * b[i - 1];
+  }
+  double xm = sub[n - 1] / diag[n - 2];
+  diag[n - 1] = diag[n - 1] - xm * sup[n - 2];
+  ypp[n - 1] = (b[n - 1] - xm * b[n - 2]) / diag[n - 1];
+  for (int i = n - 2; i >= 0; --i) ypp[i] = (b[i] - sup[i] * ypp[i + 1]) / diag[i];
+}
+
+// ====================================================================== WGC tables
+extern "C" int ofd_set_wgc_table(ofd_ctx* c, int nk, const double* eta, const double* w, const double* w1,
+                                 const double* w2) {
+  if (!c || !eta || !w || !w1 || !w2 || nk < 3) return fail(c, "bad WGC table arguments");
+  if (c->kind != kWGC8) return fail(c, "context is not a WGC functional");
+  try {
+    // int_w1 = eta w', int_w2 = eta^2 w'' and natural-type splines with ibc 0
+    // (KEDF_WGCkernel.f90:172-179)
+    std::vector<double> iw1(nk), iw2(nk), pp0(nk), pp1(nk), pp2(nk);
+    for (int i = 0; i < nk; ++i) {
+      iw1[i] = eta[i] * w1[i];
+      iw2[i] = eta[i] * eta[i] * w2[i];
+    }
+    spline_set_host(nk, eta, w, 0, 0.0, 0, 0.0, pp0.data());
+    spline_set_host(nk, eta, iw1.data(), 0, 0.0, 0, 0.0, pp1.data());
+    spline_set_host(nk, eta, iw2.data(), 0, 0.0, 0, 0.0, pp2.data());
+    double* d = dalloc<double>(7 * (size_t)nk);
+    const double* src[7] = {eta, w, iw1.data(), iw2.data(), pp0.data(), pp1.data(), pp2.data()};
+    for (int k = 0; k < 7; ++k)
+      HIPCHK(hipMemcpyAsync(d + (size_t)k * nk, src[k], nk * sizeof(double), hipMemcpyHostToDevice, c->st));
+    const double tkfs = 2.0 * std::pow(3.0 * kPi * kPi * c->rhoS, 1.0 / 3.0);   // KEDF_WGCkernel.f90:93
+    int rc = build_wgc_kernels(c->sp, c->rb, d, d + nk, d + 2 * (size_t)nk, d + 3 * (size_t)nk, d + 4 * (size_t)nk,
+                               d + 5 * (size_t)nk, d + 6 * (size_t)nk, nk, tkfs, c->gamma, c->rhoS, c->wgck, c->st);
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipFree(d));
+    if (rc) return fail(c, "WGC kernel fill failed");
+    c->have_wgc = true;
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+  return 0;
+}
+
+// The WGC kernel ODE in eta (right-hand side of KEDF_KernelODE.f90:481-537):
+//   w'' = [20 (F_L(eta) - 3 eta^2 - 1) - (gamma + 1 - 10) eta w' - 36 (5/3 - beta) beta w] / eta^2
+// integrated from eta = 100 (w = wInf, w' = 0) toward 0 by an adaptive
+// Dormand-Prince 5(4) pair at a tight tolerance, recording w, w', w'' on the
+// knots of makeKernel2 (KEDF_KernelODE.f90:316-349, zero at the last knot).
+static void wgc_rhs(double eta, const double* y, double* yp, double beta, double gamma) {
+  yp[0] = y[1];
+  double fl;
+  if (eta == 1.0) fl = 2.0;
+  else if (eta == 0.0) fl = 1.0;
+  else fl = 1.0 / (0.5 + (1.0 - eta * eta) / (4.0 * eta) * std::log(std::fabs((1.0 + eta) / (1.0 - eta))));
+  double v = 20.0 * (fl - 3.0 * eta * eta - 1.0) - (gamma + 1.0 - 10.0) * eta * y[1] - 36.0 * (5.0 / 3.0 - beta) * beta * y[0];
+  yp[1] = v / (eta * eta);
+}
+
+static int wgc_ode_table(double alpha, double beta, double gamma, std::vector<double>& eta, std::vector<double>& w,
+                         std::vector<double>& w1, std::vector<double>& w2) {
+  const double ts[2] = {1e-3, 3.0 + 1e-3}, te[2] = {3.0, 100.0}, dt[2] = {1e-3, 1e-2};
+  int np[2];
+  for (int i = 0; i < 2; ++i) np[i] = (int)std::floor((te[i] - ts[i]) / dt[i]) - 1;
+  const int n = np[0] + np[1];
+  eta.resize(n); w.assign(n, 0.0); w1.assign(n, 0.0); w2.assign(n, 0.0);
+  for (int i = 0; i < np[0]; ++i) eta[i] = ts[0] + (double)i * dt[0];
+  for (int i = np[0]; i < n; ++i) eta[i] = ts[1] + (double)(i - np[0]) * dt[1];
+  // Dormand-Prince coefficients
+  const double c2 = 1. / 5, c3 = 3. / 10, c4 = 4. / 5, c5 = 8. / 9;
+  const double a21 = 1. / 5, a31 = 3. / 40, a32 = 9. / 40, a41 = 44. / 45, a42 = -56. / 15, a43 = 32. / 9;
+  const double a51 = 19372. / 6561, a52 = -25360. / 2187, a53 = 64448. / 6561, a54 = -212. / 729;
+  const double a61 = 9017. / 3168, a62 = -355. / 33, a63 = 46732. / 5247, a64 = 49. / 176, a65 = -5103. / 18656;
+  const double b1 = 35. / 384, b3 = 500. / 1113, b4 = 125. / 192, b5 = -2187. / 6784, b6 = 11. / 84;
+  const double e1 = 71. / 57600, e3 = -71. / 16695, e4 = 71. / 1920, e5 = -17253. / 339200, e6 = 22. / 525, e7 = -1. / 40;
+  double y[2] = {-1.6 * 20.0 / (36.0 * alpha * beta), 0.0};
+  double t = eta[n - 1];
+  double h = -1e-3;
+  const double rtol = 1e-11, atol = 1e-13;
+  for (int L = n - 2; L >= 0; --L) {
+    const double tw = eta[L];
+    while (t > tw) {
+      if (t + h < tw) h = tw - t;
+      double k1[2], k2[2], k3[2], k4[2], k5[2], k6[2], k7[2], yt[2], y5[2];
+      wgc_rhs(t, y, k1, beta, gamma);
+      for (int i = 0; i < 2; ++i) yt[i] = y[i] + h * a21 * k1[i];
+      wgc_rhs(t + c2 * h, yt, k2, beta, gamma);
+      for (int i = 0; i < 2; ++i) yt[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
+      wgc_rhs(t + c3 * h, yt, k3, beta, gamma);
+      for (int i = 0; i < 2; ++i) yt[i] = y[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]);
+      wgc_rhs(t + c4 * h, yt, k4, beta, gamma);
+      for (int i = 0; i < 2; ++i) yt[i] = y[i] + h * (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]);
+      wgc_rhs(t + c5 * h, yt, k5, beta, gamma);
+      for (int i = 0; i < 2; ++i) yt[i] = y[i] + h * (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]);
+      wgc_rhs(t + h, yt, k6, beta, gamma);
+      for (int i = 0; i < 2; ++i) y5[i] = y[i] + h * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5[i] + b6 * k6[i]);
+      wgc_rhs(t + h, y5, k7, beta, gamma);
+      double err = 0.0;
+      for (int i = 0; i < 2; ++i) {
+        double ei = h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] + e7 * k7[i]);
+        double sc = atol + rtol * std::fmax(std::fabs(y[i]), std::fabs(y5[i]));
+        err = std::fmax(err, std::fabs(ei) / sc);
+      }
+      if (err <= 1.0) {
+        t += h;
+        y[0] = y5[0];
+        y[1] = y5[1];
+      }
+      double fac = err > 0.0 ? 0.9 * std::pow(err, -0.2) : 5.0;
+      fac = std::fmin(5.0, std::fmax(0.2, fac));
+      h *= fac;
+      if (std::fabs(h) < 1e-14) return -1;
+    }
+    double yp[2];
+    wgc_rhs(tw, y, yp, beta, gamma);
+    w[L] = y[0];
+    w1[L] = y[1];
+    w2[L] = yp[1];
+  }
+  return n;
+}
+
+extern "C" int ofd_build_wgc_table(ofd_ctx* c, int* nk_out) {
+  if (!c) return -1;
+  if (c->kind != kWGC8) return fail(c, "context is not a WGC functional");
+  std::vector<double> eta, w, w1, w2;
+  int n = wgc_ode_table(c->alpha, c->beta, c->gamma, eta, w, w1, w2);
+  if (n <= 0) return fail(c, "WGC ODE integration failed");
+  if (nk_out) *nk_out = n;
+  return ofd_set_wgc_table(c, n, eta.data(), w.data(), w1.data(), w2.data());
+}
+
+// ====================================================================== ion terms on the device
+extern "C" int ofd_build_ion_potential(ofd_ctx* c, int nspecies, const int* type_start, const double* frac,
+                                       const double* maxg_invA, const int* npts, const int* poff,
+                                       const double* values, int order, double eii) {
+  if (!c || nspecies < 1 || !type_start || !frac || !maxg_invA || !npts || !poff || !values)
+    return fail(c, "null argument");
+  if (order <= 0) order = 10;
+  if (order % 2 == 1) order += 1;   // SetupKEDF.f90:176-183
+  if (order > 16) return fail(c, "spline order above 16 is not supported");
+  try {
+    const double bohr = 0.529177208607388;
+    const double qElec = 1.602176462e-19, light = 299792458.0, mu0 = 4.0 * kPi * 1.0e-7;
+    const double eps0 = 1.0 / (mu0 * light * light);
+    const double h2ev = qElec * 1.0e10 / (4.0 * kPi * eps0 * bohr);
+    c->type_start.assign(type_start, type_start + nspecies + 1);
+    const int nion = type_start[nspecies];
+    c->frac.assign(frac, frac + 3 * (size_t)nion);
+    c->psp_off.clear(); c->psp_npts.clear(); c->psp_charge.clear(); c->psp_maxg.clear(); c->psp_v0.clear();
+    c->psp_vqs.clear(); c->psp_dd.clear();
+    for (int s = 0; s < nspecies; ++s) {
+      // ReadPseudo (ReadIonFile.f90:848-976)
+      const int n = npts[s];
+      if (n < 3) return fail(c, "pseudopotential table too short");
+      const double maxG = maxg_invA[s] * bohr;
+      std::vector<double> pv(n), t(n), vqs(n), dd(n);
+      for (int i = 0; i < n; ++i) pv[i] = values[poff[s] + i] / (bohr * bohr * bohr * h2ev);
+      const int charge = (int)std::lround(-(pv[1] - pv[0]) * std::pow(maxG / (double)(n - 1), 2.0) / (4.0 * kPi));
+      for (int i = 0; i < n; ++i) t[i] = (double)(i + 1);
+      vqs[0] = pv[0];
+      for (int i = 1; i < n; ++i)
+        vqs[i] = pv[i] + 4.0 * kPi * charge / std::pow((t[i] - 1.0) / (double)(n - 1) * maxG, 2.0);
+      spline_set_host(n, t.data(), vqs.data(), 1, 0.0, 1, 0.0, dd.data());
+      c->psp_off.push_back((int)c->psp_vqs.size());
+      c->psp_npts.push_back(n);
+      c->psp_charge.push_back(charge);
+      c->psp_maxg.push_back(maxG);
+      c->psp_v0.push_back(pv[0]);
+      c->psp_vqs.insert(c->psp_vqs.end(), vqs.begin(), vqs.end());
+      c->psp_dd.insert(c->psp_dd.end(), dd.begin(), dd.end());
+    }
+    c->spline_order = order;
+    hipFree(c->d_frac); hipFree(c->d_psp_vqs); hipFree(c->d_psp_dd);
+    c->d_frac = dalloc<double>(3 * (size_t)nion);
+    c->d_psp_vqs = dalloc<double>(c->psp_vqs.size());
+    c->d_psp_dd = dalloc<double>(c->psp_dd.size());
+    HIPCHK(hipMemcpy(c->d_frac, c->frac.data(), sizeof(double) * c->frac.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_psp_vqs, c->psp_vqs.data(), sizeof(double) * c->psp_vqs.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_psp_dd, c->psp_dd.data(), sizeof(double) * c->psp_dd.size(), hipMemcpyHostToDevice));
+    const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
+    void* sc = dalloc<char>(scratch);
+    int rc = build_ion_potential(c->sp, c->rb, c->vol, nspecies, c->type_start.data(), c->d_frac, nion, c->d_psp_vqs,
+                                 c->d_psp_dd, c->psp_off.data(), c->psp_npts.data(), c->psp_maxg.data(),
+                                 c->psp_v0.data(), c->psp_charge.data(), order, c->vion, sc, c->st);
+    hipFree(sc);
+    if (rc) return fail(c, "ion potential build failed");
+    c->e_ionion = eii;
+    c->have_vion = true;
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+  return 0;
+}
+
+extern "C" int ofd_ion_forces(ofd_ctx* c, const double* rho, double* forces) {
+  if (!c || !rho || !forces) return fail(c, "null argument");
+  if (c->type_start.empty()) return fail(c, "ions not set (ofd_build_ion_potential)");
+  try {
+    const int ns = (int)c->type_start.size() - 1;
+    const int nion = c->type_start[ns];
+    double cinv[9], rb[9], vol;
+    cell_setup(c->lat, rb, &vol, cinv);
+    HIPCHK(hipMemcpyAsync(c->rho, rho, c->nr() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    double* df = dalloc<double>(3 * (size_t)nion);
+    const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
+    void* sc = dalloc<char>(scratch);
+    int rc = ion_forces_spline(c->sp, c->rb, cinv, ns, c->type_start.data(), c->d_frac, nion, c->d_psp_vqs,
+                               c->d_psp_dd, c->psp_off.data(), c->psp_npts.data(), c->psp_maxg.data(),
+                               c->psp_v0.data(), c->psp_charge.data(), c->spline_order, c->rho, df, sc, c->st);
+    HIPCHK(hipMemcpy(forces, df, sizeof(double) * 3 * nion, hipMemcpyDeviceToHost));
+    hipFree(sc);
+    hipFree(df);
+    if (rc) return fail(c, "force kernels failed");
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+  return 0;
+}
+
+// ====================================================================== request pool
+struct ofd_pool {
+  std::string err;
+  int n1, n2, n3, capacity, kinetic, precision;
+  hipStream_t st = nullptr;
+  struct Slot {
+    bool used = false;
+    bool done = false;
+    ofd_ctx* c = nullptr;
+    hipStream_t saved = nullptr;
+    NtnSolver S;
+    std::string err;
+    std::chrono::steady_clock::time_point t0;
+  };
+  std::vector<Slot> slots;
+  ReqDesc* d_desc = nullptr;
+  double* d_sums = nullptr;
+  std::vector<ReqDesc> h_desc;
+  std::vector<double> h_sums;
+};
+
+extern "C" long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int precision) {
+  const long long nr = (long long)n1 * n2 * n3;
+  const long long nc = (long long)(n1 / 2 + 1) * n2 * n3;
+  const int nf = kinetic == 5 ? 8 : 3;
+  long long b = 8 * nr * 3;            // vion, rho staging, pot staging
+  b += 16 * nc * nf;                   // complex work (FP64 size; FP32 uses half)
+  b += 8 * 8 * nr;                     // optimizer vectors
+  b += 8 * (long long)n2 * n3 * kNSum; // partial sums
+  if (kinetic == 5) b += 8 * 4 * nc;   // WGC kernels
+  (void)precision;
+  return b;
+}
+
+extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic, int precision, ofd_pool** out) {
+  if (!out || capacity < 1) return -1;
+  ofd_pool* p = new ofd_pool();
+  p->n1 = n1; p->n2 = n2; p->n3 = n3; p->capacity = capacity; p->kinetic = kinetic; p->precision = precision;
+  p->slots.resize(capacity);
+  p->h_desc.resize(capacity);
+  p->h_sums.resize((size_t)capacity * kNSum);
+  if (hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&p->d_desc, sizeof(ReqDesc) * capacity) != hipSuccess ||
+      hipMalloc(&p->d_sums, sizeof(double) * kNSum * capacity) != hipSuccess) {
+    p->err = "pool allocation failed";
+    *out = p;
+    return -1;
+  }
+  *out = p;
+  return 0;
+}
+
+extern "C" void ofd_pool_destroy(ofd_pool* p) {
+  if (!p) return;
+  hipStreamSynchronize(p->st);
+  for (auto& s : p->slots)
+    if (s.used && s.c) {
+      s.c->st = s.saved;
+      ofd_destroy(s.c);
+    }
+  hipFree(p->d_desc);
+  hipFree(p->d_sums);
+  hipStreamDestroy(p->st);
+  delete p;
+}
+
+extern "C" const char* ofd_pool_last_error(const ofd_pool* p) { return p ? p->err.c_str() : "null pool"; }
+
+extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const ofd_opt_params* prm) {
+  if (!p || !c || !rho) return -1;
+  if (c->sp.d.n1 != p->n1 || c->sp.d.n2 != p->n2 || c->sp.d.n3 != p->n3 || c->kinetic != p->kinetic) {
+    p->err = "request grid or functional does not match the pool";
+    return -1;
+  }
+  if (c->kind == kWGC8 && !c->have_wgc) {
+    p->err = "WGC table not set";
+    return -1;
+  }
+  int slot = -1;
+  for (int i = 0; i < p->capacity; ++i)
+    if (!p->slots[i].used) { slot = i; break; }
+  if (slot < 0) return -2;
+  auto& s = p->slots[slot];
+  try {
+    HIPCHK(hipStreamSynchronize(c->st));
+    s = ofd_pool::Slot();
+    s.used = true;
+    s.c = c;
+    s.saved = c->st;
+    c->st = p->st;   // every request of the pool runs on the pool's stream
+    s.t0 = std::chrono::steady_clock::now();
+    alloc_opt(c);
+    const long long n = c->nr();
+    HIPCHK(hipMemcpyAsync(c->rho, rho, n * sizeof(double), hipMemcpyHostToDevice, p->st));
+    s.S.c = c;
+    fill_params(prm, &s.S.prm);
+    std::vector<double> ones(n, 1.0);
+    HIPCHK(hipMemcpyAsync(c->tmp, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, p->st));
+    s.S.tot_ele = dsum(c, c->rho, c->tmp) * c->dV;
+    vec_sqrt_abs(c->phi, c->rho, n, p->st);
+    s.S.state = S_INIT;
+  } catch (const std::exception& e) {
+    p->err = e.what();
+    s.used = false;
+    c->st = s.saved;
+    return -1;
+  }
+  return slot;
+}
+
+extern "C" int ofd_pool_step(ofd_pool* p) {
+  if (!p) return -1;
+  // group the pending evaluations by (output mode, precision)
+  std::vector<int> groups[3][2];
+  std::vector<EvalReq> reqs(p->capacity);
+  int active = 0;
+  for (int i = 0; i < p->capacity; ++i) {
+    auto& s = p->slots[i];
+    if (!s.used || s.done) continue;
+    if (!s.S.need_eval(&reqs[i])) {
+      s.done = true;
+      continue;
+    }
+    groups[reqs[i].outmode][s.S.fp32 ? 1 : 0].push_back(i);
+    ++active;
+  }
+  if (active == 0) return 0;
+  try {
+    int off = 0;
+    std::vector<std::pair<int, int>> where(p->capacity, {-1, -1});
+    for (int om = 0; om < 3; ++om)
+      for (int pr = 0; pr < 2; ++pr) {
+        auto& g = groups[om][pr];
+        if (g.empty()) continue;
+        ofd_ctx* c0 = p->slots[g[0]].c;
+        for (size_t k = 0; k < g.size(); ++k) {
+          p->h_desc[off + k] = desc_for(p->slots[g[k]].c, reqs[g[k]]);
+          where[g[k]] = {off + (int)k, 0};
+        }
+        HIPCHK(hipMemcpyAsync(p->d_desc + off, &p->h_desc[off], sizeof(ReqDesc) * g.size(), hipMemcpyHostToDevice,
+                              p->st));
+        if (run_eval(c0->sp, c0->kind, kInPhi, om, pr, p->d_desc + off, (int)g.size(), p->st))
+          throw std::runtime_error("batched evaluation launch failed");
+        if (run_finalize(c0->sp, c0->kind, pr, p->d_desc + off, (int)g.size(), p->d_sums + (size_t)off * kNSum,
+                         p->st))
+          throw std::runtime_error("batched finalize failed");
+        off += (int)g.size();
+      }
+    HIPCHK(hipMemcpyAsync(p->h_sums.data(), p->d_sums, sizeof(double) * kNSum * off, hipMemcpyDeviceToHost, p->st));
+    HIPCHK(hipStreamSynchronize(p->st));
+    for (int i = 0; i < p->capacity; ++i) {
+      if (where[i].first < 0) continue;
+      auto& s = p->slots[i];
+      try {
+        s.S.resume(&p->h_sums[(size_t)where[i].first * kNSum]);
+      } catch (const std::exception& e) {
+        s.err = e.what();
+        s.S.outcome = 1;
+        s.S.state = S_DONE;
+      }
+      if (s.S.state == S_DONE) s.done = true;
+    }
+  } catch (const std::exception& e) {
+    p->err = e.what();
+    return -1;
+  }
+  int still = 0;
+  for (auto& s : p->slots)
+    if (s.used && !s.done) ++still;
+  return still;
+}
+
+extern "C" int ofd_pool_collect(ofd_pool* p, int slot, double* rho_out, double* energy9, int* outcome,
+                                ofd_opt_stats* stats, ofd_ctx** ctx_out) {
+  if (!p || slot < 0 || slot >= p->capacity) return -1;
+  auto& s = p->slots[slot];
+  if (!s.used) return -1;
+  if (!s.done) return 0;
+  ofd_ctx* c = s.c;
+  try {
+    const long long n = c->nr();
+    vec_square(c->rho, c->phi, n, p->st);
+    if (rho_out) HIPCHK(hipMemcpyAsync(rho_out, c->rho, n * sizeof(double), hipMemcpyDeviceToHost, p->st));
+    HIPCHK(hipStreamSynchronize(p->st));
+  } catch (const std::exception& e) {
+    p->err = e.what();
+    return -1;
+  }
+  if (energy9) std::memcpy(energy9, s.S.energy, sizeof(s.S.energy));
+  if (outcome) *outcome = s.err.empty() ? s.S.outcome : 1;
+  if (stats) {
+    *stats = s.S.stats;
+    stats->final_pot_norm = s.S.pot_norm;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - s.t0).count();
+  }
+  c->st = s.saved;
+  if (!s.err.empty()) c->err = s.err;
+  if (ctx_out) *ctx_out = c;
+  else ofd_destroy(c);
+  s = ofd_pool::Slot();
+  return 1;
+}

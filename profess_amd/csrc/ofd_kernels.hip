@@ -1,0 +1,652 @@
+// HIP kernels for one PROFESS energy + potential evaluation on gfx950.
+//
+// One evaluation = five passes over the grid, batched over requests that share
+// a grid shape (blockIdx.y/z indexes the request descriptor):
+//   1 k_xfwd   reads rho (or phi = ca*u + cb*v and squares it), forms the NF
+//              real fields of the functional pointwise, runs the x-direction
+//              R2C (half-length complex FFT + split) in LDS, writes NF half rows
+//   2 k_cpass  complex FFT along y, KB adjacent x-columns per workgroup
+//   3 k_zgop   complex FFT along z, the G-space operator (Hartree 4 pi/G^2,
+//              Wang-Teter kernel K(|G|) on the fly or the four WGC kernels,
+//              vW G^2, the 1/N forward normalization), inverse FFT along z
+//   4 k_cpass  inverse complex FFT along y
+//   5 k_xinv   x-direction C2R of the NI fields in LDS, then the potential
+//              epilogue (ion + Hartree + LDA + TF + nonlocal, chain rule
+//              2 sqrt(rho), + mu*vW) and the per-workgroup FP64 energy sums
+// CalPotential.f90:41-647 is the reference for what is computed; the dataflow
+// (3 forward + 3 inverse transforms for WT with alpha == beta instead of 4+4)
+// is this file's own.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "ofd_internal.h"
+
+namespace ofd {
+
+constexpr double kPi = 3.141592653589793238462643;
+constexpr double kCTF = 2.87123400018819;   // KEDF_TF.f90:42
+
+// ------------------------------------------------------------ pointwise math
+// LDA Perdew-Zunger, numSpin = 1 (XC_LDA.f90:199-310 potential, :51-196 energy)
+__device__ __forceinline__ double lda_pot(double r13) {
+  const double cX = -0.98474502184269641, cC = 0.62035049089940009;
+  const double a1 = 3.11e-2, cC1 = -5.83666666666666666e-2, cC2 = 1.33333333333333333e-3, cC3 = -8.4e-3;
+  const double g1 = -1.423e-1, cC4 = -0.174798948333333333, cC5 = -6.325709333333333333e-2;
+  const double b1 = 1.0529, b2 = 3.334e-1;
+  double rs = cC / r13;
+  if (rs < 1.0) return cX * r13 + log(rs) * (a1 + cC2 * rs) + cC1 + cC3 * rs;
+  double sq = sqrt(rs);
+  double den = 1.0 + b1 * sq + b2 * rs;
+  return cX * r13 + (g1 + cC4 * sq + cC5 * rs) / (den * den);
+}
+__device__ __forceinline__ double lda_edens(double rho, double r13) {
+  const double cX = -0.73855876638202234, cC = 0.62035049089940009;
+  const double a = 3.11e-2, b = -4.8e-2, c = 2e-3, dd = -1.16e-2, g = -1.423e-1, b1 = 1.0529, b2 = 3.334e-1;
+  double rs = cC / r13;
+  double ec;
+  if (rs < 1.0) ec = log(rs) * (a + c * rs) + b + dd * rs;
+  else ec = g / (1.0 + b1 * sqrt(rs) + b2 * rs);
+  return cX * rho * r13 + rho * ec;
+}
+
+// LindG (MathFunctions.f90:440-528)
+__device__ __forceinline__ double lind_g(double eta, double lam, double mu) {
+  if (eta < 0.0) return 0.0;
+  if (eta < 1e-10) return 1.0 - lam + eta * eta * (1.0 / 3.0 - 3.0 * mu);
+  if (fabs(eta - 1.0) < 1e-10) return 2.0 - lam - 3.0 * mu + 20.0 * (eta - 1.0);
+  if (eta > 3.65) {
+    double e2 = eta * eta, i2 = 1.0 / e2;
+    double acc = -3.00624946457977689e-3;
+    acc = -3.38745061493813488e-3 + i2 * acc;
+    acc = -3.84763737842981233e-3 + i2 * acc;
+    acc = -4.41060829979912465e-3 + i2 * acc;
+    acc = -5.10997527675418131e-3 + i2 * acc;
+    acc = -5.9945117538835746e-3 + i2 * acc;
+    acc = -7.1372762502456763e-3 + i2 * acc;
+    acc = -8.65192783339199453e-3 + i2 * acc;
+    acc = -1.07236045520990083e-2 + i2 * acc;
+    acc = -1.36715733124818332e-2 + i2 * acc;
+    acc = -1.80879839616166146e-2 + i2 * acc;
+    acc = -2.51824061652633074e-2 + i2 * acc;
+    acc = -3.77825602968460128e-2 + i2 * acc;
+    acc = -6.39999999999999875e-2 + i2 * acc;
+    acc = -0.13714285714285712 + i2 * acc;
+    return 3.0 * (1.0 - mu) * e2 - lam - 0.6 + i2 * acc;
+  }
+  return 1.0 / (0.5 + 0.25 * (1.0 - eta * eta) * log((1.0 + eta) / fabs(1.0 - eta)) / eta) - 3.0 * mu * eta * eta -
+         lam;
+}
+
+__device__ __forceinline__ double fsign(double x) { return signbit(x) ? -1.0 : 1.0; }
+
+// rho^p with a fast exact-enough path for p = 5/6 (the WT default)
+struct Pows {
+  double rho, phi, r13;
+  double ra, ram1;   // rho^alpha, rho^(alpha-1)
+  double rb, rbm1;   // rho^beta,  rho^(beta-1)
+};
+__device__ __forceinline__ void make_pows(double rho, double alpha, double beta, Pows& p) {
+  p.rho = rho;
+  p.phi = sqrt(rho);
+  p.r13 = cbrt(rho);
+  const double f56 = 5.0 / 6.0;
+  if (alpha == f56 && beta == f56) {
+    double r16 = sqrt(p.r13);
+    p.ra = rho / r16;
+    p.ram1 = 1.0 / r16;
+    p.rb = p.ra;
+    p.rbm1 = p.ram1;
+  } else {
+    p.ra = pow(rho, alpha);
+    p.ram1 = pow(rho, alpha - 1.0);
+    p.rb = pow(rho, beta);
+    p.rbm1 = pow(rho, beta - 1.0);
+  }
+}
+
+template <int KIND> struct NFields;
+template <> struct NFields<kWT3> { static constexpr int v = 3; };
+template <> struct NFields<kWT4> { static constexpr int v = 4; };
+template <> struct NFields<kWGC8> { static constexpr int v = 8; };
+constexpr int kPlain = 9;   // one real field, no functional
+template <> struct NFields<kPlain> { static constexpr int v = 1; };
+
+template <int KIND>
+__device__ __forceinline__ void gen_fields(const ReqDesc& D, double rho, double* f) {
+  if (KIND == kPlain) { f[0] = rho; return; }
+  Pows p;
+  make_pows(rho, D.alpha, D.beta, p);
+  if (KIND == kWT3) {
+    f[0] = rho; f[1] = p.rb; f[2] = p.phi;
+  } else if (KIND == kWT4) {
+    f[0] = rho; f[1] = p.rb; f[2] = p.ra; f[3] = p.phi;
+  } else {
+    double th = rho - D.rhoS;
+    f[0] = rho; f[1] = p.phi;
+    f[2] = p.ra; f[3] = th * p.ra; f[4] = th * th * p.ra / 2.0;
+    f[5] = p.rb; f[6] = th * p.rb; f[7] = th * th * p.rb / 2.0;
+  }
+}
+
+// ------------------------------------------------------------ block reduce
+template <int N>
+__device__ __forceinline__ void block_sum_store(double (&acc)[N], double* out) {
+  // wave (64 lanes) butterfly then across waves through LDS, fixed order
+  __shared__ double red[16][N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    acc[k] = v;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) red[wv][k] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[w][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------ pass 1: x R2C
+// rows = n2*n3 lines of n1 reals per request; `rows_per` rows per workgroup.
+// LDS: 2 buffers of rows_per*NF lines, ld = M+1 complex each.
+template <typename R, int KIND, int INMODE>
+__global__ void __launch_bounds__(256) k_xfwd(const ReqDesc* __restrict__ descs, GridDims d, Plan1D p,
+                                              const typename V2<R>::T* __restrict__ tw,
+                                              const typename V2<R>::T* __restrict__ tws, int rows_per) {
+  using C = typename V2<R>::T;
+  constexpr int NF = NFields<KIND>::v;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const ReqDesc& D = descs[blockIdx.y];
+  const int M = d.M, ld = M + 1, n1 = d.n1;
+  const long long nrows = (long long)d.n2 * d.n3;
+  const long long row0 = (long long)blockIdx.x * rows_per;
+  const int nrow = (int)min((long long)rows_per, nrows - row0);
+  const int nl = nrow * NF;
+  C* A = reinterpret_cast<C*>(smem);
+  C* B = A + (size_t)rows_per * NF * ld;
+  R* Ar = reinterpret_cast<R*>(A);
+  // load + pointwise fields; real x at complex slot x/2, component x&1
+  for (int e = threadIdx.x; e < nrow * n1; e += blockDim.x) {
+    const int r = e / n1, i1 = e - r * n1;
+    const long long g = (row0 + r) * n1 + i1;
+    double rho;
+    if (INMODE == kInRho) {
+      rho = D.u[g];
+    } else {
+      const double t = D.cb != 0.0 ? fma(D.cb, D.v[g], D.ca * D.u[g]) : D.ca * D.u[g];
+      rho = t * t;
+    }
+    double f[NF];
+    gen_fields<KIND>(D, rho, f);
+#pragma unroll
+    for (int j = 0; j < NF; ++j) Ar[2 * ((r * NF + j) * ld + (i1 >> 1)) + (i1 & 1)] = (R)f[j];
+  }
+  __syncthreads();
+  C* res = lds_fft<-1>(A, B, p, ld, nl, tw);
+  // split: X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2
+  C* work = reinterpret_cast<C*>(D.work);
+  const int K1 = d.K1;
+  for (int e = threadIdx.x; e < nl * K1; e += blockDim.x) {
+    const int l = e / K1, k = e - l * K1;
+    const int r = l / NF, j = l - r * NF;
+    const C* z = res + l * ld;
+    C zk = z[k == M ? 0 : k];
+    C zm = z[k == 0 ? 0 : M - k];
+    R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
+    R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);   // O = -i(Z - conj Zm)/2
+    C w = tws[k];
+    C x;
+    x.x = er + (w.x * orr - w.y * oi);
+    x.y = ei + (w.x * oi + w.y * orr);
+    work[(size_t)j * d.nc + (size_t)(row0 + r) * K1 + k] = x;
+  }
+}
+
+// ------------------------------------------------------------ passes 2/4: strided complex FFT
+// Transforms along an axis with element stride `es` (in complex units) for
+// `kb` adjacent x-columns; blockIdx.x = outer*nchunk + chunk, blockIdx.y = field,
+// blockIdx.z = request. outer index advances by `os`.
+template <typename R, int S>
+__global__ void __launch_bounds__(256) k_cpass(const ReqDesc* __restrict__ descs, GridDims d, Plan1D p,
+                                               const typename V2<R>::T* __restrict__ tw, int kb, int nchunk,
+                                               long long es, long long os) {
+  using C = typename V2<R>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const ReqDesc& D = descs[blockIdx.z];
+  const int chunk = blockIdx.x % nchunk, outer = blockIdx.x / nchunk;
+  const int c0 = chunk * kb;
+  const int ncol = min(kb, d.K1 - c0);
+  const int n = p.n, ld = n + 1;
+  C* A = reinterpret_cast<C*>(smem);
+  C* B = A + (size_t)kb * ld;
+  C* base = reinterpret_cast<C*>(D.work) + (size_t)blockIdx.y * d.nc + (size_t)outer * os + c0;
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    const int i = e / ncol, c = e - i * ncol;
+    A[c * ld + i] = base[(size_t)i * es + c];
+  }
+  __syncthreads();
+  C* res = lds_fft<S>(A, B, p, ld, ncol, tw);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    const int i = e / ncol, c = e - i * ncol;
+    base[(size_t)i * es + c] = res[c * ld + i];
+  }
+}
+
+// ------------------------------------------------------------ pass 3: z FFT + G-space operator + z IFFT
+template <int KIND>
+__device__ __forceinline__ void gop(const ReqDesc& D, const double* __restrict__ wk, size_t gidx, size_t nc,
+                                    double q2, double invn, double (&x)[8][2]) {
+  // x[f][re/im] in, out overwrite (NI == NF for all kinds)
+  const double hart = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
+  if (KIND == kWT3 || KIND == kWT4) {
+    const double K = D.coef * lind_g(sqrt(q2) / D.tkf, D.lam, D.mu) * invn;
+    const double vw = q2 * invn;
+    x[0][0] *= hart; x[0][1] *= hart;
+    x[1][0] *= K; x[1][1] *= K;
+    if (KIND == kWT3) {
+      x[2][0] *= vw; x[2][1] *= vw;
+    } else {
+      x[2][0] *= K; x[2][1] *= K;
+      x[3][0] *= vw; x[3][1] *= vw;
+    }
+  } else if (KIND == kWGC8) {
+    const double k1 = wk[gidx] * invn, k2 = wk[nc + gidx] * invn, k3 = wk[2 * nc + gidx] * invn,
+                 k4 = wk[3 * nc + gidx] * invn;
+    const double vw = q2 * invn;
+    x[0][0] *= hart; x[0][1] *= hart;
+    x[1][0] *= vw; x[1][1] *= vw;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int b = 2 + 3 * h;   // ra block at 2..4, rb block at 5..7
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        double fr = x[b][c], ft = x[b + 1][c], f2 = x[b + 2][c];
+        x[b][c] = k1 * fr + k2 * ft + k3 * f2;
+        x[b + 1][c] = k2 * fr + k4 * ft;
+        x[b + 2][c] = k3 * fr;
+      }
+    }
+  }
+}
+
+template <typename R, int KIND>
+__global__ void __launch_bounds__(256) k_zgop(const ReqDesc* __restrict__ descs, GridDims d, Plan1D p,
+                                              const typename V2<R>::T* __restrict__ tw, int kb, int nchunk) {
+  using C = typename V2<R>::T;
+  constexpr int NF = NFields<KIND>::v;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const ReqDesc& D = descs[blockIdx.y];
+  const int chunk = blockIdx.x % nchunk, i2 = blockIdx.x / nchunk;
+  const int c0 = chunk * kb;
+  const int ncol = min(kb, d.K1 - c0);
+  const int n3 = d.n3, ld = n3 + 1;
+  C* A = reinterpret_cast<C*>(smem);
+  C* B = A + (size_t)NF * kb * ld;
+  C* work = reinterpret_cast<C*>(D.work);
+  const size_t es = (size_t)d.n2 * d.K1;
+  const size_t off = (size_t)i2 * d.K1 + c0;
+  for (int e = threadIdx.x; e < NF * n3 * ncol; e += blockDim.x) {
+    const int f = e / (n3 * ncol);
+    const int rem = e - f * n3 * ncol;
+    const int i3 = rem / ncol, c = rem - i3 * ncol;
+    A[(f * ncol + c) * ld + i3] = work[(size_t)f * d.nc + (size_t)i3 * es + off + c];
+  }
+  __syncthreads();
+  C* res = lds_fft<-1>(A, B, p, ld, NF * ncol, tw);
+  C* oth = (res == A) ? B : A;
+  const double invn = 1.0 / (double)d.nr;
+  const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
+  for (int e = threadIdx.x; e < ncol * n3; e += blockDim.x) {
+    const int c = e / n3, i3 = e - c * n3;
+    const int m1 = c0 + c;
+    const int m3 = i3 > n3 / 2 ? i3 - n3 : i3;
+    const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
+    const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
+    const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
+    const double q2 = gx * gx + gy * gy + gz * gz;
+    double x[8][2];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      C v = res[(f * ncol + c) * ld + i3];
+      x[f][0] = (double)v.x;
+      x[f][1] = (double)v.y;
+    }
+    gop<KIND>(D, D.wgck, (size_t)i3 * es + off + c, (size_t)d.nc, q2, invn, x);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      C v;
+      v.x = (R)x[f][0];
+      v.y = (R)x[f][1];
+      res[(f * ncol + c) * ld + i3] = v;
+    }
+  }
+  __syncthreads();
+  C* res2 = lds_fft<+1>(res, oth, p, ld, NF * ncol, tw);
+  for (int e = threadIdx.x; e < NF * n3 * ncol; e += blockDim.x) {
+    const int f = e / (n3 * ncol);
+    const int rem = e - f * n3 * ncol;
+    const int i3 = rem / ncol, c = rem - i3 * ncol;
+    work[(size_t)f * d.nc + (size_t)i3 * es + off + c] = res2[(f * ncol + c) * ld + i3];
+  }
+}
+
+// ------------------------------------------------------------ pass 5: x C2R + epilogue
+template <int KIND>
+__device__ __forceinline__ double epi_point(const ReqDesc& D, double rho, double vion, const double* o,
+                                            double (&acc)[kNSum]) {
+  Pows p;
+  make_pows(rho, D.alpha, D.beta, p);
+  const double vxc = lda_pot(p.r13);
+  const double r23 = p.r13 * p.r13;
+  const double vtf = (5.0 / 3.0 * kCTF * D.lam) * r23;
+  double vnl, t, enl;
+  const double vh = o[0];
+  if (KIND == kWT3) {
+    vnl = kCTF * (D.alpha * p.ram1 * o[1] + D.beta * p.rbm1 * o[1]);
+    enl = p.ra * o[1];
+    t = o[2];
+  } else if (KIND == kWT4) {
+    vnl = kCTF * (D.alpha * p.ram1 * o[1] + D.beta * p.rbm1 * o[2]);
+    enl = p.ra * o[1];
+    t = o[3];
+  } else {
+    const double th = rho - D.rhoS, a = D.alpha, b = D.beta;
+    t = o[1];
+    enl = p.rb * (o[2] + th * o[3] + th * th / 2.0 * o[4]);
+    vnl = kCTF * (p.rbm1 * (b * o[2] + (rho + b * th) * o[3] + th * (rho + b * th / 2.0) * o[4]) +
+                  p.ram1 * (a * o[5] + (rho + a * th) * o[6] + th * (rho + a * th / 2.0) * o[7]));
+  }
+  const double loc = vion + vh + vxc + vtf + vnl;
+  acc[0] += rho * vion;
+  acc[1] += vh * rho;
+  acc[2] += lda_edens(rho, p.r13);
+  acc[3] += rho * r23;
+  acc[4] += p.phi * t;
+  acc[5] += enl;
+  if (D.opt_sqrt) return 2.0 * p.phi * loc + D.mu * t;
+  return loc + D.mu * t / (2.0 * p.phi);
+}
+
+template <typename R, int KIND, int INMODE, int OUTMODE>
+__global__ void __launch_bounds__(256) k_xinv(const ReqDesc* __restrict__ descs, GridDims d, Plan1D p,
+                                              const typename V2<R>::T* __restrict__ tw,
+                                              const typename V2<R>::T* __restrict__ tws, int rows_per) {
+  using C = typename V2<R>::T;
+  constexpr int NF = (KIND == kPlain) ? 1 : NFields<KIND>::v;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const ReqDesc& D = descs[blockIdx.y];
+  const int M = d.M, ld = M + 1, n1 = d.n1, K1 = d.K1;
+  const long long nrows = (long long)d.n2 * d.n3;
+  const long long row0 = (long long)blockIdx.x * rows_per;
+  const int nrow = (int)min((long long)rows_per, nrows - row0);
+  const int nl = nrow * NF;
+  C* A = reinterpret_cast<C*>(smem);
+  C* B = A + (size_t)rows_per * NF * ld;
+  const C* work = reinterpret_cast<const C*>(D.work);
+  for (int e = threadIdx.x; e < nl * K1; e += blockDim.x) {
+    const int l = e / K1, k = e - l * K1;
+    const int r = l / NF, j = l - r * NF;
+    B[l * ld + k] = work[(size_t)j * d.nc + (size_t)(row0 + r) * K1 + k];
+  }
+  __syncthreads();
+  // Z[k] = (X[k] + conj X[M-k]) + i W^{-k} (X[k] - conj X[M-k]); Im X[0] = Im X[M] = 0
+  for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
+    const int l = e / M, k = e - l * M;
+    const C* x = B + l * ld;
+    C xk = x[k], xm = x[M - k];
+    if (k == 0) { xk.y = (R)0; xm.y = (R)0; }
+    R er = xk.x + xm.x, ei = xk.y - xm.y;
+    R dr = xk.x - xm.x, di = xk.y + xm.y;
+    C w = tws[k];   // W^k; use conj
+    R orr = w.x * dr + w.y * di, oi = w.x * di - w.y * dr;
+    C z;
+    z.x = er - oi;
+    z.y = ei + orr;
+    A[l * ld + k] = z;
+  }
+  __syncthreads();
+  C* res = lds_fft<+1>(A, B, p, ld, nl, tw);
+  const R* rr = reinterpret_cast<const R*>(res);
+  double acc[kNSum];
+#pragma unroll
+  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
+  for (int e = threadIdx.x; e < nrow * n1; e += blockDim.x) {
+    const int r = e / n1, i1 = e - r * n1;
+    const size_t g = (size_t)(row0 + r) * n1 + i1;
+    double o[8];
+#pragma unroll
+    for (int j = 0; j < NF; ++j) o[j] = (double)rr[2 * ((r * NF + j) * ld + (i1 >> 1)) + (i1 & 1)];
+    if (KIND == kPlain) {
+      D.out[g] = o[0];
+      continue;
+    }
+    double tphi = 0.0, rho;
+    if (INMODE == kInRho) {
+      rho = D.u[g];
+    } else {
+      tphi = D.cb != 0.0 ? fma(D.cb, D.v[g], D.ca * D.u[g]) : D.ca * D.u[g];
+      rho = tphi * tphi;
+    }
+    double pot = epi_point<KIND>(D, rho, D.vion[g], o, acc);
+    if (OUTMODE == kOutPot) {
+      D.out[g] = pot;
+    } else if (OUTMODE == kOutSigned) {
+      const double de = pot * fsign(tphi);
+      D.out[g] = de;
+      const double dir = D.s1 * D.u[g] + (D.v != nullptr ? D.s2 * D.v[g] : 0.0);
+      acc[6] += de * dir;
+      const double dl = de - 2.0 * tphi * D.mu0;
+      acc[7] += dl * dl;
+      acc[8] += de * tphi;
+    } else {
+      const double de = pot * fsign(tphi);
+      const double ap = ((de - D.mu0 * 2.0 * tphi) - D.bvec[g]) / D.eps;
+      D.out[g] = ap;
+      acc[6] += D.v[g] * ap;
+    }
+  }
+  if (KIND != kPlain) block_sum_store<kNSum>(acc, D.sums + (size_t)blockIdx.x * kNSum);
+}
+
+// ------------------------------------------------------------ finalize: sum partials
+__global__ void __launch_bounds__(256) k_finalize(const ReqDesc* __restrict__ descs, int nwg, double* out) {
+  const ReqDesc& D = descs[blockIdx.x];
+  double acc[kNSum];
+#pragma unroll
+  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
+  for (int w = threadIdx.x; w < nwg; w += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < kNSum; ++k) acc[k] += D.sums[(size_t)w * kNSum + k];
+  }
+  block_sum_store<kNSum>(acc, out + (size_t)blockIdx.x * kNSum);
+}
+
+// ============================================================ host launchers
+template <typename R>
+static const typename V2<R>::T* twp(const ShapePlan& sp, int which);
+template <>
+const double2* twp<double>(const ShapePlan& sp, int w) {
+  return w == 0 ? sp.tw_x : w == 1 ? sp.tw_y : w == 2 ? sp.tw_z : sp.tw_s;
+}
+template <>
+const float2* twp<float>(const ShapePlan& sp, int w) {
+  return w == 0 ? sp.tw_x32 : w == 1 ? sp.tw_y32 : w == 2 ? sp.tw_z32 : sp.tw_s32;
+}
+
+template <typename R, int KIND, int INMODE, int OUTMODE>
+static int eval_t(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st) {
+  using C = typename V2<R>::T;
+  constexpr int NF = NFields<KIND>::v;
+  const GridDims& d = sp.d;
+  const Tiles t = tiles_for(sp, NF, (int)sizeof(C));
+  size_t lds_x = 2ull * t.rows * NF * (d.M + 1) * sizeof(C);
+  hipLaunchKernelGGL((k_xfwd<R, KIND, INMODE>), dim3(t.ntile, nb), dim3(256), lds_x, st, dd, d, sp.px,
+                     twp<R>(sp, 0), twp<R>(sp, 3), t.rows);
+  const int nchy = (d.K1 + t.kby - 1) / t.kby;
+  size_t ldsy = 2ull * t.kby * (d.n2 + 1) * sizeof(C);
+  hipLaunchKernelGGL((k_cpass<R, -1>), dim3(nchy * d.n3, NF, nb), dim3(256), ldsy, st, dd, d, sp.py,
+                     twp<R>(sp, 1), t.kby, nchy, (long long)d.K1, (long long)d.n2 * d.K1);
+  const int nchz = (d.K1 + t.kbz - 1) / t.kbz;
+  size_t ldsz = 2ull * NF * t.kbz * (d.n3 + 1) * sizeof(C);
+  hipLaunchKernelGGL((k_zgop<R, KIND>), dim3(nchz * d.n2, nb), dim3(256), ldsz, st, dd, d, sp.pz, twp<R>(sp, 2),
+                     t.kbz, nchz);
+  hipLaunchKernelGGL((k_cpass<R, +1>), dim3(nchy * d.n3, NF, nb), dim3(256), ldsy, st, dd, d, sp.py,
+                     twp<R>(sp, 1), t.kby, nchy, (long long)d.K1, (long long)d.n2 * d.K1);
+  hipLaunchKernelGGL((k_xinv<R, KIND, INMODE, OUTMODE>), dim3(t.ntile, nb), dim3(256), lds_x, st, dd, d, sp.px,
+                     twp<R>(sp, 0), twp<R>(sp, 3), t.rows);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename R, int KIND>
+static int eval_k(const ShapePlan& sp, int inmode, int outmode, const ReqDesc* dd, int nb, hipStream_t st) {
+  if (inmode == kInRho) {
+    if (outmode != kOutPot) return -2;
+    return eval_t<R, KIND, kInRho, kOutPot>(sp, dd, nb, st);
+  }
+  switch (outmode) {
+    case kOutPot: return eval_t<R, KIND, kInPhi, kOutPot>(sp, dd, nb, st);
+    case kOutSigned: return eval_t<R, KIND, kInPhi, kOutSigned>(sp, dd, nb, st);
+    default: return eval_t<R, KIND, kInPhi, kOutNewton>(sp, dd, nb, st);
+  }
+}
+
+template <typename R>
+static int eval_r(const ShapePlan& sp, int kind, int inmode, int outmode, const ReqDesc* dd, int nb,
+                  hipStream_t st) {
+  switch (kind) {
+    case kWT3: return eval_k<R, kWT3>(sp, inmode, outmode, dd, nb, st);
+    case kWT4: return eval_k<R, kWT4>(sp, inmode, outmode, dd, nb, st);
+    case kWGC8: return eval_k<R, kWGC8>(sp, inmode, outmode, dd, nb, st);
+    default: return -3;
+  }
+}
+
+int run_eval(const ShapePlan& sp, int kind, int inmode, int outmode, int fp32, const ReqDesc* dd, int nb,
+             hipStream_t st) {
+  return fp32 ? eval_r<float>(sp, kind, inmode, outmode, dd, nb, st)
+              : eval_r<double>(sp, kind, inmode, outmode, dd, nb, st);
+}
+
+int run_finalize(const ShapePlan& sp, int kind, int fp32, const ReqDesc* dd, int nb, double* out, hipStream_t st) {
+  const Tiles t = tiles_for(sp, func_nf(kind), fp32 ? (int)sizeof(float2) : (int)sizeof(double2));
+  hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, st, dd, t.ntile, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------ plain 3D transforms (per-geometry tables)
+// Forward R2C of the real field D.u into D.work (no 1/N), backward C2R of
+// D.work into D.out (unnormalized). Used by the ion potential and forces.
+int run_fft3d_plain(const ShapePlan& sp, int forward, const ReqDesc* dd, int nb, hipStream_t st) {
+  using C = double2;
+  const GridDims& d = sp.d;
+  const Tiles t = tiles_for(sp, 1, (int)sizeof(C));
+  size_t lds_x = 2ull * t.rows * (d.M + 1) * sizeof(C);
+  const int nchy = (d.K1 + t.kby - 1) / t.kby;
+  size_t ldsy = 2ull * t.kby * (d.n2 + 1) * sizeof(C);
+  const int nchz = (d.K1 + t.kbz - 1) / t.kbz;
+  size_t ldsz = 2ull * t.kbz * (d.n3 + 1) * sizeof(C);
+  if (forward) {
+    hipLaunchKernelGGL((k_xfwd<double, kPlain, kInRho>), dim3(t.ntile, nb), dim3(256), lds_x, st, dd, d, sp.px,
+                       sp.tw_x, sp.tw_s, t.rows);
+    hipLaunchKernelGGL((k_cpass<double, -1>), dim3(nchy * d.n3, 1, nb), dim3(256), ldsy, st, dd, d, sp.py, sp.tw_y,
+                       t.kby, nchy, (long long)d.K1, (long long)d.n2 * d.K1);
+    hipLaunchKernelGGL((k_cpass<double, -1>), dim3(nchz * d.n2, 1, nb), dim3(256), ldsz, st, dd, d, sp.pz, sp.tw_z,
+                       t.kbz, nchz, (long long)d.n2 * d.K1, (long long)d.K1);
+  } else {
+    hipLaunchKernelGGL((k_cpass<double, +1>), dim3(nchz * d.n2, 1, nb), dim3(256), ldsz, st, dd, d, sp.pz, sp.tw_z,
+                       t.kbz, nchz, (long long)d.n2 * d.K1, (long long)d.K1);
+    hipLaunchKernelGGL((k_cpass<double, +1>), dim3(nchy * d.n3, 1, nb), dim3(256), ldsy, st, dd, d, sp.py, sp.tw_y,
+                       t.kby, nchy, (long long)d.K1, (long long)d.n2 * d.K1);
+    hipLaunchKernelGGL((k_xinv<double, kPlain, kInRho, kOutPot>), dim3(t.ntile, nb), dim3(256), lds_x, st, dd, d,
+                       sp.px, sp.tw_x, sp.tw_s, t.rows);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ============================================================ vector kernels
+__global__ void k_axpby(double* y, const double* x, double a, double b, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = fma(a, x[i], b * y[i]);
+}
+__global__ void k_lincomb(double* z, const double* x, const double* y, double a, double b, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    z[i] = fma(b, y[i], a * x[i]);
+}
+__global__ void k_sqrt_abs(double* y, const double* x, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = sqrt(fabs(x[i]));
+}
+__global__ void k_square(double* y, const double* x, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i] * x[i];
+}
+__global__ void k_scale_sign(double* y, const double* x, const double* s, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i] * fsign(s[i]);
+}
+struct DotArgs {
+  const double* a[4];
+  const double* b[4];
+};
+constexpr int kDotBlocks = 512;
+__global__ void __launch_bounds__(256) k_dots(DotArgs args, int np, long long n, double* partial) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < np) acc[k] += args.a[k][i] * args.b[k][i];
+  }
+  block_sum_store<4>(acc, partial + (size_t)blockIdx.x * 4);
+}
+__global__ void __launch_bounds__(256) k_dots_fin(const double* partial, int nblk, double* out) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int w = threadIdx.x; w < nblk; w += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] += partial[(size_t)w * 4 + k];
+  }
+  block_sum_store<4>(acc, out);
+}
+
+static int nblocks(long long n) {
+  long long b = (n + 255) / 256;
+  return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
+}
+void vec_axpby(double* y, const double* x, double a, double b, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(256), 0, st, y, x, a, b, n);
+}
+void vec_lincomb(double* z, const double* x, const double* y, double a, double b, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_lincomb, dim3(nblocks(n)), dim3(256), 0, st, z, x, y, a, b, n);
+}
+void vec_sqrt_abs(double* y, const double* x, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_sqrt_abs, dim3(nblocks(n)), dim3(256), 0, st, y, x, n);
+}
+void vec_square(double* y, const double* x, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_square, dim3(nblocks(n)), dim3(256), 0, st, y, x, n);
+}
+void vec_scale_sign(double* y, const double* x, const double* s, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_sign, dim3(nblocks(n)), dim3(256), 0, st, y, x, s, n);
+}
+void vec_dots(const double* const* a, const double* const* b, int np, long long n, double* d_partial,
+              double* d_out, hipStream_t st) {
+  DotArgs args;
+  for (int k = 0; k < 4; ++k) {
+    args.a[k] = k < np ? a[k] : a[0];
+    args.b[k] = k < np ? b[k] : b[0];
+  }
+  long long want = (n + 255) / 256;
+  int nb = (int)(want > kDotBlocks ? kDotBlocks : (want < 1 ? 1 : want));
+  hipLaunchKernelGGL(k_dots, dim3(nb), dim3(256), 0, st, args, np, n, d_partial);
+  hipLaunchKernelGGL(k_dots_fin, dim3(1), dim3(256), 0, st, d_partial, nb, d_out);
+}
+
+}  // namespace ofd

@@ -1,0 +1,416 @@
+// Per-geometry tables on the device:
+//  * WGC kernels K1..K4 on the half grid from the 1D eta table
+//    (FillWGC_ReciprocalSpace, KEDF_WGCkernel.f90:109-277)
+//  * the ionic potential from cardinal B-splines (IonElectronPotRecipSpline,
+//    IonElectronSpline.f90:83-216) and the ion-electron forces
+//    (IonElectronForcesSpline / CalculateSplineForces, :219-355, :642-753)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ofd_internal.h"
+
+namespace ofd {
+
+constexpr double kPi = 3.141592653589793238462643;
+
+// rvec_bracket (MathSplines.f90:498-545): bisection, 0-based left/right
+__device__ __forceinline__ void bracket(const double* __restrict__ t, int n, double x, int& left, int& right) {
+  int i1 = 1, i3 = n, i2 = (i3 + i1) / 2;
+  while (i2 != i1) {
+    const double x2 = t[i2 - 1];
+    if (x == x2) {
+      left = i2 - 1;
+      right = i2;
+      return;
+    } else if (x < x2) {
+      i3 = i2;
+    } else if (x > x2) {
+      i1 = i2;
+    }
+    i2 = (i3 + i1) / 2;
+  }
+  if (x == t[i3 - 1]) { left = i3 - 1; right = i3; return; }
+  if (x == t[i1 - 1]) { left = i1 - 1; right = i1; return; }
+  left = i1 - 1;
+  right = i3 - 1;
+}
+
+// spline_cubic_val (MathSplines.f90:395-470), value only
+__device__ __forceinline__ double spline_val(const double* __restrict__ t, const double* __restrict__ y,
+                                             const double* __restrict__ ypp, int left, int right, double x) {
+  const double dt = x - t[left];
+  const double h = t[right] - t[left];
+  const double tmp1 = (y[right] - y[left]) / h;
+  const double tmp2 = -(ypp[right] / 6.0 + ypp[left] / 3.0) * h;
+  const double tmp3 = ypp[right] - ypp[left];
+  return y[left] + dt * (tmp1 + tmp2 + dt * (0.5 * ypp[left] + dt * (tmp3 / (6.0 * h))));
+}
+
+__device__ __forceinline__ double gnorm2(const double* rb, int m1, int m2, int m3) {
+  const double gx = rb[0] * m1 + rb[1] * m2 + rb[2] * m3;
+  const double gy = rb[3] * m1 + rb[4] * m2 + rb[5] * m3;
+  const double gz = rb[6] * m1 + rb[7] * m2 + rb[8] * m3;
+  return gx * gx + gy * gy + gz * gz;
+}
+
+struct RB {
+  double v[9];
+};
+
+__global__ void k_wgc_fill(GridDims d, RB rb, const double* __restrict__ eta, const double* __restrict__ w,
+                           const double* __restrict__ iw1, const double* __restrict__ iw2,
+                           const double* __restrict__ pp0, const double* __restrict__ pp1,
+                           const double* __restrict__ pp2, int nk, double tkfs, double gamma, double rhoS,
+                           double* __restrict__ out) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= d.nc) return;
+  const int k1 = (int)(idx % d.K1);
+  const long long rest = idx / d.K1;
+  const int i2 = (int)(rest % d.n2), i3 = (int)(rest / d.n2);
+  const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
+  const int m3 = i3 > d.n3 / 2 ? i3 - d.n3 : i3;
+  const double q = sqrt(gnorm2(rb.v, k1, m2, m3));
+  const double e = q / tkfs;
+  int l, r;
+  bracket(eta, nk, e, l, r);
+  const double K1 = spline_val(eta, w, pp0, l, r, e);
+  const double s1 = spline_val(eta, iw1, pp1, l, r, e);
+  const double s2 = spline_val(eta, iw2, pp2, l, r, e);
+  out[idx] = K1;
+  out[d.nc + idx] = -s1 / (6.0 * rhoS);
+  out[2 * d.nc + idx] = (s2 + (7 - gamma) * s1) / (36.0 * rhoS * rhoS);
+  out[3 * d.nc + idx] = (s2 + (1 + gamma) * s1) / (36.0 * rhoS * rhoS);
+}
+
+int build_wgc_kernels(const ShapePlan& sp, const double* rb, const double* d_eta, const double* d_w,
+                      const double* d_w1, const double* d_w2, const double* d_pp0, const double* d_pp1,
+                      const double* d_pp2, int nknots, double tkfs, double gamma, double rhoS, double* d_out,
+                      hipStream_t st) {
+  RB r;
+  for (int i = 0; i < 9; ++i) r.v[i] = rb[i];
+  const long long n = sp.d.nc;
+  hipLaunchKernelGGL(k_wgc_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sp.d, r, d_eta, d_w, d_w1, d_w2,
+                     d_pp0, d_pp1, d_pp2, nknots, tkfs, gamma, rhoS, d_out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------------ B-splines
+// GetCardinalBSpline (CBSpline.f90:42-90): s[0..order], s[k] for k = 1..order
+template <int MAXO>
+__device__ __forceinline__ void card_bspline(double* s, int order, double x) {
+  for (int i = 0; i <= order; ++i) s[i] = 0.0;
+  s[1] = x;
+  s[2] = 1.0 - x;
+  for (int n = 3; n <= order; ++n) {
+    for (int k = 0; k < n; ++k) {
+      s[n - k] = ((x + (double)(n - k - 1)) * s[n - k] + ((double)(k + 1) - x) * s[n - k - 1]) / (double)(n - 1);
+    }
+  }
+}
+
+constexpr int kMaxOrder = 16;
+
+// FillQIonTable (IonElectronSpline.f90:537-638): one thread per (ion, i3, i2)
+// stencil line; weights added with FP64 atomics.
+__global__ void k_qion_scatter(GridDims d, const double* __restrict__ frac, int ion0, int nion, int order,
+                               double* __restrict__ q) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long total = (long long)nion * order * order;
+  if (idx >= total) return;
+  const int ion = ion0 + (int)(idx / (order * order));
+  const int rem = (int)(idx % (order * order));
+  const int i3 = rem / order, i2 = rem % order;
+  const double g1 = frac[3 * ion] * d.n1, g2 = frac[3 * ion + 1] * d.n2, g3 = frac[3 * ion + 2] * d.n3;
+  const double f1 = floor(g1), f2 = floor(g2), f3 = floor(g3);
+  double M1[kMaxOrder + 1], M2[kMaxOrder + 1], M3[kMaxOrder + 1];
+  card_bspline<kMaxOrder>(M1, order, g1 - f1);
+  card_bspline<kMaxOrder>(M2, order, g2 - f2);
+  card_bspline<kMaxOrder>(M3, order, g3 - f3);
+  auto md = [](long long a, int n) { long long r = a % n; return (int)(r < 0 ? r + n : r); };
+  const int l3 = md(-(long long)f3 + i3, d.n3);
+  const int l2 = md(-(long long)f2 + i2, d.n2);
+  for (int ix = 0; ix < order; ++ix) {
+    const int l1 = md(-(long long)f1 + ix, d.n1);
+    atomicAdd(&q[((size_t)l3 * d.n2 + l2) * d.n1 + l1], M1[ix + 1] * M2[i2 + 1] * M3[i3 + 1]);
+  }
+}
+
+struct PspDev {
+  const double* vqs;
+  const double* dd;
+  double maxg, v0, spacing;
+  int npts, charge;
+};
+
+// PspRecipCubicSpline (LocalPseudoPot.f90:136-198)
+__device__ __forceinline__ double psp_lookup(const PspDev& p, double q) {
+  if (q == 0.0) return p.v0;
+  if (q >= p.maxg) return 0.0;
+  const double pos = q / p.spacing + 1.0;
+  int left = (int)floor(pos);
+  if (left > p.npts - 1) left = p.npts - 1;   // q just below maxG rounding up
+  const double dt = pos - (double)left;
+  const double yl = p.vqs[left - 1], yr = p.vqs[left];
+  const double pl = p.dd[left - 1], pr = p.dd[left];
+  const double yval = yl + dt * (yr - yl - (pr / 6.0 + pl / 3.0) + dt * (0.5 * pl + dt * ((pr - pl) / 6.0)));
+  return yval - 4.0 * kPi * (double)p.charge / pow(q, 2.0);
+}
+
+// acc(G) (+)= V_loc(|G|) * SF(G) / N  (SF = unnormalized R2C of the table)
+__global__ void k_ion_accum(GridDims d, RB rb, PspDev p, const double2* __restrict__ sf, double2* __restrict__ acc,
+                            int first) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= d.nc) return;
+  const int k1 = (int)(idx % d.K1);
+  const long long rest = idx / d.K1;
+  const int i2 = (int)(rest % d.n2), i3 = (int)(rest / d.n2);
+  const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
+  const int m3 = i3 > d.n3 / 2 ? i3 - d.n3 : i3;
+  const double q = sqrt(gnorm2(rb.v, k1, m2, m3));
+  const double v = psp_lookup(p, q);
+  const double invn = 1.0 / (double)d.nr;
+  double2 s = sf[idx];
+  double2 a = first ? make_double2(0.0, 0.0) : acc[idx];
+  a.x += v * (s.x * invn);
+  a.y += v * (s.y * invn);
+  acc[idx] = a;
+}
+
+// V(G) = conj(acc * b1 b2 b3) / vol * N   (IonElectronSpline.f90:200-203)
+__global__ void k_ion_finish(GridDims d, const double2* __restrict__ b1, const double2* __restrict__ b2,
+                             const double2* __restrict__ b3, double vol, double2* __restrict__ acc) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= d.nc) return;
+  const int k1 = (int)(idx % d.K1);
+  const long long rest = idx / d.K1;
+  const int i2 = (int)(rest % d.n2), i3 = (int)(rest / d.n2);
+  double2 a = acc[idx];
+  auto mul = [](double2 x, double2 y) { return make_double2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x); };
+  a = mul(mul(mul(a, b1[k1]), b2[i2]), b3[i3]);
+  const double sc = (double)d.nr / vol;
+  acc[idx] = make_double2(a.x / vol * (double)d.nr, -a.y / vol * (double)d.nr);
+  (void)sc;
+}
+
+// rhoRecip * pspRecip * b1 b2 b3 (IonElectronSpline.f90:315-331), G = 0 set to 0
+__global__ void k_force_recip(GridDims d, RB rb, PspDev p, const double2* __restrict__ rho_g,
+                              const double2* __restrict__ b1, const double2* __restrict__ b2,
+                              const double2* __restrict__ b3, double2* __restrict__ out) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= d.nc) return;
+  const int k1 = (int)(idx % d.K1);
+  const long long rest = idx / d.K1;
+  const int i2 = (int)(rest % d.n2), i3 = (int)(rest / d.n2);
+  const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
+  const int m3 = i3 > d.n3 / 2 ? i3 - d.n3 : i3;
+  double v = idx == 0 ? 0.0 : psp_lookup(p, sqrt(gnorm2(rb.v, k1, m2, m3)));
+  const double invn = 1.0 / (double)d.nr;
+  double2 r = rho_g[idx];
+  double2 a = make_double2(r.x * invn * v, r.y * invn * v);
+  auto mul = [](double2 x, double2 y) { return make_double2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x); };
+  out[idx] = mul(mul(mul(a, b1[k1]), b2[i2]), b3[i3]);
+}
+
+// CalculateSplineForces (IonElectronSpline.f90:642-753): one workgroup per ion,
+// fixed-order block reduction of the order^3 stencil.
+__global__ void __launch_bounds__(256) k_force_gather(GridDims d, const double* __restrict__ frac, int ion0,
+                                                      int order, RB cinv, const double* __restrict__ factor,
+                                                      double* __restrict__ forces) {
+  const int ion = ion0 + blockIdx.x;
+  const double g1 = frac[3 * ion] * d.n1, g2 = frac[3 * ion + 1] * d.n2, g3 = frac[3 * ion + 2] * d.n3;
+  const double f1 = floor(g1), f2 = floor(g2), f3 = floor(g3);
+  double M1[kMaxOrder + 1], M2[kMaxOrder + 1], M3[kMaxOrder + 1];
+  double D1[kMaxOrder + 1], D2[kMaxOrder + 1], D3[kMaxOrder + 1];
+  card_bspline<kMaxOrder>(M1, order, g1 - f1);
+  card_bspline<kMaxOrder>(M2, order, g2 - f2);
+  card_bspline<kMaxOrder>(M3, order, g3 - f3);
+  card_bspline<kMaxOrder>(D1, order - 1, g1 - f1);
+  card_bspline<kMaxOrder>(D2, order - 1, g2 - f2);
+  card_bspline<kMaxOrder>(D3, order - 1, g3 - f3);
+  D1[order] = 0.0; D2[order] = 0.0; D3[order] = 0.0;
+  auto md = [](long long a, int n) { long long r = a % n; return (int)(r < 0 ? r + n : r); };
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int tot = order * order * order;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int iz = e / (order * order), rem = e % (order * order);
+    const int iy = rem / order, ix = rem % order;
+    const int l3 = md((long long)f3 - iz, d.n3);
+    const int l2 = md((long long)f2 - iy, d.n2);
+    const int l1 = md((long long)f1 - ix, d.n1);
+    const double q1 = M2[iy + 1] * M3[iz + 1] * (double)d.n1 * (D1[ix + 1] - D1[ix]);
+    const double q2 = M1[ix + 1] * M3[iz + 1] * (double)d.n2 * (D2[iy + 1] - D2[iy]);
+    const double q3 = M1[ix + 1] * M2[iy + 1] * (double)d.n3 * (D3[iz + 1] - D3[iz]);
+    const double fac = factor[((size_t)l3 * d.n2 + l2) * d.n1 + l1];
+    // splineForces -= Vecmul(cellInv, qIonDeriv) * Factor
+    for (int c = 0; c < 3; ++c) acc[c] -= (cinv.v[3 * c] * q1 + cinv.v[3 * c + 1] * q2 + cinv.v[3 * c + 2] * q3) * fac;
+  }
+  __shared__ double red[4][3];
+  for (int c = 0; c < 3; ++c) {
+    double v = acc[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    acc[c] = v;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int c = 0; c < 3; ++c) red[wv][c] = acc[c];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w][threadIdx.x];
+    forces[3 * ion + threadIdx.x] = s;
+  }
+}
+
+// FillB (CBSpline.f90:93-196) on the host in long double
+static void fill_b_host(int order, int size, int tot, std::vector<double2>& b) {
+  // M = GetCardinalBSpline(order, 1.0)
+  std::vector<long double> s(order + 1, 0.0L);
+  const long double x = 1.0L;
+  s[1] = x;
+  s[2] = 1.0L - x;
+  for (int n = 3; n <= order; ++n)
+    for (int k = 0; k < n; ++k)
+      s[n - k] = ((x + (long double)(n - k - 1)) * s[n - k] + ((long double)(k + 1) - x) * s[n - k - 1]) /
+                 (long double)(n - 1);
+  const long double tp = 6.283185307179586476925286766559005768L;
+  b.resize(size);
+  for (int ix = 0; ix < size; ++ix) {
+    long double dr = 0, di = 0;
+    for (int k = 0; k <= order - 2; ++k) {
+      long double a = tp * (long double)ix * (long double)k / (long double)tot;
+      dr += s[k + 1] * cosl(a);
+      di += s[k + 1] * sinl(a);
+    }
+    long double a = tp * (long double)(order - 1) * (long double)ix / (long double)tot;
+    long double nr = cosl(a), ni = sinl(a);
+    long double den = dr * dr + di * di;
+    b[ix].x = (double)((nr * dr + ni * di) / den);
+    b[ix].y = (double)((ni * dr - nr * di) / den);
+  }
+}
+
+struct BTables {
+  double2 *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
+  void make(const GridDims& d, int order, hipStream_t st) {
+    std::vector<double2> h1, h2, h3;
+    fill_b_host(order, d.K1, d.n1, h1);
+    fill_b_host(order, d.n2, d.n2, h2);
+    fill_b_host(order, d.n3, d.n3, h3);
+    hipMalloc(&b1, sizeof(double2) * h1.size());
+    hipMalloc(&b2, sizeof(double2) * h2.size());
+    hipMalloc(&b3, sizeof(double2) * h3.size());
+    hipMemcpyAsync(b1, h1.data(), sizeof(double2) * h1.size(), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(b2, h2.data(), sizeof(double2) * h2.size(), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(b3, h3.data(), sizeof(double2) * h3.size(), hipMemcpyHostToDevice, st);
+    hipStreamSynchronize(st);
+  }
+  ~BTables() {
+    hipFree(b1);
+    hipFree(b2);
+    hipFree(b3);
+  }
+};
+
+static PspDev psp_dev(int s, const double* d_vqs, const double* d_dd, const int* off, const int* npts,
+                      const double* maxg, const double* v0, const int* charge) {
+  PspDev p;
+  p.vqs = d_vqs + off[s];
+  p.dd = d_dd + off[s];
+  p.maxg = maxg[s];
+  p.v0 = v0[s];
+  p.npts = npts[s];
+  p.spacing = maxg[s] / (double)(npts[s] - 1);
+  p.charge = charge[s];
+  return p;
+}
+
+// scratch layout: [real grid nr doubles][complex sf nc][complex acc nc]
+int build_ion_potential(const ShapePlan& sp, const double* rb, double vol, int ntypes, const int* h_type_start,
+                        const double* d_frac, int nion, const double* d_psp_vqs, const double* d_psp_dd,
+                        const int* h_psp_off, const int* h_psp_npts, const double* h_psp_maxg,
+                        const double* h_psp_v0, const int* h_psp_charge, int order, double* d_vion,
+                        void* d_scratch, hipStream_t st) {
+  const GridDims& d = sp.d;
+  if (order < 2 || order > kMaxOrder) return -2;
+  RB r;
+  for (int i = 0; i < 9; ++i) r.v[i] = rb[i];
+  double* q = (double*)d_scratch;
+  double2* sf = (double2*)(q + d.nr);
+  double2* acc = sf + d.nc;
+  ReqDesc D;
+  std::memset(&D, 0, sizeof(D));
+  ReqDesc* dd;
+  if (hipMalloc(&dd, sizeof(ReqDesc)) != hipSuccess) return -1;
+  const unsigned gb = (unsigned)((d.nc + 255) / 256);
+  for (int s = 0; s < ntypes; ++s) {
+    const int i0 = h_type_start[s], n_s = h_type_start[s + 1] - i0;
+    hipMemsetAsync(q, 0, sizeof(double) * d.nr, st);
+    const long long tot = (long long)n_s * order * order;
+    if (tot > 0)
+      hipLaunchKernelGGL(k_qion_scatter, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, d, d_frac, i0, n_s,
+                         order, q);
+    D.u = q;
+    D.work = sf;
+    hipMemcpyAsync(dd, &D, sizeof(D), hipMemcpyHostToDevice, st);
+    run_fft3d_plain(sp, 1, dd, 1, st);
+    PspDev p = psp_dev(s, d_psp_vqs, d_psp_dd, h_psp_off, h_psp_npts, h_psp_maxg, h_psp_v0, h_psp_charge);
+    hipLaunchKernelGGL(k_ion_accum, dim3(gb), dim3(256), 0, st, d, r, p, sf, acc, s == 0 ? 1 : 0);
+  }
+  BTables bt;
+  bt.make(d, order, st);
+  hipLaunchKernelGGL(k_ion_finish, dim3(gb), dim3(256), 0, st, d, bt.b1, bt.b2, bt.b3, vol, acc);
+  D.work = acc;
+  D.out = d_vion;
+  hipMemcpyAsync(dd, &D, sizeof(D), hipMemcpyHostToDevice, st);
+  run_fft3d_plain(sp, 0, dd, 1, st);
+  hipError_t e = hipStreamSynchronize(st);
+  hipFree(dd);
+  return e == hipSuccess ? 0 : -1;
+}
+
+int ion_forces_spline(const ShapePlan& sp, const double* rb, const double* cell_inv_t, int ntypes,
+                      const int* h_type_start, const double* d_frac, int nion, const double* d_psp_vqs,
+                      const double* d_psp_dd, const int* h_psp_off, const int* h_psp_npts, const double* h_psp_maxg,
+                      const double* h_psp_v0, const int* h_psp_charge, int order, const double* d_rho,
+                      double* d_forces, void* d_scratch, hipStream_t st) {
+  const GridDims& d = sp.d;
+  if (order < 2 || order > kMaxOrder) return -2;
+  RB r, ci;
+  for (int i = 0; i < 9; ++i) {
+    r.v[i] = rb[i];
+    ci.v[i] = cell_inv_t[i];
+  }
+  double* factor = (double*)d_scratch;
+  double2* rg = (double2*)(factor + d.nr);
+  double2* tmp = rg + d.nc;
+  ReqDesc D;
+  std::memset(&D, 0, sizeof(D));
+  ReqDesc* dd;
+  if (hipMalloc(&dd, sizeof(ReqDesc)) != hipSuccess) return -1;
+  D.u = d_rho;
+  D.work = rg;
+  hipMemcpyAsync(dd, &D, sizeof(D), hipMemcpyHostToDevice, st);
+  run_fft3d_plain(sp, 1, dd, 1, st);
+  BTables bt;
+  bt.make(d, order, st);
+  const unsigned gb = (unsigned)((d.nc + 255) / 256);
+  for (int s = 0; s < ntypes; ++s) {
+    const int i0 = h_type_start[s], n_s = h_type_start[s + 1] - i0;
+    PspDev p = psp_dev(s, d_psp_vqs, d_psp_dd, h_psp_off, h_psp_npts, h_psp_maxg, h_psp_v0, h_psp_charge);
+    hipLaunchKernelGGL(k_force_recip, dim3(gb), dim3(256), 0, st, d, r, p, rg, bt.b1, bt.b2, bt.b3, tmp);
+    D.work = tmp;
+    D.out = factor;
+    hipMemcpyAsync(dd, &D, sizeof(D), hipMemcpyHostToDevice, st);
+    run_fft3d_plain(sp, 0, dd, 1, st);
+    if (n_s > 0)
+      hipLaunchKernelGGL(k_force_gather, dim3(n_s), dim3(256), 0, st, d, d_frac, i0, order, ci, factor, d_forces);
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  hipFree(dd);
+  return e == hipSuccess ? 0 : -1;
+}
+
+}  // namespace ofd

@@ -1,0 +1,60 @@
+"""CPU checks of the drop-in boundary: the library loads and exports every
+entry point include/ofd.h declares (no compute call: there is no GPU here),
+and the host-side helpers behave."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "ofd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ofd_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declares_the_reference_surface():
+    names = _declared()
+    for n in ["ofd_calpot", "ofd_rho_opt", "ofd_create", "ofd_set_ion_terms", "ofd_ion_forces",
+              "ofd_build_ion_potential", "ofd_set_wgc_table"]:
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from profess_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == _declared()
+
+
+def test_request_bytes_is_monotone_and_host_only():
+    from profess_amd import _lib, request_bytes
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    b32 = request_bytes((32, 32, 32))
+    b128 = request_bytes((128, 128, 128))
+    bw = request_bytes((128, 128, 128), kinetic=5)
+    assert 0 < b32 < b128 < bw
+    # about 25 real grids at 128^3 (SURVEY footprint estimate)
+    assert 0.2e9 < b128 < 0.6e9
+
+
+def test_input_generators():
+    from profess_amd import inputs
+    lat, frac = inputs.fcc_supercell(7.65, 2)
+    assert frac.shape == (32, 3) and np.allclose(np.diag(lat), 15.3)
+    lat, frac = inputs.hcp_supercell(6.06, 9.84, (2, 2, 1))
+    assert frac.shape == (8, 3)
+    c = inputs.al_fcc_case((1, 1, 1), (16, 16, 16))
+    vol = abs(np.linalg.det(c["lattice"]))
+    rho0 = c["num_electrons"] / vol
+    assert c["rho"].shape == (16, 16, 16)
+    assert np.all(c["rho"] >= 0.7 * rho0 - 1e-15) and np.all(c["rho"] <= 1.3 * rho0 + 1e-15)
+    mg, v = inputs.synthetic_recpot(3, 1.2)
+    assert v.size == 15001 and abs(mg * inputs.BOHR - 30.0) < 1e-12
