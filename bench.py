@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""Benchmark: Wang-Teter energy + potential evaluations per second at 128^3 (FP64).
+
+One step = one complete CalculatePotentialPlus evaluation (TF + vW + WT, LDA,
+Hartree, ion-electron, all nine eTable entries) of fcc Al 4x4x4 (256 atoms,
+Gaussian displacements sigma = 0.1 bohr, synthetic V_loc Z = 3, rc = 1.2 bohr)
+on a 128^3 grid, density rho0 (1 + 0.3 f) already resident in HBM; evaluations
+are enqueued back to back on the library's stream and timed with HIP events on
+that stream. Prints one JSON line (see the keys below).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128] [--batch B]
+                  [--no-cpu] [--no-serving]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def b_wt(n1, n2, n3):
+    """Algorithmic HBM bytes of one WT evaluation (alpha == beta): rho read
+    twice, V_ion read once, the potential written once (4 real grids) and
+    three half-complex spectra each written once and read once (6 complex)."""
+    nr = n1 * n2 * n3
+    nc = (n1 // 2 + 1) * n2 * n3
+    return 4 * 8 * nr + 6 * 16 * nc
+
+
+class Hip:
+    def __init__(self):
+        self.L = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipFree.argtypes = [C.c_void_p]
+        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.L.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.L.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed (%d)" % (what, rc))
+
+    def malloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.L.hipMalloc(C.byref(p), nbytes), "hipMalloc")
+        return p.value
+
+    def h2d(self, dst, arr):
+        self.check(self.L.hipMemcpy(C.c_void_p(dst), arr.ctypes.data_as(C.c_void_p), arr.nbytes, 1), "hipMemcpy")
+
+    def d2h(self, arr, src):
+        self.check(self.L.hipMemcpy(arr.ctypes.data_as(C.c_void_p), C.c_void_p(src), arr.nbytes, 2), "hipMemcpy")
+
+    def event(self):
+        e = C.c_void_p()
+        self.check(self.L.hipEventCreate(C.byref(e)), "hipEventCreate")
+        return e.value
+
+    def record(self, e, stream):
+        self.check(self.L.hipEventRecord(C.c_void_p(e), C.c_void_p(stream)), "hipEventRecord")
+
+    def elapsed_ms(self, e0, e1):
+        self.check(self.L.hipEventSynchronize(C.c_void_p(e1)), "hipEventSynchronize")
+        ms = C.c_float()
+        self.check(self.L.hipEventElapsedTime(C.byref(ms), C.c_void_p(e0), C.c_void_p(e1)), "hipEventElapsedTime")
+        return float(ms.value)
+
+
+def make_case(n):
+    from profess_amd import inputs
+    reps = max(1, n // 32)
+    return inputs.al_fcc_case((reps, reps, reps), (n, n, n), sigma=0.1, seed_ion=7, seed_rho=11)
+
+
+def time_single(pa, hip, case, steps, warmup):
+    L = pa.lib()
+    s = pa.OFDSystem(case["lattice"], case["grid"], case["num_electrons"], kinetic=4)
+    mg, vals = case["recpot"]
+    s.build_ion_potential([(mg, vals, case["frac"])])
+    rho = np.ascontiguousarray(case["rho"])
+    d_rho = hip.malloc(rho.nbytes)
+    d_pot = hip.malloc(rho.nbytes)
+    d_sum = hip.malloc(9 * 8)
+    hip.h2d(d_rho, rho)
+    stream = L.ofd_get_stream(s._h)
+
+    def enq():
+        s._check(L.ofd_calpot_enqueue(s._h, C.c_void_p(d_rho), 1, C.c_void_p(d_pot), C.c_void_p(d_sum)))
+
+    for _ in range(warmup):
+        enq()
+    s._check(L.ofd_sync(s._h))
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    for _ in range(steps):
+        enq()
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1)
+    sums = np.zeros(9)
+    hip.d2h(sums, d_sum)
+    et = np.zeros(9)
+    L.ofd_sums_to_etable(s._h, sums.ctypes.data_as(C.POINTER(C.c_double)), et.ctypes.data_as(C.POINTER(C.c_double)))
+    if not np.all(np.isfinite(et)):
+        raise RuntimeError("non-finite energies")
+    vion = s.ion_potential()
+    return s, ms / steps, et, vion, (d_rho, d_pot, d_sum)
+
+
+def time_batched(pa, hip, s0, case, vion, nb, steps, warmup):
+    L = pa.lib()
+    systems = [s0]
+    for _ in range(nb - 1):
+        t = pa.OFDSystem(case["lattice"], case["grid"], case["num_electrons"], kinetic=4)
+        t.set_ion_terms(vion, 0.0)
+        systems.append(t)
+    rho = np.ascontiguousarray(case["rho"])
+    rp, pp = [], []
+    for _ in range(nb):
+        r = hip.malloc(rho.nbytes)
+        hip.h2d(r, rho)
+        rp.append(r)
+        pp.append(hip.malloc(rho.nbytes))
+    hs = (C.c_void_p * nb)(*[x._h.value for x in systems])
+    ra = (C.c_void_p * nb)(*rp)
+    pa_ = (C.c_void_p * nb)(*pp)
+    stream = L.ofd_get_stream(s0._h)
+
+    def run():
+        rc = L.ofd_calpot_many(hs, nb, ra, pa_, 1, None)
+        if rc != 0:
+            raise RuntimeError(L.ofd_last_error(s0._h).decode())
+
+    for _ in range(warmup):
+        run()
+    s0._check(L.ofd_sync(s0._h))
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    for _ in range(steps):
+        run()
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1)
+    return ms / steps, systems
+
+
+def cpu_baseline(case, vion, budget_s=15.0):
+    """The oracle's WT evaluation (numpy, single thread) on the same inputs."""
+    from oracle import profess_oracle as O
+    n1, n2, n3 = case["grid"]
+    cell = O.Cell(case["lattice"], n1, n2, n3)
+    f = O.Functional(kinetic=4).resolve(case["num_electrons"], cell.vol)
+    sysm = O.System(cell, f, vion)
+    rho = case["rho"]
+    t0 = time.perf_counter()
+    n = 0
+    et = None
+    while True:
+        _, et = O.calculate_potential_plus(rho, sysm, True, True)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 50:
+            break
+    dt = time.perf_counter() - t0
+    return n / dt, n, dt, et
+
+
+def read_traffic():
+    """HBM bytes per evaluation measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by the copy-kernel factor), from profiles/."""
+    p = os.path.join(ROOT, "profiles", "wt128_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    for ln in open(p):
+        if ln.startswith("traffic_bytes_per_eval"):
+            try:
+                return float(ln.split("=")[1].split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def serving_sample(pa, nreq=24, seed=5):
+    """Small continuous-batching sample: fcc Al cells of 4-32 atoms on 32^3/48^3
+    grids through profess_amd.serve (FP32-then-FP64 NTN to 1e-5)."""
+    from profess_amd import serve
+    return serve.run_workload(serve.make_requests(nreq, seed=seed, grids=(32, 48), max_atoms=32, rate=200.0),
+                              mixed=True, verbose=False)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the evaluation is a single-GPU path; "
+              "multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
+        sys.exit(2)
+
+    import profess_amd as pa
+    pa.lib()
+    hip = Hip()
+    n = args.grid
+    case = make_case(n)
+    s, ms1, et, vion, bufs = time_single(pa, hip, case, args.steps, args.warmup)
+    evals_s = 1000.0 / ms1
+    bw = b_wt(n, n, n)
+    achieved = bw * evals_s / 1e9
+    out = {
+        "metric": "WT energy+potential evals/s at 128^3",
+        "value": evals_s,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms1,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "WT (TF+vW+WT a=b=5/6) + LDA + Hartree + ion-electron energy and potential, "
+                               "fcc Al %dx%dx%d (%d atoms, sigma 0.1 bohr), %d^3 grid, FP64, device-resident rho"
+                               % (max(1, n // 32), max(1, n // 32), max(1, n // 32), len(case["frac"]), n),
+                   "grid": [n, n, n], "atoms": len(case["frac"]), "batch": 1},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "kernel": "one evaluation (5 fused passes + partial-sum finalize)",
+                     "algorithmic_bytes": bw},
+    }
+    if args.batch > 1:
+        msb, _ = time_batched(pa, hip, s, case, vion, args.batch, max(10, args.steps // args.batch), args.warmup)
+        out["batched"] = {"batch": args.batch, "evals_per_s": args.batch * 1000.0 / msb, "ms_per_batch": msb,
+                          "roofline_frac": bw * args.batch * 1000.0 / msb / 1e9 / HBM_PEAK_GBS}
+    if not args.no_serving:
+        try:
+            out["serving"] = serving_sample(pa)
+        except Exception as e:   # reported, never hidden
+            out["serving"] = {"error": str(e)}
+    if not args.no_cpu:
+        rate, nev, dt, et_cpu = cpu_baseline(case, vion)
+        out["cpu_baseline"] = {"value": rate, "unit": "evals/s", "cores": 1, "kind": "port",
+                               "sample": "%d full WT evaluations of the same %d^3 workload by the numpy oracle "
+                                         "(oracle/profess_oracle.py) in %.1f s" % (nev, n, dt)}
+        rel = max(abs(et[k] - et_cpu[k]) / max(abs(et_cpu[k]), 1e-300) for k in range(9) if abs(et_cpu[k]) > 1e-12)
+        out["cpu_baseline"]["max_term_rel_diff_vs_gpu"] = rel
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -15,13 +15,24 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libprofess_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["ofd_kernels.hip", "ofd_tables.hip", "ofd_plan.cpp", "ofd_api.cpp"]
+SOURCES = ["ofd_kernels.hip", "ofd_fast.hip", "ofd_tables.hip", "ofd_plan.cpp", "ofd_api.cpp"]
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wno-unused-result", "-Wno-unused-value"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def _compile(src):
+def _stale(src, out):
+    if not os.path.exists(out):
+        return True
+    t = os.path.getmtime(out)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    deps.append(os.path.join(os.path.dirname(HERE), "include", "ofd.h"))
+    return any(os.path.getmtime(p) > t for p in deps if os.path.exists(p))
+
+
+def _compile(src, force=True):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    if not force and not _stale(src, out):
+        return out
     cmd = [HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -29,10 +40,11 @@ def _compile(src):
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, force=True):
+    """Compile every source (force=True, the default) or only stale ones."""
     os.makedirs(OBJ, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=4) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    with cf.ThreadPoolExecutor(max_workers=5) as ex:
+        objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -44,6 +56,30 @@ def build(verbose=False):
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose=False):
+    """ISO_C_BINDING drop-in modules + the Fortran host test program, linked
+    against libprofess_amd.so (amdflang)."""
+    if not os.path.exists(FC):
+        return None
+    out = os.path.join(FORTRAN, "ofd_fortran_test")
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [FC, "-O2", "-module-dir", mod, "-I" + mod, os.path.join(FORTRAN, "ofd_profess.f90"),
+           os.path.join(FORTRAN, "ofd_fortran_test.f90"), "-o", out, "-L" + HERE, "-lprofess_amd",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("amdflang failed:\n" + r.stderr[-4000:])
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, force="--incremental" not in sys.argv)
+    build_fortran(verbose=True)
     sys.exit(0)

@@ -167,3 +167,17 @@ def IonElectronForcesSpline(system: OFDSystem, rho):
 
 def request_bytes(grid, kinetic=4, precision=0):
     return int(lib().ofd_request_bytes(int(grid[0]), int(grid[1]), int(grid[2]), int(kinetic), int(precision)))
+
+
+def CalculatePotentialPlusMany(systems, rho_ptrs, pot_ptrs, optSqrt=True, eTable=True):
+    """One batched evaluation of several systems of one grid shape/functional
+    (device-resident rho/potential addresses). Returns eTables (nb x 9) or None."""
+    nb = len(systems)
+    hs = (C.c_void_p * nb)(*[s._h.value for s in systems])
+    ra = (C.c_void_p * nb)(*rho_ptrs)
+    pa_ = (C.c_void_p * nb)(*pot_ptrs)
+    et = np.zeros((nb, 9)) if eTable else None
+    rc = lib().ofd_calpot_many(hs, nb, ra, pa_, 1 if optSqrt else 0, dptr(et) if eTable else None)
+    if rc != 0:
+        raise OFDError(lib().ofd_last_error(systems[0]._h).decode())
+    return et

@@ -213,6 +213,7 @@ struct ofd_ctx {
   void* work = nullptr;      // NF x nc complex (FP64 size)
   double* sums = nullptr;    // partials
   double* fin = nullptr;     // kNSum finalized sums
+  unsigned int* cnt = nullptr;  // epilogue arrival counter (self-resetting)
   double* wgck = nullptr;    // 4 x nc
   ReqDesc* desc = nullptr;   // one descriptor
   ReqDesc last_desc{};       // what `desc` holds after an enqueue
@@ -242,6 +243,8 @@ struct ofd_ctx {
     D.sums = sums;
     D.wgck = wgck;
     D.work = work;
+    D.fin = fin;
+    D.cnt = cnt;
     for (int i = 0; i < 9; ++i) D.rb[i] = rb[i];
     D.lam = lam; D.mu = mu; D.alpha = alpha; D.beta = beta; D.rhoS = rhoS; D.coef = coef; D.tkf = tkf;
     D.opt_sqrt = 1;
@@ -343,6 +346,8 @@ extern "C" int ofd_create(const ofd_grid* g, const ofd_functional* f, double num
     const long long nrows = (long long)g->n2 * g->n3;
     c->sums = dalloc<double>((size_t)nrows * kNSum);
     c->fin = dalloc<double>(kNSum);
+    c->cnt = dalloc<unsigned int>(1);
+    HIPCHK(hipMemset(c->cnt, 0, sizeof(unsigned int)));
     if (c->kind == kWGC8) c->wgck = dalloc<double>(4 * nc);
     c->desc = dalloc<ReqDesc>(1);
     c->dot_part = dalloc<double>(512 * 4);
@@ -370,7 +375,7 @@ extern "C" void ofd_destroy(ofd_ctx* c) {
   free_opt(c);
   hipFree(c->vion); hipFree(c->rho); hipFree(c->pot); hipFree(c->work); hipFree(c->sums); hipFree(c->fin);
   hipFree(c->wgck); hipFree(c->desc); hipFree(c->dot_part); hipFree(c->dot_out);
-  hipFree(c->many_desc); hipFree(c->many_sums);
+  hipFree(c->many_desc); hipFree(c->many_sums); hipFree(c->cnt);
   hipFree(c->d_frac); hipFree(c->d_psp_vqs); hipFree(c->d_psp_dd);
   if (c->sp.tw_x) shape_plan_free(&c->sp);
   if (c->st) hipStreamDestroy(c->st);
@@ -452,6 +457,7 @@ extern "C" int ofd_calpot_enqueue(ofd_ctx* c, const double* rho, int opt_sqrt, d
   D.u = rho;
   D.out = pot;
   D.opt_sqrt = opt_sqrt ? 1 : 0;
+  if (sums_dev) D.fin = sums_dev;
   // the descriptor is uploaded only when it changes (blocking copy, so the
   // host copy may go out of scope at once)
   if (std::memcmp(&D, &c->last_desc, sizeof(D)) != 0) {
@@ -503,8 +509,9 @@ extern "C" int ofd_calpot_many(ofd_ctx* const* cs, int nb, const double* const* 
       h[i].u = rho[i];
       h[i].out = pot[i];
       h[i].opt_sqrt = opt_sqrt ? 1 : 0;
+      h[i].fin = c0->many_sums + (size_t)i * kNSum;
     }
-    if (std::memcmp(h.data(), c0->many_last.data(), 0) != 0 || c0->many_last.size() != (size_t)nb ||
+    if (c0->many_last.size() != (size_t)nb ||
         std::memcmp(h.data(), c0->many_last.data(), sizeof(ReqDesc) * nb) != 0) {
       HIPCHK(hipStreamSynchronize(c0->st));
       HIPCHK(hipMemcpy(c0->many_desc, h.data(), sizeof(ReqDesc) * nb, hipMemcpyHostToDevice));
@@ -1275,6 +1282,7 @@ extern "C" int ofd_pool_step(ofd_pool* p) {
         ofd_ctx* c0 = p->slots[g[0]].c;
         for (size_t k = 0; k < g.size(); ++k) {
           p->h_desc[off + k] = desc_for(p->slots[g[k]].c, reqs[g[k]]);
+          p->h_desc[off + k].fin = p->d_sums + (size_t)(off + k) * kNSum;
           where[g[k]] = {off + (int)k, 0};
         }
         HIPCHK(hipMemcpyAsync(p->d_desc + off, &p->h_desc[off], sizeof(ReqDesc) * g.size(), hipMemcpyHostToDevice,

@@ -45,6 +45,8 @@ struct ReqDesc {
   double* out;          // potential / dEdphi / Ap
   double* sums;         // kNSum partials per epilogue workgroup
   const double* wgck;   // WGC kernels K1..K4 (4 x Nc doubles), kWGC8 only
+  double* fin;          // kNSum finalized sums of this evaluation (device)
+  unsigned int* cnt;    // arrival counter of the epilogue workgroups (zeroed, self-resetting)
   void* work;           // complex work buffer, NF fields x Nc
   double ca, cb;        // phi_t = ca*u + cb*v
   double s1, s2;        // kOutSigned direction weights
@@ -86,6 +88,7 @@ struct ShapePlan {
   int kb_z = 0;     // columns per workgroup in the z pass
   int lds_cap = 65536;     // device limit per workgroup
   int lds_target = 65536;  // soft budget used for the automatic tiles
+  int force_generic = 0;   // 1: never use the register-resident fast path
 };
 
 struct Tiles {
@@ -105,6 +108,12 @@ int run_eval(const ShapePlan& sp, int kind, int inmode, int outmode, int fp32, c
 // Sums the per-workgroup partials of every request: sums_out[req*kNSum + k].
 int run_finalize(const ShapePlan& sp, int kind, int fp32, const ReqDesc* d_descs, int nb, double* d_sums_out,
                  hipStream_t st);
+// Register-resident fast path (power-of-two WT grids; ofd_fast.hip)
+bool fast_path_ok(const ShapePlan& sp, int kind);
+int fast_eval(const ShapePlan& sp, int kind, int inmode, int outmode, int fp32, const ReqDesc* d_descs, int nb,
+              hipStream_t st);
+int fast_finalize(const ShapePlan& sp, int kind, int fp32, const ReqDesc* d_descs, int nb, double* out,
+                  hipStream_t st);
 // Plain 3D transforms of D.u -> D.work (forward, no 1/N) or D.work -> D.out.
 int run_fft3d_plain(const ShapePlan& sp, int forward, const ReqDesc* d_descs, int nb, hipStream_t st);
 

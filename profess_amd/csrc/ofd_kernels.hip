@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(256) k_xinv(const ReqDesc* __restrict__ descs,
 }
 
 // ------------------------------------------------------------ finalize: sum partials
-__global__ void __launch_bounds__(256) k_finalize(const ReqDesc* __restrict__ descs, int nwg, double* out) {
+__global__ void __launch_bounds__(256) k_finalize(const ReqDesc* __restrict__ descs, int nwg, double* /*unused*/) {
   const ReqDesc& D = descs[blockIdx.x];
   double acc[kNSum];
 #pragma unroll
@@ -468,7 +468,7 @@ __global__ void __launch_bounds__(256) k_finalize(const ReqDesc* __restrict__ de
 #pragma unroll
     for (int k = 0; k < kNSum; ++k) acc[k] += D.sums[(size_t)w * kNSum + k];
   }
-  block_sum_store<kNSum>(acc, out + (size_t)blockIdx.x * kNSum);
+  block_sum_store<kNSum>(acc, D.fin);
 }
 
 // ============================================================ host launchers
@@ -533,11 +533,13 @@ static int eval_r(const ShapePlan& sp, int kind, int inmode, int outmode, const 
 
 int run_eval(const ShapePlan& sp, int kind, int inmode, int outmode, int fp32, const ReqDesc* dd, int nb,
              hipStream_t st) {
+  if (!sp.force_generic && fast_path_ok(sp, kind)) return fast_eval(sp, kind, inmode, outmode, fp32, dd, nb, st);
   return fp32 ? eval_r<float>(sp, kind, inmode, outmode, dd, nb, st)
               : eval_r<double>(sp, kind, inmode, outmode, dd, nb, st);
 }
 
 int run_finalize(const ShapePlan& sp, int kind, int fp32, const ReqDesc* dd, int nb, double* out, hipStream_t st) {
+  if (!sp.force_generic && fast_path_ok(sp, kind)) return fast_finalize(sp, kind, fp32, dd, nb, out, st);
   const Tiles t = tiles_for(sp, func_nf(kind), fp32 ? (int)sizeof(float2) : (int)sizeof(double2));
   hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, st, dd, t.ntile, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;

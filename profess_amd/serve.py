@@ -1,0 +1,208 @@
+"""Continuous-batching server for density optimizations of uneven cells/grids.
+
+Requests (fcc Al / hcp Mg supercells, synthetic V_loc, uniform start density)
+arrive over time. A request is admitted as soon as (a) it has arrived, (b) the
+pool for its grid shape has a free slot and (c) its device footprint
+(``ofd_request_bytes``) fits in the HBM budget; its rho, ion potential, WGC
+kernels and optimizer state then live on the device until it converges. Every
+server step advances every active request of every pool by one evaluation:
+one batched launch set per (grid shape, optimizer phase, precision), so
+requests that arrive while others run share those launches. Latency is
+arrival -> converged result.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import inputs
+from ._lib import OFDError, dptr, lib, ofd_opt_stats
+from .calpot import OFDSystem, opt_params
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float
+    species: str          # "Al" (fcc) or "Mg" (hcp)
+    reps: tuple
+    grid: tuple
+    sigma: float
+    seed: int
+    # filled by the server
+    admitted: float = -1.0
+    finished: float = -1.0
+    energy: np.ndarray = field(default=None, repr=False)
+    outcome: int = -1
+    stats: dict = field(default=None)
+    rho: np.ndarray = field(default=None, repr=False)
+
+
+def _reps_for(species, n, max_atoms):
+    if species == "Al":
+        r = max(1, int(round(n * 0.24 / inputs.AL_A)))
+        while 4 * r ** 3 > max_atoms and r > 1:
+            r -= 1
+        return (r, r, r)
+    r = max(1, int(round(n * 0.24 / inputs.MG_A)))
+    rc = max(1, int(round(n * 0.24 / inputs.MG_C)))
+    while 2 * r * r * rc > max_atoms and r > 1:
+        r -= 1
+    return (r, r, rc)
+
+
+def make_requests(n, seed=0, grids=(32, 48, 64, 80, 96, 120, 128), max_atoms=500, rate=50.0):
+    """Poisson arrivals at `rate` per second; species, grid and displacement
+    amplitude (0.05-0.15 bohr) drawn from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        sp = "Al" if rng.random() < 0.5 else "Mg"
+        g = int(rng.choice(grids))
+        out.append(Request(i, t, sp, _reps_for(sp, g, max_atoms), (g, g, g), float(rng.uniform(0.05, 0.15)),
+                           int(rng.integers(1 << 30))))
+    return out
+
+
+def build_system(req: Request, precision=1):
+    if req.species == "Al":
+        lat, frac = inputs.fcc_supercell(inputs.AL_A, req.reps)
+        Z, rc = inputs.AL_Z, inputs.AL_RC
+    else:
+        lat, frac = inputs.hcp_supercell(inputs.MG_A, inputs.MG_C, req.reps)
+        Z, rc = inputs.MG_Z, inputs.MG_RC
+    frac = inputs.displace(lat, frac, req.sigma, req.seed)
+    ne = Z * frac.shape[0]
+    s = OFDSystem(lat, req.grid, ne, kinetic=4, precision=precision)
+    mg, vals = inputs.synthetic_recpot(Z, rc)
+    s.build_ion_potential([(mg, vals, frac)])
+    rho = np.full(s.shape, ne / s.vol)
+    return s, rho
+
+
+class Pool:
+    def __init__(self, grid, capacity, precision):
+        L = lib()
+        h = C.c_void_p()
+        if L.ofd_pool_create(grid[0], grid[1], grid[2], capacity, 4, precision, C.byref(h)) != 0:
+            raise OFDError("pool create failed")
+        self.h = h
+        self.grid = grid
+        self.capacity = capacity
+        self.slots = {}   # slot -> Request
+
+    def close(self):
+        if self.h is not None:
+            lib().ofd_pool_destroy(self.h)
+            self.h = None
+
+    def admit(self, req, system, rho, params):
+        L = lib()
+        r = np.ascontiguousarray(rho)
+        ctx = system._h
+        slot = L.ofd_pool_admit(self.h, ctx, dptr(r), C.byref(params))
+        if slot < 0:
+            return -1
+        system.detach()
+        self.slots[slot] = req
+        return slot
+
+    def step(self):
+        n = lib().ofd_pool_step(self.h)
+        if n < 0:
+            raise OFDError(lib().ofd_pool_last_error(self.h).decode())
+        return n
+
+    def collect(self, now):
+        L = lib()
+        done = []
+        for slot, req in list(self.slots.items()):
+            n = self.grid[0] * self.grid[1] * self.grid[2]
+            rho = np.empty(n)
+            et = np.zeros(9)
+            oc = C.c_int(-1)
+            st = ofd_opt_stats()
+            got = L.ofd_pool_collect(self.h, slot, dptr(rho), dptr(et), C.byref(oc), C.byref(st), None)
+            if got == 1:
+                req.finished = now()
+                req.energy = et
+                req.outcome = oc.value
+                req.stats = {k: getattr(st, k) for k, _ in st._fields_}
+                req.rho = rho.reshape(self.grid[2], self.grid[1], self.grid[0])
+                del self.slots[slot]
+                done.append(req)
+        return done
+
+
+def capacity_for(grid):
+    n = grid[0] * grid[1] * grid[2]
+    return int(max(2, min(32, (64 ** 3 * 16) // n)))
+
+
+def run_workload(requests, mixed=True, hbm_budget=64e9, pot_tol=1e-5, mixed_switch=1e-3, verbose=False,
+                 keep_rho=False):
+    """Serve `requests` (arrival times in seconds from the start) to completion.
+    Returns throughput and latency statistics."""
+    precision = 1 if mixed else 0
+    params = opt_params(pot_tol=pot_tol, conv_check="POT", mixed_switch=mixed_switch if mixed else 0.0)
+    pools = {}
+    pending = sorted(requests, key=lambda r: r.arrival)
+    in_use = 0.0
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0   # noqa: E731
+    finished = []
+    qi = 0
+    waiting = []
+    while len(finished) < len(requests):
+        t = now()
+        while qi < len(pending) and pending[qi].arrival <= t:
+            waiting.append(pending[qi])
+            qi += 1
+        # admission: FIFO among arrived requests that fit
+        still = []
+        for req in waiting:
+            g = req.grid
+            pool = pools.get(g)
+            if pool is None:
+                pool = pools[g] = Pool(g, capacity_for(g), precision)
+            need = float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
+            if len(pool.slots) < pool.capacity and in_use + need <= hbm_budget:
+                s, rho = build_system(req, precision)
+                if pool.admit(req, s, rho, params) >= 0:
+                    req.admitted = now()
+                    in_use += need
+                    continue
+                s.close()
+            still.append(req)
+        waiting = still
+        active = 0
+        for pool in pools.values():
+            if pool.slots:
+                active += pool.step()
+        for g, pool in pools.items():
+            for req in pool.collect(now):
+                in_use -= float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
+                if not keep_rho:
+                    req.rho = None
+                finished.append(req)
+                if verbose:
+                    print("req %d grid %s done in %.3f s (E = %.8f Ha, outcome %d)"
+                          % (req.rid, g, req.finished - req.arrival, req.energy[0], req.outcome), flush=True)
+        if active == 0 and not waiting and qi < len(pending):
+            time.sleep(max(0.0, min(0.01, pending[qi].arrival - now())))
+    makespan = now()
+    for p in pools.values():
+        p.close()
+    lat = np.array([r.finished - r.arrival for r in finished])
+    ok = sum(1 for r in finished if r.outcome == 0)
+    return {"requests": len(requests), "converged": ok, "makespan_s": makespan,
+            "solves_per_s": len(requests) / makespan,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mixed_precision": bool(mixed),
+            "grids": sorted({r.grid[0] for r in requests})}

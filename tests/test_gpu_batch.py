@@ -1,0 +1,73 @@
+"""GPU: batched evaluation and serving. A result must be the same alone and in
+a batch (bitwise for single evaluations; converged energies identical for the
+pool), and FP32-then-FP64 solves must land within 1e-5 eV/atom of FP64."""
+import numpy as np
+import pytest
+
+from oracle import profess_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import profess_amd
+    profess_amd.lib()
+    return profess_amd
+
+
+def _system(pa, grid, seed, precision=0, kinetic=4):
+    from profess_amd import inputs
+    rng = np.random.default_rng(seed)
+    lat = np.array([[7.3, 0.2, 0.1], [0.3, 7.5, 0.0], [-0.2, 0.4, 7.6]]).T * (1.0 + 0.05 * rng.random())
+    frac = rng.random((4, 3))
+    s = pa.OFDSystem(lat, grid, 12.0, kinetic=kinetic, precision=precision)
+    mg, v = inputs.synthetic_recpot(3, 1.2)
+    s.build_ion_potential([(mg, v, frac)])
+    rho = 12.0 / s.vol * (1.0 + 0.3 * (rng.random(s.shape) - 0.5))
+    return s, rho
+
+
+@pytest.mark.parametrize("grid", [(64, 32, 64), (24, 20, 18)])
+def test_batched_evaluation_equals_alone(pa, grid):
+    from profess_amd.devmem import DeviceArray
+    systems, rhos = zip(*[_system(pa, grid, seed) for seed in range(3)])
+    alone = [pa.CalculatePotentialPlus(s, r) for s, r in zip(systems, rhos)]
+    dr = [DeviceArray.from_host(r) for r in rhos]
+    dp = [DeviceArray(r.shape) for r in rhos]
+    for _ in range(2):   # twice: the reduction counters must reset themselves
+        et = pa.calpot.CalculatePotentialPlusMany(list(systems), [d.ptr for d in dr], [d.ptr for d in dp])
+        for i in range(3):
+            assert np.array_equal(et[i], alone[i][1])
+            assert np.array_equal(dp[i].download(), alone[i][0])
+
+
+def test_pool_results_same_alone_and_mixed(pa):
+    from profess_amd import serve
+    reqs = serve.make_requests(6, seed=3, grids=(32, 48), max_atoms=8, rate=1e6)
+    mixed = serve.run_workload(reqs, mixed=False, keep_rho=True)
+    assert mixed["converged"] == len(reqs)
+    e_mixed = {r.rid: r.energy.copy() for r in reqs}
+    for r in reqs[:3]:
+        one = serve.make_requests(6, seed=3, grids=(32, 48), max_atoms=8, rate=1e6)[r.rid]
+        serve.run_workload([one], mixed=False)
+        assert np.array_equal(one.energy, e_mixed[r.rid]), (one.energy, e_mixed[r.rid])
+
+
+def test_mixed_precision_solve_within_1e5_ev_per_atom(pa):
+    from profess_amd import inputs
+    c = inputs.al_fcc_case((2, 2, 2), (64, 64, 64), sigma=0.1, seed_ion=5)
+    out = {}
+    for prec in (0, 1):
+        s = pa.OFDSystem(c["lattice"], c["grid"], c["num_electrons"], kinetic=4, precision=prec)
+        mg, v = c["recpot"]
+        s.build_ion_potential([(mg, v, c["frac"])])
+        rho0 = np.full(s.shape, c["num_electrons"] / s.vol)
+        rho, et, oc, st = pa.ProjNormRhoMinimization(s, rho0, conv_check="POT", pot_tol=1e-5,
+                                                       mixed_switch=1e-3 if prec else 0.0)
+        assert oc == 0
+        out[prec] = (et, st)
+    natom = len(c["frac"])
+    de = abs(out[1][0][0] - out[0][0][0]) * O.HARTREE_TO_EV / natom
+    assert de < 1e-5, de
+    assert out[1][1]["fp32_evals"] > 0

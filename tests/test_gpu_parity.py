@@ -60,7 +60,10 @@ def _case(grid, lattice, kinetic=4, alpha=-100.0, beta=-100.0, seed=0, nion=3):
     return cell, frac, vion, ne, f, rho
 
 
-GRIDS = [(16, 16, 16), (24, 20, 18), (14, 21, 10), (32, 30, 28), (6, 5, 4)]
+# generic LDS path (mixed radices, odd lengths) and register-resident fast path
+# (n1/2, n2, n3 in {32, 64, 128, 256})
+GRIDS = [(16, 16, 16), (24, 20, 18), (14, 21, 10), (32, 30, 28), (6, 5, 4),
+         (64, 32, 64), (128, 64, 32), (64, 128, 256), (512, 32, 32)]
 
 
 @pytest.mark.parametrize("grid", GRIDS)
@@ -80,8 +83,8 @@ def test_calpot_wt_matches_oracle(pa, grid):
     assert et2 is None and np.max(np.abs(pot2 - pot_ref)) <= POT_RTOL * np.max(np.abs(pot_ref))
 
 
-def test_calpot_wt_alpha_ne_beta(pa):
-    grid = (16, 18, 20)
+@pytest.mark.parametrize("grid", [(16, 18, 20), (64, 64, 32)])
+def test_calpot_wt_alpha_ne_beta(pa, grid):
     cell, frac, vion, ne, f, rho = _case(grid, _tri_lattice(), alpha=0.9, beta=0.7)
     sysm = O.System(cell, f, vion)
     pr, er = O.calculate_potential_plus(rho, sysm, True, True)
