@@ -11,6 +11,7 @@
 // RhoLineSearch.f90).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -544,7 +545,12 @@ extern "C" int ofd_calpot(ofd_ctx* c, const double* rho, int opt_sqrt, double* p
 }
 
 // ====================================================================== NTN solver
-// ProjNormRhoMinimization with dir_type = 1 (NTN), numSpin = 1.
+// ProjNormRhoMinimization with dir_type = 1 (NTN), numSpin = 1, as a state
+// machine. Each state either queues vector operations (run later, in order,
+// by the driver) or stops on a "need": one evaluation of the potential, or a
+// few dot products. The driver (ofd_rho_opt for one request, ofd_pool_step
+// for many) launches what the solvers need, batched across requests, and
+// hands the results back through deliver().
 namespace {
 
 struct EvalReq {
@@ -555,156 +561,183 @@ struct EvalReq {
   bool energy;
 };
 
-enum SState { S_INIT, S_AFTER_INIT, S_NEWTON_ITER, S_AFTER_NEWTON, S_LINE_EVAL_DONE, S_DONE };
+enum NeedKind { NEED_NONE = 0, NEED_EVAL = 1, NEED_DOTS = 2 };
+struct Need {
+  NeedKind kind = NEED_NONE;
+  EvalReq ev{};
+  int np = 0;
+  const double* a[4] = {nullptr, nullptr, nullptr, nullptr};
+  const double* b[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+enum SState {
+  ST_INIT_EVAL, ST_INIT_DOTS, ST_NEWTON_PP, ST_NEWTON_EVAL, ST_NEWTON_RR, ST_LP1, ST_LP2, ST_LP3, ST_LINE_EVAL,
+  ST_ACC_DOTS, ST_DONE
+};
 
 struct NtnSolver {
   ofd_ctx* c = nullptr;
-  ofd_opt_params prm;
-  // scalars
+  ofd_opt_params prm{};
+  std::vector<VecOp> ops;   // queued vector operations (device pointers)
+  Need need;
+  SState st = ST_INIT_EVAL;
+  std::string err;
+  // scalars (RhoOptN.f90:120-170)
   double tot_ele = 0, mu0 = 0, theta = 0.2, energy[9] = {0};
   double e_hist[3] = {0, 0, 0};
   int it = 1, outcome = -1, success = 1;
-  // Newton
+  // NewtonDirection (RhoDirNew.f90:54-75)
   int nit = 0, ntflag = 0;
   double rr = 0, rr_last = 0, criteria = 0, ry = 0, ry_last = 0, ss1 = 0, eps = 0;
-  // line search
+  // getNextGradient / line search
+  bool switched = false;
+  double norm_phi_dir = 0;
   LineSearch ls;
   int nline = 0;
   double f_cur = 0, g_cur = 0;
-  double last_sums[kNSum];
+  double last_sums[kNSum] = {0};
   double pot_norm = 0;
   bool fp32 = false;
-  SState state = S_INIT;
+  bool force64 = false;   // set when an FP32 phase stalls: the rest of the run is FP64
+  double mach = 2.220446049250313e-16;
   ofd_opt_stats stats{};
   std::vector<double> hist_energy, hist_potnorm;
-  double mach = 2.220446049250313e-16;
 
-  void dots(int np, const double* a0, const double* b0, const double* a1, const double* b1, const double* a2,
-            const double* b2, double* out) {
-    const double* A[4] = {a0, a1, a2, a0};
-    const double* B[4] = {b0, b1, b2, b0};
-    vec_dots(A, B, np, c->nr(), c->dot_part, c->dot_out, c->st);
-    HIPCHK(hipMemcpyAsync(out, c->dot_out, sizeof(double) * np, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+  // ---- queued vector operations
+  void lincomb(double* z, const double* x, const double* y, double a, double b) {   // z = fma(b, y, a x)
+    ops.push_back(VecOp{kOpLincomb, z, x, y, a, b});
+  }
+  void axpby(double* y, const double* x, double a, double b) {   // y = fma(a, x, b y)
+    ops.push_back(VecOp{kOpAxpby, y, x, y, a, b});
+  }
+  void zero(double* z) { ops.push_back(VecOp{kOpZero, z, nullptr, nullptr, 0.0, 0.0}); }
+
+  void need_dots(int np, const double* a0, const double* b0, const double* a1 = nullptr, const double* b1 = nullptr) {
+    need = Need();
+    need.kind = NEED_DOTS;
+    need.np = np;
+    need.a[0] = a0; need.b[0] = b0;
+    need.a[1] = a1; need.b[1] = b1;
+  }
+  void need_eval(const EvalReq& q) {
+    need = Need();
+    need.kind = NEED_EVAL;
+    need.ev = q;
+  }
+  void finish() {
+    need = Need();
+    st = ST_DONE;
+  }
+  // An FP32 phase that stalls (checkGradient gives up, or the line search
+  // fails) does not end the run: re-evaluate at the current phi in FP64 and
+  // continue from there as the reference would from this density.
+  void fall_back_fp64() {
+    force64 = true;
+    fp32 = false;
+    mach = 2.220446049250313e-16;
+    e_hist[0] = e_hist[1] = e_hist[2] = 0.0;
+    success = 1;
+    outcome = -1;
+    st = ST_INIT_EVAL;
+    need_eval(EvalReq{kOutSigned, c->phi, nullptr, nullptr, c->dedphi, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, true});
+  }
+  void fail_with(const char* m) {
+    err = m;
+    outcome = 1;
+    finish();
   }
 
   void set_precision_for(double pnorm) {
-    fp32 = (c->precision == 1) && !(prm.mixed_switch > 0.0 && pnorm < prm.mixed_switch);
+    fp32 = (c->precision == 1) && !force64 && !(prm.mixed_switch > 0.0 && pnorm < prm.mixed_switch);
     mach = fp32 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
   }
 
-  bool need_eval(EvalReq* q) {
-    switch (state) {
-      case S_INIT:
-        *q = EvalReq{kOutSigned, c->phi, nullptr, nullptr, c->dedphi, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, true};
-        return true;
-      case S_NEWTON_ITER: {
-        *q = EvalReq{kOutNewton, c->phi, c->p, c->dldphi, c->ap, 1.0, eps, 0.0, 0.0, mu0, eps, false};
-        return true;
-      }
-      case S_LINE_EVAL_DONE: {
-        const double ct = std::cos(theta), st = std::sin(theta);
-        *q = EvalReq{kOutSigned, c->phi, c->dir, nullptr, c->dedphi, ct, st, -st, ct, mu0, 0.0, true};
-        return true;
-      }
-      default:
-        return false;
-    }
+  // the first evaluation at phi = sqrt|rhoR| (RhoOptN.f90:352-357)
+  void start() {
+    st = ST_INIT_EVAL;
+    need_eval(EvalReq{kOutSigned, c->phi, nullptr, nullptr, c->dedphi, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, true});
   }
 
-  double dV() const { return c->dV; }
-
-  // begin a Newton direction (RhoDirNew.f90:95-115)
-  void newton_begin() {
-    const long long n = c->nr();
-    vec_lincomb(c->r, c->dldphi, c->dldphi, -1.0, 0.0, n, c->st);   // r = -b
-    HIPCHK(hipMemsetAsync(c->dir, 0, n * sizeof(double), c->st));
-    double d2[2];
-    dots(2, c->r, c->r, c->phi, c->phi, nullptr, nullptr, d2);
-    rr = d2[0];
-    ss1 = d2[1];
+  // NewtonDirection start (RhoDirNew.f90:95-115); rr0 = sum r^2 = sum b^2, s1 = sum phi^2
+  void newton_begin(double rr0, double s1) {
+    lincomb(c->r, c->dldphi, c->dldphi, -1.0, 0.0);   // r = -b
+    zero(c->dir);
+    rr = rr0;
+    ss1 = s1;
     criteria = rr;
     rr_last = rr * 2.0;
     nit = 1;
     ry = 0.0;
     ntflag = -1;
-    newton_prep_p();
+    newton_prep();
   }
-
-  // y = r; p = y (+ ry/ryLast p); eps (RhoDirNew.f90:116-147)
-  void newton_prep_p() {
-    const long long n = c->nr();
+  // y = r; p = y (+ ry/ryLast p) (RhoDirNew.f90:116-130)
+  void newton_prep() {
     ry_last = ry;
-    ry = rr;   // sum(r*y) with y = r
-    if (nit == 1) vec_lincomb(c->p, c->r, c->r, 1.0, 0.0, n, c->st);
-    else vec_axpby(c->p, c->r, 1.0, ry / ry_last, n, c->st);   // p = r + (ry/ryLast) p
-    double s2;
-    dots(1, c->p, c->p, nullptr, nullptr, nullptr, nullptr, &s2);
-    eps = 2.0 * std::sqrt(mach) * (1.0 + std::sqrt(ss1)) / std::sqrt(s2);
-    state = S_NEWTON_ITER;
-  }
-
-  // getNextGradient + checkGradient loop, then start the line search
-  // (RhoOptN.f90:577-591, RhoLineSearch.f90:40-309)
-  bool line_prep() {
-    const long long n = c->nr();
-    bool switched = false;
-    for (;;) {
-      vec_lincomb(c->tmp, c->phi, c->dir, 1.0, 1.0, n, c->st);
-      double d3[3];
-      dots(2, c->tmp, c->tmp, c->phi, c->dir, nullptr, nullptr, d3);
-      const double norm_phi_dir = std::sqrt(d3[0] * c->dV);
-      const double proj = d3[1] * c->dV;
-      vec_axpby(c->dir, c->phi, -(proj / tot_ele), 1.0, n, c->st);   // dir = dir - phi*<phi,dir>/Ne
-      double dd;
-      dots(1, c->dir, c->dir, nullptr, nullptr, nullptr, nullptr, &dd);
-      const double norm_perp = std::sqrt(dd * c->dV);
-      theta = std::fmin(theta, norm_perp / norm_phi_dir);
-      vec_axpby(c->dir, c->dir, 0.0, std::sqrt(tot_ele) / norm_perp, n, c->st);
-      double gd;
-      dots(1, c->dedphi, c->dir, nullptr, nullptr, nullptr, nullptr, &gd);
-      const double gradth = c->dV * gd;
-      if (gradth > 1.0e5) throw std::runtime_error("The gradient is too large");
-      if (gradth >= 0.0) {
-        if (switched) {
-          outcome = 1;
-          return false;
-        }
-        switched = true;
-        vec_lincomb(c->dir, c->dldphi, c->dldphi, -1.0, 0.0, n, c->st);
-        continue;
-      }
-      // LineSearchTN start (RhoLineSearch.f90:369-374)
-      const float ftol32 = 1e-4f, gtol32 = 2e-1f;   // default-real literals (RhoOptN.f90:255-258)
-      ls = LineSearch();
-      ls.ftol = (double)ftol32;
-      ls.gtol = (double)gtol32;
-      ls.xtol = 1e-12;
-      ls.stpmin = 0.0;
-      ls.stpmax = kPi;
-      nline = 0;
-      f_cur = energy[0];
-      g_cur = gradth;
-      return true;
+    ry = rr;
+    if (nit == 1) {
+      lincomb(c->p, c->r, c->r, 1.0, 0.0);   // p = r exactly: sum p^2 = rr
+      newton_eval(rr);
+    } else {
+      axpby(c->p, c->r, 1.0, ry / ry_last);
+      st = ST_NEWTON_PP;
+      need_dots(1, c->p, c->p);
     }
   }
-
-  // drives dcsrch until it asks for an evaluation or finishes
-  bool line_advance() {
-    int task = ls.next(theta, f_cur, g_cur);
+  // epsilon (RhoDirNew.f90:143-147) and the finite-difference product
+  void newton_eval(double pp) {
+    eps = 2.0 * std::sqrt(mach) * (1.0 + std::sqrt(ss1)) / std::sqrt(pp);
+    st = ST_NEWTON_EVAL;
+    need_eval(EvalReq{kOutNewton, c->phi, c->p, c->dldphi, c->ap, 1.0, eps, 0.0, 0.0, mu0, eps, false});
+  }
+  void newton_done() {
+    stats.newton_iters_total += nit;
+    switched = false;
+    line_prep1();
+  }
+  // getNextGradient (RhoLineSearch.f90:40-139) in three dot rounds
+  void line_prep1() {
+    lincomb(c->tmp, c->phi, c->dir, 1.0, 1.0);
+    st = ST_LP1;
+    need_dots(2, c->tmp, c->tmp, c->phi, c->dir);
+  }
+  void line_start(double gradth) {
+    // LineSearchTN start (RhoLineSearch.f90:369-374); ftol/gtol are
+    // default-real literals (RhoOptN.f90:255-258)
+    const float ftol32 = 1e-4f, gtol32 = 2e-1f;
+    ls = LineSearch();
+    ls.ftol = (double)ftol32;
+    ls.gtol = (double)gtol32;
+    ls.xtol = 1e-12;
+    ls.stpmin = 0.0;
+    ls.stpmax = kPi;
+    nline = 0;
+    f_cur = energy[0];
+    g_cur = gradth;
+    line_advance();
+  }
+  void line_advance() {
+    const int task = ls.next(theta, f_cur, g_cur);
     if (task == 1) {
-      state = S_LINE_EVAL_DONE;
-      return true;
+      const double ct = std::cos(theta), sn = std::sin(theta);
+      st = ST_LINE_EVAL;
+      need_eval(EvalReq{kOutSigned, c->phi, c->dir, nullptr, c->dedphi, ct, sn, -sn, ct, mu0, 0.0, true});
+      return;
     }
     success = task == 2 ? 1 : (task == 3 ? -2 : -1);
     accept();
-    return false;
   }
-
   // StepInformation + CheckExit + update (RhoOptN.f90:595-627, :692-977)
   void accept() {
+    if (success == -1 && fp32) {
+      fall_back_fp64();
+      return;
+    }
+    if (success == -1 && it > prm.max_iter) {
+      fail_with("Fail rho optimizations");
+      return;
+    }
     const long long n = c->nr();
-    if (success == -1 && it > prm.max_iter) throw std::runtime_error("Fail rho optimizations");
     pot_norm = std::sqrt(last_sums[7] / (double)n);
     hist_energy.push_back(energy[0]);
     hist_potnorm.push_back(pot_norm);
@@ -720,97 +753,146 @@ struct NtnSolver {
     if (prm.conv_check == 3 && (pconv || econv)) { outcome = 0; exit_flag = true; }
     if (outcome == 1) exit_flag = true;
     if (it >= prm.max_iter) { outcome = 1; exit_flag = true; }
-    // mixed precision: an FP32 phase never ends the run
+    // an FP32 phase never ends the run
     if (exit_flag && fp32 && outcome == 0) {
       exit_flag = false;
       outcome = -1;
     }
-    // phi = tempPhi; mu; dLdphi
-    const double ct = std::cos(theta), st = std::sin(theta);
-    vec_lincomb(c->tmp, c->phi, c->dir, ct, st, n, c->st);
+    // phi = tempPhi (same fma as inside the evaluation), mu, dL/dphi
+    const double ct = std::cos(theta), sn = std::sin(theta);
+    lincomb(c->tmp, c->phi, c->dir, ct, sn);
     std::swap(c->phi, c->tmp);
     mu0 = c->dV * last_sums[8] / (2.0 * tot_ele);
-    vec_lincomb(c->dldphi, c->dedphi, c->phi, 1.0, -2.0 * mu0, n, c->st);
+    lincomb(c->dldphi, c->dedphi, c->phi, 1.0, -2.0 * mu0);
     stats.iterations = it;
     if (exit_flag) {
-      state = S_DONE;
+      finish();
       return;
     }
     const bool was32 = fp32;
     set_precision_for(pot_norm);
-    if (was32 && !fp32) {
-      // entering FP64 refinement: FP32 energies never count toward ENE
-      e_hist[0] = e_hist[1] = e_hist[2] = 0.0;
-    }
+    if (was32 && !fp32) e_hist[0] = e_hist[1] = e_hist[2] = 0.0;   // FP32 energies never count toward ENE
     ++it;
-    newton_begin();
+    st = ST_ACC_DOTS;
+    need_dots(2, c->dldphi, c->dldphi, c->phi, c->phi);
   }
 
-  // consume the sums of the evaluation requested by need_eval
-  void resume(const double* s) {
-    std::memcpy(last_sums, s, sizeof(last_sums));
-    ++stats.num_evals;
-    if (fp32) ++stats.fp32_evals;
+  // results of the current need: kNSum sums (evaluation) or np dot products
+  void deliver(const double* v) {
     const long long n = c->nr();
-    switch (state) {
-      case S_INIT: {
-        etable_from_sums(c, s, energy);
-        mu0 = c->dV * s[8] / (2.0 * tot_ele);   // ChemicalPotential
-        vec_lincomb(c->dldphi, c->dedphi, c->phi, 1.0, -2.0 * mu0, n, c->st);
-        double pn;
-        dots(1, c->dldphi, c->dldphi, nullptr, nullptr, nullptr, nullptr, &pn);
-        pot_norm = std::sqrt(pn / (double)n);
+    switch (st) {
+      case ST_INIT_EVAL: {
+        std::memcpy(last_sums, v, sizeof(last_sums));
+        ++stats.num_evals;
+        if (fp32) ++stats.fp32_evals;
+        etable_from_sums(c, v, energy);
+        mu0 = c->dV * v[8] / (2.0 * tot_ele);   // ChemicalPotential (RhoDirBFGS.f90:337)
+        lincomb(c->dldphi, c->dedphi, c->phi, 1.0, -2.0 * mu0);
+        st = ST_INIT_DOTS;
+        need_dots(2, c->dldphi, c->dldphi, c->phi, c->phi);
+        return;
+      }
+      case ST_INIT_DOTS: {
+        pot_norm = std::sqrt(v[0] / (double)n);
         hist_energy.push_back(energy[0]);
         hist_potnorm.push_back(pot_norm);
         e_hist[0] = energy[0];
         set_precision_for(pot_norm);
-        it = 1;
-        newton_begin();
+        newton_begin(v[0], v[1]);
         return;
       }
-      case S_NEWTON_ITER: {
-        const double pap = s[6];
-        bool done = false;
+      case ST_NEWTON_PP:
+        newton_eval(v[0]);
+        return;
+      case ST_NEWTON_EVAL: {
+        std::memcpy(last_sums, v, sizeof(last_sums));
+        ++stats.num_evals;
+        if (fp32) ++stats.fp32_evals;
+        const double pap = v[6];
         if (pap < 0.0) {
-          if (nit == 1) vec_axpby(c->dir, c->p, ry / pap, -1.0, n, c->st);   // dir = -(dir - ry/pAp p)
+          if (nit == 1) axpby(c->dir, c->p, ry / pap, -1.0);   // dir = -(dir - ry/pAp p)
           ntflag = -2;
-          done = true;
+          newton_done();
+          return;
+        }
+        const double alpha = ry / pap;
+        axpby(c->dir, c->p, alpha, 1.0);
+        axpby(c->r, c->ap, -alpha, 1.0);
+        rr_last = rr;
+        st = ST_NEWTON_RR;
+        need_dots(1, c->r, c->r);
+        return;
+      }
+      case ST_NEWTON_RR: {
+        rr = v[0];
+        bool done = false;
+        if (rr < 0.1 * criteria) { ntflag = 0; done = true; }
+        else if (nit > 50) { ntflag = 1; done = true; }
+        else if (std::fabs(rr - rr_last) / rr < 0.01 && nit > 9) { ntflag = 2; done = true; }
+        if (done) {
+          newton_done();
         } else {
-          const double alpha = ry / pap;
-          vec_axpby(c->dir, c->p, alpha, 1.0, n, c->st);
-          vec_axpby(c->r, c->ap, -alpha, 1.0, n, c->st);
-          rr_last = rr;
-          dots(1, c->r, c->r, nullptr, nullptr, nullptr, nullptr, &rr);
-          if (rr < 0.1 * criteria) { ntflag = 0; done = true; }
-          else if (nit > 50) { ntflag = 1; done = true; }
-          else if (std::fabs(rr - rr_last) / rr < 0.01 && nit > 9) { ntflag = 2; done = true; }
-        }
-        if (!done) {
           ++nit;
-          newton_prep_p();
-          return;
+          newton_prep();
         }
-        stats.newton_iters_total += nit;
-        if (!line_prep()) {
-          // checkGradient gave up (rhoOutcome = 1)
-          state = S_DONE;
-          return;
-        }
-        line_advance();
         return;
       }
-      case S_LINE_EVAL_DONE: {
-        etable_from_sums(c, s, energy);
+      case ST_LP1: {
+        norm_phi_dir = std::sqrt(v[0] * c->dV);
+        const double proj = v[1] * c->dV;
+        axpby(c->dir, c->phi, -(proj / tot_ele), 1.0);   // dir = dir - phi <phi,dir>/Ne
+        st = ST_LP2;
+        need_dots(1, c->dir, c->dir);
+        return;
+      }
+      case ST_LP2: {
+        const double norm_perp = std::sqrt(v[0] * c->dV);
+        theta = std::fmin(theta, norm_perp / norm_phi_dir);
+        axpby(c->dir, c->dir, 0.0, std::sqrt(tot_ele) / norm_perp);
+        st = ST_LP3;
+        need_dots(1, c->dedphi, c->dir);
+        return;
+      }
+      case ST_LP3: {
+        const double gradth = c->dV * v[0];   // gradOftheta at theta = 0
+        // checkGradient (RhoLineSearch.f90:201-309)
+        if (gradth > 1.0e5) {
+          fail_with("The gradient is too large");
+          return;
+        }
+        if (gradth >= 0.0) {
+          if (switched && fp32) {
+            fall_back_fp64();
+            return;
+          }
+          if (switched) {
+            outcome = 1;
+            finish();
+            return;
+          }
+          switched = true;
+          lincomb(c->dir, c->dldphi, c->dldphi, -1.0, 0.0);   // steepest descent
+          line_prep1();
+          return;
+        }
+        line_start(gradth);
+        return;
+      }
+      case ST_LINE_EVAL: {
+        std::memcpy(last_sums, v, sizeof(last_sums));
+        ++stats.num_evals;
+        if (fp32) ++stats.fp32_evals;
+        etable_from_sums(c, v, energy);
         f_cur = energy[0];
-        g_cur = c->dV * s[6];
+        g_cur = c->dV * v[6];
         ++nline;
-        if (nline > prm.max_line) {
-          accept();
-          return;
-        }
-        line_advance();
+        if (nline > prm.max_line) accept();
+        else line_advance();
         return;
       }
+      case ST_ACC_DOTS:
+        newton_begin(v[0], v[1]);
+        return;
       default:
         return;
     }
@@ -849,18 +931,102 @@ ReqDesc desc_for(ofd_ctx* c, const EvalReq& q) {
   return D;
 }
 
-}  // namespace
+// Device scratch for batched vector ops and dots, grown on demand.
+struct BatchScratch {
+  VecOp* d_ops = nullptr;
+  int ops_cap = 0;
+  DotDesc* d_dots = nullptr;
+  double* d_part = nullptr;
+  double* d_res = nullptr;
+  int dots_cap = 0;
+  std::vector<double> h_res;
+  void free_all() {
+    hipFree(d_ops); hipFree(d_dots); hipFree(d_part); hipFree(d_res);
+    *this = BatchScratch();
+  }
+};
 
-// sum of a device vector (deterministic): dot with a ones vector
-static double dsum(ofd_ctx* c, const double* x, double* ones) {
-  const double* A[4] = {x, x, x, x};
-  const double* B[4] = {ones, ones, ones, ones};
-  vec_dots(A, B, 1, c->nr(), c->dot_part, c->dot_out, c->st);
+// Runs the queued ops of all solvers, layer by layer (each solver's ops stay in order).
+void flush_ops(std::vector<NtnSolver*>& ss, long long n, BatchScratch& bs, hipStream_t st) {
+  size_t layers = 0;
+  for (auto* s : ss) layers = std::max(layers, s->ops.size());
+  if (layers == 0) return;
+  std::vector<VecOp> all;
+  std::vector<int> off(layers + 1, 0);
+  for (size_t L = 0; L < layers; ++L) {
+    off[L] = (int)all.size();
+    for (auto* s : ss)
+      if (L < s->ops.size()) all.push_back(s->ops[L]);
+  }
+  off[layers] = (int)all.size();
+  if ((int)all.size() > bs.ops_cap) {
+    hipFree(bs.d_ops);
+    bs.ops_cap = (int)all.size() * 2;
+    bs.d_ops = dalloc<VecOp>(bs.ops_cap);
+  }
+  HIPCHK(hipMemcpy(bs.d_ops, all.data(), sizeof(VecOp) * all.size(), hipMemcpyHostToDevice));
+  for (size_t L = 0; L < layers; ++L) vec_ops_batched(bs.d_ops + off[L], off[L + 1] - off[L], n, st);
+  for (auto* s : ss) s->ops.clear();
+}
+
+// Runs the dot needs of `ss` (all NEED_DOTS) and delivers the results.
+void run_dots(std::vector<NtnSolver*>& ss, long long n, BatchScratch& bs, hipStream_t st) {
+  std::vector<DotDesc> dd;
+  for (auto* s : ss)
+    for (int k = 0; k < s->need.np; ++k) dd.push_back(DotDesc{s->need.a[k], s->need.b[k]});
+  if (dd.empty()) return;
+  if ((int)dd.size() > bs.dots_cap) {
+    hipFree(bs.d_dots); hipFree(bs.d_part); hipFree(bs.d_res);
+    bs.dots_cap = (int)dd.size() * 2;
+    bs.d_dots = dalloc<DotDesc>(bs.dots_cap);
+    bs.d_part = dalloc<double>((size_t)bs.dots_cap * kDotBlocksBatched);
+    bs.d_res = dalloc<double>(bs.dots_cap);
+  }
+  HIPCHK(hipMemcpy(bs.d_dots, dd.data(), sizeof(DotDesc) * dd.size(), hipMemcpyHostToDevice));
+  vec_dots_batched(bs.d_dots, (int)dd.size(), n, bs.d_part, bs.d_res, st);
+  bs.h_res.resize(dd.size());
+  HIPCHK(hipMemcpyAsync(bs.h_res.data(), bs.d_res, sizeof(double) * dd.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  size_t k = 0;
+  for (auto* s : ss) {
+    const int np = s->need.np;
+    s->deliver(&bs.h_res[k]);
+    k += np;
+  }
+}
+
+// sum(x) over a device vector, deterministic (dot with a ones vector)
+double dsum(ofd_ctx* c, const double* x, double* ones, BatchScratch& bs, hipStream_t st) {
+  DotDesc d{x, ones};
+  if (bs.dots_cap < 1) {
+    bs.dots_cap = 8;
+    bs.d_dots = dalloc<DotDesc>(bs.dots_cap);
+    bs.d_part = dalloc<double>((size_t)bs.dots_cap * kDotBlocksBatched);
+    bs.d_res = dalloc<double>(bs.dots_cap);
+  }
+  HIPCHK(hipMemcpy(bs.d_dots, &d, sizeof(d), hipMemcpyHostToDevice));
+  vec_dots_batched(bs.d_dots, 1, c->nr(), bs.d_part, bs.d_res, st);
   double s;
-  HIPCHK(hipMemcpyAsync(&s, c->dot_out, sizeof(double), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpyAsync(&s, bs.d_res, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return s;
 }
+
+// Prepares phi = sqrt|rhoR| and totEleNum = sum(rhoR) dV (RhoOptN.f90:326-352)
+// from a density already on the device in c->rho.
+void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchScratch& bs, hipStream_t st) {
+  S = NtnSolver();
+  S.c = c;
+  fill_params(prm, &S.prm);
+  const long long n = c->nr();
+  std::vector<double> ones(n, 1.0);
+  HIPCHK(hipMemcpyAsync(c->tmp, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  S.tot_ele = dsum(c, c->rho, c->tmp, bs, st) * c->dV;
+  vec_sqrt_abs(c->phi, c->rho, n, st);
+  S.start();
+}
+
+}  // namespace
 
 extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt_params* prm, double* energy9,
                            int* rho_outcome, ofd_opt_stats* stats) {
@@ -868,32 +1034,34 @@ extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt
   if (dir_type != 1) return fail(c, "only dir_type 1 (NTN) is implemented");
   if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set");
   auto t0 = std::chrono::steady_clock::now();
+  BatchScratch bs;
   try {
     alloc_opt(c);
     const long long n = c->nr();
     HIPCHK(hipMemcpyAsync(c->rho, rhoR, n * sizeof(double), hipMemcpyHostToDevice, c->st));
     NtnSolver S;
-    S.c = c;
-    fill_params(prm, &S.prm);
-    // totEleNum = sum(rhoR) * dV  (RhoOptN.f90:326-329)
-    {
-      std::vector<double> ones(n, 1.0);
-      HIPCHK(hipMemcpyAsync(c->tmp, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
-      S.tot_ele = dsum(c, c->rho, c->tmp) * c->dV;
-    }
-    vec_sqrt_abs(c->phi, c->rho, n, c->st);   // phi = SQRT(ABS(rhoR)) (:352)
-    S.state = S_INIT;
-    EvalReq q;
+    solver_prepare(S, c, prm, bs, c->st);
+    std::vector<NtnSolver*> one{&S};
     double s[kNSum];
-    while (S.need_eval(&q)) {
-      ReqDesc D = desc_for(c, q);
-      if (eval_one(c, D, kInPhi, q.outmode, S.fp32 ? 1 : 0, s)) return -1;
-      S.resume(s);
+    while (S.need.kind != NEED_NONE) {
+      flush_ops(one, n, bs, c->st);
+      if (S.need.kind == NEED_DOTS) {
+        run_dots(one, n, bs, c->st);
+      } else {
+        ReqDesc D = desc_for(c, S.need.ev);
+        if (eval_one(c, D, kInPhi, S.need.ev.outmode, S.fp32 ? 1 : 0, s)) {
+          bs.free_all();
+          return -1;
+        }
+        S.deliver(s);
+      }
     }
+    flush_ops(one, n, bs, c->st);
     // rhoR = phi**2 (:495)
     vec_square(c->rho, c->phi, n, c->st);
     HIPCHK(hipMemcpyAsync(rhoR, c->rho, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    bs.free_all();
     if (energy9) std::memcpy(energy9, S.energy, sizeof(S.energy));
     if (rho_outcome) *rho_outcome = S.outcome;
     if (stats) {
@@ -901,7 +1069,9 @@ extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt
       stats->final_pot_norm = S.pot_norm;
       stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+    if (!S.err.empty()) return fail(c, S.err);
   } catch (const std::exception& e) {
+    bs.free_all();
     return fail(c, e.what());
   }
   return 0;
@@ -1156,7 +1326,6 @@ struct ofd_pool {
     ofd_ctx* c = nullptr;
     hipStream_t saved = nullptr;
     NtnSolver S;
-    std::string err;
     std::chrono::steady_clock::time_point t0;
   };
   std::vector<Slot> slots;
@@ -1164,6 +1333,7 @@ struct ofd_pool {
   double* d_sums = nullptr;
   std::vector<ReqDesc> h_desc;
   std::vector<double> h_sums;
+  BatchScratch bs;
 };
 
 extern "C" long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int precision) {
@@ -1205,6 +1375,7 @@ extern "C" void ofd_pool_destroy(ofd_pool* p) {
       s.c->st = s.saved;
       ofd_destroy(s.c);
     }
+  p->bs.free_all();
   hipFree(p->d_desc);
   hipFree(p->d_sums);
   hipStreamDestroy(p->st);
@@ -1237,15 +1408,8 @@ extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const 
     c->st = p->st;   // every request of the pool runs on the pool's stream
     s.t0 = std::chrono::steady_clock::now();
     alloc_opt(c);
-    const long long n = c->nr();
-    HIPCHK(hipMemcpyAsync(c->rho, rho, n * sizeof(double), hipMemcpyHostToDevice, p->st));
-    s.S.c = c;
-    fill_params(prm, &s.S.prm);
-    std::vector<double> ones(n, 1.0);
-    HIPCHK(hipMemcpyAsync(c->tmp, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, p->st));
-    s.S.tot_ele = dsum(c, c->rho, c->tmp) * c->dV;
-    vec_sqrt_abs(c->phi, c->rho, n, p->st);
-    s.S.state = S_INIT;
+    HIPCHK(hipMemcpyAsync(c->rho, rho, c->nr() * sizeof(double), hipMemcpyHostToDevice, p->st));
+    solver_prepare(s.S, c, prm, p->bs, p->st);
   } catch (const std::exception& e) {
     p->err = e.what();
     s.used = false;
@@ -1255,38 +1419,45 @@ extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const 
   return slot;
 }
 
+// Advances every active request by one evaluation: dot rounds (batched over
+// all requests that need dots) until every request waits on an evaluation,
+// then the evaluations, batched per (output mode, precision).
 extern "C" int ofd_pool_step(ofd_pool* p) {
   if (!p) return -1;
-  // group the pending evaluations by (output mode, precision)
-  std::vector<int> groups[3][2];
-  std::vector<EvalReq> reqs(p->capacity);
-  int active = 0;
-  for (int i = 0; i < p->capacity; ++i) {
-    auto& s = p->slots[i];
-    if (!s.used || s.done) continue;
-    if (!s.S.need_eval(&reqs[i])) {
-      s.done = true;
-      continue;
-    }
-    groups[reqs[i].outmode][s.S.fp32 ? 1 : 0].push_back(i);
-    ++active;
-  }
-  if (active == 0) return 0;
+  const long long n = (long long)p->n1 * p->n2 * p->n3;
   try {
+    std::vector<NtnSolver*> act;
+    for (auto& s : p->slots)
+      if (s.used && !s.done) act.push_back(&s.S);
+    if (act.empty()) return 0;
+    for (int guard = 0; guard < 64; ++guard) {
+      flush_ops(act, n, p->bs, p->st);
+      std::vector<NtnSolver*> dots;
+      for (auto* s : act)
+        if (s->need.kind == NEED_DOTS) dots.push_back(s);
+      if (dots.empty()) break;
+      run_dots(dots, n, p->bs, p->st);
+    }
+    flush_ops(act, n, p->bs, p->st);
+    std::vector<int> groups[3][2];
+    for (int i = 0; i < p->capacity; ++i) {
+      auto& s = p->slots[i];
+      if (!s.used || s.done) continue;
+      if (s.S.need.kind == NEED_EVAL) groups[s.S.need.ev.outmode][s.S.fp32 ? 1 : 0].push_back(i);
+    }
     int off = 0;
-    std::vector<std::pair<int, int>> where(p->capacity, {-1, -1});
+    std::vector<int> where(p->capacity, -1);
     for (int om = 0; om < 3; ++om)
       for (int pr = 0; pr < 2; ++pr) {
         auto& g = groups[om][pr];
         if (g.empty()) continue;
         ofd_ctx* c0 = p->slots[g[0]].c;
         for (size_t k = 0; k < g.size(); ++k) {
-          p->h_desc[off + k] = desc_for(p->slots[g[k]].c, reqs[g[k]]);
+          p->h_desc[off + k] = desc_for(p->slots[g[k]].c, p->slots[g[k]].S.need.ev);
           p->h_desc[off + k].fin = p->d_sums + (size_t)(off + k) * kNSum;
-          where[g[k]] = {off + (int)k, 0};
+          where[g[k]] = off + (int)k;
         }
-        HIPCHK(hipMemcpyAsync(p->d_desc + off, &p->h_desc[off], sizeof(ReqDesc) * g.size(), hipMemcpyHostToDevice,
-                              p->st));
+        HIPCHK(hipMemcpy(p->d_desc + off, &p->h_desc[off], sizeof(ReqDesc) * g.size(), hipMemcpyHostToDevice));
         if (run_eval(c0->sp, c0->kind, kInPhi, om, pr, p->d_desc + off, (int)g.size(), p->st))
           throw std::runtime_error("batched evaluation launch failed");
         if (run_finalize(c0->sp, c0->kind, pr, p->d_desc + off, (int)g.size(), p->d_sums + (size_t)off * kNSum,
@@ -1294,19 +1465,16 @@ extern "C" int ofd_pool_step(ofd_pool* p) {
           throw std::runtime_error("batched finalize failed");
         off += (int)g.size();
       }
-    HIPCHK(hipMemcpyAsync(p->h_sums.data(), p->d_sums, sizeof(double) * kNSum * off, hipMemcpyDeviceToHost, p->st));
-    HIPCHK(hipStreamSynchronize(p->st));
+    if (off > 0) {
+      HIPCHK(hipMemcpyAsync(p->h_sums.data(), p->d_sums, sizeof(double) * kNSum * off, hipMemcpyDeviceToHost,
+                            p->st));
+      HIPCHK(hipStreamSynchronize(p->st));
+    }
     for (int i = 0; i < p->capacity; ++i) {
-      if (where[i].first < 0) continue;
       auto& s = p->slots[i];
-      try {
-        s.S.resume(&p->h_sums[(size_t)where[i].first * kNSum]);
-      } catch (const std::exception& e) {
-        s.err = e.what();
-        s.S.outcome = 1;
-        s.S.state = S_DONE;
-      }
-      if (s.S.state == S_DONE) s.done = true;
+      if (!s.used) continue;
+      if (where[i] >= 0) s.S.deliver(&p->h_sums[(size_t)where[i] * kNSum]);
+      if (s.S.need.kind == NEED_NONE) s.done = true;
     }
   } catch (const std::exception& e) {
     p->err = e.what();
@@ -1326,6 +1494,8 @@ extern "C" int ofd_pool_collect(ofd_pool* p, int slot, double* rho_out, double* 
   if (!s.done) return 0;
   ofd_ctx* c = s.c;
   try {
+    std::vector<NtnSolver*> one{&s.S};
+    flush_ops(one, c->nr(), p->bs, p->st);
     const long long n = c->nr();
     vec_square(c->rho, c->phi, n, p->st);
     if (rho_out) HIPCHK(hipMemcpyAsync(rho_out, c->rho, n * sizeof(double), hipMemcpyDeviceToHost, p->st));
@@ -1335,14 +1505,14 @@ extern "C" int ofd_pool_collect(ofd_pool* p, int slot, double* rho_out, double* 
     return -1;
   }
   if (energy9) std::memcpy(energy9, s.S.energy, sizeof(s.S.energy));
-  if (outcome) *outcome = s.err.empty() ? s.S.outcome : 1;
+  if (outcome) *outcome = s.S.err.empty() ? s.S.outcome : 1;
   if (stats) {
     *stats = s.S.stats;
     stats->final_pot_norm = s.S.pot_norm;
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - s.t0).count();
   }
   c->st = s.saved;
-  if (!s.err.empty()) c->err = s.err;
+  if (!s.S.err.empty()) c->err = s.S.err;
   if (ctx_out) *ctx_out = c;
   else ofd_destroy(c);
   s = ofd_pool::Slot();

@@ -38,6 +38,7 @@ template <> struct P2<128> { static constexpr int R1 = 16, R2 = 8; };
 template <> struct P2<256> { static constexpr int R1 = 16, R2 = 16; };
 
 template <int N> __device__ __forceinline__ constexpr int padded(int j) { return j + j / P2<N>::R1; }
+__device__ __forceinline__ int t_plus(int t, int T, int m) { return t + T * m; }
 template <int N> constexpr int line_lds() { return N + N / P2<N>::R1; }
 
 // Two-stage FFT of one line; x[m] holds point t + T*m on entry and exit.
@@ -279,6 +280,152 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
   }
 }
 
+// ------------------------------------------------------------ fused x + y forward (one plane)
+// One workgroup per (z-plane i3, field f, request). The plane's half
+// spectrum (K1 x n2) of the field stays in LDS between the x R2C and the y
+// FFT: rows of stride S >= max(M + M/R1x, M + 1) complex, n2 + n2/R1y rows so
+// that the y transform can use the padded column layout as its scratch.
+template <int M, int N2> struct PlaneGeo {
+  static constexpr int LLX = M + M / P2<M>::R1;
+  static constexpr int S = LLX > M + 1 ? LLX : M + 1;
+  static constexpr int ROWS = N2 + N2 / P2<N2>::R1;
+  static constexpr int TX = P2<M>::R2, TY = P2<N2>::R2;
+  static constexpr int NTX = N2 * TX, NTY = (M + 1) * TY;
+  static constexpr int THREADS = ((NTX > NTY ? NTX : NTY) + 63) / 64 * 64;
+};
+
+template <int KIND>
+__device__ __forceinline__ double field_of(const ReqDesc& D, int f, double rho) {
+  if (f == 0) return rho;
+  if (KIND == kWT3) {
+    if (f == 2) return sqrt(rho);
+  } else {
+    if (f == 3) return sqrt(rho);
+    if (f == 2) {
+      if (D.alpha == 5.0 / 6.0) return rho / sqrt(cbrt(rho));
+      return pow(rho, D.alpha);
+    }
+  }
+  if (D.beta == 5.0 / 6.0) return rho / sqrt(cbrt(rho));
+  return pow(rho, D.beta);
+}
+
+template <int M, int N2, int KIND, int INMODE, typename R>
+__global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ descs, GridDims d,
+                                                    const typename V2<R>::T* __restrict__ twx,
+                                                    const typename V2<R>::T* __restrict__ tws,
+                                                    const typename V2<R>::T* __restrict__ twy) {
+  using C = typename V2<R>::T;
+  using G = PlaneGeo<M, N2>;
+  constexpr int R1X = P2<M>::R1, TX = G::TX, R1Y = P2<N2>::R1, TY = G::TY, S = G::S;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* P = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.z];
+  const int f = blockIdx.y;
+  const int i3 = blockIdx.x;
+  // ---- x step: thread (row i2, lane t)
+  const int row = threadIdx.x / TX, tx = threadIdx.x % TX;
+  const bool ax = row < N2;
+  C z[R1X];
+  if (ax) {
+    const size_t base = ((size_t)i3 * N2 + row) * (2 * M);
+#pragma unroll
+    for (int m = 0; m < R1X; ++m) {
+      const int k = t_plus(tx, TX, m);
+      const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
+      double a, b;
+      if (INMODE == kInRho) {
+        a = uu.x;
+        b = uu.y;
+      } else {
+        double ta = D.ca * uu.x, tb = D.ca * uu.y;
+        if (D.cb != 0.0) {
+          const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
+          ta = fma(D.cb, vv.x, ta);
+          tb = fma(D.cb, vv.y, tb);
+        }
+        a = ta * ta;
+        b = tb * tb;
+      }
+      z[m].x = (R)field_of<KIND>(D, f, a);
+      z[m].y = (R)field_of<KIND>(D, f, b);
+    }
+  }
+  C* myrow = P + (size_t)row * S;
+  fft2<M, -1>(z, myrow, tx, twx, ax);
+  if (ax) {
+#pragma unroll
+    for (int m = 0; m < R1X; ++m) myrow[tx + TX * m] = z[m];
+  }
+  __syncthreads();
+  // split, values first into registers, then written in place
+  C xo[R1X + 1];
+  if (ax) {
+#pragma unroll
+    for (int m = 0; m <= R1X; ++m) {
+      const int k = m < R1X ? tx + TX * m : M;
+      const C zk = myrow[k == M ? 0 : k];
+      const C zm = myrow[k == 0 ? 0 : M - k];
+      const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
+      const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
+      const C w = tws[k];
+      xo[m].x = er + (w.x * orr - w.y * oi);
+      xo[m].y = ei + (w.x * oi + w.y * orr);
+    }
+  }
+  __syncthreads();
+  if (ax) {
+#pragma unroll
+    for (int m = 0; m < R1X; ++m) myrow[tx + TX * m] = xo[m];
+    if (tx == 0) myrow[M] = xo[R1X];
+  }
+  __syncthreads();
+  // ---- y step: thread (column c, lane t), c fastest
+  const int c = threadIdx.x % (M + 1), ty = threadIdx.x / (M + 1);
+  const bool ay = ty < TY;
+  C y[R1Y];
+  if (ay) {
+#pragma unroll
+    for (int m = 0; m < R1Y; ++m) y[m] = P[(size_t)(ty + TY * m) * S + c];
+  }
+  __syncthreads();
+  // the column's padded scratch: element j at row j + j/R1Y
+  {
+    constexpr int R1 = R1Y, R2 = TY, Q = R1 / R2;
+    if (ay) {
+      Bfly<R1, -1, C>::run(y);
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        const int j = ty * R1 + m;
+        P[(size_t)(j + j / R1) * S + c] = y[m];
+      }
+    }
+    __syncthreads();
+    if (ay) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int b2 = ty + TY * q;
+        C w2[R2];
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          const int j = b2 + m * R1;
+          w2[m] = P[(size_t)(j + j / R1) * S + c];
+        }
+#pragma unroll
+        for (int m = 1; m < R2; ++m) w2[m] = cmul(w2[m], twy[m * b2]);
+        Bfly<R2, -1, C>::run(w2);
+#pragma unroll
+        for (int m = 0; m < R2; ++m) y[q + Q * m] = w2[m];
+      }
+    }
+  }
+  if (ay) {
+    C* out = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)i3 * N2 * (M + 1) + c;
+#pragma unroll
+    for (int m = 0; m < R1Y; ++m) out[(size_t)(ty + TY * m) * (M + 1)] = y[m];
+  }
+}
+
 // ------------------------------------------------------------ y columns
 // columns flattened over (i3, k1); blockDim = KB * T, thread = (t, c), c fastest
 template <int N, int S, typename R>
@@ -309,6 +456,9 @@ __global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, G
 }
 
 // ------------------------------------------------------------ z columns + G-space
+// One field per workgroup (blockIdx.y): for the Wang-Teter layouts the
+// G-space operator is diagonal in the fields (Hartree 4 pi/G^2 on F[rho],
+// K(|G|) on F[rho^b] (and F[rho^a]), G^2 on F[sqrt rho]).
 template <int N, int KIND, typename R>
 __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
                                                 const typename V2<R>::T* __restrict__ tw, int kb) {
@@ -317,7 +467,8 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   constexpr int LL = line_lds<N>();
   extern __shared__ __align__(16) unsigned char smem[];
   C* lds = reinterpret_cast<C*>(smem);
-  const ReqDesc& D = descs[blockIdx.y];
+  const ReqDesc& D = descs[blockIdx.z];
+  const int f = blockIdx.y;
   const int c = threadIdx.x % kb, t = threadIdx.x / kb;
   const long long col = (long long)blockIdx.x * kb + c;
   const long long ncol = (long long)d.K1 * d.n2;
@@ -327,39 +478,34 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
   const size_t es = (size_t)d.n2 * d.K1;
   const double invn = 1.0 / (double)d.nr;
-  C* lbuf = lds + (size_t)c * LL;
-#pragma unroll 1
-  for (int f = 0; f < NF; ++f) {
-    C* base = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)col;
-    C x[R1];
-    if (active) {
+  C* base = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)col;
+  C x[R1];
+  if (active) {
 #pragma unroll
-      for (int m = 0; m < R1; ++m) x[m] = base[(size_t)(t + T * m) * es];
-    }
-    fft2<N, -1>(x, lbuf, t, tw, active);
-    if (active) {
+    for (int m = 0; m < R1; ++m) x[m] = base[(size_t)(t + T * m) * es];
+  }
+  fft2<N, -1>(x, lds + (size_t)c * LL, t, tw, active);
+  if (active) {
 #pragma unroll
-      for (int m = 0; m < R1; ++m) {
-        // G-space multiplier of this point (FP64), 1/N folded in
-        const int i3 = t + T * m;
-        const int m3 = i3 > N / 2 ? i3 - N : i3;
-        const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
-        const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
-        const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
-        const double q2 = gx * gx + gy * gy + gz * gz;
-        double s;
-        if (f == 0) s = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
-        else if (f == NF - 1) s = q2 * invn;
-        else s = D.coef * lind_g(sqrt(q2) / D.tkf, D.lam, D.mu) * invn;
-        x[m].x = (R)((double)x[m].x * s);
-        x[m].y = (R)((double)x[m].y * s);
-      }
+    for (int m = 0; m < R1; ++m) {
+      const int i3 = t + T * m;
+      const int m3 = i3 > N / 2 ? i3 - N : i3;
+      const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
+      const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
+      const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
+      const double q2 = gx * gx + gy * gy + gz * gz;
+      double sc;
+      if (f == 0) sc = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
+      else if (f == NF - 1) sc = q2 * invn;
+      else sc = D.coef * lind_g(sqrt(q2) / D.tkf, D.lam, D.mu) * invn;
+      x[m].x = (R)((double)x[m].x * sc);
+      x[m].y = (R)((double)x[m].y * sc);
     }
-    fft2<N, +1>(x, lbuf, t, tw, active);
-    if (active) {
+  }
+  fft2<N, +1>(x, lds + (size_t)c * LL, t, tw, active);
+  if (active) {
 #pragma unroll
-      for (int m = 0; m < R1; ++m) base[(size_t)(t + T * m) * es] = x[m];
-    }
+    for (int m = 0; m < R1; ++m) base[(size_t)(t + T * m) * es] = x[m];
   }
 }
 
@@ -392,43 +538,52 @@ __device__ __forceinline__ double epi(const ReqDesc& D, double rho, double vion,
   return loc + D.mu * t / (2.0 * p.phi);
 }
 
+// One row per T threads; thread t keeps the reals 2k, 2k+1 (k = t + T*m) of
+// every field in registers, so the epilogue runs on the thread's own points
+// and LDS holds only one padded line per row for the FFT's lane exchange.
+template <int EPI_OUT, int KIND>
+__device__ __forceinline__ void epi_store(const ReqDesc& D, size_t g, double rho, double tphi, double vion,
+                                          const double* o, double bv, double dirv, double (&acc)[kNSum],
+                                          double& res) {
+  const double pot = epi<KIND>(D, rho, vion, o, acc);
+  if (EPI_OUT == kOutPot) {
+    res = pot;
+  } else if (EPI_OUT == kOutSigned) {
+    const double de = pot * fsign(tphi);
+    res = de;
+    acc[6] += de * dirv;
+    const double dl = de - 2.0 * tphi * D.mu0;
+    acc[7] += dl * dl;
+    acc[8] += de * tphi;
+  } else {
+    const double de = pot * fsign(tphi);
+    const double ap = ((de - D.mu0 * 2.0 * tphi) - bv) / D.eps;
+    res = ap;
+    acc[6] += dirv * ap;
+  }
+}
+
 template <int M, int KIND, int INMODE, int OUTMODE, typename R>
 __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ descs, GridDims d,
                                                 const typename V2<R>::T* __restrict__ tw,
                                                 const typename V2<R>::T* __restrict__ tws, int rpb) {
   using C = typename V2<R>::T;
   constexpr int R1 = P2<M>::R1, T = P2<M>::R2, NF = NF_<KIND>::v;
-  constexpr int LL = line_lds<M>() > M + 1 ? line_lds<M>() : M + 1;
+  constexpr int LL = line_lds<M>();
   extern __shared__ __align__(16) unsigned char smem[];
   C* lds = reinterpret_cast<C*>(smem);
   const ReqDesc& D = descs[blockIdx.y];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const long long nrows = (long long)d.n2 * d.n3;
-  const long long row0 = (long long)blockIdx.x * rpb;
-  const long long row = row0 + r;
+  const long long row = (long long)blockIdx.x * rpb + r;
   const bool active = row < nrows;
-  const int nrow = (int)min((long long)rpb, nrows - row0);
-  C* mine = lds + (size_t)r * NF * LL;
-  // 1: the NF half rows of this workgroup's rows into LDS (coalesced)
-  {
-    const C* work = reinterpret_cast<const C*>(D.work);
-    const int per = nrow * (M + 1);
-    for (int j = 0; j < NF; ++j) {
-      const C* src = work + (size_t)j * d.nc + (size_t)row0 * (M + 1);
-      for (int e = threadIdx.x; e < per; e += blockDim.x) {
-        const int rr = e / (M + 1), k = e - rr * (M + 1);
-        lds[((size_t)rr * NF + j) * LL + k] = src[e];
-      }
-    }
-  }
-  __syncthreads();
-  // 2: per field, C2R merge into registers, inverse FFT, real outputs back
-  //    into the field's LDS line (as R values, point i at index i)
-#pragma unroll 1
+  C* mine = lds + (size_t)r * LL;
+  double o[NF][2 * R1];
+#pragma unroll
   for (int j = 0; j < NF; ++j) {
     C z[R1];
     if (active) {
-      const C* X = mine + j * LL;
+      const C* X = reinterpret_cast<const C*>(D.work) + (size_t)j * d.nc + (size_t)row * (M + 1);
 #pragma unroll
       for (int m = 0; m < R1; ++m) {
         const int k = t + T * m;
@@ -442,50 +597,60 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
         z[m].y = ei + orr;
       }
     }
-    __syncthreads();
-    fft2<M, +1>(z, mine + j * LL, t, tw, active);
-    if (active) {
-      C* o = mine + j * LL;
+    fft2<M, +1>(z, mine, t, tw, active);
 #pragma unroll
-      for (int m = 0; m < R1; ++m) o[t + T * m] = z[m];   // complex k holds reals 2k, 2k+1
+    for (int m = 0; m < R1; ++m) {
+      o[j][2 * m] = (double)z[m].x;       // complex k holds reals 2k, 2k+1
+      o[j][2 * m + 1] = (double)z[m].y;
     }
   }
-  __syncthreads();
-  // 3: epilogue, consecutive threads on consecutive points of the tile
   double acc[kNSum];
 #pragma unroll
   for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
-  const int npts = nrow * 2 * M;
-  const size_t gbase = (size_t)row0 * 2 * M;
-  const R* lr = reinterpret_cast<const R*>(lds);
-  for (int e = threadIdx.x; e < npts; e += blockDim.x) {
-    const int rr = e / (2 * M), i = e - rr * (2 * M);
-    const size_t g = gbase + e;
-    double o[NF];
+  if (active) {
+    const size_t base = (size_t)row * (2 * M);
 #pragma unroll
-    for (int j = 0; j < NF; ++j) o[j] = (double)lr[2 * ((size_t)rr * NF + j) * LL + i];
-    double tphi;
-    const double rho = load_rho<INMODE>(D, g, tphi);
-    const double pot = epi<KIND>(D, rho, D.vion[g], o, acc);
-    if (OUTMODE == kOutPot) {
-      D.out[g] = pot;
-    } else if (OUTMODE == kOutSigned) {
-      const double de = pot * fsign(tphi);
-      D.out[g] = de;
-      const double dir = D.s1 * D.u[g] + (D.v != nullptr ? D.s2 * D.v[g] : 0.0);
-      acc[6] += de * dir;
-      const double dl = de - 2.0 * tphi * D.mu0;
-      acc[7] += dl * dl;
-      acc[8] += de * tphi;
-    } else {
-      const double de = pot * fsign(tphi);
-      const double ap = ((de - D.mu0 * 2.0 * tphi) - D.bvec[g]) / D.eps;
-      D.out[g] = ap;
-      acc[6] += D.v[g] * ap;
+    for (int m = 0; m < R1; ++m) {
+      const size_t g = base + 2 * (size_t)(t + T * m);
+      double rho0, rho1, tp0 = 0.0, tp1 = 0.0;
+      const double2 uu = *reinterpret_cast<const double2*>(D.u + g);
+      double dir0 = 0.0, dir1 = 0.0, bv0 = 0.0, bv1 = 0.0;
+      if (INMODE == kInRho) {
+        rho0 = uu.x;
+        rho1 = uu.y;
+      } else {
+        double2 vv = make_double2(0.0, 0.0);
+        if (D.v != nullptr) vv = *reinterpret_cast<const double2*>(D.v + g);
+        tp0 = D.cb != 0.0 ? fma(D.cb, vv.x, D.ca * uu.x) : D.ca * uu.x;
+        tp1 = D.cb != 0.0 ? fma(D.cb, vv.y, D.ca * uu.y) : D.ca * uu.y;
+        rho0 = tp0 * tp0;
+        rho1 = tp1 * tp1;
+        if (OUTMODE == kOutSigned) {
+          dir0 = D.s1 * uu.x + (D.v != nullptr ? D.s2 * vv.x : 0.0);
+          dir1 = D.s1 * uu.y + (D.v != nullptr ? D.s2 * vv.y : 0.0);
+        } else if (OUTMODE == kOutNewton) {
+          dir0 = vv.x;
+          dir1 = vv.y;
+          const double2 bb = *reinterpret_cast<const double2*>(D.bvec + g);
+          bv0 = bb.x;
+          bv1 = bb.y;
+        }
+      }
+      const double2 vi = *reinterpret_cast<const double2*>(D.vion + g);
+      double oa[NF], ob[NF];
+#pragma unroll
+      for (int j = 0; j < NF; ++j) {
+        oa[j] = o[j][2 * m];
+        ob[j] = o[j][2 * m + 1];
+      }
+      double2 res;
+      epi_store<OUTMODE, KIND>(D, g, rho0, tp0, vi.x, oa, bv0, dir0, acc, res.x);
+      epi_store<OUTMODE, KIND>(D, g + 1, rho1, tp1, vi.y, ob, bv1, dir1, acc, res.y);
+      *reinterpret_cast<double2*>(D.out + g) = res;
     }
   }
   block_sum<kNSum>(acc, D.sums + (size_t)blockIdx.x * kNSum);
-  // 4: the last workgroup of this request sums the partials in a fixed order
+  // the last workgroup of this request sums the partials in a fixed order
   __shared__ unsigned int is_last;
   __threadfence();
   __syncthreads();
@@ -572,10 +737,12 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
     hipLaunchKernelGGL((k_fx_fwd<M, KIND, INMODE, R>), dim3(ft.ntile, nb), dim3(threads), lds, st, dd, sp.d, tw, tws,
                        ft.rpb);
   } else {
-    constexpr int LL = line_lds<M>() > M + 1 ? line_lds<M>() : M + 1;
-    size_t lds = (size_t)ft.rpb * NF * LL * sizeof(C);
-    hipLaunchKernelGGL((k_fx_inv<M, KIND, INMODE, OUTMODE, R>), dim3(ft.ntile, nb), dim3(threads), lds, st, dd, sp.d,
-                       tw, tws, ft.rpb);
+    const int rpb = 256 / T;
+    const long long nrows = (long long)sp.d.n2 * sp.d.n3;
+    const int ntile = (int)((nrows + rpb - 1) / rpb);
+    size_t lds = (size_t)rpb * line_lds<M>() * sizeof(C);
+    hipLaunchKernelGGL((k_fx_inv<M, KIND, INMODE, OUTMODE, R>), dim3(ntile, nb), dim3(rpb * T), lds, st, dd, sp.d,
+                       tw, tws, rpb);
   }
 }
 
@@ -598,7 +765,35 @@ static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hip
   const long long ncol = (long long)sp.d.K1 * sp.d.n2;
   const int nblk = (int)((ncol + kb - 1) / kb);
   size_t lds = (size_t)kb * line_lds<N>() * sizeof(C);
-  hipLaunchKernelGGL((k_fz_gop<N, KIND, R>), dim3(nblk, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb);
+  hipLaunchKernelGGL((k_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d,
+                     tw, kb);
+}
+
+template <typename R, int M, int N2, int KIND, int INMODE>
+static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st,
+                         const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
+  using namespace fast;
+  using C = typename V2<R>::T;
+  using G = PlaneGeo<M, N2>;
+  const size_t lds = (size_t)G::ROWS * G::S * sizeof(C);
+  if (lds > (size_t)sp.lds_cap || G::THREADS > 1024) return false;
+  auto kern = k_plane_fwd<M, N2, KIND, INMODE, R>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(sp.d.n3, NF_<KIND>::v, nb), dim3(G::THREADS), lds, st, dd, sp.d, twx, tws, twy);
+  return true;
+}
+
+template <typename R, int KIND, int INMODE>
+static bool try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st, const typename V2<R>::T* twx,
+                      const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
+  if (sp.no_plane) return false;
+  if (sp.d.M == 64 && sp.d.n2 == 128) return launch_plane<R, 64, 128, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  if (sp.d.M == 32 && sp.d.n2 == 64) return launch_plane<R, 32, 64, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  return false;
 }
 
 template <typename R, int KIND, int INMODE, int OUTMODE, int M>
@@ -606,7 +801,6 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
                        const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy,
                        const typename V2<R>::T* twz) {
   constexpr int NF = fast::NF_<KIND>::v;
-  launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, true, twx, tws);
   auto ylaunch = [&](auto sgn) {
     constexpr int S = decltype(sgn)::value;
     switch (sp.d.n2) {
@@ -616,7 +810,10 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
       default: launch_y<R, 256, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
     }
   };
-  ylaunch(std::integral_constant<int, -1>());
+  if (!try_plane<R, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy)) {
+    launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, true, twx, tws);
+    ylaunch(std::integral_constant<int, -1>());
+  }
   switch (sp.d.n3) {
     case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 64: launch_z<R, 64, KIND>(sp, ft.kbz, dd, nb, st, twz); break;

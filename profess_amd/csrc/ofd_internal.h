@@ -89,6 +89,7 @@ struct ShapePlan {
   int lds_cap = 65536;     // device limit per workgroup
   int lds_target = 65536;  // soft budget used for the automatic tiles
   int force_generic = 0;   // 1: never use the register-resident fast path
+  int no_plane = 0;        // 1: never use the fused x+y plane kernel
 };
 
 struct Tiles {
@@ -126,6 +127,25 @@ void vec_square(double* y, const double* x, long long n, hipStream_t st);       
 void vec_dots(const double* const* a, const double* const* b, int npairs, long long n, double* d_partial,
               double* d_out, hipStream_t st);
 void vec_scale_sign(double* y, const double* x, const double* s, long long n, hipStream_t st);  // y = x*sign(s)
+
+// Batched vector operations over many requests of one grid size.
+enum VecOpKind : int { kOpLincomb = 0, kOpAxpby = 1, kOpZero = 2 };
+struct VecOp {
+  int kind;
+  double* z;
+  const double* x;
+  const double* y;
+  double a, b;
+};
+// kOpLincomb: z = fma(b, y, a*x); kOpAxpby: z = fma(a, x, b*z); kOpZero: z = 0
+void vec_ops_batched(const VecOp* d_ops, int nops, long long n, hipStream_t st);
+struct DotDesc {
+  const double* a;
+  const double* b;
+};
+constexpr int kDotBlocksBatched = 256;
+// res[k] = sum a_k . b_k, fixed-order reduction (kDotBlocksBatched partials each)
+void vec_dots_batched(const DotDesc* d_dd, int ndots, long long n, double* d_part, double* d_res, hipStream_t st);
 
 // Per-geometry tables.
 int build_wgc_kernels(const ShapePlan& sp, const double* rb, const double* d_eta, const double* d_w,

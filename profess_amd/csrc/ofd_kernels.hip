@@ -619,6 +619,38 @@ __global__ void __launch_bounds__(256) k_dots_fin(const double* partial, int nbl
   block_sum_store<4>(acc, out);
 }
 
+__global__ void __launch_bounds__(256) k_vecops(const VecOp* __restrict__ ops, long long n) {
+  const VecOp o = ops[blockIdx.y];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    if (o.kind == kOpLincomb) o.z[i] = fma(o.b, o.y[i], o.a * o.x[i]);
+    else if (o.kind == kOpAxpby) o.z[i] = fma(o.a, o.x[i], o.b * o.z[i]);
+    else o.z[i] = 0.0;
+  }
+}
+__global__ void __launch_bounds__(256) k_dotsb(const DotDesc* __restrict__ dd, long long n, double* part) {
+  const DotDesc q = dd[blockIdx.y];
+  double acc[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    acc[0] += q.a[i] * q.b[i];
+  block_sum_store<1>(acc, part + (size_t)blockIdx.y * kDotBlocksBatched + blockIdx.x);
+}
+__global__ void __launch_bounds__(256) k_dotsb_fin(const double* part, double* res) {
+  double acc[1] = {0.0};
+  for (int w = threadIdx.x; w < kDotBlocksBatched; w += blockDim.x) acc[0] += part[(size_t)blockIdx.x * kDotBlocksBatched + w];
+  block_sum_store<1>(acc, res + blockIdx.x);
+}
+void vec_ops_batched(const VecOp* d_ops, int nops, long long n, hipStream_t st) {
+  if (nops <= 0) return;
+  long long b = (n + 255) / 256;
+  int nb = (int)(b > 512 ? 512 : (b < 1 ? 1 : b));
+  hipLaunchKernelGGL(k_vecops, dim3(nb, nops), dim3(256), 0, st, d_ops, n);
+}
+void vec_dots_batched(const DotDesc* d_dd, int ndots, long long n, double* d_part, double* d_res, hipStream_t st) {
+  if (ndots <= 0) return;
+  hipLaunchKernelGGL(k_dotsb, dim3(kDotBlocksBatched, ndots), dim3(256), 0, st, d_dd, n, d_part);
+  hipLaunchKernelGGL(k_dotsb_fin, dim3(ndots), dim3(256), 0, st, d_part, d_res);
+}
+
 static int nblocks(long long n) {
   long long b = (n + 255) / 256;
   return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b));

@@ -88,6 +88,7 @@ int shape_plan_init(ShapePlan* sp, int n1, int n2, int n3, std::string* err) {
     if (v >= 4096) sp->lds_target = v < sp->lds_cap ? v : sp->lds_cap;
   }
   if (const char* e = std::getenv("OFD_GENERIC")) sp->force_generic = std::atoi(e) != 0;
+  if (const char* e = std::getenv("OFD_NO_PLANE")) sp->no_plane = std::atoi(e) != 0;
   const size_t cb = sizeof(double2);
   if (2ull * 8 * (d.M + 1) * cb > (size_t)sp->lds_cap || 2ull * 8 * (n3 + 1) * cb > (size_t)sp->lds_cap ||
       2ull * (n2 + 1) * cb > (size_t)sp->lds_cap) {

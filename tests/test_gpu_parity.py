@@ -63,7 +63,8 @@ def _case(grid, lattice, kinetic=4, alpha=-100.0, beta=-100.0, seed=0, nion=3):
 # generic LDS path (mixed radices, odd lengths) and register-resident fast path
 # (n1/2, n2, n3 in {32, 64, 128, 256})
 GRIDS = [(16, 16, 16), (24, 20, 18), (14, 21, 10), (32, 30, 28), (6, 5, 4),
-         (64, 32, 64), (128, 64, 32), (64, 128, 256), (512, 32, 32)]
+         (64, 32, 64), (128, 64, 32), (64, 128, 256), (512, 32, 32),
+         (128, 128, 32), (64, 64, 32)]   # the last two take the fused x+y plane kernel
 
 
 @pytest.mark.parametrize("grid", GRIDS)
@@ -195,3 +196,19 @@ def test_ntn_optDen_wgc(pa):
     rho, et, outcome, st = pa.ProjNormRhoMinimization(s, rho0)
     assert outcome == 0
     assert abs(et[0] * O.HARTREE_TO_EV - S.OPTDEN_STEP_EV[-1]) / 4 < 5e-6
+
+
+def test_plane_kernel_matches_separate_passes(pa, monkeypatch):
+    """The fused x+y plane pass and the separate x and y passes agree to rounding."""
+    grid = (128, 128, 32)
+    cell, frac, vion, ne, f, rho = _case(grid, _tri_lattice(), seed=4)
+    res = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("OFD_NO_PLANE", flag)
+        s = pa.OFDSystem(cell.lattice, grid, ne, kinetic=4)
+        s.set_ion_terms(vion, 0.0)
+        res.append(pa.CalculatePotentialPlus(s, rho))
+        s.close()
+    (p0, e0), (p1, e1) = res
+    assert np.max(np.abs(p0 - p1)) <= 1e-13 * np.max(np.abs(p1))
+    _check_terms(e0, e1, rtol=1e-13)
