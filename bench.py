@@ -10,6 +10,10 @@ that stream. Prints one JSON line (see the keys below).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128] [--batch B]
                   [--no-cpu] [--no-serving]
+  python bench.py --probe [--steps K]     # K evaluations + 3 device copies of
+                                          # 256 MiB, for rocprofv3 --pmc runs
+  python bench.py --pmc-summary CSV       # per-kernel FETCH_SIZE/WRITE_SIZE of
+                                          # such a run, scaled by the copy kernel
 """
 from __future__ import annotations
 
@@ -49,6 +53,7 @@ class Hip:
         self.L.hipEventSynchronize.argtypes = [C.c_void_p]
         self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         self.L.hipEventDestroy.argtypes = [C.c_void_p]
+        self.L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
 
     def check(self, rc, what):
         if rc != 0:
@@ -156,6 +161,57 @@ def time_batched(pa, hip, s0, case, vion, nb, steps, warmup):
     return ms / steps, systems
 
 
+COPY_BYTES = 256 << 20
+
+
+def probe(pa, hip, n, steps):
+    """Evaluations followed by plain device-to-device copies of known size:
+    the copy kernel calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    case = make_case(n)
+    s, ms, et, vion, bufs = time_single(pa, hip, case, steps, 3)
+    a, b = hip.malloc(COPY_BYTES), hip.malloc(COPY_BYTES)
+    hip.check(hip.L.hipMemset(C.c_void_p(a), 1, COPY_BYTES), "hipMemset")
+    for _ in range(3):
+        hip.check(hip.L.hipMemcpy(C.c_void_p(b), C.c_void_p(a), COPY_BYTES, 3), "hipMemcpy")
+    hip.check(hip.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    print("probe: %d evaluations at %d^3 (%.1f us each), 3 copies of %d bytes" % (steps, n, ms * 1e3, COPY_BYTES),
+          flush=True)
+
+
+def pmc_summary(path, n=128):
+    """Per-kernel average of each counter (KiB per dispatch) in a rocprofv3
+    counter_collection.csv, and the evaluation's HBM bytes scaled by the copy
+    kernel's reading (a copy of COPY_BYTES reads and writes COPY_BYTES)."""
+    import csv
+    from collections import defaultdict
+    tot = defaultdict(float)
+    cnt = defaultdict(set)
+    for r in csv.DictReader(open(path)):
+        key = (r["Kernel_Name"], r["Counter_Name"])
+        tot[key] += float(r["Counter_Value"])
+        cnt[key].add(r.get("Dispatch_Id", r.get("Correlation_Id", "")))
+    per = {k: tot[k] / max(1, len(cnt[k])) for k in tot}
+    counters = sorted({k[1] for k in per})
+    copy = {c: v for (kn, c), v in per.items() if "copyBuffer" in kn}
+    ndisp = {kn: len(cnt[(kn, c)]) for (kn, c) in per}
+    nmax = max(v for kn, v in ndisp.items() if "rocclr" not in kn)
+    lines = ["evaluation kernels: the %d kernels dispatched %d times" % (
+        sum(1 for kn, v in ndisp.items() if v == nmax and "rocclr" not in kn), nmax)]
+    for c in counters:
+        factor = COPY_BYTES / 1024.0 / copy[c] if copy.get(c) else float("nan")
+        lines.append("%s copy-kernel factor = %.4f (copy of %d B read as %.1f KiB)"
+                     % (c, factor, COPY_BYTES, copy.get(c, float("nan"))))
+        evsum = 0.0
+        for (kn, cc), v in sorted(per.items()):
+            if cc != c or "rocclr" in kn or ndisp[kn] != nmax:
+                continue
+            evsum += v
+            lines.append("  %-70s %12.1f KiB/dispatch  scaled %14.0f B" % (kn[:70], v, v * 1024 * factor))
+        lines.append("  %s per evaluation (sum over its kernels), scaled: %.0f B" % (c, evsum * 1024 * factor))
+        lines.append("")
+    print("\n".join(lines))
+
+
 def cpu_baseline(case, vion, budget_s=15.0):
     """The oracle's WT evaluation (numpy, single thread) on the same inputs."""
     from oracle import profess_oracle as O
@@ -208,7 +264,12 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--pmc-summary", default=None)
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary, args.grid)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the evaluation is a single-GPU path; "
               "multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
@@ -218,13 +279,16 @@ def main():
     pa.lib()
     hip = Hip()
     n = args.grid
+    if args.probe:
+        probe(pa, hip, n, args.steps)
+        return
     case = make_case(n)
     s, ms1, et, vion, bufs = time_single(pa, hip, case, args.steps, args.warmup)
     evals_s = 1000.0 / ms1
     bw = b_wt(n, n, n)
     achieved = bw * evals_s / 1e9
     out = {
-        "metric": "WT energy+potential evals/s at 128^3",
+        "metric": "WT energy+potential evals/s at 128\u00b3; batched converged solves/s + p99 latency",
         "value": evals_s,
         "unit": "evals/s",
         "n_gpus": 1,

@@ -165,6 +165,22 @@ int ofd_pool_collect(ofd_pool* pool, int slot, double* rho_out, double* energy9,
 /* Device bytes one request of this grid needs (admission by HBM footprint). */
 long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int precision);
 
+/* ---- autotuner (no reference counterpart: FFTW chooses the reference's
+ *      plans inside FFTW_PLAN_DFT_R2C_3D, Fourier_NEW.f90) ---- */
+typedef struct ofd_tune_result {
+  int rows_x;         /* rows per workgroup of the generic x passes (0 = derived) */
+  int kb_y, kb_z;     /* columns per workgroup of the y and z passes (0 = derived) */
+  int plane;          /* 1 when the fused x+y forward plane pass is used */
+  int configs_tried;
+  double us_default;  /* one evaluation with the derived tiles, microseconds */
+  double us_best;     /* one evaluation with the chosen tiles */
+} ofd_tune_result;
+/* Times `reps` evaluations per candidate tile shape on the context's stream
+ * (coordinate descent over plane pass, z tiles, y tiles, x rows), keeps the
+ * fastest in the context and records it for every context created later
+ * with the same grid, functional kind and precision. */
+int ofd_autotune(ofd_ctx* ctx, int reps, ofd_tune_result* out);
+
 #ifdef __cplusplus
 }
 #endif

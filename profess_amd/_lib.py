@@ -32,6 +32,11 @@ class ofd_opt_params(C.Structure):
                 ("max_line", C.c_int), ("mixed_switch", C.c_double)]
 
 
+class ofd_tune_result(C.Structure):
+    _fields_ = [("rows_x", C.c_int), ("kb_y", C.c_int), ("kb_z", C.c_int), ("plane", C.c_int),
+                ("configs_tried", C.c_int), ("us_default", C.c_double), ("us_best", C.c_double)]
+
+
 class ofd_opt_stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("num_evals", C.c_int), ("newton_iters_total", C.c_int),
                 ("fp32_evals", C.c_int), ("final_pot_norm", C.c_double), ("seconds", C.c_double)]
@@ -45,7 +50,7 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_calpot", "ofd_calpot_device", "ofd_rho_opt", "ofd_ion_forces", "ofd_pool_create",
            "ofd_pool_destroy", "ofd_pool_last_error", "ofd_pool_admit", "ofd_pool_step", "ofd_pool_collect",
            "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
-           "ofd_calpot_many"]
+           "ofd_calpot_many", "ofd_autotune"]
 
 
 def lib():
@@ -89,6 +94,7 @@ def lib():
     L.ofd_get_stream.restype = C.c_void_p
     L.ofd_sync.argtypes = [vp]
     L.ofd_calpot_many.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, dp]
+    L.ofd_autotune.argtypes = [vp, C.c_int, C.POINTER(ofd_tune_result)]
     _lib = L
     return L
 

@@ -15,7 +15,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libprofess_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["ofd_kernels.hip", "ofd_fast.hip", "ofd_tables.hip", "ofd_plan.cpp", "ofd_api.cpp"]
+SOURCES = ["ofd_kernels.hip", "ofd_fast.hip", "ofd_fast_d3.hip", "ofd_fast_d4.hip", "ofd_fast_f3.hip",
+           "ofd_fast_f4.hip", "ofd_tables.hip", "ofd_plan.cpp", "ofd_api.cpp"]
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wno-unused-result", "-Wno-unused-value"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -43,7 +44,7 @@ def _compile(src, force=True):
 def build(verbose=False, force=True):
     """Compile every source (force=True, the default) or only stale ones."""
     os.makedirs(OBJ, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=5) as ex:
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
         objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs

@@ -108,6 +108,14 @@ class OFDSystem:
         self._check(lib().ofd_get_ion_potential(self._h, dptr(out)))
         return out
 
+    def autotune(self, reps=5):
+        """Pick the evaluation passes' tile shapes for this grid by timing
+        (ofd_autotune); later systems of the same shape start from them."""
+        from ._lib import ofd_tune_result
+        r = ofd_tune_result()
+        self._check(lib().ofd_autotune(self._h, int(reps), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_}
+
     def set_wgc_table(self, eta, w, w1, w2):
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (eta, w, w1, w2)]
         self._check(lib().ofd_set_wgc_table(self._h, a[0].size, *[dptr(x) for x in a]))

@@ -13,8 +13,12 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -193,6 +197,82 @@ struct LineSearch {
 
 }  // namespace
 
+// ====================================================================== launch graphs
+// The launch chain of one batched evaluation (five passes + finalize) is
+// captured once per (input mode, output mode, precision, descriptor array,
+// batch width) and replayed: the kernels read their per-request parameters
+// from the descriptor array in device memory, so a replay only needs the
+// descriptors uploaded first. The first use of a key runs eagerly (so that
+// one-time function attributes are set outside any capture). No memset or
+// copy is ever recorded. OFD_NO_GRAPH=1 disables the capture.
+struct GraphKey {
+  int inmode, outmode, fp32, nb;
+  const void* desc;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(inmode, outmode, fp32, nb, desc) < std::tie(o.inmode, o.outmode, o.fp32, o.nb, o.desc);
+  }
+};
+struct GraphCache {
+  std::map<GraphKey, hipGraphExec_t> exec;
+  std::map<GraphKey, int> seen;
+};
+static void graphs_clear(GraphCache& g) {
+  for (auto& kv : g.exec) hipGraphExecDestroy(kv.second);
+  g.exec.clear();
+  g.seen.clear();
+}
+static bool graphs_enabled() {
+  const char* e = std::getenv("OFD_NO_GRAPH");
+  return !(e && std::atoi(e) != 0);
+}
+// Runs run_eval + run_finalize for `nb` descriptors at `dd` on `st`,
+// through a cached graph when possible. Returns 0 on success.
+static int launch_eval(GraphCache& gc, const ShapePlan& sp, int kind, int inmode, int outmode, int fp32,
+                       const ReqDesc* dd, int nb, double* fin_out, hipStream_t st) {
+  const GraphKey key{inmode, outmode, fp32, nb, dd};
+  auto direct = [&]() -> int {
+    if (run_eval(sp, kind, inmode, outmode, fp32, dd, nb, st)) return -1;
+    return run_finalize(sp, kind, fp32, dd, nb, fin_out, st);
+  };
+  if (!graphs_enabled()) return direct();
+  auto it = gc.exec.find(key);
+  if (it == gc.exec.end()) {
+    if (gc.seen[key]++ == 0) return direct();   // first use: eager
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return direct();
+    const int rc = direct();
+    if (hipStreamEndCapture(st, &g) != hipSuccess || rc != 0 || g == nullptr) {
+      if (g) hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return direct();
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      (void)hipGetLastError();
+      return direct();
+    }
+    it = gc.exec.emplace(key, ex).first;
+  }
+  return hipGraphLaunch(it->second, st) == hipSuccess ? 0 : -1;
+}
+
+// ====================================================================== tuned launch shapes
+// Tiles picked by ofd_autotune, per (grid, functional kind, precision); new
+// contexts of a tuned shape start from them.
+struct TuneKey {
+  int n1, n2, n3, kind, fp32;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(n1, n2, n3, kind, fp32) < std::tie(o.n1, o.n2, o.n3, o.kind, o.fp32);
+  }
+};
+struct TuneChoice {
+  int rows_x, kb_y, kb_z, no_plane;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneChoice> g_tuned;
+
 // ====================================================================== context
 struct ofd_ctx {
   std::string err;
@@ -214,7 +294,6 @@ struct ofd_ctx {
   void* work = nullptr;      // NF x nc complex (FP64 size)
   double* sums = nullptr;    // partials
   double* fin = nullptr;     // kNSum finalized sums
-  unsigned int* cnt = nullptr;  // epilogue arrival counter (self-resetting)
   double* wgck = nullptr;    // 4 x nc
   ReqDesc* desc = nullptr;   // one descriptor
   ReqDesc last_desc{};       // what `desc` holds after an enqueue
@@ -222,6 +301,7 @@ struct ofd_ctx {
   double* many_sums = nullptr;
   int many_cap = 0;
   std::vector<ReqDesc> many_last;
+  GraphCache graphs;
   // optimizer vectors
   double *phi = nullptr, *dir = nullptr, *dedphi = nullptr, *dldphi = nullptr, *r = nullptr, *p = nullptr,
          *ap = nullptr, *tmp = nullptr;
@@ -245,7 +325,6 @@ struct ofd_ctx {
     D.wgck = wgck;
     D.work = work;
     D.fin = fin;
-    D.cnt = cnt;
     for (int i = 0; i < 9; ++i) D.rb[i] = rb[i];
     D.lam = lam; D.mu = mu; D.alpha = alpha; D.beta = beta; D.rhoS = rhoS; D.coef = coef; D.tkf = tkf;
     D.opt_sqrt = 1;
@@ -347,12 +426,20 @@ extern "C" int ofd_create(const ofd_grid* g, const ofd_functional* f, double num
     const long long nrows = (long long)g->n2 * g->n3;
     c->sums = dalloc<double>((size_t)nrows * kNSum);
     c->fin = dalloc<double>(kNSum);
-    c->cnt = dalloc<unsigned int>(1);
-    HIPCHK(hipMemset(c->cnt, 0, sizeof(unsigned int)));
     if (c->kind == kWGC8) c->wgck = dalloc<double>(4 * nc);
     c->desc = dalloc<ReqDesc>(1);
     c->dot_part = dalloc<double>(512 * 4);
     c->dot_out = dalloc<double>(4);
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tuned.find(TuneKey{g->n1, g->n2, g->n3, c->kind, c->precision == 1 ? 1 : 0});
+      if (it != g_tuned.end()) {
+        c->sp.rows_x = it->second.rows_x;
+        c->sp.kb_y = it->second.kb_y;
+        c->sp.kb_z = it->second.kb_z;
+        c->sp.no_plane = it->second.no_plane != 0;
+      }
+    }
   } catch (const std::exception& e) {
     c->err = e.what();
     *out = c;
@@ -376,7 +463,8 @@ extern "C" void ofd_destroy(ofd_ctx* c) {
   free_opt(c);
   hipFree(c->vion); hipFree(c->rho); hipFree(c->pot); hipFree(c->work); hipFree(c->sums); hipFree(c->fin);
   hipFree(c->wgck); hipFree(c->desc); hipFree(c->dot_part); hipFree(c->dot_out);
-  hipFree(c->many_desc); hipFree(c->many_sums); hipFree(c->cnt);
+  hipFree(c->many_desc); hipFree(c->many_sums);
+  graphs_clear(c->graphs);
   hipFree(c->d_frac); hipFree(c->d_psp_vqs); hipFree(c->d_psp_dd);
   if (c->sp.tw_x) shape_plan_free(&c->sp);
   if (c->st) hipStreamDestroy(c->st);
@@ -428,9 +516,8 @@ static int eval_one(ofd_ctx* c, const ReqDesc& D, int inmode, int outmode, int f
   if (hipMemcpy(c->desc, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess)
     return fail(c, "descriptor upload failed");
   c->last_desc = D;
-  int rc = run_eval(c->sp, c->kind, inmode, outmode, fp32, c->desc, 1, c->st);
-  if (rc) return fail(c, "kernel launch failed");
-  if (run_finalize(c->sp, c->kind, fp32, c->desc, 1, c->fin, c->st)) return fail(c, "finalize launch failed");
+  if (launch_eval(c->graphs, c->sp, c->kind, inmode, outmode, fp32, c->desc, 1, c->fin, c->st))
+    return fail(c, "kernel launch failed");
   if (h_sums) {
     if (hipMemcpyAsync(h_sums, c->fin, kNSum * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess)
       return fail(c, "sum copy failed");
@@ -468,9 +555,9 @@ extern "C" int ofd_calpot_enqueue(ofd_ctx* c, const double* rho, int opt_sqrt, d
     c->last_desc = D;
   }
   const int fp32 = c->precision == 1;
-  if (run_eval(c->sp, c->kind, kInRho, kOutPot, fp32, c->desc, 1, c->st)) return fail(c, "kernel launch failed");
-  if (run_finalize(c->sp, c->kind, fp32, c->desc, 1, sums_dev ? sums_dev : c->fin, c->st))
-    return fail(c, "finalize launch failed");
+  if (launch_eval(c->graphs, c->sp, c->kind, kInRho, kOutPot, fp32, c->desc, 1, sums_dev ? sums_dev : c->fin,
+                  c->st))
+    return fail(c, "kernel launch failed");
   return 0;
 }
 
@@ -498,6 +585,8 @@ extern "C" int ofd_calpot_many(ofd_ctx* const* cs, int nb, const double* const* 
   }
   try {
     if ((int)c0->many_cap < nb) {
+      HIPCHK(hipStreamSynchronize(c0->st));
+      graphs_clear(c0->graphs);
       hipFree(c0->many_desc);
       hipFree(c0->many_sums);
       c0->many_desc = dalloc<ReqDesc>(nb);
@@ -519,10 +608,8 @@ extern "C" int ofd_calpot_many(ofd_ctx* const* cs, int nb, const double* const* 
       c0->many_last = h;
     }
     const int fp32 = c0->precision == 1;
-    if (run_eval(c0->sp, c0->kind, kInRho, kOutPot, fp32, c0->many_desc, nb, c0->st))
+    if (launch_eval(c0->graphs, c0->sp, c0->kind, kInRho, kOutPot, fp32, c0->many_desc, nb, c0->many_sums, c0->st))
       return fail(c0, "kernel launch failed");
-    if (run_finalize(c0->sp, c0->kind, fp32, c0->many_desc, nb, c0->many_sums, c0->st))
-      return fail(c0, "finalize failed");
     if (ets) {
       std::vector<double> s((size_t)nb * kNSum);
       HIPCHK(hipMemcpyAsync(s.data(), c0->many_sums, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c0->st));
@@ -569,6 +656,8 @@ struct Need {
   const double* a[4] = {nullptr, nullptr, nullptr, nullptr};
   const double* b[4] = {nullptr, nullptr, nullptr, nullptr};
 };
+
+constexpr int kFp32MaxLine = 20;
 
 enum SState {
   ST_INIT_EVAL, ST_INIT_DOTS, ST_NEWTON_PP, ST_NEWTON_EVAL, ST_NEWTON_RR, ST_LP1, ST_LP2, ST_LP3, ST_LINE_EVAL,
@@ -769,6 +858,13 @@ struct NtnSolver {
       finish();
       return;
     }
+    // an FP32 step that does not lower the energy means FP32 has reached its
+    // noise floor: continue in FP64 from the accepted density
+    if (fp32 && e_hist[0] >= e_hist[1]) {
+      ++it;
+      fall_back_fp64();
+      return;
+    }
     const bool was32 = fp32;
     set_precision_for(pot_norm);
     if (was32 && !fp32) e_hist[0] = e_hist[1] = e_hist[2] = 0.0;   // FP32 energies never count toward ENE
@@ -886,6 +982,11 @@ struct NtnSolver {
         f_cur = energy[0];
         g_cur = c->dV * v[6];
         ++nline;
+        if (fp32 && nline > kFp32MaxLine) {
+          // FP32 noise keeps the Wolfe conditions from being met: refine in FP64
+          fall_back_fp64();
+          return;
+        }
         if (nline > prm.max_line) accept();
         else line_advance();
         return;
@@ -1315,6 +1416,112 @@ extern "C" int ofd_ion_forces(ofd_ctx* c, const double* rho, double* forces) {
   return 0;
 }
 
+// ====================================================================== autotuner
+// Times one evaluation (uniform density at the context's electron count) per
+// candidate tile shape on the context's stream and keeps the fastest, by
+// coordinate descent over: the fused x+y plane pass (on/off), the z-pass
+// columns per workgroup, the y-pass columns per workgroup and (generic
+// passes) the x rows per workgroup.
+static double time_current(ofd_ctx* c, int reps) {
+  ReqDesc D = c->base_desc();
+  D.u = c->rho;
+  D.out = c->pot;
+  D.opt_sqrt = 1;
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(c->desc, &D, sizeof(D), hipMemcpyHostToDevice));
+  c->last_desc = D;
+  const int fp32 = c->precision == 1;
+  auto once = [&]() {
+    if (run_eval(c->sp, c->kind, kInRho, kOutPot, fp32, c->desc, 1, c->st) ||
+        run_finalize(c->sp, c->kind, fp32, c->desc, 1, c->fin, c->st))
+      throw std::runtime_error("evaluation launch failed while tuning");
+  };
+  once();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->st));
+  for (int i = 0; i < reps; ++i) once();
+  HIPCHK(hipEventRecord(e1, c->st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return 1e3 * (double)ms / reps;
+}
+
+extern "C" int ofd_autotune(ofd_ctx* c, int reps, ofd_tune_result* res) {
+  if (!c) return -1;
+  if (reps < 1) reps = 5;
+  try {
+    const long long n = c->nr();
+    std::vector<double> h(n, c->num_ele / c->vol);
+    HIPCHK(hipMemcpy(c->rho, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    graphs_clear(c->graphs);
+    const bool fast = !c->sp.force_generic && fast_path_ok(c->sp, c->kind);
+    const int fp32 = c->precision == 1;
+    std::vector<int> cy, cz, cx, cp;
+    if (fast) {
+      cy = fast_tile_candidates(c->sp, c->kind, fp32, 1);
+      cz = fast_tile_candidates(c->sp, c->kind, fp32, 2);
+      cx = {0};
+      if (fast_plane_available(c->sp, c->kind, fp32)) cp = {0, 1};
+      else cp = {1};
+    } else {
+      cy = {1, 2, 4, 8, 16};
+      cz = {1, 2, 4, 8};
+      cx = {1, 2, 4, 8, 16};
+      cp = {1};
+    }
+    cy.insert(cy.begin(), 0);
+    cz.insert(cz.begin(), 0);
+    ShapePlan& sp = c->sp;
+    sp.rows_x = 0; sp.kb_y = 0; sp.kb_z = 0; sp.no_plane = false;
+    if (cp.size() == 1) sp.no_plane = cp[0] != 0;
+    const double t_def = time_current(c, reps);
+    double best = t_def;
+    int tried = 1;
+    // coordinate descent, one dimension at a time
+    int cur_plane = sp.no_plane ? 1 : 0, cur_z = 0, cur_y = 0, cur_x = 0;
+    auto run_dim = [&](std::vector<int>& cand, int& cur, auto set) {
+      int bestv = cur;
+      for (int v : cand) {
+        if (v == cur) continue;
+        set(v);
+        const double t = time_current(c, reps);
+        ++tried;
+        if (t < best * 0.995) { best = t; bestv = v; }
+      }
+      cur = bestv;
+      set(bestv);
+    };
+    run_dim(cp, cur_plane, [&](int v) { sp.no_plane = v != 0; });
+    run_dim(cz, cur_z, [&](int v) { sp.kb_z = v; });
+    run_dim(cy, cur_y, [&](int v) { sp.kb_y = v; });
+    if (!fast) run_dim(cx, cur_x, [&](int v) { sp.rows_x = v; });
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tuned[TuneKey{sp.d.n1, sp.d.n2, sp.d.n3, c->kind, fp32}] = TuneChoice{sp.rows_x, sp.kb_y, sp.kb_z,
+                                                                             sp.no_plane ? 1 : 0};
+    }
+    graphs_clear(c->graphs);
+    std::memset(&c->last_desc, 0, sizeof(c->last_desc));
+    if (res) {
+      res->rows_x = sp.rows_x;
+      res->kb_y = sp.kb_y;
+      res->kb_z = sp.kb_z;
+      res->plane = sp.no_plane ? 0 : 1;
+      res->configs_tried = tried;
+      res->us_default = t_def;
+      res->us_best = best;
+    }
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+  return 0;
+}
+
 // ====================================================================== request pool
 struct ofd_pool {
   std::string err;
@@ -1334,6 +1541,7 @@ struct ofd_pool {
   std::vector<ReqDesc> h_desc;
   std::vector<double> h_sums;
   BatchScratch bs;
+  GraphCache graphs;
 };
 
 extern "C" long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int precision) {
@@ -1376,6 +1584,7 @@ extern "C" void ofd_pool_destroy(ofd_pool* p) {
       ofd_destroy(s.c);
     }
   p->bs.free_all();
+  graphs_clear(p->graphs);
   hipFree(p->d_desc);
   hipFree(p->d_sums);
   hipStreamDestroy(p->st);
@@ -1458,11 +1667,9 @@ extern "C" int ofd_pool_step(ofd_pool* p) {
           where[g[k]] = off + (int)k;
         }
         HIPCHK(hipMemcpy(p->d_desc + off, &p->h_desc[off], sizeof(ReqDesc) * g.size(), hipMemcpyHostToDevice));
-        if (run_eval(c0->sp, c0->kind, kInPhi, om, pr, p->d_desc + off, (int)g.size(), p->st))
+        if (launch_eval(p->graphs, c0->sp, c0->kind, kInPhi, om, pr, p->d_desc + off, (int)g.size(),
+                        p->d_sums + (size_t)off * kNSum, p->st))
           throw std::runtime_error("batched evaluation launch failed");
-        if (run_finalize(c0->sp, c0->kind, pr, p->d_desc + off, (int)g.size(), p->d_sums + (size_t)off * kNSum,
-                         p->st))
-          throw std::runtime_error("batched finalize failed");
         off += (int)g.size();
       }
     if (off > 0) {

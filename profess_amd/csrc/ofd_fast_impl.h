@@ -1,0 +1,733 @@
+// Register-resident two-stage FFT passes for power-of-two lines (n = 16..256).
+//
+// A line of n = R1*R2 complex points is owned by T = R2 threads; thread t
+// holds the R1 points t + T*m (m = 0..R1-1) in registers. Stage 1 is one
+// radix-R1 butterfly per thread (no twiddles), the results cross lanes once
+// through a padded LDS buffer, stage 2 is R1/R2 radix-R2 butterflies per
+// thread. After stage 2 thread t again holds the points t + T*m' of the
+// (autosorted) transform, so a pointwise G-space operator and the inverse
+// transform of the same line run without touching memory in between.
+//
+// Kernels (the fast path of the five evaluation passes, Wang-Teter functionals):
+//   k_fx_fwd  rows along x: fields from rho, half-length complex FFT, R2C split
+//   k_fy      columns along y, flattened over (i3, k1)
+//   k_fz_gop  columns along z, flattened over (i2, k1): FFT, diagonal G-space
+//             multiplier per field, inverse FFT
+//   k_fx_inv  rows along x: C2R merge, inverse FFT, potential epilogue and
+//             per-workgroup energy sums
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+
+#include "ofd_internal.h"
+#include "ofd_point.h"
+
+namespace ofd {
+
+namespace fast {
+
+constexpr double kPi = 3.141592653589793238462643;
+constexpr double kCTF = 2.87123400018819;
+
+template <int N> struct P2;
+template <> struct P2<16> { static constexpr int R1 = 4, R2 = 4; };
+template <> struct P2<32> { static constexpr int R1 = 8, R2 = 4; };
+template <> struct P2<64> { static constexpr int R1 = 8, R2 = 8; };
+template <> struct P2<128> { static constexpr int R1 = 16, R2 = 8; };
+template <> struct P2<256> { static constexpr int R1 = 16, R2 = 16; };
+
+template <int N> __device__ __forceinline__ constexpr int padded(int j) { return j + j / P2<N>::R1; }
+__device__ __forceinline__ int t_plus(int t, int T, int m) { return t + T * m; }
+template <int N> constexpr int line_lds() { return N + N / P2<N>::R1; }
+// Line stride of the column passes (y, z), whose consecutive lanes sit on
+// consecutive columns: an odd number of 16-byte slots puts the eight lanes
+// served together by one LDS cycle on distinct banks (an even multiple of the
+// bank period would make it an 8-way conflict on every exchange).
+template <int N> constexpr int col_stride() { return line_lds<N>() | 1; }
+
+// Two-stage FFT of one line; x[m] holds point t + T*m on entry and exit.
+// `lds` is this line's padded buffer. Contains two workgroup barriers.
+template <int N, int S, typename C>
+__device__ __forceinline__ void fft2(C (&x)[P2<N>::R1], C* __restrict__ lds, int t, const C* __restrict__ tw,
+                                     bool active) {
+  constexpr int R1 = P2<N>::R1, R2 = P2<N>::R2, T = R2, Q = R1 / R2;
+  if (active) {
+    Bfly<R1, S, C>::run(x);
+#pragma unroll
+    for (int m = 0; m < R1; ++m) lds[padded<N>(t * R1 + m)] = x[m];
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b2 = t + T * q;
+      C y[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) y[m] = lds[padded<N>(b2 + m * R1)];
+#pragma unroll
+      for (int m = 1; m < R2; ++m) {
+        C w = tw[m * b2];
+        if (S > 0) w.y = -w.y;
+        y[m] = cmul(y[m], w);
+      }
+      Bfly<R2, S, C>::run(y);
+#pragma unroll
+      for (int m = 0; m < R2; ++m) x[q + Q * m] = y[m];
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------ pointwise
+template <int KIND> struct NF_;
+template <> struct NF_<kWT3> { static constexpr int v = 3; };
+template <> struct NF_<kWT4> { static constexpr int v = 4; };
+
+template <int KIND>
+__device__ __forceinline__ void fields(const ReqDesc& D, double rho, double* f) {
+  double ra, rb;
+  fwd_pows(rho, D.alpha, D.beta, ra, rb);
+  f[0] = rho;
+  f[1] = rb;
+  if (KIND == kWT3) {
+    f[2] = sqrt(rho);
+  } else {
+    f[2] = ra;
+    f[3] = sqrt(rho);
+  }
+}
+
+template <int INMODE>
+__device__ __forceinline__ double load_rho(const ReqDesc& D, size_t g, double& tphi) {
+  if (INMODE == kInRho) {
+    tphi = 0.0;
+    return D.u[g];
+  }
+  tphi = D.cb != 0.0 ? fma(D.cb, D.v[g], D.ca * D.u[g]) : D.ca * D.u[g];
+  return tphi * tphi;
+}
+
+// Fixed-order workgroup sum: wave butterflies, then the waves in order.
+// out[k * stride] = sum over the workgroup of acc[k].
+template <int N>
+__device__ __forceinline__ void block_sum_strided(double (&acc)[N], double* out, int stride) {
+  __shared__ double red[16][N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    acc[k] = v;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) red[wv][k] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[w][threadIdx.x];
+    out[(size_t)threadIdx.x * stride] = s;
+  }
+}
+
+// ------------------------------------------------------------ x forward (R2C)
+// blockDim = RPB * T threads; thread = (row r, lane t), t fastest.
+template <int M, int KIND, int INMODE, typename R>
+__global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ descs, GridDims d,
+                                                const typename V2<R>::T* __restrict__ tw,
+                                                const typename V2<R>::T* __restrict__ tws, int rpb) {
+  using C = typename V2<R>::T;
+  constexpr int R1 = P2<M>::R1, T = P2<M>::R2, NF = NF_<KIND>::v;
+  constexpr int LL = line_lds<M>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.y];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const long long nrows = (long long)d.n2 * d.n3;
+  const long long row = (long long)blockIdx.x * rpb + r;
+  const bool active = row < nrows;
+  C z[NF][R1];
+  if (active) {
+    const size_t base = (size_t)row * (2 * M);
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const int k = t + T * m;
+      double a, b;
+      {
+        const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
+        if (INMODE == kInRho) {
+          a = uu.x;
+          b = uu.y;
+        } else {
+          double ta = D.ca * uu.x, tb = D.ca * uu.y;
+          if (D.cb != 0.0) {
+            const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
+            ta = fma(D.cb, vv.x, ta);
+            tb = fma(D.cb, vv.y, tb);
+          }
+          a = ta * ta;
+          b = tb * tb;
+        }
+      }
+      double fa[NF], fb[NF];
+      fields<KIND>(D, a, fa);
+      fields<KIND>(D, b, fb);
+#pragma unroll
+      for (int j = 0; j < NF; ++j) {
+        z[j][m].x = (R)fa[j];
+        z[j][m].y = (R)fb[j];
+      }
+    }
+  }
+  C* mine = lds + (size_t)r * NF * LL;
+#pragma unroll
+  for (int j = 0; j < NF; ++j) fft2<M, -1>(z[j], mine + j * LL, t, tw, active);
+  // Z to LDS (unpadded, per field), then the split
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < NF; ++j)
+#pragma unroll
+      for (int m = 0; m < R1; ++m) mine[j * LL + t + T * m] = z[j][m];
+  }
+  __syncthreads();
+  if (active) {
+    C* work = reinterpret_cast<C*>(D.work);
+    const size_t ob = (size_t)row * (M + 1);
+#pragma unroll
+    for (int j = 0; j < NF; ++j) {
+      const C* Z = mine + j * LL;
+#pragma unroll
+      for (int m = 0; m <= R1; ++m) {
+        const int k = m < R1 ? t + T * m : M;
+        if (m == R1 && t != 0) continue;
+        const C zk = Z[k == M ? 0 : k];
+        const C zm = Z[k == 0 ? 0 : M - k];
+        const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
+        const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
+        const C w = tws[k];
+        C x;
+        x.x = er + (w.x * orr - w.y * oi);
+        x.y = ei + (w.x * oi + w.y * orr);
+        work[(size_t)j * d.nc + ob + k] = x;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ fused x + y forward (one plane)
+// One workgroup per (z-plane i3, field f, request). The plane's half
+// spectrum (K1 x n2) of the field stays in LDS between the x R2C and the y
+// FFT: rows of stride S >= max(M + M/R1x, M + 1) complex, n2 + n2/R1y rows so
+// that the y transform can use the padded column layout as its scratch.
+template <int M, int N2> struct PlaneGeo {
+  static constexpr int LLX = M + M / P2<M>::R1;
+  static constexpr int S = LLX > M + 1 ? LLX : M + 1;
+  static constexpr int ROWS = N2 + N2 / P2<N2>::R1;
+  static constexpr int TX = P2<M>::R2, TY = P2<N2>::R2;
+  static constexpr int NTX = N2 * TX, NTY = (M + 1) * TY;
+  static constexpr int THREADS = ((NTX > NTY ? NTX : NTY) + 63) / 64 * 64;
+};
+
+template <int KIND>
+__device__ __forceinline__ double field_of(const ReqDesc& D, int f, double rho) {
+  if (f == 0) return rho;
+  if (f == NF_<KIND>::v - 1) return sqrt(rho);
+  double ra, rb;
+  fwd_pows(rho, D.alpha, D.beta, ra, rb);
+  return (KIND == kWT4 && f == 2) ? ra : rb;
+}
+
+template <int M, int N2, int KIND, int INMODE, typename R>
+__global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ descs, GridDims d,
+                                                    const typename V2<R>::T* __restrict__ twx,
+                                                    const typename V2<R>::T* __restrict__ tws,
+                                                    const typename V2<R>::T* __restrict__ twy) {
+  using C = typename V2<R>::T;
+  using G = PlaneGeo<M, N2>;
+  constexpr int R1X = P2<M>::R1, TX = G::TX, R1Y = P2<N2>::R1, TY = G::TY, S = G::S;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* P = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.z];
+  const int f = blockIdx.y;
+  const int i3 = blockIdx.x;
+  // ---- x step: thread (row i2, lane t)
+  const int row = threadIdx.x / TX, tx = threadIdx.x % TX;
+  const bool ax = row < N2;
+  C z[R1X];
+  if (ax) {
+    const size_t base = ((size_t)i3 * N2 + row) * (2 * M);
+#pragma unroll
+    for (int m = 0; m < R1X; ++m) {
+      const int k = t_plus(tx, TX, m);
+      const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
+      double a, b;
+      if (INMODE == kInRho) {
+        a = uu.x;
+        b = uu.y;
+      } else {
+        double ta = D.ca * uu.x, tb = D.ca * uu.y;
+        if (D.cb != 0.0) {
+          const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
+          ta = fma(D.cb, vv.x, ta);
+          tb = fma(D.cb, vv.y, tb);
+        }
+        a = ta * ta;
+        b = tb * tb;
+      }
+      z[m].x = (R)field_of<KIND>(D, f, a);
+      z[m].y = (R)field_of<KIND>(D, f, b);
+    }
+  }
+  C* myrow = P + (size_t)row * S;
+  fft2<M, -1>(z, myrow, tx, twx, ax);
+  if (ax) {
+#pragma unroll
+    for (int m = 0; m < R1X; ++m) myrow[tx + TX * m] = z[m];
+  }
+  __syncthreads();
+  // split, values first into registers, then written in place
+  C xo[R1X + 1];
+  if (ax) {
+#pragma unroll
+    for (int m = 0; m <= R1X; ++m) {
+      const int k = m < R1X ? tx + TX * m : M;
+      const C zk = myrow[k == M ? 0 : k];
+      const C zm = myrow[k == 0 ? 0 : M - k];
+      const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
+      const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
+      const C w = tws[k];
+      xo[m].x = er + (w.x * orr - w.y * oi);
+      xo[m].y = ei + (w.x * oi + w.y * orr);
+    }
+  }
+  __syncthreads();
+  if (ax) {
+#pragma unroll
+    for (int m = 0; m < R1X; ++m) myrow[tx + TX * m] = xo[m];
+    if (tx == 0) myrow[M] = xo[R1X];
+  }
+  __syncthreads();
+  // ---- y step: thread (column c, lane t), c fastest
+  const int c = threadIdx.x % (M + 1), ty = threadIdx.x / (M + 1);
+  const bool ay = ty < TY;
+  C y[R1Y];
+  if (ay) {
+#pragma unroll
+    for (int m = 0; m < R1Y; ++m) y[m] = P[(size_t)(ty + TY * m) * S + c];
+  }
+  __syncthreads();
+  // the column's padded scratch: element j at row j + j/R1Y
+  {
+    constexpr int R1 = R1Y, R2 = TY, Q = R1 / R2;
+    if (ay) {
+      Bfly<R1, -1, C>::run(y);
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        const int j = ty * R1 + m;
+        P[(size_t)(j + j / R1) * S + c] = y[m];
+      }
+    }
+    __syncthreads();
+    if (ay) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int b2 = ty + TY * q;
+        C w2[R2];
+#pragma unroll
+        for (int m = 0; m < R2; ++m) {
+          const int j = b2 + m * R1;
+          w2[m] = P[(size_t)(j + j / R1) * S + c];
+        }
+#pragma unroll
+        for (int m = 1; m < R2; ++m) w2[m] = cmul(w2[m], twy[m * b2]);
+        Bfly<R2, -1, C>::run(w2);
+#pragma unroll
+        for (int m = 0; m < R2; ++m) y[q + Q * m] = w2[m];
+      }
+    }
+  }
+  if (ay) {
+    C* out = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)i3 * N2 * (M + 1) + c;
+#pragma unroll
+    for (int m = 0; m < R1Y; ++m) out[(size_t)(ty + TY * m) * (M + 1)] = y[m];
+  }
+}
+
+// ------------------------------------------------------------ y columns
+// columns flattened over (i3, k1); blockDim = KB * T, thread = (t, c), c fastest
+template <int N, int S, typename R>
+__global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, GridDims d,
+                                            const typename V2<R>::T* __restrict__ tw, int kb) {
+  using C = typename V2<R>::T;
+  constexpr int R1 = P2<N>::R1, T = P2<N>::R2;
+  constexpr int LL = col_stride<N>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.z];
+  const int c = threadIdx.x % kb, t = threadIdx.x / kb;
+  const long long col = (long long)blockIdx.x * kb + c;
+  const long long ncol = (long long)d.K1 * d.n3;
+  const bool active = col < ncol;
+  const long long i3 = active ? col / d.K1 : 0, k1 = active ? col - i3 * d.K1 : 0;
+  C* base = reinterpret_cast<C*>(D.work) + (size_t)blockIdx.y * d.nc + (size_t)i3 * d.n2 * d.K1 + k1;
+  C x[R1];
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) x[m] = base[(size_t)(t + T * m) * d.K1];
+  }
+  fft2<N, S>(x, lds + (size_t)c * LL, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) base[(size_t)(t + T * m) * d.K1] = x[m];
+  }
+}
+
+// ------------------------------------------------------------ z columns + G-space
+// One field per workgroup (blockIdx.y): for the Wang-Teter layouts the
+// G-space operator is diagonal in the fields (Hartree 4 pi/G^2 on F[rho],
+// K(|G|) on F[rho^b] (and F[rho^a]), G^2 on F[sqrt rho]).
+template <int N, int KIND, typename R>
+__global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
+                                                const typename V2<R>::T* __restrict__ tw, int kb) {
+  using C = typename V2<R>::T;
+  constexpr int R1 = P2<N>::R1, T = P2<N>::R2, NF = NF_<KIND>::v;
+  constexpr int LL = col_stride<N>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.z];
+  const int f = blockIdx.y;
+  const int c = threadIdx.x % kb, t = threadIdx.x / kb;
+  const long long col = (long long)blockIdx.x * kb + c;
+  const long long ncol = (long long)d.K1 * d.n2;
+  const bool active = col < ncol;
+  const int i2 = active ? (int)(col / d.K1) : 0;
+  const int m1 = active ? (int)(col - (long long)i2 * d.K1) : 0;
+  const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
+  const size_t es = (size_t)d.n2 * d.K1;
+  const double invn = 1.0 / (double)d.nr;
+  C* base = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)col;
+  C x[R1];
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) x[m] = base[(size_t)(t + T * m) * es];
+  }
+  fft2<N, -1>(x, lds + (size_t)c * LL, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const int i3 = t + T * m;
+      const int m3 = i3 > N / 2 ? i3 - N : i3;
+      const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
+      const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
+      const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
+      const double q2 = gx * gx + gy * gy + gz * gz;
+      double sc;
+      if (f == 0) sc = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
+      else if (f == NF - 1) sc = q2 * invn;
+      else sc = D.coef * lind_g(sqrt(q2) / D.tkf, D.lam, D.mu) * invn;
+      x[m].x = (R)((double)x[m].x * sc);
+      x[m].y = (R)((double)x[m].y * sc);
+    }
+  }
+  fft2<N, +1>(x, lds + (size_t)c * LL, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) base[(size_t)(t + T * m) * es] = x[m];
+  }
+}
+
+// ------------------------------------------------------------ x inverse (C2R) + epilogue
+template <int KIND>
+__device__ __forceinline__ double epi(const ReqDesc& D, double rho, double vion, const double* o, double (&acc)[kNSum]) {
+  PtVals p;
+  point_vals(rho, D.alpha, D.beta, p);
+  const double r23 = p.r13 * p.r13;
+  const double vtf = (5.0 / 3.0 * kCTF * D.lam) * r23;
+  double vnl, t, enl;
+  if (KIND == kWT3) {
+    vnl = kCTF * (D.alpha * p.ram1 * o[1] + D.beta * p.rbm1 * o[1]);
+    enl = p.ra * o[1];
+    t = o[2];
+  } else {
+    vnl = kCTF * (D.alpha * p.ram1 * o[1] + D.beta * p.rbm1 * o[2]);
+    enl = p.ra * o[1];
+    t = o[3];
+  }
+  const double loc = vion + o[0] + p.vxc + vtf + vnl;
+  acc[0] += rho * vion;
+  acc[1] += o[0] * rho;
+  acc[2] += p.exc;
+  acc[3] += rho * r23;
+  acc[4] += p.phi * t;
+  acc[5] += enl;
+  if (D.opt_sqrt) return 2.0 * p.phi * loc + D.mu * t;
+  return loc + D.mu * t / (2.0 * p.phi);
+}
+
+// Rows per workgroup of the x-inverse pass: RPB * NF lines of T threads,
+// whole wavefronts, at most 256 threads.
+__host__ __device__ constexpr int inv_rpb(int nf, int t) {
+  int r = 256 / (nf * t);
+  while (r > 1 && (r * nf * t) % 64 != 0) --r;
+  return r;
+}
+
+// One workgroup = RPB rows x NF fields; thread (line = row*NF + field, lane t).
+// Every line is merged (C2R), transformed and left in LDS (complex k holds the
+// reals 2k, 2k+1), then all threads run the epilogue over the workgroup's
+// points, consecutive threads on consecutive points.
+template <int M, int KIND, int INMODE, int OUTMODE, typename R>
+__global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ descs, GridDims d,
+                                                const typename V2<R>::T* __restrict__ tw,
+                                                const typename V2<R>::T* __restrict__ tws) {
+  using C = typename V2<R>::T;
+  constexpr int R1 = P2<M>::R1, T = P2<M>::R2, NF = NF_<KIND>::v;
+  constexpr int LL = line_lds<M>();
+  constexpr int RPB = inv_rpb(NF, T);
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.y];
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int r = line / NF, f = line - r * NF;
+  const long long nrows = (long long)d.n2 * d.n3;
+  const long long row0 = (long long)blockIdx.x * RPB;
+  const long long row = row0 + r;
+  const bool active = row < nrows;
+  C* mine = lds + (size_t)line * LL;
+  C z[R1];
+  if (active) {
+    const C* X = reinterpret_cast<const C*>(D.work) + (size_t)f * d.nc + (size_t)row * (M + 1);
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const int k = t + T * m;
+      C xk = X[k], xm = X[M - k];
+      if (k == 0) { xk.y = (R)0; xm.y = (R)0; }
+      const R er = xk.x + xm.x, ei = xk.y - xm.y;
+      const R dr = xk.x - xm.x, di = xk.y + xm.y;
+      const C w = tws[k];
+      const R orr = w.x * dr + w.y * di, oi = w.x * di - w.y * dr;
+      z[m].x = er - oi;
+      z[m].y = ei + orr;
+    }
+  }
+  fft2<M, +1>(z, mine, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) mine[t + T * m] = z[m];
+  }
+  __syncthreads();
+  double acc[kNSum];
+#pragma unroll
+  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
+  const int nrow = (int)min((long long)RPB, nrows - row0);
+  const int npts = nrow * 2 * M;
+  const size_t gbase = (size_t)row0 * 2 * M;
+  const R* lr = reinterpret_cast<const R*>(lds);
+  for (int e = threadIdx.x; e < npts; e += blockDim.x) {
+    const int rr = e / (2 * M), i = e - rr * (2 * M);
+    const size_t g = gbase + e;
+    double o[NF];
+#pragma unroll
+    for (int j = 0; j < NF; ++j) o[j] = (double)lr[2 * (size_t)(rr * NF + j) * LL + i];
+    double tphi;
+    const double rho = load_rho<INMODE>(D, g, tphi);
+    const double pot = epi<KIND>(D, rho, D.vion[g], o, acc);
+    if (OUTMODE == kOutPot) {
+      D.out[g] = pot;
+    } else if (OUTMODE == kOutSigned) {
+      const double de = pot * fsign(tphi);
+      D.out[g] = de;
+      const double dir = D.s1 * D.u[g] + (D.v != nullptr ? D.s2 * D.v[g] : 0.0);
+      acc[6] += de * dir;
+      const double dl = de - 2.0 * tphi * D.mu0;
+      acc[7] += dl * dl;
+      acc[8] += de * tphi;
+    } else {
+      const double de = pot * fsign(tphi);
+      const double ap = ((de - D.mu0 * 2.0 * tphi) - D.bvec[g]) / D.eps;
+      D.out[g] = ap;
+      acc[6] += D.v[g] * ap;
+    }
+  }
+  // partials laid out [slot][workgroup]; the finalize kernel sums each slot
+  // in a fixed order (no device-scope fence or atomic here: on a multi-XCD
+  // part a release fence per workgroup writes back the whole L2)
+  block_sum_strided<kNSum>(acc, D.sums + blockIdx.x, gridDim.x);
+}
+
+}  // namespace fast
+
+// ------------------------------------------------------------ dispatch
+static inline bool is_fast_len(int n) { return n == 32 || n == 64 || n == 128 || n == 256; }
+
+inline bool fast_path_ok_impl(const ShapePlan& sp, int kind) {
+  return (kind == kWT3 || kind == kWT4) && is_fast_len(sp.d.M) && is_fast_len(sp.d.n2) && is_fast_len(sp.d.n3);
+}
+
+struct FastTiles {
+  int rpb, kby, kbz, ntile;
+};
+
+static inline int p2_T(int n) { return n == 16 ? 4 : n == 32 ? 4 : n == 64 ? 8 : n == 128 ? 8 : 16; }
+static inline int p2_R1(int n) { return n == 16 ? 4 : n == 32 ? 8 : n == 64 ? 8 : 16; }
+static inline int p2_ll(int n) { return n + n / p2_R1(n); }
+static inline int p2_cs(int n) { return p2_ll(n) | 1; }
+
+// Tiles of the fast passes: at most 256 threads per workgroup and at most
+// lds_target bytes of LDS (the tuner may override rows_x/kb_y/kb_z).
+static inline FastTiles fast_tiles(const ShapePlan& sp, int nf, int cbytes) {
+  FastTiles ft;
+  const long long tgt = sp.lds_target;
+  const int M = sp.d.M;
+  const int llx = std::max(p2_ll(M), M + 1);
+  // whole wavefronts only: lines per workgroup are multiples of 64/T
+  const int wx = 64 / p2_T(M), wy = 64 / p2_T(sp.d.n2), wz = 64 / p2_T(sp.d.n3);
+  ft.rpb = sp.rows_x > 0 ? sp.rows_x : 256 / p2_T(M);
+  while (ft.rpb > wx && ((long long)ft.rpb * nf * llx * cbytes > tgt || ft.rpb * p2_T(M) > 256)) ft.rpb -= wx;
+  // a tuned value (sp.kb_y/kb_z > 0) is bound by the LDS capacity, a derived
+  // one by the LDS target
+  const long long ty = sp.kb_y > 0 ? sp.lds_cap : tgt, tz = sp.kb_z > 0 ? sp.lds_cap : tgt;
+  ft.kby = sp.kb_y > 0 ? std::max(wy, sp.kb_y / wy * wy) : 256 / p2_T(sp.d.n2);
+  while (ft.kby > wy && ((long long)ft.kby * p2_cs(sp.d.n2) * cbytes > ty || ft.kby * p2_T(sp.d.n2) > 256)) ft.kby -= wy;
+  ft.kbz = sp.kb_z > 0 ? std::max(wz, sp.kb_z / wz * wz) : 256 / p2_T(sp.d.n3);
+  while (ft.kbz > wz && ((long long)ft.kbz * p2_cs(sp.d.n3) * cbytes > tz || ft.kbz * p2_T(sp.d.n3) > 256)) ft.kbz -= wz;
+  const long long nrows = (long long)sp.d.n2 * sp.d.n3;
+  ft.ntile = (int)((nrows + ft.rpb - 1) / ft.rpb);
+  return ft;
+}
+
+template <typename R, int M, int KIND, int INMODE, int OUTMODE>
+static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd, int nb, hipStream_t st, bool fwd,
+                     const typename V2<R>::T* tw, const typename V2<R>::T* tws) {
+  using namespace fast;
+  using C = typename V2<R>::T;
+  constexpr int T = P2<M>::R2, NF = NF_<KIND>::v;
+  const int threads = ft.rpb * T;
+  if (fwd) {
+    size_t lds = (size_t)ft.rpb * NF * line_lds<M>() * sizeof(C);
+    hipLaunchKernelGGL((k_fx_fwd<M, KIND, INMODE, R>), dim3(ft.ntile, nb), dim3(threads), lds, st, dd, sp.d, tw, tws,
+                       ft.rpb);
+  } else {
+    constexpr int RPB = fast::inv_rpb(NF, T);
+    const long long nrows = (long long)sp.d.n2 * sp.d.n3;
+    const int ntile = (int)((nrows + RPB - 1) / RPB);
+    size_t lds = (size_t)RPB * NF * line_lds<M>() * sizeof(C);
+    hipLaunchKernelGGL((k_fx_inv<M, KIND, INMODE, OUTMODE, R>), dim3(ntile, nb), dim3(RPB * NF * T), lds, st, dd,
+                       sp.d, tw, tws);
+  }
+}
+
+template <typename R, int N, int S>
+static void launch_y(const ShapePlan& sp, int kb, int nf, const ReqDesc* dd, int nb, hipStream_t st,
+                     const typename V2<R>::T* tw) {
+  using namespace fast;
+  using C = typename V2<R>::T;
+  const long long ncol = (long long)sp.d.K1 * sp.d.n3;
+  const int nblk = (int)((ncol + kb - 1) / kb);
+  size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
+  hipLaunchKernelGGL((k_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb);
+}
+
+template <typename R, int N, int KIND>
+static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hipStream_t st,
+                     const typename V2<R>::T* tw) {
+  using namespace fast;
+  using C = typename V2<R>::T;
+  const long long ncol = (long long)sp.d.K1 * sp.d.n2;
+  const int nblk = (int)((ncol + kb - 1) / kb);
+  size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
+  hipLaunchKernelGGL((k_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d,
+                     tw, kb);
+}
+
+template <typename R, int M, int N2, int KIND, int INMODE>
+static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st,
+                         const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
+  using namespace fast;
+  using C = typename V2<R>::T;
+  using G = PlaneGeo<M, N2>;
+  const size_t lds = (size_t)G::ROWS * G::S * sizeof(C);
+  if (lds > (size_t)sp.lds_cap || G::THREADS > 1024) return false;
+  auto kern = k_plane_fwd<M, N2, KIND, INMODE, R>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(sp.d.n3, NF_<KIND>::v, nb), dim3(G::THREADS), lds, st, dd, sp.d, twx, tws, twy);
+  return true;
+}
+
+template <typename R, int KIND, int INMODE>
+static bool try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st, const typename V2<R>::T* twx,
+                      const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
+  if (sp.no_plane) return false;
+  if (sp.d.M == 64 && sp.d.n2 == 128) return launch_plane<R, 64, 128, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  if (sp.d.M == 32 && sp.d.n2 == 64) return launch_plane<R, 32, 64, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  return false;
+}
+
+template <typename R, int KIND, int INMODE, int OUTMODE, int M>
+static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd, int nb, hipStream_t st,
+                       const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy,
+                       const typename V2<R>::T* twz) {
+  constexpr int NF = fast::NF_<KIND>::v;
+  auto ylaunch = [&](auto sgn) {
+    constexpr int S = decltype(sgn)::value;
+    switch (sp.d.n2) {
+      case 32: launch_y<R, 32, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 64: launch_y<R, 64, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 128: launch_y<R, 128, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      default: launch_y<R, 256, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+    }
+  };
+  if (!try_plane<R, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy)) {
+    launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, true, twx, tws);
+    ylaunch(std::integral_constant<int, -1>());
+  }
+  switch (sp.d.n3) {
+    case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 64: launch_z<R, 64, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 128: launch_z<R, 128, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    default: launch_z<R, 256, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+  }
+  ylaunch(std::integral_constant<int, +1>());
+  launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, false, twx, tws);
+}
+
+template <typename R, int KIND, int INMODE, int OUTMODE>
+static int fast_eval_t(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st) {
+  const FastTiles ft = fast_tiles(sp, fast::NF_<KIND>::v, (int)sizeof(typename V2<R>::T));
+  const bool f32 = sizeof(R) == 4;
+  const typename V2<R>::T *twx, *tws, *twy, *twz;
+  if constexpr (sizeof(R) == 4) {
+    twx = sp.tw_x32; tws = sp.tw_s32; twy = sp.tw_y32; twz = sp.tw_z32;
+  } else {
+    twx = sp.tw_x; tws = sp.tw_s; twy = sp.tw_y; twz = sp.tw_z;
+  }
+  (void)f32;
+  switch (sp.d.M) {
+    case 32: fast_chain<R, KIND, INMODE, OUTMODE, 32>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    case 64: fast_chain<R, KIND, INMODE, OUTMODE, 64>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    case 128: fast_chain<R, KIND, INMODE, OUTMODE, 128>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    default: fast_chain<R, KIND, INMODE, OUTMODE, 256>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename R, int KIND>
+int fast_eval_k(const ShapePlan& sp, int inmode, int outmode, const ReqDesc* dd, int nb, hipStream_t st) {
+  if (inmode == kInRho) return fast_eval_t<R, KIND, kInRho, kOutPot>(sp, dd, nb, st);
+  if (outmode == kOutPot) return fast_eval_t<R, KIND, kInPhi, kOutPot>(sp, dd, nb, st);
+  if (outmode == kOutSigned) return fast_eval_t<R, KIND, kInPhi, kOutSigned>(sp, dd, nb, st);
+  return fast_eval_t<R, KIND, kInPhi, kOutNewton>(sp, dd, nb, st);
+}
+
+}  // namespace ofd

@@ -46,7 +46,6 @@ struct ReqDesc {
   double* sums;         // kNSum partials per epilogue workgroup
   const double* wgck;   // WGC kernels K1..K4 (4 x Nc doubles), kWGC8 only
   double* fin;          // kNSum finalized sums of this evaluation (device)
-  unsigned int* cnt;    // arrival counter of the epilogue workgroups (zeroed, self-resetting)
   void* work;           // complex work buffer, NF fields x Nc
   double ca, cb;        // phi_t = ca*u + cb*v
   double s1, s2;        // kOutSigned direction weights
@@ -127,6 +126,13 @@ void vec_square(double* y, const double* x, long long n, hipStream_t st);       
 void vec_dots(const double* const* a, const double* const* b, int npairs, long long n, double* d_partial,
               double* d_out, hipStream_t st);
 void vec_scale_sign(double* y, const double* x, const double* s, long long n, hipStream_t st);  // y = x*sign(s)
+
+// Tile candidates of the fast passes for the autotuner: columns per
+// workgroup of the y pass (which = 1) or the z pass (which = 2), whole
+// wavefronts within the LDS limit.
+std::vector<int> fast_tile_candidates(const ShapePlan& sp, int kind, int fp32, int which);
+// Whether the fused x+y plane pass exists for this shape and fits in LDS.
+bool fast_plane_available(const ShapePlan& sp, int kind, int fp32);
 
 // Batched vector operations over many requests of one grid size.
 enum VecOpKind : int { kOpLincomb = 0, kOpAxpby = 1, kOpZero = 2 };

@@ -21,89 +21,12 @@
 #include <cmath>
 
 #include "ofd_internal.h"
+#include "ofd_point.h"
 
 namespace ofd {
 
 constexpr double kPi = 3.141592653589793238462643;
 constexpr double kCTF = 2.87123400018819;   // KEDF_TF.f90:42
-
-// ------------------------------------------------------------ pointwise math
-// LDA Perdew-Zunger, numSpin = 1 (XC_LDA.f90:199-310 potential, :51-196 energy)
-__device__ __forceinline__ double lda_pot(double r13) {
-  const double cX = -0.98474502184269641, cC = 0.62035049089940009;
-  const double a1 = 3.11e-2, cC1 = -5.83666666666666666e-2, cC2 = 1.33333333333333333e-3, cC3 = -8.4e-3;
-  const double g1 = -1.423e-1, cC4 = -0.174798948333333333, cC5 = -6.325709333333333333e-2;
-  const double b1 = 1.0529, b2 = 3.334e-1;
-  double rs = cC / r13;
-  if (rs < 1.0) return cX * r13 + log(rs) * (a1 + cC2 * rs) + cC1 + cC3 * rs;
-  double sq = sqrt(rs);
-  double den = 1.0 + b1 * sq + b2 * rs;
-  return cX * r13 + (g1 + cC4 * sq + cC5 * rs) / (den * den);
-}
-__device__ __forceinline__ double lda_edens(double rho, double r13) {
-  const double cX = -0.73855876638202234, cC = 0.62035049089940009;
-  const double a = 3.11e-2, b = -4.8e-2, c = 2e-3, dd = -1.16e-2, g = -1.423e-1, b1 = 1.0529, b2 = 3.334e-1;
-  double rs = cC / r13;
-  double ec;
-  if (rs < 1.0) ec = log(rs) * (a + c * rs) + b + dd * rs;
-  else ec = g / (1.0 + b1 * sqrt(rs) + b2 * rs);
-  return cX * rho * r13 + rho * ec;
-}
-
-// LindG (MathFunctions.f90:440-528)
-__device__ __forceinline__ double lind_g(double eta, double lam, double mu) {
-  if (eta < 0.0) return 0.0;
-  if (eta < 1e-10) return 1.0 - lam + eta * eta * (1.0 / 3.0 - 3.0 * mu);
-  if (fabs(eta - 1.0) < 1e-10) return 2.0 - lam - 3.0 * mu + 20.0 * (eta - 1.0);
-  if (eta > 3.65) {
-    double e2 = eta * eta, i2 = 1.0 / e2;
-    double acc = -3.00624946457977689e-3;
-    acc = -3.38745061493813488e-3 + i2 * acc;
-    acc = -3.84763737842981233e-3 + i2 * acc;
-    acc = -4.41060829979912465e-3 + i2 * acc;
-    acc = -5.10997527675418131e-3 + i2 * acc;
-    acc = -5.9945117538835746e-3 + i2 * acc;
-    acc = -7.1372762502456763e-3 + i2 * acc;
-    acc = -8.65192783339199453e-3 + i2 * acc;
-    acc = -1.07236045520990083e-2 + i2 * acc;
-    acc = -1.36715733124818332e-2 + i2 * acc;
-    acc = -1.80879839616166146e-2 + i2 * acc;
-    acc = -2.51824061652633074e-2 + i2 * acc;
-    acc = -3.77825602968460128e-2 + i2 * acc;
-    acc = -6.39999999999999875e-2 + i2 * acc;
-    acc = -0.13714285714285712 + i2 * acc;
-    return 3.0 * (1.0 - mu) * e2 - lam - 0.6 + i2 * acc;
-  }
-  return 1.0 / (0.5 + 0.25 * (1.0 - eta * eta) * log((1.0 + eta) / fabs(1.0 - eta)) / eta) - 3.0 * mu * eta * eta -
-         lam;
-}
-
-__device__ __forceinline__ double fsign(double x) { return signbit(x) ? -1.0 : 1.0; }
-
-// rho^p with a fast exact-enough path for p = 5/6 (the WT default)
-struct Pows {
-  double rho, phi, r13;
-  double ra, ram1;   // rho^alpha, rho^(alpha-1)
-  double rb, rbm1;   // rho^beta,  rho^(beta-1)
-};
-__device__ __forceinline__ void make_pows(double rho, double alpha, double beta, Pows& p) {
-  p.rho = rho;
-  p.phi = sqrt(rho);
-  p.r13 = cbrt(rho);
-  const double f56 = 5.0 / 6.0;
-  if (alpha == f56 && beta == f56) {
-    double r16 = sqrt(p.r13);
-    p.ra = rho / r16;
-    p.ram1 = 1.0 / r16;
-    p.rb = p.ra;
-    p.rbm1 = p.ram1;
-  } else {
-    p.ra = pow(rho, alpha);
-    p.ram1 = pow(rho, alpha - 1.0);
-    p.rb = pow(rho, beta);
-    p.rbm1 = pow(rho, beta - 1.0);
-  }
-}
 
 template <int KIND> struct NFields;
 template <> struct NFields<kWT3> { static constexpr int v = 3; };
@@ -115,17 +38,18 @@ template <> struct NFields<kPlain> { static constexpr int v = 1; };
 template <int KIND>
 __device__ __forceinline__ void gen_fields(const ReqDesc& D, double rho, double* f) {
   if (KIND == kPlain) { f[0] = rho; return; }
-  Pows p;
-  make_pows(rho, D.alpha, D.beta, p);
+  double ra, rb;
+  fwd_pows(rho, D.alpha, D.beta, ra, rb);
+  const double phi = sqrt(rho);
   if (KIND == kWT3) {
-    f[0] = rho; f[1] = p.rb; f[2] = p.phi;
+    f[0] = rho; f[1] = rb; f[2] = phi;
   } else if (KIND == kWT4) {
-    f[0] = rho; f[1] = p.rb; f[2] = p.ra; f[3] = p.phi;
+    f[0] = rho; f[1] = rb; f[2] = ra; f[3] = phi;
   } else {
     double th = rho - D.rhoS;
-    f[0] = rho; f[1] = p.phi;
-    f[2] = p.ra; f[3] = th * p.ra; f[4] = th * th * p.ra / 2.0;
-    f[5] = p.rb; f[6] = th * p.rb; f[7] = th * th * p.rb / 2.0;
+    f[0] = rho; f[1] = phi;
+    f[2] = ra; f[3] = th * ra; f[4] = th * th * ra / 2.0;
+    f[5] = rb; f[6] = th * rb; f[7] = th * th * rb / 2.0;
   }
 }
 
@@ -344,9 +268,9 @@ __global__ void __launch_bounds__(256) k_zgop(const ReqDesc* __restrict__ descs,
 template <int KIND>
 __device__ __forceinline__ double epi_point(const ReqDesc& D, double rho, double vion, const double* o,
                                             double (&acc)[kNSum]) {
-  Pows p;
-  make_pows(rho, D.alpha, D.beta, p);
-  const double vxc = lda_pot(p.r13);
+  PtVals p;
+  point_vals(rho, D.alpha, D.beta, p);
+  const double vxc = p.vxc;
   const double r23 = p.r13 * p.r13;
   const double vtf = (5.0 / 3.0 * kCTF * D.lam) * r23;
   double vnl, t, enl;
@@ -369,7 +293,7 @@ __device__ __forceinline__ double epi_point(const ReqDesc& D, double rho, double
   const double loc = vion + vh + vxc + vtf + vnl;
   acc[0] += rho * vion;
   acc[1] += vh * rho;
-  acc[2] += lda_edens(rho, p.r13);
+  acc[2] += p.exc;
   acc[3] += rho * r23;
   acc[4] += p.phi * t;
   acc[5] += enl;

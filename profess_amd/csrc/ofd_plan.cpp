@@ -89,6 +89,8 @@ int shape_plan_init(ShapePlan* sp, int n1, int n2, int n3, std::string* err) {
   }
   if (const char* e = std::getenv("OFD_GENERIC")) sp->force_generic = std::atoi(e) != 0;
   if (const char* e = std::getenv("OFD_NO_PLANE")) sp->no_plane = std::atoi(e) != 0;
+  if (const char* e = std::getenv("OFD_KB_Y")) sp->kb_y = std::atoi(e);
+  if (const char* e = std::getenv("OFD_KB_Z")) sp->kb_z = std::atoi(e);
   const size_t cb = sizeof(double2);
   if (2ull * 8 * (d.M + 1) * cb > (size_t)sp->lds_cap || 2ull * 8 * (n3 + 1) * cb > (size_t)sp->lds_cap ||
       2ull * (n2 + 1) * cb > (size_t)sp->lds_cap) {

@@ -1,0 +1,99 @@
+// Pointwise math of the evaluation passes, shared by the generic and the
+// fast kernels:
+//   LDA Perdew-Zunger, numSpin = 1: potential XC_LDA.f90:199-310, energy
+//   density XC_LDA.f90:51-196 (the constants below are the formula's);
+//   LindG, MathFunctions.f90:440-528 (including its 15 Taylor terms for
+//   eta > 3.65); the rho powers of the Wang-Teter terms.
+// rho^(1/3), rho^(-1/3), rs and sqrt(rs) are computed once per point and
+// shared between the kinetic and the exchange-correlation terms.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ofd {
+
+struct PtVals {
+  double phi;          // rho^(1/2)
+  double r13;          // rho^(1/3)
+  double ra, ram1;     // rho^alpha, rho^(alpha-1)
+  double rb, rbm1;     // rho^beta,  rho^(beta-1)
+  double vxc;          // LDA potential
+  double exc;          // LDA energy density times rho
+};
+
+__device__ __forceinline__ void point_vals(double rho, double alpha, double beta, PtVals& p) {
+  constexpr double cC = 0.62035049089940009;       // (3/(4 pi))^(1/3)
+  constexpr double sqrt_cC = 0.78762331789974330;  // sqrt(cC)
+  constexpr double cXp = -0.98474502184269641, cXe = -0.73855876638202234;
+  p.phi = sqrt(rho);
+  p.r13 = cbrt(rho);
+  const double i13 = 1.0 / p.r13;
+  const double rs = cC * i13;
+  double sq;
+  if (alpha == 5.0 / 6.0 && beta == 5.0 / 6.0) {
+    const double i16 = sqrt(i13);   // rho^(-1/6)
+    p.ram1 = i16;
+    p.ra = p.phi * p.r13;           // rho^(5/6) = rho^(1/2) rho^(1/3)
+    p.rb = p.ra;
+    p.rbm1 = i16;
+    sq = sqrt_cC * i16;
+  } else {
+    p.ra = pow(rho, alpha);
+    p.ram1 = pow(rho, alpha - 1.0);
+    p.rb = pow(rho, beta);
+    p.rbm1 = pow(rho, beta - 1.0);
+    sq = sqrt(rs);
+  }
+  double ec;
+  if (rs < 1.0) {
+    const double lg = log(rs);
+    p.vxc = cXp * p.r13 + lg * (3.11e-2 + 1.33333333333333333e-3 * rs) - 5.83666666666666666e-2 - 8.4e-3 * rs;
+    ec = lg * (3.11e-2 + 2e-3 * rs) - 4.8e-2 - 1.16e-2 * rs;
+  } else {
+    const double iden = 1.0 / (1.0 + 1.0529 * sq + 3.334e-1 * rs);
+    p.vxc = cXp * p.r13 + (-1.423e-1 - 0.174798948333333333 * sq - 6.325709333333333333e-2 * rs) * iden * iden;
+    ec = -1.423e-1 * iden;
+  }
+  p.exc = cXe * rho * p.r13 + rho * ec;
+}
+
+// rho^alpha and rho^beta for the forward fields
+__device__ __forceinline__ void fwd_pows(double rho, double alpha, double beta, double& ra, double& rb) {
+  if (alpha == 5.0 / 6.0 && beta == 5.0 / 6.0) {
+    ra = sqrt(rho) * cbrt(rho);
+    rb = ra;
+  } else {
+    ra = pow(rho, alpha);
+    rb = pow(rho, beta);
+  }
+}
+
+__device__ __forceinline__ double lind_g(double eta, double lam, double mu) {
+  if (eta < 0.0) return 0.0;
+  if (eta < 1e-10) return 1.0 - lam + eta * eta * (1.0 / 3.0 - 3.0 * mu);
+  if (fabs(eta - 1.0) < 1e-10) return 2.0 - lam - 3.0 * mu + 20.0 * (eta - 1.0);
+  if (eta > 3.65) {
+    const double e2 = eta * eta, i2 = 1.0 / e2;
+    double acc = -3.00624946457977689e-3;
+    acc = -3.38745061493813488e-3 + i2 * acc;
+    acc = -3.84763737842981233e-3 + i2 * acc;
+    acc = -4.41060829979912465e-3 + i2 * acc;
+    acc = -5.10997527675418131e-3 + i2 * acc;
+    acc = -5.9945117538835746e-3 + i2 * acc;
+    acc = -7.1372762502456763e-3 + i2 * acc;
+    acc = -8.65192783339199453e-3 + i2 * acc;
+    acc = -1.07236045520990083e-2 + i2 * acc;
+    acc = -1.36715733124818332e-2 + i2 * acc;
+    acc = -1.80879839616166146e-2 + i2 * acc;
+    acc = -2.51824061652633074e-2 + i2 * acc;
+    acc = -3.77825602968460128e-2 + i2 * acc;
+    acc = -6.39999999999999875e-2 + i2 * acc;
+    acc = -0.13714285714285712 + i2 * acc;
+    return 3.0 * (1.0 - mu) * e2 - lam - 0.6 + i2 * acc;
+  }
+  return 1.0 / (0.5 + 0.25 * (1.0 - eta * eta) * log((1.0 + eta) / fabs(1.0 - eta)) / eta) - 3.0 * mu * eta * eta -
+         lam;
+}
+
+__device__ __forceinline__ double fsign(double x) { return signbit(x) ? -1.0 : 1.0; }
+
+}  // namespace ofd

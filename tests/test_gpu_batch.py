@@ -71,3 +71,36 @@ def test_mixed_precision_solve_within_1e5_ev_per_atom(pa):
     de = abs(out[1][0][0] - out[0][0][0]) * O.HARTREE_TO_EV / natom
     assert de < 1e-5, de
     assert out[1][1]["fp32_evals"] > 0
+
+
+def test_graph_replays_match_eager(pa, monkeypatch):
+    """A density optimization replays the captured evaluation graphs many
+    times with eager vector work and synchronizations between the replays;
+    every result must equal the eager (uncaptured) run bit for bit."""
+    res = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("OFD_NO_GRAPH", mode)
+        s, rho = _system(pa, (32, 32, 32), 11)
+        r, et, oc, st = pa.ProjNormRhoMinimization(s, rho, conv_check="POT", pot_tol=1e-6)
+        e2 = [pa.CalculatePotentialPlus(s, r)[1] for _ in range(3)]
+        res[mode] = (et, r, st["num_evals"], e2)
+        s.close()
+    assert res["0"][2] == res["1"][2] and res["0"][2] > 10
+    assert np.array_equal(res["0"][0], res["1"][0])
+    assert np.array_equal(res["0"][1], res["1"][1])
+    for a, b in zip(res["0"][3], res["1"][3]):
+        assert np.array_equal(a, b)
+
+
+def test_autotune_keeps_results(pa):
+    """Tuned tiles change launch shapes only: energies stay within 1e-12."""
+    s, rho = _system(pa, (64, 64, 64), 4)
+    pot0, et0 = pa.CalculatePotentialPlus(s, rho)
+    r = s.autotune(reps=3)
+    assert r["configs_tried"] >= 3 and r["us_best"] <= r["us_default"] * 1.0001
+    pot1, et1 = pa.CalculatePotentialPlus(s, rho)
+    np.testing.assert_allclose(et1, et0, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(pot1, pot0, rtol=1e-12, atol=1e-12 * np.abs(pot0).max())
+    s2, _ = _system(pa, (64, 64, 64), 5)   # a new system of the same shape starts tuned
+    pot2, et2 = pa.CalculatePotentialPlus(s2, rho)
+    assert np.all(np.isfinite(et2))
