@@ -688,6 +688,7 @@ struct NtnSolver {
   double pot_norm = 0;
   bool fp32 = false;
   bool force64 = false;   // set when an FP32 phase stalls: the rest of the run is FP64
+  bool trace = false;     // OFD_TRACE=1: one stderr line per accepted step
   double mach = 2.220446049250313e-16;
   ofd_opt_stats stats{};
   std::vector<double> hist_energy, hist_potnorm;
@@ -717,9 +718,11 @@ struct NtnSolver {
     need = Need();
     st = ST_DONE;
   }
-  // An FP32 phase that stalls (checkGradient gives up, or the line search
-  // fails) does not end the run: re-evaluate at the current phi in FP64 and
-  // continue from there as the reference would from this density.
+  // Ends the FP32 phase for good: re-evaluate at the current phi in FP64 and
+  // continue from there as the reference would from this density. Taken when
+  // the potential norm reaches the switch threshold, and when an FP32 phase
+  // stalls (checkGradient gives up, the line search fails or runs long, or a
+  // step does not lower the energy).
   void fall_back_fp64() {
     force64 = true;
     fp32 = false;
@@ -830,6 +833,12 @@ struct NtnSolver {
     pot_norm = std::sqrt(last_sums[7] / (double)n);
     hist_energy.push_back(energy[0]);
     hist_potnorm.push_back(pot_norm);
+    if (trace) {
+      // one line per accepted step, the columns of the reference's RhoOptN
+      // iteration table (RhoOptN.f90:595-627) plus the precision
+      std::fprintf(stderr, "ofd-ntn %p it %4d %s E %.12f potnorm %.4e theta %.4e nline %3d newton %2d flag %2d ok %2d\n",
+                   (void*)c, it, fp32 ? "fp32" : "fp64", energy[0], pot_norm, theta, nline, nit, ntflag, success);
+    }
     bool exit_flag = false;
     const bool pconv = pot_norm < prm.pot_tol;
     e_hist[2] = e_hist[1];
@@ -867,7 +876,15 @@ struct NtnSolver {
     }
     const bool was32 = fp32;
     set_precision_for(pot_norm);
-    if (was32 && !fp32) e_hist[0] = e_hist[1] = e_hist[2] = 0.0;   // FP32 energies never count toward ENE
+    if (was32 && !fp32) {
+      // FP64 refinement starts from an FP64 evaluation at the accepted
+      // density: the line search must not compare FP64 energies against an
+      // FP32 one (a baseline a little too low makes it return theta = 0,
+      // after which theta = MIN(theta, ...) can never grow again)
+      ++it;
+      fall_back_fp64();
+      return;
+    }
     ++it;
     st = ST_ACC_DOTS;
     need_dots(2, c->dldphi, c->dldphi, c->phi, c->phi);
@@ -1118,6 +1135,7 @@ double dsum(ofd_ctx* c, const double* x, double* ones, BatchScratch& bs, hipStre
 void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchScratch& bs, hipStream_t st) {
   S = NtnSolver();
   S.c = c;
+  if (const char* e = std::getenv("OFD_TRACE")) S.trace = std::atoi(e) != 0;
   fill_params(prm, &S.prm);
   const long long n = c->nr();
   std::vector<double> ones(n, 1.0);

@@ -46,7 +46,7 @@ std::vector<int> fast_tile_candidates(const ShapePlan& sp, int kind, int fp32, i
   const int cb = fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
   std::vector<int> out;
   for (int kb = w; kb * T <= 256; kb += w)
-    if ((long long)kb * p2_cs(n) * cb <= sp.lds_cap) out.push_back(kb);
+    if ((long long)kb * p2_cs(n) * (which == 1 ? cb / 2 : cb) <= sp.lds_cap) out.push_back(kb);
   (void)kind;
   return out;
 }
@@ -60,6 +60,7 @@ bool fast_plane_available(const ShapePlan& sp, int kind, int fp32) {
   };
   if (sp.d.M == 64 && sp.d.n2 == 128) return fits(64, 128);
   if (sp.d.M == 32 && sp.d.n2 == 64) return fits(32, 64);
+  if (sp.d.M == 16 && sp.d.n2 == 32) return fits(16, 32);
   return false;
 }
 

@@ -42,10 +42,10 @@ template <> struct P2<256> { static constexpr int R1 = 16, R2 = 16; };
 template <int N> __device__ __forceinline__ constexpr int padded(int j) { return j + j / P2<N>::R1; }
 __device__ __forceinline__ int t_plus(int t, int T, int m) { return t + T * m; }
 template <int N> constexpr int line_lds() { return N + N / P2<N>::R1; }
-// Line stride of the column passes (y, z), whose consecutive lanes sit on
-// consecutive columns: an odd number of 16-byte slots puts the eight lanes
-// served together by one LDS cycle on distinct banks (an even multiple of the
-// bank period would make it an 8-way conflict on every exchange).
+// Line stride of the column passes (y, z), in real components: their
+// consecutive lanes sit on consecutive columns, and an odd stride puts the
+// lanes served together by one LDS cycle on distinct banks (a multiple of
+// the bank period would turn every exchange into a many-way conflict).
 template <int N> constexpr int col_stride() { return line_lds<N>() | 1; }
 
 // Two-stage FFT of one line; x[m] holds point t + T*m on entry and exit.
@@ -82,6 +82,65 @@ __device__ __forceinline__ void fft2(C (&x)[P2<N>::R1], C* __restrict__ lds, int
 }
 
 // ------------------------------------------------------------ pointwise
+// fft2 with the lane exchange done one real component at a time through a
+// scalar LDS line (half the LDS of fft2, two more barriers). Used by the
+// column passes, whose occupancy is bound by LDS.
+template <int N, int S, typename C, typename R>
+__device__ __forceinline__ void fft2s(C (&x)[P2<N>::R1], R* __restrict__ lds, int t, const C* __restrict__ tw,
+                                      bool active) {
+  constexpr int R1 = P2<N>::R1, R2 = P2<N>::R2, T = R2, Q = R1 / R2;
+  R re[R1];   // stage-2 inputs, real parts, in (q, m) order
+  if (active) {
+    Bfly<R1, S, C>::run(x);
+#pragma unroll
+    for (int m = 0; m < R1; ++m) lds[padded<N>(t * R1 + m)] = x[m].x;
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int m = 0; m < R2; ++m) re[q * R2 + m] = lds[padded<N>(t + T * q + m * R1)];
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) lds[padded<N>(t * R1 + m)] = x[m].y;
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b2 = t + T * q;
+      C y[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        y[m].x = re[q * R2 + m];
+        y[m].y = lds[padded<N>(b2 + m * R1)];
+      }
+#pragma unroll
+      for (int m = 1; m < R2; ++m) {
+        C w = tw[m * b2];
+        if (S > 0) w.y = -w.y;
+        y[m] = cmul(y[m], w);
+      }
+      Bfly<R2, S, C>::run(y);
+#pragma unroll
+      for (int m = 0; m < R2; ++m) x[q + Q * m] = y[m];
+    }
+  }
+  __syncthreads();
+}
+
+// Workgroups are handed to the eight XCDs round-robin (id mod 8), and each
+// XCD has its own L2. Renumbering so that every XCD gets one contiguous range
+// of logical blocks keeps the cache lines shared by neighbouring blocks
+// (line-segment boundaries of the column passes) inside one L2.
+__device__ __forceinline__ int xcd_block(int w, int n) {
+  const int q = n >> 3, r = n & 7, x = w & 7;
+  return x * q + min(x, r) + (w >> 3);
+}
+
 template <int KIND> struct NF_;
 template <> struct NF_<kWT3> { static constexpr int v = 3; };
 template <> struct NF_<kWT4> { static constexpr int v = 4; };
@@ -137,86 +196,90 @@ __device__ __forceinline__ void block_sum_strided(double (&acc)[N], double* out,
   }
 }
 
+template <int KIND>
+__device__ __forceinline__ double field_of(const ReqDesc& D, int f, double rho) {
+  if (f == 0) return rho;
+  if (f == NF_<KIND>::v - 1) return sqrt(rho);
+  double ra, rb;
+  fwd_pows(rho, D.alpha, D.beta, ra, rb);
+  return (KIND == kWT4 && f == 2) ? ra : rb;
+}
+
+// Rows per workgroup of the x passes: RPB * NF lines of T threads,
+// whole wavefronts, at most 256 threads.
+__host__ __device__ constexpr int inv_rpb(int nf, int t) {
+  int r = 256 / (nf * t);
+  while (r > 1 && (r * nf * t) % 64 != 0) --r;
+  return r;
+}
+
 // ------------------------------------------------------------ x forward (R2C)
-// blockDim = RPB * T threads; thread = (row r, lane t), t fastest.
+// One workgroup = RPB rows x NF fields; thread (line = row*NF + field, lane t):
+// every line forms its field from rho, runs the half-length complex FFT in
+// registers and LDS, and writes its R2C-split half row.
 template <int M, int KIND, int INMODE, typename R>
 __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ descs, GridDims d,
                                                 const typename V2<R>::T* __restrict__ tw,
-                                                const typename V2<R>::T* __restrict__ tws, int rpb) {
+                                                const typename V2<R>::T* __restrict__ tws) {
   using C = typename V2<R>::T;
   constexpr int R1 = P2<M>::R1, T = P2<M>::R2, NF = NF_<KIND>::v;
   constexpr int LL = line_lds<M>();
+  constexpr int RPB = inv_rpb(NF, T);
   extern __shared__ __align__(16) unsigned char smem[];
   C* lds = reinterpret_cast<C*>(smem);
   const ReqDesc& D = descs[blockIdx.y];
-  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int r = line / NF, f = line - r * NF;
   const long long nrows = (long long)d.n2 * d.n3;
-  const long long row = (long long)blockIdx.x * rpb + r;
+  const long long row = (long long)xcd_block(blockIdx.x, gridDim.x) * RPB + r;
   const bool active = row < nrows;
-  C z[NF][R1];
+  C z[R1];
   if (active) {
     const size_t base = (size_t)row * (2 * M);
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
       const int k = t + T * m;
+      const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
       double a, b;
-      {
-        const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
-        if (INMODE == kInRho) {
-          a = uu.x;
-          b = uu.y;
-        } else {
-          double ta = D.ca * uu.x, tb = D.ca * uu.y;
-          if (D.cb != 0.0) {
-            const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
-            ta = fma(D.cb, vv.x, ta);
-            tb = fma(D.cb, vv.y, tb);
-          }
-          a = ta * ta;
-          b = tb * tb;
+      if (INMODE == kInRho) {
+        a = uu.x;
+        b = uu.y;
+      } else {
+        double ta = D.ca * uu.x, tb = D.ca * uu.y;
+        if (D.cb != 0.0) {
+          const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
+          ta = fma(D.cb, vv.x, ta);
+          tb = fma(D.cb, vv.y, tb);
         }
+        a = ta * ta;
+        b = tb * tb;
       }
-      double fa[NF], fb[NF];
-      fields<KIND>(D, a, fa);
-      fields<KIND>(D, b, fb);
-#pragma unroll
-      for (int j = 0; j < NF; ++j) {
-        z[j][m].x = (R)fa[j];
-        z[j][m].y = (R)fb[j];
-      }
+      z[m].x = (R)field_of<KIND>(D, f, a);
+      z[m].y = (R)field_of<KIND>(D, f, b);
     }
   }
-  C* mine = lds + (size_t)r * NF * LL;
-#pragma unroll
-  for (int j = 0; j < NF; ++j) fft2<M, -1>(z[j], mine + j * LL, t, tw, active);
-  // Z to LDS (unpadded, per field), then the split
+  C* mine = lds + (size_t)line * LL;
+  fft2<M, -1>(z, mine, t, tw, active);
   if (active) {
 #pragma unroll
-    for (int j = 0; j < NF; ++j)
-#pragma unroll
-      for (int m = 0; m < R1; ++m) mine[j * LL + t + T * m] = z[j][m];
+    for (int m = 0; m < R1; ++m) mine[t + T * m] = z[m];
   }
   __syncthreads();
   if (active) {
-    C* work = reinterpret_cast<C*>(D.work);
-    const size_t ob = (size_t)row * (M + 1);
+    C* out = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)row * (M + 1);
 #pragma unroll
-    for (int j = 0; j < NF; ++j) {
-      const C* Z = mine + j * LL;
-#pragma unroll
-      for (int m = 0; m <= R1; ++m) {
-        const int k = m < R1 ? t + T * m : M;
-        if (m == R1 && t != 0) continue;
-        const C zk = Z[k == M ? 0 : k];
-        const C zm = Z[k == 0 ? 0 : M - k];
-        const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
-        const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
-        const C w = tws[k];
-        C x;
-        x.x = er + (w.x * orr - w.y * oi);
-        x.y = ei + (w.x * oi + w.y * orr);
-        work[(size_t)j * d.nc + ob + k] = x;
-      }
+    for (int m = 0; m <= R1; ++m) {
+      const int k = m < R1 ? t + T * m : M;
+      if (m == R1 && t != 0) continue;
+      const C zk = mine[k == M ? 0 : k];
+      const C zm = mine[k == 0 ? 0 : M - k];
+      const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
+      const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
+      const C w = tws[k];
+      C x;
+      x.x = er + (w.x * orr - w.y * oi);
+      x.y = ei + (w.x * oi + w.y * orr);
+      out[k] = x;
     }
   }
 }
@@ -234,15 +297,6 @@ template <int M, int N2> struct PlaneGeo {
   static constexpr int NTX = N2 * TX, NTY = (M + 1) * TY;
   static constexpr int THREADS = ((NTX > NTY ? NTX : NTY) + 63) / 64 * 64;
 };
-
-template <int KIND>
-__device__ __forceinline__ double field_of(const ReqDesc& D, int f, double rho) {
-  if (f == 0) return rho;
-  if (f == NF_<KIND>::v - 1) return sqrt(rho);
-  double ra, rb;
-  fwd_pows(rho, D.alpha, D.beta, ra, rb);
-  return (KIND == kWT4 && f == 2) ? ra : rb;
-}
 
 template <int M, int N2, int KIND, int INMODE, typename R>
 __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ descs, GridDims d,
@@ -369,10 +423,10 @@ __global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, G
   constexpr int R1 = P2<N>::R1, T = P2<N>::R2;
   constexpr int LL = col_stride<N>();
   extern __shared__ __align__(16) unsigned char smem[];
-  C* lds = reinterpret_cast<C*>(smem);
+  R* lds = reinterpret_cast<R*>(smem);
   const ReqDesc& D = descs[blockIdx.z];
   const int c = threadIdx.x % kb, t = threadIdx.x / kb;
-  const long long col = (long long)blockIdx.x * kb + c;
+  const long long col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + c;
   const long long ncol = (long long)d.K1 * d.n3;
   const bool active = col < ncol;
   const long long i3 = active ? col / d.K1 : 0, k1 = active ? col - i3 * d.K1 : 0;
@@ -382,7 +436,7 @@ __global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, G
 #pragma unroll
     for (int m = 0; m < R1; ++m) x[m] = base[(size_t)(t + T * m) * d.K1];
   }
-  fft2<N, S>(x, lds + (size_t)c * LL, t, tw, active);
+  fft2s<N, S>(x, lds + (size_t)c * LL, t, tw, active);
   if (active) {
 #pragma unroll
     for (int m = 0; m < R1; ++m) base[(size_t)(t + T * m) * d.K1] = x[m];
@@ -404,7 +458,7 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   const ReqDesc& D = descs[blockIdx.z];
   const int f = blockIdx.y;
   const int c = threadIdx.x % kb, t = threadIdx.x / kb;
-  const long long col = (long long)blockIdx.x * kb + c;
+  const long long col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + c;
   const long long ncol = (long long)d.K1 * d.n2;
   const bool active = col < ncol;
   const int i2 = active ? (int)(col / d.K1) : 0;
@@ -471,14 +525,6 @@ __device__ __forceinline__ double epi(const ReqDesc& D, double rho, double vion,
   return loc + D.mu * t / (2.0 * p.phi);
 }
 
-// Rows per workgroup of the x-inverse pass: RPB * NF lines of T threads,
-// whole wavefronts, at most 256 threads.
-__host__ __device__ constexpr int inv_rpb(int nf, int t) {
-  int r = 256 / (nf * t);
-  while (r > 1 && (r * nf * t) % 64 != 0) --r;
-  return r;
-}
-
 // One workgroup = RPB rows x NF fields; thread (line = row*NF + field, lane t).
 // Every line is merged (C2R), transformed and left in LDS (complex k holds the
 // reals 2k, 2k+1), then all threads run the epilogue over the workgroup's
@@ -497,7 +543,8 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
   const int line = threadIdx.x / T, t = threadIdx.x % T;
   const int r = line / NF, f = line - r * NF;
   const long long nrows = (long long)d.n2 * d.n3;
-  const long long row0 = (long long)blockIdx.x * RPB;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const long long row0 = (long long)blk * RPB;
   const long long row = row0 + r;
   const bool active = row < nrows;
   C* mine = lds + (size_t)line * LL;
@@ -559,7 +606,7 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
   // partials laid out [slot][workgroup]; the finalize kernel sums each slot
   // in a fixed order (no device-scope fence or atomic here: on a multi-XCD
   // part a release fence per workgroup writes back the whole L2)
-  block_sum_strided<kNSum>(acc, D.sums + blockIdx.x, gridDim.x);
+  block_sum_strided<kNSum>(acc, D.sums + blk, gridDim.x);
 }
 
 }  // namespace fast
@@ -568,7 +615,8 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
 static inline bool is_fast_len(int n) { return n == 32 || n == 64 || n == 128 || n == 256; }
 
 inline bool fast_path_ok_impl(const ShapePlan& sp, int kind) {
-  return (kind == kWT3 || kind == kWT4) && is_fast_len(sp.d.M) && is_fast_len(sp.d.n2) && is_fast_len(sp.d.n3);
+  return (kind == kWT3 || kind == kWT4) && (sp.d.M == 16 || is_fast_len(sp.d.M)) && is_fast_len(sp.d.n2) &&
+         is_fast_len(sp.d.n3);
 }
 
 struct FastTiles {
@@ -578,7 +626,7 @@ struct FastTiles {
 static inline int p2_T(int n) { return n == 16 ? 4 : n == 32 ? 4 : n == 64 ? 8 : n == 128 ? 8 : 16; }
 static inline int p2_R1(int n) { return n == 16 ? 4 : n == 32 ? 8 : n == 64 ? 8 : 16; }
 static inline int p2_ll(int n) { return n + n / p2_R1(n); }
-static inline int p2_cs(int n) { return p2_ll(n) | 1; }
+static inline int p2_cs(int n) { return p2_ll(n) | 1; }   // in real components (column passes)
 
 // Tiles of the fast passes: at most 256 threads per workgroup and at most
 // lds_target bytes of LDS (the tuner may override rows_x/kb_y/kb_z).
@@ -595,7 +643,7 @@ static inline FastTiles fast_tiles(const ShapePlan& sp, int nf, int cbytes) {
   // one by the LDS target
   const long long ty = sp.kb_y > 0 ? sp.lds_cap : tgt, tz = sp.kb_z > 0 ? sp.lds_cap : tgt;
   ft.kby = sp.kb_y > 0 ? std::max(wy, sp.kb_y / wy * wy) : 256 / p2_T(sp.d.n2);
-  while (ft.kby > wy && ((long long)ft.kby * p2_cs(sp.d.n2) * cbytes > ty || ft.kby * p2_T(sp.d.n2) > 256)) ft.kby -= wy;
+  while (ft.kby > wy && ((long long)ft.kby * p2_cs(sp.d.n2) * (cbytes / 2) > ty || ft.kby * p2_T(sp.d.n2) > 256)) ft.kby -= wy;
   ft.kbz = sp.kb_z > 0 ? std::max(wz, sp.kb_z / wz * wz) : 256 / p2_T(sp.d.n3);
   while (ft.kbz > wz && ((long long)ft.kbz * p2_cs(sp.d.n3) * cbytes > tz || ft.kbz * p2_T(sp.d.n3) > 256)) ft.kbz -= wz;
   const long long nrows = (long long)sp.d.n2 * sp.d.n3;
@@ -609,11 +657,14 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
   using namespace fast;
   using C = typename V2<R>::T;
   constexpr int T = P2<M>::R2, NF = NF_<KIND>::v;
-  const int threads = ft.rpb * T;
+  (void)ft;
   if (fwd) {
-    size_t lds = (size_t)ft.rpb * NF * line_lds<M>() * sizeof(C);
-    hipLaunchKernelGGL((k_fx_fwd<M, KIND, INMODE, R>), dim3(ft.ntile, nb), dim3(threads), lds, st, dd, sp.d, tw, tws,
-                       ft.rpb);
+    constexpr int RPB = fast::inv_rpb(NF, T);
+    const long long nrows = (long long)sp.d.n2 * sp.d.n3;
+    const int ntile = (int)((nrows + RPB - 1) / RPB);
+    size_t lds = (size_t)RPB * NF * line_lds<M>() * sizeof(C);
+    hipLaunchKernelGGL((k_fx_fwd<M, KIND, INMODE, R>), dim3(ntile, nb), dim3(RPB * NF * T), lds, st, dd, sp.d, tw,
+                       tws);
   } else {
     constexpr int RPB = fast::inv_rpb(NF, T);
     const long long nrows = (long long)sp.d.n2 * sp.d.n3;
@@ -631,7 +682,7 @@ static void launch_y(const ShapePlan& sp, int kb, int nf, const ReqDesc* dd, int
   using C = typename V2<R>::T;
   const long long ncol = (long long)sp.d.K1 * sp.d.n3;
   const int nblk = (int)((ncol + kb - 1) / kb);
-  size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
+  size_t lds = (size_t)kb * col_stride<N>() * sizeof(R);
   hipLaunchKernelGGL((k_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb);
 }
 
@@ -671,6 +722,7 @@ static bool try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_
   if (sp.no_plane) return false;
   if (sp.d.M == 64 && sp.d.n2 == 128) return launch_plane<R, 64, 128, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   if (sp.d.M == 32 && sp.d.n2 == 64) return launch_plane<R, 32, 64, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  if (sp.d.M == 16 && sp.d.n2 == 32) return launch_plane<R, 16, 32, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   return false;
 }
 
@@ -714,6 +766,7 @@ static int fast_eval_t(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream
   }
   (void)f32;
   switch (sp.d.M) {
+    case 16: fast_chain<R, KIND, INMODE, OUTMODE, 16>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 32: fast_chain<R, KIND, INMODE, OUTMODE, 32>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 64: fast_chain<R, KIND, INMODE, OUTMODE, 64>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 128: fast_chain<R, KIND, INMODE, OUTMODE, 128>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;

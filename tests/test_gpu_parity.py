@@ -61,10 +61,10 @@ def _case(grid, lattice, kinetic=4, alpha=-100.0, beta=-100.0, seed=0, nion=3):
 
 
 # generic LDS path (mixed radices, odd lengths) and register-resident fast path
-# (n1/2, n2, n3 in {32, 64, 128, 256})
+# (n1/2 in {16, 32, 64, 128, 256}, n2 and n3 in {32, 64, 128, 256})
 GRIDS = [(16, 16, 16), (24, 20, 18), (14, 21, 10), (32, 30, 28), (6, 5, 4),
-         (64, 32, 64), (128, 64, 32), (64, 128, 256), (512, 32, 32),
-         (128, 128, 32), (64, 64, 32)]   # the last two take the fused x+y plane kernel
+         (64, 32, 64), (128, 64, 32), (64, 128, 256), (512, 32, 32), (32, 64, 128),
+         (128, 128, 32), (64, 64, 32), (32, 32, 32)]   # the last three take the fused x+y plane kernel
 
 
 @pytest.mark.parametrize("grid", GRIDS)
