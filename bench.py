@@ -12,8 +12,8 @@ that stream. Prints one JSON line (see the keys below).
                   [--no-cpu] [--no-serving]
   python bench.py --probe [--steps K]     # K evaluations + 3 device copies of
                                           # 256 MiB, for rocprofv3 --pmc runs
-  python bench.py --pmc-summary CSV       # per-kernel FETCH_SIZE/WRITE_SIZE of
-                                          # such a run, scaled by the copy kernel
+  python bench.py --pmc-summary CSV...    # per-kernel FETCH_SIZE/WRITE_SIZE of
+                                          # such runs, scaled by the copy kernel
 """
 from __future__ import annotations
 
@@ -178,37 +178,48 @@ def probe(pa, hip, n, steps):
           flush=True)
 
 
-def pmc_summary(path, n=128):
+def pmc_summary(paths, n=128):
     """Per-kernel average of each counter (KiB per dispatch) in a rocprofv3
     counter_collection.csv, and the evaluation's HBM bytes scaled by the copy
-    kernel's reading (a copy of COPY_BYTES reads and writes COPY_BYTES)."""
+    kernel's reading: the probe's device copies of COPY_BYTES read and write
+    COPY_BYTES each, so COPY_BYTES / reading is the counter's scale factor."""
     import csv
     from collections import defaultdict
-    tot = defaultdict(float)
-    cnt = defaultdict(set)
-    for r in csv.DictReader(open(path)):
-        key = (r["Kernel_Name"], r["Counter_Name"])
-        tot[key] += float(r["Counter_Value"])
-        cnt[key].add(r.get("Dispatch_Id", r.get("Correlation_Id", "")))
-    per = {k: tot[k] / max(1, len(cnt[k])) for k in tot}
-    counters = sorted({k[1] for k in per})
-    copy = {c: v for (kn, c), v in per.items() if "copyBuffer" in kn}
-    ndisp = {kn: len(cnt[(kn, c)]) for (kn, c) in per}
-    nmax = max(v for kn, v in ndisp.items() if "rocclr" not in kn)
-    lines = ["evaluation kernels: the %d kernels dispatched %d times" % (
-        sum(1 for kn, v in ndisp.items() if v == nmax and "rocclr" not in kn), nmax)]
-    for c in counters:
-        factor = COPY_BYTES / 1024.0 / copy[c] if copy.get(c) else float("nan")
-        lines.append("%s copy-kernel factor = %.4f (copy of %d B read as %.1f KiB)"
-                     % (c, factor, COPY_BYTES, copy.get(c, float("nan"))))
+    rows = [r for path in paths for r in csv.DictReader(open(path))]
+    copies = [r for r in rows if "copyBuffer" in r["Kernel_Name"]]
+    gmax = max(int(r["Grid_Size"]) for r in copies)
+    copy_val = defaultdict(list)
+    for r in copies:
+        if int(r["Grid_Size"]) == gmax:
+            copy_val[r["Counter_Name"]].append(float(r["Counter_Value"]))
+    vals = defaultdict(list)
+    for r in rows:
+        if "rocclr" in r["Kernel_Name"]:
+            continue
+        vals[(r["Kernel_Name"], r["Counter_Name"])].append(float(r["Counter_Value"]))
+    ndisp = defaultdict(int)
+    for (kn, c), v in vals.items():
+        ndisp[kn] = max(ndisp[kn], len(v))
+    nmax = max(ndisp.values())
+    ev_kernels = sorted(kn for kn, v in ndisp.items() if v == nmax)
+    lines = ["evaluation kernels: the %d kernels dispatched %d times (one per evaluation)" % (len(ev_kernels), nmax)]
+    total = {}
+    for c in sorted(copy_val):
+        cv = sum(copy_val[c]) / len(copy_val[c])
+        factor = COPY_BYTES / 1024.0 / cv
+        lines.append("%s: copy of %d B reads as %.1f KiB -> scale factor %.4f" % (c, COPY_BYTES, cv, factor))
         evsum = 0.0
-        for (kn, cc), v in sorted(per.items()):
-            if cc != c or "rocclr" in kn or ndisp[kn] != nmax:
+        for kn in ev_kernels:
+            v = vals.get((kn, c))
+            if not v:
                 continue
-            evsum += v
-            lines.append("  %-70s %12.1f KiB/dispatch  scaled %14.0f B" % (kn[:70], v, v * 1024 * factor))
-        lines.append("  %s per evaluation (sum over its kernels), scaled: %.0f B" % (c, evsum * 1024 * factor))
-        lines.append("")
+            avg = sum(v) / len(v)
+            evsum += avg
+            lines.append("  %-72s %11.1f KiB/dispatch  %12.0f B scaled" % (kn[:72], avg, avg * 1024 * factor))
+        total[c] = evsum * 1024 * factor
+        lines.append("  %s per evaluation, scaled: %.0f B" % (c, total[c]))
+    if "FETCH_SIZE" in total and "WRITE_SIZE" in total:
+        lines.append("traffic_bytes_per_eval = %.0f" % (total["FETCH_SIZE"] + total["WRITE_SIZE"]))
     print("\n".join(lines))
 
 
@@ -265,7 +276,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--probe", action="store_true")
-    ap.add_argument("--pmc-summary", default=None)
+    ap.add_argument("--pmc-summary", nargs="+", default=None)
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, args.grid)

@@ -199,17 +199,19 @@ struct LineSearch {
 
 // ====================================================================== launch graphs
 // The launch chain of one batched evaluation (five passes + finalize) is
-// captured once per (input mode, output mode, precision, descriptor array,
-// batch width) and replayed: the kernels read their per-request parameters
+// captured once per (functional kind, input mode, output mode, precision,
+// descriptor array, batch width) and replayed. The ShapePlan passed with a
+// key must outlive the cache (its twiddle tables are kernel arguments baked
+// into the graph): the kernels read their per-request parameters
 // from the descriptor array in device memory, so a replay only needs the
 // descriptors uploaded first. The first use of a key runs eagerly (so that
 // one-time function attributes are set outside any capture). No memset or
 // copy is ever recorded. OFD_NO_GRAPH=1 disables the capture.
 struct GraphKey {
-  int inmode, outmode, fp32, nb;
+  int kind, inmode, outmode, fp32, nb;
   const void* desc;
   bool operator<(const GraphKey& o) const {
-    return std::tie(inmode, outmode, fp32, nb, desc) < std::tie(o.inmode, o.outmode, o.fp32, o.nb, o.desc);
+    return std::tie(kind, inmode, outmode, fp32, nb, desc) < std::tie(o.kind, o.inmode, o.outmode, o.fp32, o.nb, o.desc);
   }
 };
 struct GraphCache {
@@ -229,7 +231,7 @@ static bool graphs_enabled() {
 // through a cached graph when possible. Returns 0 on success.
 static int launch_eval(GraphCache& gc, const ShapePlan& sp, int kind, int inmode, int outmode, int fp32,
                        const ReqDesc* dd, int nb, double* fin_out, hipStream_t st) {
-  const GraphKey key{inmode, outmode, fp32, nb, dd};
+  const GraphKey key{kind, inmode, outmode, fp32, nb, dd};
   auto direct = [&]() -> int {
     if (run_eval(sp, kind, inmode, outmode, fp32, dd, nb, st)) return -1;
     return run_finalize(sp, kind, fp32, dd, nb, fin_out, st);
@@ -1545,6 +1547,9 @@ struct ofd_pool {
   std::string err;
   int n1, n2, n3, capacity, kinetic, precision;
   hipStream_t st = nullptr;
+  // the pool's own plan (twiddle tables live as long as the pool and its
+  // captured graphs; a request's context may be destroyed at collection)
+  ShapePlan sp;
   struct Slot {
     bool used = false;
     bool done = false;
@@ -1582,6 +1587,20 @@ extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic
   p->slots.resize(capacity);
   p->h_desc.resize(capacity);
   p->h_sums.resize((size_t)capacity * kNSum);
+  if (shape_plan_init(&p->sp, n1, n2, n3, &p->err) != 0) {
+    *out = p;
+    return -1;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(TuneKey{n1, n2, n3, kWT3, precision == 1 ? 1 : 0});
+    if (it != g_tuned.end()) {
+      p->sp.rows_x = it->second.rows_x;
+      p->sp.kb_y = it->second.kb_y;
+      p->sp.kb_z = it->second.kb_z;
+      p->sp.no_plane = it->second.no_plane != 0;
+    }
+  }
   if (hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&p->d_desc, sizeof(ReqDesc) * capacity) != hipSuccess ||
       hipMalloc(&p->d_sums, sizeof(double) * kNSum * capacity) != hipSuccess) {
@@ -1603,6 +1622,7 @@ extern "C" void ofd_pool_destroy(ofd_pool* p) {
     }
   p->bs.free_all();
   graphs_clear(p->graphs);
+  if (p->sp.tw_x) shape_plan_free(&p->sp);
   hipFree(p->d_desc);
   hipFree(p->d_sums);
   hipStreamDestroy(p->st);
@@ -1666,30 +1686,31 @@ extern "C" int ofd_pool_step(ofd_pool* p) {
       run_dots(dots, n, p->bs, p->st);
     }
     flush_ops(act, n, p->bs, p->st);
-    std::vector<int> groups[3][2];
+    // one batched launch set per (functional kind, output mode, precision)
+    std::vector<int> groups[3][3][2];
     for (int i = 0; i < p->capacity; ++i) {
       auto& s = p->slots[i];
       if (!s.used || s.done) continue;
-      if (s.S.need.kind == NEED_EVAL) groups[s.S.need.ev.outmode][s.S.fp32 ? 1 : 0].push_back(i);
+      if (s.S.need.kind == NEED_EVAL) groups[s.c->kind][s.S.need.ev.outmode][s.S.fp32 ? 1 : 0].push_back(i);
     }
     int off = 0;
     std::vector<int> where(p->capacity, -1);
-    for (int om = 0; om < 3; ++om)
-      for (int pr = 0; pr < 2; ++pr) {
-        auto& g = groups[om][pr];
-        if (g.empty()) continue;
-        ofd_ctx* c0 = p->slots[g[0]].c;
-        for (size_t k = 0; k < g.size(); ++k) {
-          p->h_desc[off + k] = desc_for(p->slots[g[k]].c, p->slots[g[k]].S.need.ev);
-          p->h_desc[off + k].fin = p->d_sums + (size_t)(off + k) * kNSum;
-          where[g[k]] = off + (int)k;
+    for (int kd = 0; kd < 3; ++kd)
+      for (int om = 0; om < 3; ++om)
+        for (int pr = 0; pr < 2; ++pr) {
+          auto& g = groups[kd][om][pr];
+          if (g.empty()) continue;
+          for (size_t k = 0; k < g.size(); ++k) {
+            p->h_desc[off + k] = desc_for(p->slots[g[k]].c, p->slots[g[k]].S.need.ev);
+            p->h_desc[off + k].fin = p->d_sums + (size_t)(off + k) * kNSum;
+            where[g[k]] = off + (int)k;
+          }
+          HIPCHK(hipMemcpy(p->d_desc + off, &p->h_desc[off], sizeof(ReqDesc) * g.size(), hipMemcpyHostToDevice));
+          if (launch_eval(p->graphs, p->sp, kd, kInPhi, om, pr, p->d_desc + off, (int)g.size(),
+                          p->d_sums + (size_t)off * kNSum, p->st))
+            throw std::runtime_error("batched evaluation launch failed");
+          off += (int)g.size();
         }
-        HIPCHK(hipMemcpy(p->d_desc + off, &p->h_desc[off], sizeof(ReqDesc) * g.size(), hipMemcpyHostToDevice));
-        if (launch_eval(p->graphs, c0->sp, c0->kind, kInPhi, om, pr, p->d_desc + off, (int)g.size(),
-                        p->d_sums + (size_t)off * kNSum, p->st))
-          throw std::runtime_error("batched evaluation launch failed");
-        off += (int)g.size();
-      }
     if (off > 0) {
       HIPCHK(hipMemcpyAsync(p->h_sums.data(), p->d_sums, sizeof(double) * kNSum * off, hipMemcpyDeviceToHost,
                             p->st));

@@ -104,3 +104,38 @@ def test_autotune_keeps_results(pa):
     s2, _ = _system(pa, (64, 64, 64), 5)   # a new system of the same shape starts tuned
     pot2, et2 = pa.CalculatePotentialPlus(s2, rho)
     assert np.all(np.isfinite(et2))
+
+
+def test_pool_replays_after_requests_leave(pa):
+    """Requests admitted while others finish (the first admitted context is
+    destroyed at collection while launch graphs of the same batch width are
+    replayed later): every result equals the request's solve run alone."""
+    from profess_amd import serve
+    reqs = serve.make_requests(6, seed=9, grids=(32,), max_atoms=8, rate=1e6)
+    params = pa.calpot.opt_params(pot_tol=1e-5, conv_check="POT")
+    pool = serve.Pool((32, 32, 32), 4, 0)
+    clock = iter(range(10 ** 9))
+    now = lambda: float(next(clock))   # noqa: E731
+    done = []
+    queue = list(reqs)
+    try:
+        for _ in range(2):
+            r = queue.pop(0)
+            s, rho = serve.build_system(r, 0)
+            assert pool.admit(r, s, rho, params) >= 0
+        while len(done) < len(reqs):
+            pool.step()
+            got = pool.collect(now)
+            done += got
+            for _ in got:   # refill: the batch width goes back to an earlier value
+                if queue:
+                    r = queue.pop(0)
+                    s, rho = serve.build_system(r, 0)
+                    assert pool.admit(r, s, rho, params) >= 0
+    finally:
+        pool.close()
+    assert all(r.outcome == 0 for r in done)
+    for r in reqs:
+        one = serve.make_requests(6, seed=9, grids=(32,), max_atoms=8, rate=1e6)[r.rid]
+        serve.run_workload([one], mixed=False)
+        assert np.array_equal(one.energy, r.energy), (one.energy, r.energy)
