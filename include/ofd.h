@@ -51,7 +51,13 @@ typedef struct {
  * 2 = POT, 3 = BOTH (ReadInputFile.f90:591-611). Negative pot_tol/tole/max_iter
  * select the defaults (1.E-5 and 1.E-6 as single-precision literals, 500).
  * mixed_switch: with precision = 1, potential norm below which the optimizer
- * switches from FP32 to FP64 passes (<= 0: stay in the context's precision). */
+ * switches from FP32 to FP64 passes (<= 0: stay in the context's precision).
+ * fused_dots: 0 computes every inner product of the NTN loop as the reference
+ * does (RhoDirNew.f90, RhoOptN.f90); 1 takes the CG residual norms, the
+ * Newton step's |p|^2 and the next |dL/dphi|^2, |phi|^2 from sums the
+ * evaluation already produces and from the CG recurrences (one host round
+ * trip per inner iteration instead of three; direct products whenever the
+ * recurrence would lose accuracy). */
 typedef struct {
   int max_iter;
   double pot_tol;
@@ -59,6 +65,7 @@ typedef struct {
   int conv_check;
   int max_line;
   double mixed_switch;
+  int fused_dots;
 } ofd_opt_params;
 
 typedef struct {

@@ -146,9 +146,10 @@ def CalculatePotentialPlusDevice(system: OFDSystem, rho_ptr, pot_ptr, optSqrt=Tr
     return et
 
 
-def opt_params(max_iter=-1, pot_tol=-1.0, tole=-1.0, conv_check="ENE", max_line=-1, mixed_switch=0.0):
+def opt_params(max_iter=-1, pot_tol=-1.0, tole=-1.0, conv_check="ENE", max_line=-1, mixed_switch=0.0,
+               fused_dots=False):
     return ofd_opt_params(int(max_iter), float(pot_tol), float(tole), CONV[conv_check.upper()], int(max_line),
-                          float(mixed_switch))
+                          float(mixed_switch), int(bool(fused_dots)))
 
 
 def ProjNormRhoMinimization(system: OFDSystem, rhoR, dir_type=1, **kw):

@@ -648,6 +648,7 @@ struct EvalReq {
   double* out;
   double ca, cb, s1, s2, mu0, eps;
   bool energy;
+  const double* w = nullptr;   // CG residual for the fused Newton sums
 };
 
 enum NeedKind { NEED_NONE = 0, NEED_EVAL = 1, NEED_DOTS = 2 };
@@ -680,6 +681,9 @@ struct NtnSolver {
   // NewtonDirection (RhoDirNew.f90:54-75)
   int nit = 0, ntflag = 0;
   double rr = 0, rr_last = 0, criteria = 0, ry = 0, ry_last = 0, ss1 = 0, eps = 0;
+  // fused_dots: |p|^2, r.p and r_new.p_old carried by the CG recurrences
+  bool fused = false;
+  double pp = 0, rp = 0, rp_after = 0;
   // getNextGradient / line search
   bool switched = false;
   double norm_phi_dir = 0;
@@ -771,18 +775,48 @@ struct NtnSolver {
     ry = rr;
     if (nit == 1) {
       lincomb(c->p, c->r, c->r, 1.0, 0.0);   // p = r exactly: sum p^2 = rr
+      pp = rr;
+      rp = rr;
       newton_eval(rr);
-    } else {
-      axpby(c->p, c->r, 1.0, ry / ry_last);
-      st = ST_NEWTON_PP;
-      need_dots(1, c->p, c->p);
+      return;
     }
+    const double beta = ry / ry_last;
+    axpby(c->p, c->r, 1.0, beta);
+    if (fused) {
+      // p' = r + beta p: |p'|^2 = rr + 2 beta r.p + beta^2 |p|^2, r.p' = rr + beta r.p
+      const double pp_new = rr + 2.0 * beta * rp_after + beta * beta * pp;
+      if (pp_new > 1e-6 * (rr + beta * beta * pp)) {
+        pp = pp_new;
+        rp = rr + beta * rp_after;
+        newton_eval(pp);
+        return;
+      }
+    }
+    st = ST_NEWTON_PP;
+    if (fused) need_dots(2, c->p, c->p, c->r, c->p);
+    else need_dots(1, c->p, c->p);
   }
   // epsilon (RhoDirNew.f90:143-147) and the finite-difference product
   void newton_eval(double pp) {
     eps = 2.0 * std::sqrt(mach) * (1.0 + std::sqrt(ss1)) / std::sqrt(pp);
     st = ST_NEWTON_EVAL;
-    need_eval(EvalReq{kOutNewton, c->phi, c->p, c->dldphi, c->ap, 1.0, eps, 0.0, 0.0, mu0, eps, false});
+    EvalReq q{kOutNewton, c->phi, c->p, c->dldphi, c->ap, 1.0, eps, 0.0, 0.0, mu0, eps, false};
+    if (fused) q.w = c->r;
+    need_eval(q);
+  }
+  // CG exit tests after the residual update (RhoDirNew.f90:187-200)
+  void newton_after_rr(double rr_new) {
+    rr = rr_new;
+    bool done = false;
+    if (rr < 0.1 * criteria) { ntflag = 0; done = true; }
+    else if (nit > 50) { ntflag = 1; done = true; }
+    else if (std::fabs(rr - rr_last) / rr < 0.01 && nit > 9) { ntflag = 2; done = true; }
+    if (done) {
+      newton_done();
+    } else {
+      ++nit;
+      newton_prep();
+    }
   }
   void newton_done() {
     stats.newton_iters_total += nit;
@@ -889,7 +923,20 @@ struct NtnSolver {
     }
     ++it;
     st = ST_ACC_DOTS;
+    double bb, ff;
+    if (fused && fused_bb(last_sums, bb, ff)) {
+      newton_begin(bb, ff);
+      return;
+    }
     need_dots(2, c->dldphi, c->dldphi, c->phi, c->phi);
+  }
+  // |dL/dphi|^2 and |phi|^2 at the point the sums `v` were evaluated at, with
+  // dL/dphi = dE/dphi - 2 mu0 phi: |b|^2 = S(de^2) - 4 mu0 S(de phi) + 4 mu0^2 S(phi^2).
+  // False when the cancellation would cost more than about 8 digits.
+  bool fused_bb(const double* v, double& bb, double& ff) const {
+    ff = v[10];
+    bb = v[9] - 4.0 * mu0 * v[8] + 4.0 * mu0 * mu0 * v[10];
+    return bb > 1e-8 * v[9];
   }
 
   // results of the current need: kNSum sums (evaluation) or np dot products
@@ -904,6 +951,12 @@ struct NtnSolver {
         mu0 = c->dV * v[8] / (2.0 * tot_ele);   // ChemicalPotential (RhoDirBFGS.f90:337)
         lincomb(c->dldphi, c->dedphi, c->phi, 1.0, -2.0 * mu0);
         st = ST_INIT_DOTS;
+        double bb, ff;
+        if (fused && fused_bb(v, bb, ff)) {
+          const double d2[2] = {bb, ff};
+          deliver(d2);
+          return;
+        }
         need_dots(2, c->dldphi, c->dldphi, c->phi, c->phi);
         return;
       }
@@ -917,7 +970,9 @@ struct NtnSolver {
         return;
       }
       case ST_NEWTON_PP:
-        newton_eval(v[0]);
+        pp = v[0];
+        if (fused) rp = v[1];
+        newton_eval(pp);
         return;
       case ST_NEWTON_EVAL: {
         std::memcpy(last_sums, v, sizeof(last_sums));
@@ -934,24 +989,24 @@ struct NtnSolver {
         axpby(c->dir, c->p, alpha, 1.0);
         axpby(c->r, c->ap, -alpha, 1.0);
         rr_last = rr;
-        st = ST_NEWTON_RR;
-        need_dots(1, c->r, c->r);
-        return;
-      }
-      case ST_NEWTON_RR: {
-        rr = v[0];
-        bool done = false;
-        if (rr < 0.1 * criteria) { ntflag = 0; done = true; }
-        else if (nit > 50) { ntflag = 1; done = true; }
-        else if (std::fabs(rr - rr_last) / rr < 0.01 && nit > 9) { ntflag = 2; done = true; }
-        if (done) {
-          newton_done();
-        } else {
-          ++nit;
-          newton_prep();
+        if (fused) {
+          // r' = r - alpha Ap: |r'|^2 = rr - 2 alpha r.Ap + alpha^2 |Ap|^2, r'.p = r.p - alpha pAp
+          const double rr_new = rr - 2.0 * alpha * v[7] + alpha * alpha * v[8];
+          rp_after = rp - alpha * pap;
+          if (rr_new > 1e-6 * rr) {
+            newton_after_rr(rr_new);
+            return;
+          }
         }
+        st = ST_NEWTON_RR;
+        if (fused) need_dots(2, c->r, c->r, c->r, c->p);
+        else need_dots(1, c->r, c->r);
         return;
       }
+      case ST_NEWTON_RR:
+        if (fused) rp_after = v[1];
+        newton_after_rr(v[0]);
+        return;
       case ST_LP1: {
         norm_phi_dir = std::sqrt(v[0] * c->dV);
         const double proj = v[1] * c->dV;
@@ -1026,6 +1081,7 @@ void fill_params(const ofd_opt_params* in, ofd_opt_params* p) {
   p->conv_check = 1;
   p->max_line = 200;
   p->mixed_switch = 0.0;
+  p->fused_dots = 0;
   if (!in) return;
   if (in->max_iter > 0) p->max_iter = in->max_iter;
   if (in->pot_tol > 0) p->pot_tol = in->pot_tol;
@@ -1033,6 +1089,7 @@ void fill_params(const ofd_opt_params* in, ofd_opt_params* p) {
   if (in->conv_check >= 1 && in->conv_check <= 3) p->conv_check = in->conv_check;
   if (in->max_line > 0) p->max_line = in->max_line;
   p->mixed_switch = in->mixed_switch;
+  p->fused_dots = in->fused_dots != 0 ? 1 : 0;
 }
 
 void alloc_opt(ofd_ctx* c) {
@@ -1047,6 +1104,7 @@ ReqDesc desc_for(ofd_ctx* c, const EvalReq& q) {
   ReqDesc D = c->base_desc();
   D.u = q.u; D.v = q.v; D.bvec = q.bvec; D.out = q.out;
   D.ca = q.ca; D.cb = q.cb; D.s1 = q.s1; D.s2 = q.s2; D.mu0 = q.mu0; D.eps = q.eps;
+  D.w = q.w;
   D.opt_sqrt = 1;
   return D;
 }
@@ -1139,6 +1197,7 @@ void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchSc
   S.c = c;
   if (const char* e = std::getenv("OFD_TRACE")) S.trace = std::atoi(e) != 0;
   fill_params(prm, &S.prm);
+  S.fused = S.prm.fused_dots != 0;
   const long long n = c->nr();
   std::vector<double> ones(n, 1.0);
   HIPCHK(hipMemcpyAsync(c->tmp, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, st));

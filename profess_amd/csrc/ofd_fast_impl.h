@@ -596,11 +596,17 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
       const double dl = de - 2.0 * tphi * D.mu0;
       acc[7] += dl * dl;
       acc[8] += de * tphi;
+      acc[9] += de * de;
+      acc[10] += tphi * tphi;
     } else {
       const double de = pot * fsign(tphi);
       const double ap = ((de - D.mu0 * 2.0 * tphi) - D.bvec[g]) / D.eps;
       D.out[g] = ap;
       acc[6] += D.v[g] * ap;
+      if (D.w != nullptr) {
+        acc[7] += D.w[g] * ap;
+        acc[8] += ap * ap;
+      }
     }
   }
   // partials laid out [slot][workgroup]; the finalize kernel sums each slot

@@ -33,14 +33,18 @@ enum InMode : int { kInRho = 0, kInPhi = 1 };
 enum OutMode : int { kOutPot = 0, kOutSigned = 1, kOutNewton = 2 };
 
 // Per-pass partial sums written by every epilogue workgroup.
-constexpr int kNSum = 9;
-// 0 ion-electron, 1 Hartree(x2), 2 XC, 3 TF, 4 vW, 5 nonlocal, 6..8 extras
+constexpr int kNSum = 11;
+// 0 ion-electron, 1 Hartree(x2), 2 XC, 3 TF, 4 vW, 5 nonlocal, 6..10 optimizer sums:
+//   kOutSigned: 6 sum dE/dphi*dir, 7 sum (dE/dphi - 2 mu0 phi)^2, 8 sum dE/dphi*phi,
+//               9 sum (dE/dphi)^2, 10 sum phi^2
+//   kOutNewton: 6 sum p*Ap, 7 sum r*Ap (when w is set), 8 sum Ap^2 (when w is set)
 
 struct ReqDesc {
   // inputs of one evaluation
   const double* u;      // rho (kInRho) or phi base (kInPhi)
   const double* v;      // second phi vector (kInPhi), may be null when cb == 0
   const double* bvec;   // dL/dphi at the base point (kOutNewton)
+  const double* w;      // CG residual r (kOutNewton), null when the sums 7, 8 are not wanted
   const double* vion;   // ionic potential (real grid)
   double* out;          // potential / dEdphi / Ap
   double* sums;         // kNSum partials per epilogue workgroup

@@ -31,6 +31,7 @@ MODULE ofd_c
     REAL(C_DOUBLE) :: pot_tol, tole
     INTEGER(C_INT) :: conv_check, max_line
     REAL(C_DOUBLE) :: mixed_switch
+    INTEGER(C_INT) :: fused_dots
   END TYPE ofd_opt_params_t
 
   TYPE, BIND(C) :: ofd_opt_stats_t
@@ -195,6 +196,7 @@ CONTAINS
     IF (conv_check(1:4) == "BOTH") prm%conv_check = 3
     prm%max_line = 200
     prm%mixed_switch = 0.d0
+    prm%fused_dots = 0
     st = ofd_rho_opt(ofd_context, C_LOC(r), INT(dir_type, C_INT), prm, C_LOC(et), oc, stt)
     CALL ofd_check(ofd_context, st, 'ProjNormRhoMinimization')
     rhoR(:,:,:,1) = r

@@ -146,11 +146,14 @@ def capacity_for(grid):
 
 
 def run_workload(requests, mixed=True, hbm_budget=64e9, pot_tol=1e-5, mixed_switch=1e-3, verbose=False,
-                 keep_rho=False):
+                 keep_rho=False, fused_dots=True):
     """Serve `requests` (arrival times in seconds from the start) to completion.
-    Returns throughput and latency statistics."""
+    Returns throughput and latency statistics. fused_dots: the optimizer's
+    inner products from the evaluation's sums and the CG recurrences (one
+    host round trip per inner iteration)."""
     precision = 1 if mixed else 0
-    params = opt_params(pot_tol=pot_tol, conv_check="POT", mixed_switch=mixed_switch if mixed else 0.0)
+    params = opt_params(pot_tol=pot_tol, conv_check="POT", mixed_switch=mixed_switch if mixed else 0.0,
+                        fused_dots=fused_dots)
     pools = {}
     pending = sorted(requests, key=lambda r: r.arrival)
     in_use = 0.0
@@ -206,3 +209,33 @@ def run_workload(requests, mixed=True, hbm_budget=64e9, pot_tol=1e-5, mixed_swit
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mixed_precision": bool(mixed),
             "grids": sorted({r.grid[0] for r in requests})}
+
+
+def run_sequential(requests, pot_tol=1e-5, precision=0, mixed_switch=0.0, keep_rho=False):
+    """The plain drop-in path as a server: each request is solved alone with
+    ProjNormRhoMinimization (ofd_rho_opt), first come first served, one at a
+    time. Returns the same statistics as run_workload."""
+    from .calpot import ProjNormRhoMinimization
+    pending = sorted(requests, key=lambda r: r.arrival)
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0   # noqa: E731
+    for req in pending:
+        wait = req.arrival - now()
+        if wait > 0:
+            time.sleep(wait)
+        s, rho = build_system(req, precision)
+        req.admitted = now()
+        r, et, oc, st = ProjNormRhoMinimization(s, rho, conv_check="POT", pot_tol=pot_tol, mixed_switch=mixed_switch)
+        req.finished = now()
+        req.energy = et
+        req.outcome = oc
+        req.stats = st
+        req.rho = r if keep_rho else None
+        s.close()
+    makespan = now()
+    lat = np.array([r.finished - r.arrival for r in pending])
+    ok = sum(1 for r in pending if r.outcome == 0)
+    return {"requests": len(requests), "converged": ok, "makespan_s": makespan,
+            "solves_per_s": len(requests) / makespan,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mixed_precision": precision == 1, "grids": sorted({r.grid[0] for r in requests})}

@@ -112,7 +112,7 @@ def test_pool_replays_after_requests_leave(pa):
     replayed later): every result equals the request's solve run alone."""
     from profess_amd import serve
     reqs = serve.make_requests(6, seed=9, grids=(32,), max_atoms=8, rate=1e6)
-    params = pa.calpot.opt_params(pot_tol=1e-5, conv_check="POT")
+    params = pa.calpot.opt_params(pot_tol=1e-5, conv_check="POT", fused_dots=True)   # run_workload's settings
     pool = serve.Pool((32, 32, 32), 4, 0)
     clock = iter(range(10 ** 9))
     now = lambda: float(next(clock))   # noqa: E731

@@ -212,3 +212,22 @@ def test_plane_kernel_matches_separate_passes(pa, monkeypatch):
     (p0, e0), (p1, e1) = res
     assert np.max(np.abs(p0 - p1)) <= 1e-13 * np.max(np.abs(p1))
     _check_terms(e0, e1, rtol=1e-13)
+
+
+def test_ntn_fused_dots_matches_reference_products(pa):
+    """fused_dots takes the CG norms from the evaluation's sums and the
+    recurrences: the converged energy must match the reference-product solve
+    to 1e-7 eV/atom with about the same number of evaluations."""
+    cell, frac, vion, ne, f, rho = _case((32, 32, 32), _tri_lattice(), seed=6, nion=4)
+    res = []
+    for fused in (False, True):
+        s = pa.OFDSystem(cell.lattice, (32, 32, 32), ne, kinetic=4)
+        s.set_ion_terms(vion, 0.0)
+        r, et, oc, st = pa.ProjNormRhoMinimization(s, np.full(s.shape, ne / s.vol), conv_check="POT",
+                                                   pot_tol=1e-6, fused_dots=fused)
+        assert oc == 0
+        res.append((et, st))
+        s.close()
+    (e0, s0), (e1, s1) = res
+    assert abs(e0[0] - e1[0]) * O.HARTREE_TO_EV / 4 < 1e-7, (e0[0], e1[0])
+    assert abs(s0["num_evals"] - s1["num_evals"]) <= max(3, s0["num_evals"] // 10)
