@@ -91,9 +91,9 @@ def make_case(n):
     return inputs.al_fcc_case((reps, reps, reps), (n, n, n), sigma=0.1, seed_ion=7, seed_rho=11)
 
 
-def time_single(pa, hip, case, steps, warmup):
+def time_single(pa, hip, case, steps, warmup, precision=0):
     L = pa.lib()
-    s = pa.OFDSystem(case["lattice"], case["grid"], case["num_electrons"], kinetic=4)
+    s = pa.OFDSystem(case["lattice"], case["grid"], case["num_electrons"], kinetic=4, precision=precision)
     mg, vals = case["recpot"]
     s.build_ion_potential([(mg, vals, case["frac"])])
     rho = np.ascontiguousarray(case["rho"])
@@ -276,6 +276,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--fp32", action="store_true", help="FP32 FFT passes (mixed-precision form), reported apart")
     ap.add_argument("--pmc-summary", nargs="+", default=None)
     args = ap.parse_args()
     if args.pmc_summary:
@@ -294,6 +295,11 @@ def main():
         probe(pa, hip, n, args.steps)
         return
     case = make_case(n)
+    if args.fp32:
+        _, ms32, et32, _, _ = time_single(pa, hip, case, args.steps, args.warmup, precision=1)
+        print(json.dumps({"fp32_passes": True, "grid": n, "us_per_eval": ms32 * 1e3, "evals_per_s": 1000.0 / ms32,
+                          "etable": list(et32)}), flush=True)
+        return
     s, ms1, et, vion, bufs = time_single(pa, hip, case, args.steps, args.warmup)
     evals_s = 1000.0 / ms1
     bw = b_wt(n, n, n)

@@ -164,6 +164,14 @@ int ofd_pool_admit(ofd_pool* pool, ofd_ctx* ctx, const double* rho_init, const o
 /* Advance every active request by one evaluation (one batched launch set).
  * Returns the number of requests still active. */
 int ofd_pool_step(ofd_pool* pool);
+/* The same for several pools (different grid shapes) at once: their launches
+ * go to their own streams and run concurrently; the host waits once per
+ * round for all of them. Returns the number of requests still active in all
+ * pools (on error every pool's last_error is set). */
+int ofd_pools_step(ofd_pool* const* pools, int npool);
+/* Slots whose requests have finished and wait for collection: writes up to
+ * max_slots of them and returns how many there are. */
+int ofd_pool_finished(const ofd_pool* pool, int* slots, int max_slots);
 /* Collect a finished request: returns 1 and fills outputs when slot `slot`
  * has finished, 0 while it is running. The slot is freed on collection and
  * the context is handed back to the caller. */

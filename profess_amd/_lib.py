@@ -50,7 +50,7 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_calpot", "ofd_calpot_device", "ofd_rho_opt", "ofd_ion_forces", "ofd_pool_create",
            "ofd_pool_destroy", "ofd_pool_last_error", "ofd_pool_admit", "ofd_pool_step", "ofd_pool_collect",
            "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
-           "ofd_calpot_many", "ofd_autotune"]
+           "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished"]
 
 
 def lib():
@@ -95,6 +95,8 @@ def lib():
     L.ofd_sync.argtypes = [vp]
     L.ofd_calpot_many.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, dp]
     L.ofd_autotune.argtypes = [vp, C.c_int, C.POINTER(ofd_tune_result)]
+    L.ofd_pools_step.argtypes = [C.POINTER(vp), C.c_int]
+    L.ofd_pool_finished.argtypes = [vp, ip, C.c_int]
     _lib = L
     return L
 

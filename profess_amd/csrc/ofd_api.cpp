@@ -1109,84 +1109,129 @@ ReqDesc desc_for(ofd_ctx* c, const EvalReq& q) {
   return D;
 }
 
-// Device scratch for batched vector ops and dots, grown on demand.
+// Scratch for batched vector ops and dots, grown on demand. The host side is
+// pinned so that uploads and result copies are asynchronous on the caller's
+// stream; an event per staging buffer guards its reuse.
 struct BatchScratch {
   VecOp* d_ops = nullptr;
+  VecOp* h_ops = nullptr;
   int ops_cap = 0;
   DotDesc* d_dots = nullptr;
+  DotDesc* h_dots = nullptr;
   double* d_part = nullptr;
   double* d_res = nullptr;
+  double* h_res = nullptr;
   int dots_cap = 0;
-  std::vector<double> h_res;
+  int ndots = 0;   // results pending in h_res
+  hipEvent_t ev_ops = nullptr, ev_dots = nullptr;
   void free_all() {
     hipFree(d_ops); hipFree(d_dots); hipFree(d_part); hipFree(d_res);
+    hipHostFree(h_ops); hipHostFree(h_dots); hipHostFree(h_res);
+    if (ev_ops) hipEventDestroy(ev_ops);
+    if (ev_dots) hipEventDestroy(ev_dots);
     *this = BatchScratch();
+  }
+  void events() {
+    if (!ev_ops) HIPCHK(hipEventCreateWithFlags(&ev_ops, hipEventDisableTiming));
+    if (!ev_dots) HIPCHK(hipEventCreateWithFlags(&ev_dots, hipEventDisableTiming));
   }
 };
 
-// Runs the queued ops of all solvers, layer by layer (each solver's ops stay in order).
+template <typename T>
+static T* halloc(size_t n) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::runtime_error("hipHostMalloc failed");
+  return static_cast<T*>(p);
+}
+
+// Launches the queued ops of all solvers, layer by layer (each solver's ops
+// stay in order), on `st`.
 void flush_ops(std::vector<NtnSolver*>& ss, long long n, BatchScratch& bs, hipStream_t st) {
-  size_t layers = 0;
-  for (auto* s : ss) layers = std::max(layers, s->ops.size());
+  size_t layers = 0, total = 0;
+  for (auto* s : ss) {
+    layers = std::max(layers, s->ops.size());
+    total += s->ops.size();
+  }
   if (layers == 0) return;
-  std::vector<VecOp> all;
-  std::vector<int> off(layers + 1, 0);
-  for (size_t L = 0; L < layers; ++L) {
-    off[L] = (int)all.size();
-    for (auto* s : ss)
-      if (L < s->ops.size()) all.push_back(s->ops[L]);
-  }
-  off[layers] = (int)all.size();
-  if ((int)all.size() > bs.ops_cap) {
+  bs.events();
+  HIPCHK(hipEventSynchronize(bs.ev_ops));   // the previous upload has left the staging buffer
+  if ((int)total > bs.ops_cap) {
     hipFree(bs.d_ops);
-    bs.ops_cap = (int)all.size() * 2;
+    hipHostFree(bs.h_ops);
+    bs.ops_cap = (int)total * 2;
     bs.d_ops = dalloc<VecOp>(bs.ops_cap);
+    bs.h_ops = halloc<VecOp>(bs.ops_cap);
   }
-  HIPCHK(hipMemcpy(bs.d_ops, all.data(), sizeof(VecOp) * all.size(), hipMemcpyHostToDevice));
+  std::vector<int> off(layers + 1, 0);
+  int k = 0;
+  for (size_t L = 0; L < layers; ++L) {
+    off[L] = k;
+    for (auto* s : ss)
+      if (L < s->ops.size()) bs.h_ops[k++] = s->ops[L];
+  }
+  off[layers] = k;
+  HIPCHK(hipMemcpyAsync(bs.d_ops, bs.h_ops, sizeof(VecOp) * k, hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(bs.ev_ops, st));
   for (size_t L = 0; L < layers; ++L) vec_ops_batched(bs.d_ops + off[L], off[L + 1] - off[L], n, st);
   for (auto* s : ss) s->ops.clear();
 }
 
-// Runs the dot needs of `ss` (all NEED_DOTS) and delivers the results.
-void run_dots(std::vector<NtnSolver*>& ss, long long n, BatchScratch& bs, hipStream_t st) {
-  std::vector<DotDesc> dd;
-  for (auto* s : ss)
-    for (int k = 0; k < s->need.np; ++k) dd.push_back(DotDesc{s->need.a[k], s->need.b[k]});
-  if (dd.empty()) return;
-  if ((int)dd.size() > bs.dots_cap) {
+// Launches the dot needs of `ss` (all NEED_DOTS) and the copy of their
+// results to pinned memory; deliver_dots hands them over after the stream
+// has been synchronized.
+void launch_dots(std::vector<NtnSolver*>& ss, long long n, BatchScratch& bs, hipStream_t st) {
+  int cnt = 0;
+  for (auto* s : ss) cnt += s->need.np;
+  bs.ndots = cnt;
+  if (cnt == 0) return;
+  bs.events();
+  HIPCHK(hipEventSynchronize(bs.ev_dots));
+  if (cnt > bs.dots_cap) {
     hipFree(bs.d_dots); hipFree(bs.d_part); hipFree(bs.d_res);
-    bs.dots_cap = (int)dd.size() * 2;
+    hipHostFree(bs.h_dots); hipHostFree(bs.h_res);
+    bs.dots_cap = cnt * 2;
     bs.d_dots = dalloc<DotDesc>(bs.dots_cap);
     bs.d_part = dalloc<double>((size_t)bs.dots_cap * kDotBlocksBatched);
     bs.d_res = dalloc<double>(bs.dots_cap);
+    bs.h_dots = halloc<DotDesc>(bs.dots_cap);
+    bs.h_res = halloc<double>(bs.dots_cap);
   }
-  HIPCHK(hipMemcpy(bs.d_dots, dd.data(), sizeof(DotDesc) * dd.size(), hipMemcpyHostToDevice));
-  vec_dots_batched(bs.d_dots, (int)dd.size(), n, bs.d_part, bs.d_res, st);
-  bs.h_res.resize(dd.size());
-  HIPCHK(hipMemcpyAsync(bs.h_res.data(), bs.d_res, sizeof(double) * dd.size(), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  int k = 0;
+  for (auto* s : ss)
+    for (int j = 0; j < s->need.np; ++j) bs.h_dots[k++] = DotDesc{s->need.a[j], s->need.b[j]};
+  HIPCHK(hipMemcpyAsync(bs.d_dots, bs.h_dots, sizeof(DotDesc) * cnt, hipMemcpyHostToDevice, st));
+  vec_dots_batched(bs.d_dots, cnt, n, bs.d_part, bs.d_res, st);
+  HIPCHK(hipMemcpyAsync(bs.h_res, bs.d_res, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(bs.ev_dots, st));
+}
+void deliver_dots(std::vector<NtnSolver*>& ss, BatchScratch& bs) {
   size_t k = 0;
   for (auto* s : ss) {
     const int np = s->need.np;
     s->deliver(&bs.h_res[k]);
     k += np;
   }
+  bs.ndots = 0;
+}
+void run_dots(std::vector<NtnSolver*>& ss, long long n, BatchScratch& bs, hipStream_t st) {
+  launch_dots(ss, n, bs, st);
+  if (bs.ndots == 0) return;
+  HIPCHK(hipStreamSynchronize(st));
+  deliver_dots(ss, bs);
 }
 
-// sum(x) over a device vector, deterministic (dot with a ones vector)
-double dsum(ofd_ctx* c, const double* x, double* ones, BatchScratch& bs, hipStream_t st) {
-  DotDesc d{x, ones};
-  if (bs.dots_cap < 1) {
-    bs.dots_cap = 8;
-    bs.d_dots = dalloc<DotDesc>(bs.dots_cap);
-    bs.d_part = dalloc<double>((size_t)bs.dots_cap * kDotBlocksBatched);
-    bs.d_res = dalloc<double>(bs.dots_cap);
-  }
-  HIPCHK(hipMemcpy(bs.d_dots, &d, sizeof(d), hipMemcpyHostToDevice));
-  vec_dots_batched(bs.d_dots, 1, c->nr(), bs.d_part, bs.d_res, st);
-  double s;
-  HIPCHK(hipMemcpyAsync(&s, bs.d_res, sizeof(double), hipMemcpyDeviceToHost, st));
+// sum(x) over a device vector, deterministic (a dot product with no second vector)
+double dsum(const double* x, long long n, BatchScratch& bs, hipStream_t st) {
+  NtnSolver tmp;
+  tmp.need.kind = NEED_DOTS;
+  tmp.need.np = 1;
+  tmp.need.a[0] = x;
+  tmp.need.b[0] = nullptr;
+  std::vector<NtnSolver*> one{&tmp};
+  launch_dots(one, n, bs, st);
   HIPCHK(hipStreamSynchronize(st));
+  const double s = bs.h_res[0];
+  bs.ndots = 0;
   return s;
 }
 
@@ -1199,9 +1244,7 @@ void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchSc
   fill_params(prm, &S.prm);
   S.fused = S.prm.fused_dots != 0;
   const long long n = c->nr();
-  std::vector<double> ones(n, 1.0);
-  HIPCHK(hipMemcpyAsync(c->tmp, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
-  S.tot_ele = dsum(c, c->rho, c->tmp, bs, st) * c->dV;
+  S.tot_ele = dsum(c->rho, n, bs, st) * c->dV;
   vec_sqrt_abs(c->phi, c->rho, n, st);
   S.start();
 }
@@ -1620,10 +1663,14 @@ struct ofd_pool {
   std::vector<Slot> slots;
   ReqDesc* d_desc = nullptr;
   double* d_sums = nullptr;
-  std::vector<ReqDesc> h_desc;
-  std::vector<double> h_sums;
+  ReqDesc* h_desc = nullptr;   // pinned
+  double* h_sums = nullptr;    // pinned
   BatchScratch bs;
   GraphCache graphs;
+  // per step
+  std::vector<NtnSolver*> act, dots;
+  std::vector<int> where;
+  int nsum = 0;
 };
 
 extern "C" long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int precision) {
@@ -1644,8 +1691,13 @@ extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic
   ofd_pool* p = new ofd_pool();
   p->n1 = n1; p->n2 = n2; p->n3 = n3; p->capacity = capacity; p->kinetic = kinetic; p->precision = precision;
   p->slots.resize(capacity);
-  p->h_desc.resize(capacity);
-  p->h_sums.resize((size_t)capacity * kNSum);
+  p->where.resize(capacity);
+  if (hipHostMalloc((void**)&p->h_desc, sizeof(ReqDesc) * capacity, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&p->h_sums, sizeof(double) * kNSum * capacity, hipHostMallocDefault) != hipSuccess) {
+    p->err = "pinned host allocation failed";
+    *out = p;
+    return -1;
+  }
   if (shape_plan_init(&p->sp, n1, n2, n3, &p->err) != 0) {
     *out = p;
     return -1;
@@ -1682,6 +1734,8 @@ extern "C" void ofd_pool_destroy(ofd_pool* p) {
   p->bs.free_all();
   graphs_clear(p->graphs);
   if (p->sp.tw_x) shape_plan_free(&p->sp);
+  hipHostFree(p->h_desc);
+  hipHostFree(p->h_sums);
   hipFree(p->d_desc);
   hipFree(p->d_sums);
   hipStreamDestroy(p->st);
@@ -1725,35 +1779,53 @@ extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const 
   return slot;
 }
 
-// Advances every active request by one evaluation: dot rounds (batched over
-// all requests that need dots) until every request waits on an evaluation,
-// then the evaluations, batched per (output mode, precision).
-extern "C" int ofd_pool_step(ofd_pool* p) {
-  if (!p) return -1;
-  const long long n = (long long)p->n1 * p->n2 * p->n3;
-  try {
-    std::vector<NtnSolver*> act;
+// Advances every active request of every pool by one evaluation. Dot rounds
+// first: each round launches, on every pool's own stream, the batched vector
+// updates and the inner products of all requests that need them, then waits
+// once for all pools; rounds repeat until every request waits on an
+// evaluation. Then the evaluations, batched per (functional kind, output
+// mode, precision) within a pool, launched for all pools before one wait.
+// Pools on different streams run concurrently on the device.
+static int pools_step(ofd_pool* const* pools, int npool) {
+  for (int ip = 0; ip < npool; ++ip) {
+    ofd_pool* p = pools[ip];
+    p->act.clear();
     for (auto& s : p->slots)
-      if (s.used && !s.done) act.push_back(&s.S);
-    if (act.empty()) return 0;
-    for (int guard = 0; guard < 64; ++guard) {
-      flush_ops(act, n, p->bs, p->st);
-      std::vector<NtnSolver*> dots;
-      for (auto* s : act)
-        if (s->need.kind == NEED_DOTS) dots.push_back(s);
-      if (dots.empty()) break;
-      run_dots(dots, n, p->bs, p->st);
+      if (s.used && !s.done) p->act.push_back(&s.S);
+  }
+  for (int guard = 0; guard < 64; ++guard) {
+    bool any = false;
+    for (int ip = 0; ip < npool; ++ip) {
+      ofd_pool* p = pools[ip];
+      const long long n = (long long)p->n1 * p->n2 * p->n3;
+      flush_ops(p->act, n, p->bs, p->st);
+      p->dots.clear();
+      for (auto* s : p->act)
+        if (s->need.kind == NEED_DOTS) p->dots.push_back(s);
+      if (!p->dots.empty()) {
+        launch_dots(p->dots, n, p->bs, p->st);
+        any = true;
+      }
     }
-    flush_ops(act, n, p->bs, p->st);
+    if (!any) break;
+    for (int ip = 0; ip < npool; ++ip)
+      if (!pools[ip]->dots.empty()) HIPCHK(hipStreamSynchronize(pools[ip]->st));
+    for (int ip = 0; ip < npool; ++ip)
+      if (!pools[ip]->dots.empty()) deliver_dots(pools[ip]->dots, pools[ip]->bs);
+  }
+  for (int ip = 0; ip < npool; ++ip) {
+    ofd_pool* p = pools[ip];
+    const long long n = (long long)p->n1 * p->n2 * p->n3;
+    flush_ops(p->act, n, p->bs, p->st);
     // one batched launch set per (functional kind, output mode, precision)
     std::vector<int> groups[3][3][2];
     for (int i = 0; i < p->capacity; ++i) {
       auto& s = p->slots[i];
+      p->where[i] = -1;
       if (!s.used || s.done) continue;
       if (s.S.need.kind == NEED_EVAL) groups[s.c->kind][s.S.need.ev.outmode][s.S.fp32 ? 1 : 0].push_back(i);
     }
     int off = 0;
-    std::vector<int> where(p->capacity, -1);
     for (int kd = 0; kd < 3; ++kd)
       for (int om = 0; om < 3; ++om)
         for (int pr = 0; pr < 2; ++pr) {
@@ -1762,33 +1834,65 @@ extern "C" int ofd_pool_step(ofd_pool* p) {
           for (size_t k = 0; k < g.size(); ++k) {
             p->h_desc[off + k] = desc_for(p->slots[g[k]].c, p->slots[g[k]].S.need.ev);
             p->h_desc[off + k].fin = p->d_sums + (size_t)(off + k) * kNSum;
-            where[g[k]] = off + (int)k;
+            p->where[g[k]] = off + (int)k;
           }
-          HIPCHK(hipMemcpy(p->d_desc + off, &p->h_desc[off], sizeof(ReqDesc) * g.size(), hipMemcpyHostToDevice));
+          HIPCHK(hipMemcpyAsync(p->d_desc + off, p->h_desc + off, sizeof(ReqDesc) * g.size(),
+                                hipMemcpyHostToDevice, p->st));
           if (launch_eval(p->graphs, p->sp, kd, kInPhi, om, pr, p->d_desc + off, (int)g.size(),
                           p->d_sums + (size_t)off * kNSum, p->st))
             throw std::runtime_error("batched evaluation launch failed");
           off += (int)g.size();
         }
-    if (off > 0) {
-      HIPCHK(hipMemcpyAsync(p->h_sums.data(), p->d_sums, sizeof(double) * kNSum * off, hipMemcpyDeviceToHost,
-                            p->st));
-      HIPCHK(hipStreamSynchronize(p->st));
-    }
+    p->nsum = off;
+    if (off > 0)
+      HIPCHK(hipMemcpyAsync(p->h_sums, p->d_sums, sizeof(double) * kNSum * off, hipMemcpyDeviceToHost, p->st));
+  }
+  int still = 0;
+  for (int ip = 0; ip < npool; ++ip) {
+    ofd_pool* p = pools[ip];
+    if (p->nsum > 0) HIPCHK(hipStreamSynchronize(p->st));
     for (int i = 0; i < p->capacity; ++i) {
       auto& s = p->slots[i];
       if (!s.used) continue;
-      if (where[i] >= 0) s.S.deliver(&p->h_sums[(size_t)where[i] * kNSum]);
+      if (p->where[i] >= 0) s.S.deliver(&p->h_sums[(size_t)p->where[i] * kNSum]);
       if (s.S.need.kind == NEED_NONE) s.done = true;
+      if (!s.done) ++still;
     }
+  }
+  return still;
+}
+
+extern "C" int ofd_pool_step(ofd_pool* p) {
+  if (!p) return -1;
+  try {
+    return pools_step(&p, 1);
   } catch (const std::exception& e) {
     p->err = e.what();
     return -1;
   }
-  int still = 0;
-  for (auto& s : p->slots)
-    if (s.used && !s.done) ++still;
-  return still;
+}
+
+extern "C" int ofd_pools_step(ofd_pool* const* pools, int npool) {
+  if (!pools || npool < 1) return -1;
+  for (int i = 0; i < npool; ++i)
+    if (!pools[i]) return -1;
+  try {
+    return pools_step(pools, npool);
+  } catch (const std::exception& e) {
+    for (int i = 0; i < npool; ++i) pools[i]->err = e.what();
+    return -1;
+  }
+}
+
+extern "C" int ofd_pool_finished(const ofd_pool* p, int* slots, int max_slots) {
+  if (!p || (!slots && max_slots > 0)) return -1;
+  int k = 0;
+  for (int i = 0; i < p->capacity; ++i)
+    if (p->slots[i].used && p->slots[i].done) {
+      if (k < max_slots) slots[k] = i;
+      ++k;
+    }
+  return k;
 }
 
 extern "C" int ofd_pool_collect(ofd_pool* p, int slot, double* rho_out, double* energy9, int* outcome,

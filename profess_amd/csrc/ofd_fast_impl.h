@@ -132,15 +132,6 @@ __device__ __forceinline__ void fft2s(C (&x)[P2<N>::R1], R* __restrict__ lds, in
   __syncthreads();
 }
 
-// Workgroups are handed to the eight XCDs round-robin (id mod 8), and each
-// XCD has its own L2. Renumbering so that every XCD gets one contiguous range
-// of logical blocks keeps the cache lines shared by neighbouring blocks
-// (line-segment boundaries of the column passes) inside one L2.
-__device__ __forceinline__ int xcd_block(int w, int n) {
-  const int q = n >> 3, r = n & 7, x = w & 7;
-  return x * q + min(x, r) + (w >> 3);
-}
-
 template <int KIND> struct NF_;
 template <> struct NF_<kWT3> { static constexpr int v = 3; };
 template <> struct NF_<kWT4> { static constexpr int v = 4; };

@@ -239,11 +239,64 @@ __device__ __forceinline__ void stage_rad(const C* __restrict__ in, C* __restric
   }
 }
 
+// The same stage with the length, radix and stride known at compile time
+// (divisions by constants, unrolled indexing).
+template <int RAD, int S, int N, int NS, typename C>
+__device__ __forceinline__ void stage_rad_c(const C* __restrict__ in, C* __restrict__ out, int ld, int nlines,
+                                            const C* __restrict__ tw, int tid, int nthr) {
+  constexpr int nr = N / RAD;
+  constexpr int tws = N / (NS * RAD);
+  const int total = nlines * nr;
+  for (int it = tid; it < total; it += nthr) {
+    const int line = it / nr;
+    const int b = it - line * nr;
+    const int k = b % NS;
+    const C* src = in + line * ld;
+    C* dst = out + line * ld;
+    C x[RAD];
+#pragma unroll
+    for (int m = 0; m < RAD; ++m) x[m] = src[b + m * nr];
+    if (NS > 1) {
+#pragma unroll
+      for (int m = 1; m < RAD; ++m) {
+        C w = tw[m * k * tws];
+        if (S > 0) w.y = -w.y;
+        x[m] = cmul(x[m], w);
+      }
+    }
+    Bfly<RAD, S, C>::run(x);
+    const int j = (b - k) * RAD + k;
+#pragma unroll
+    for (int m = 0; m < RAD; ++m) dst[j + m * NS] = x[m];
+  }
+}
+
+template <int S, int N, int NS, typename C, int R, int... Rest>
+__device__ __forceinline__ C* stages_c(C* a, C* b, int ld, int nlines, const C* __restrict__ tw) {
+  stage_rad_c<R, S, N, NS>(a, b, ld, nlines, tw, threadIdx.x, blockDim.x);
+  __syncthreads();
+  if constexpr (sizeof...(Rest) > 0) return stages_c<S, N, NS * R, C, Rest...>(b, a, ld, nlines, tw);
+  else return b;
+}
+
 // Runs all stages of `p` on nlines lines that start in buffer a (scratch b).
 // Returns the buffer holding the result (a if p.nst is even, else b).
 // Must be called by every thread of the workgroup (it contains barriers).
+// Lengths of the serving and benchmark grids run a compile-time plan; any
+// other length runs the plan in `p`.
 template <int S, typename C>
 __device__ C* lds_fft(C* a, C* b, const Plan1D& p, int ld, int nlines, const C* __restrict__ tw) {
+  switch (p.n) {
+    case 24: return stages_c<S, 24, 1, C, 8, 3>(a, b, ld, nlines, tw);
+    case 40: return stages_c<S, 40, 1, C, 8, 5>(a, b, ld, nlines, tw);
+    case 48: return stages_c<S, 48, 1, C, 16, 3>(a, b, ld, nlines, tw);
+    case 60: return stages_c<S, 60, 1, C, 4, 3, 5>(a, b, ld, nlines, tw);
+    case 80: return stages_c<S, 80, 1, C, 16, 5>(a, b, ld, nlines, tw);
+    case 96: return stages_c<S, 96, 1, C, 8, 4, 3>(a, b, ld, nlines, tw);
+    case 120: return stages_c<S, 120, 1, C, 8, 3, 5>(a, b, ld, nlines, tw);
+    case 192: return stages_c<S, 192, 1, C, 8, 8, 3>(a, b, ld, nlines, tw);
+    default: break;
+  }
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   int Ns = 1;

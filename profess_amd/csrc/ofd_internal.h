@@ -154,7 +154,8 @@ struct DotDesc {
   const double* b;
 };
 constexpr int kDotBlocksBatched = 256;
-// res[k] = sum a_k . b_k, fixed-order reduction (kDotBlocksBatched partials each)
+// res[k] = sum a_k . b_k (sum a_k when b_k is null), fixed-order reduction
+// (kDotBlocksBatched partials each)
 void vec_dots_batched(const DotDesc* d_dd, int ndots, long long n, double* d_part, double* d_res, hipStream_t st);
 
 // Per-geometry tables.

@@ -80,6 +80,32 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[N], double* out) {
   }
 }
 
+// block_sum_store with out[k * stride] (partials laid out [slot][workgroup])
+template <int N>
+__device__ __forceinline__ void block_sum_store_strided(double (&acc)[N], double* out, int stride) {
+  __shared__ double red2[16][N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    acc[k] = v;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) red2[wv][k] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red2[w][threadIdx.x];
+    out[(size_t)threadIdx.x * stride] = s;
+  }
+}
+
 // ------------------------------------------------------------ pass 1: x R2C
 // rows = n2*n3 lines of n1 reals per request; `rows_per` rows per workgroup.
 // LDS: 2 buffers of rows_per*NF lines, ld = M+1 complex each.
@@ -93,7 +119,8 @@ __global__ void __launch_bounds__(256) k_xfwd(const ReqDesc* __restrict__ descs,
   const ReqDesc& D = descs[blockIdx.y];
   const int M = d.M, ld = M + 1, n1 = d.n1;
   const long long nrows = (long long)d.n2 * d.n3;
-  const long long row0 = (long long)blockIdx.x * rows_per;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const long long row0 = (long long)blk * rows_per;
   const int nrow = (int)min((long long)rows_per, nrows - row0);
   const int nl = nrow * NF;
   C* A = reinterpret_cast<C*>(smem);
@@ -147,7 +174,8 @@ __global__ void __launch_bounds__(256) k_cpass(const ReqDesc* __restrict__ descs
   using C = typename V2<R>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   const ReqDesc& D = descs[blockIdx.z];
-  const int chunk = blockIdx.x % nchunk, outer = blockIdx.x / nchunk;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const int chunk = bx % nchunk, outer = bx / nchunk;
   const int c0 = chunk * kb;
   const int ncol = min(kb, d.K1 - c0);
   const int n = p.n, ld = n + 1;
@@ -210,7 +238,8 @@ __global__ void __launch_bounds__(256) k_zgop(const ReqDesc* __restrict__ descs,
   constexpr int NF = NFields<KIND>::v;
   extern __shared__ __align__(16) unsigned char smem[];
   const ReqDesc& D = descs[blockIdx.y];
-  const int chunk = blockIdx.x % nchunk, i2 = blockIdx.x / nchunk;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const int chunk = bx % nchunk, i2 = bx / nchunk;
   const int c0 = chunk * kb;
   const int ncol = min(kb, d.K1 - c0);
   const int n3 = d.n3, ld = n3 + 1;
@@ -311,7 +340,8 @@ __global__ void __launch_bounds__(256) k_xinv(const ReqDesc* __restrict__ descs,
   const ReqDesc& D = descs[blockIdx.y];
   const int M = d.M, ld = M + 1, n1 = d.n1, K1 = d.K1;
   const long long nrows = (long long)d.n2 * d.n3;
-  const long long row0 = (long long)blockIdx.x * rows_per;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const long long row0 = (long long)blk * rows_per;
   const int nrow = (int)min((long long)rows_per, nrows - row0);
   const int nl = nrow * NF;
   C* A = reinterpret_cast<C*>(smem);
@@ -385,20 +415,33 @@ __global__ void __launch_bounds__(256) k_xinv(const ReqDesc* __restrict__ descs,
       }
     }
   }
-  if (KIND != kPlain) block_sum_store<kNSum>(acc, D.sums + (size_t)blockIdx.x * kNSum);
+  if (KIND != kPlain) block_sum_store_strided<kNSum>(acc, D.sums + blk, gridDim.x);
 }
 
 // ------------------------------------------------------------ finalize: sum partials
-__global__ void __launch_bounds__(256) k_finalize(const ReqDesc* __restrict__ descs, int nwg, double* /*unused*/) {
-  const ReqDesc& D = descs[blockIdx.x];
-  double acc[kNSum];
+// One workgroup per (slot, request): fin[slot] = sum of the workgroup
+// partials (laid out [slot][workgroup]), eight independent loads in flight
+// per thread, fixed order.
+__global__ void __launch_bounds__(256) k_finalize(const ReqDesc* __restrict__ descs, int nwg) {
+  const ReqDesc& D = descs[blockIdx.y];
+  const double* src = D.sums + (size_t)blockIdx.x * nwg;
+  double acc[1] = {0.0};
+  for (int base = 0; base < nwg; base += 8 * 256) {
+    double v[8];
 #pragma unroll
-  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
-  for (int w = threadIdx.x; w < nwg; w += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < kNSum; ++k) acc[k] += D.sums[(size_t)w * kNSum + k];
+    for (int j = 0; j < 8; ++j) {
+      const int w = base + threadIdx.x + 256 * j;
+      v[j] = w < nwg ? src[w] : 0.0;
+    }
+    acc[0] += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
   }
-  block_sum_store<kNSum>(acc, D.fin);
+  __shared__ double red[4];
+  double x = acc[0];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) D.fin[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // ============================================================ host launchers
@@ -471,7 +514,8 @@ int run_eval(const ShapePlan& sp, int kind, int inmode, int outmode, int fp32, c
 int run_finalize(const ShapePlan& sp, int kind, int fp32, const ReqDesc* dd, int nb, double* out, hipStream_t st) {
   if (!sp.force_generic && fast_path_ok(sp, kind)) return fast_finalize(sp, kind, fp32, dd, nb, out, st);
   const Tiles t = tiles_for(sp, func_nf(kind), fp32 ? (int)sizeof(float2) : (int)sizeof(double2));
-  hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, st, dd, t.ntile, out);
+  (void)out;
+  hipLaunchKernelGGL(k_finalize, dim3(kNSum, nb), dim3(256), 0, st, dd, t.ntile);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -560,8 +604,13 @@ __global__ void __launch_bounds__(256) k_vecops(const VecOp* __restrict__ ops, l
 __global__ void __launch_bounds__(256) k_dotsb(const DotDesc* __restrict__ dd, long long n, double* part) {
   const DotDesc q = dd[blockIdx.y];
   double acc[1] = {0.0};
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    acc[0] += q.a[i] * q.b[i];
+  if (q.b != nullptr) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+      acc[0] += q.a[i] * q.b[i];
+  } else {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+      acc[0] += q.a[i];
+  }
   block_sum_store<1>(acc, part + (size_t)blockIdx.y * kDotBlocksBatched + blockIdx.x);
 }
 __global__ void __launch_bounds__(256) k_dotsb_fin(const double* part, double* res) {

@@ -96,4 +96,14 @@ __device__ __forceinline__ double lind_g(double eta, double lam, double mu) {
 
 __device__ __forceinline__ double fsign(double x) { return signbit(x) ? -1.0 : 1.0; }
 
+// Workgroups are handed to the eight XCDs round-robin (id mod 8), and each
+// XCD has its own L2. Renumbering so that every XCD gets one contiguous range
+// of logical blocks keeps the cache lines shared by neighbouring blocks
+// (line-segment boundaries of the column passes) inside one L2.
+__device__ __forceinline__ int xcd_block(int w, int n) {
+  const int q = n >> 3, r = n & 7, x = w & 7;
+  return x * q + min(x, r) + (w >> 3);
+}
+
+
 }  // namespace ofd

@@ -119,12 +119,17 @@ class Pool:
             raise OFDError(lib().ofd_pool_last_error(self.h).decode())
         return n
 
-    def collect(self, now):
+    def collect(self, now, want_rho=True):
         L = lib()
         done = []
-        for slot, req in list(self.slots.items()):
+        fin = (C.c_int * self.capacity)()
+        nfin = L.ofd_pool_finished(self.h, fin, self.capacity)
+        if nfin <= 0:
+            return done
+        for slot in list(fin[:nfin]):
+            req = self.slots[slot]
             n = self.grid[0] * self.grid[1] * self.grid[2]
-            rho = np.empty(n)
+            rho = np.empty(n) if want_rho else None
             et = np.zeros(9)
             oc = C.c_int(-1)
             st = ofd_opt_stats()
@@ -134,10 +139,20 @@ class Pool:
                 req.energy = et
                 req.outcome = oc.value
                 req.stats = {k: getattr(st, k) for k, _ in st._fields_}
-                req.rho = rho.reshape(self.grid[2], self.grid[1], self.grid[0])
+                req.rho = rho.reshape(self.grid[2], self.grid[1], self.grid[0]) if want_rho else None
                 del self.slots[slot]
                 done.append(req)
         return done
+
+
+def step_pools(pools):
+    """One step of every pool at once (ofd_pools_step: concurrent streams,
+    one host wait per round)."""
+    arr = (C.c_void_p * len(pools))(*[p.h.value for p in pools])
+    n = lib().ofd_pools_step(arr, len(pools))
+    if n < 0:
+        raise OFDError(lib().ofd_pool_last_error(pools[0].h).decode())
+    return n
 
 
 def capacity_for(grid):
@@ -146,7 +161,7 @@ def capacity_for(grid):
 
 
 def run_workload(requests, mixed=True, hbm_budget=64e9, pot_tol=1e-5, mixed_switch=1e-3, verbose=False,
-                 keep_rho=False, fused_dots=True):
+                 keep_rho=False, fused_dots=True, tune=True):
     """Serve `requests` (arrival times in seconds from the start) to completion.
     Returns throughput and latency statistics. fused_dots: the optimizer's
     inner products from the evaluation's sums and the CG recurrences (one
@@ -172,24 +187,30 @@ def run_workload(requests, mixed=True, hbm_budget=64e9, pot_tol=1e-5, mixed_swit
         for req in waiting:
             g = req.grid
             pool = pools.get(g)
+            s = None
             if pool is None:
+                # a new grid shape: tune its pass tiles on this request's own
+                # system before the pool (which copies the tuned tiles) exists
+                s, rho = build_system(req, precision)
+                if tune:
+                    s.autotune(reps=3)
                 pool = pools[g] = Pool(g, capacity_for(g), precision)
             need = float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
             if len(pool.slots) < pool.capacity and in_use + need <= hbm_budget:
-                s, rho = build_system(req, precision)
+                if s is None:
+                    s, rho = build_system(req, precision)
                 if pool.admit(req, s, rho, params) >= 0:
                     req.admitted = now()
                     in_use += need
                     continue
+            if s is not None:
                 s.close()
             still.append(req)
         waiting = still
-        active = 0
-        for pool in pools.values():
-            if pool.slots:
-                active += pool.step()
+        busy = [pool for pool in pools.values() if pool.slots]
+        active = step_pools(busy) if busy else 0
         for g, pool in pools.items():
-            for req in pool.collect(now):
+            for req in pool.collect(now, want_rho=keep_rho):
                 in_use -= float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
                 if not keep_rho:
                     req.rho = None
