@@ -156,11 +156,13 @@ def step_pools(pools):
 
 
 def capacity_for(grid):
-    n = grid[0] * grid[1] * grid[2]
-    return int(max(2, min(32, (64 ** 3 * 16) // n)))
+    """Slots of a pool: enough concurrent requests that one batched launch
+    set fills the device even at 32^3; memory is bounded separately by the
+    HBM budget at admission."""
+    return 32
 
 
-def run_workload(requests, mixed=True, hbm_budget=64e9, pot_tol=1e-5, mixed_switch=1e-3, verbose=False,
+def run_workload(requests, mixed=True, hbm_budget=160e9, pot_tol=1e-5, mixed_switch=1e-3, verbose=False,
                  keep_rho=False, fused_dots=True, tune=True):
     """Serve `requests` (arrival times in seconds from the start) to completion.
     Returns throughput and latency statistics. fused_dots: the optimizer's

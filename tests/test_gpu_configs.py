@@ -1,0 +1,97 @@
+"""GPU: the full-size benchmark configurations.
+
+config 3: fcc Al 6x6x6 (864 atoms, sigma 0.1 bohr), 192^3. The mixed form
+(FP32 passes until the potential norm drops below 1e-3, then FP64 to 1e-6)
+must land within 1e-5 eV/atom of the FP64 solve; the per-term differences
+are printed.
+
+config 5: hcp Mg 2000 atoms (sigma 0.1 bohr), 256^3, WGC second order
+(gamma 2.7) on the reference's own RKSUITE kernel table. The NTN solve must
+converge; at the converged density every energy term must match the oracle to
+1e-10 relative and the ion-electron forces to 1e-6 Ha/bohr.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import profess_oracle as O
+from oracle import refbuild as R
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["total", "KE", "IE", "H", "XC", "II", "TF", "vW", "NL"]
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import profess_amd
+    profess_amd.lib()
+    return profess_amd
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def test_config3_mixed_precision_192(pa):
+    from profess_amd import inputs
+    _say("config 3: fcc Al 864 atoms, 192^3, FP32 -> FP64")
+    c = inputs.al_fcc_case((6, 6, 6), (192, 192, 192), sigma=0.1, seed_ion=31)
+    mg, vals = c["recpot"]
+    res = {}
+    for prec in (0, 1):
+        s = pa.OFDSystem(c["lattice"], c["grid"], c["num_electrons"], kinetic=4, precision=prec)
+        s.build_ion_potential([(mg, vals, c["frac"])])
+        rho0 = np.full(s.shape, c["num_electrons"] / s.vol)
+        r, et, oc, st = pa.ProjNormRhoMinimization(s, rho0, conv_check="POT", pot_tol=1e-6,
+                                                   mixed_switch=1e-3 if prec else 0.0)
+        assert oc == 0, st
+        res[prec] = (et, st)
+        s.close()
+    e64, e32 = res[0][0], res[1][0]
+    nat = len(c["frac"])
+    diffs = {NAMES[k]: (e32[k] - e64[k]) * O.HARTREE_TO_EV / nat for k in range(9)}
+    _say("config 3 per-term (mixed - FP64) eV/atom: " + ", ".join("%s %.2e" % kv for kv in diffs.items()))
+    _say("config 3 evaluations: FP64 %d, mixed %d (%d FP32); seconds FP64 %.2f, mixed %.2f" % (
+        res[0][1]["num_evals"], res[1][1]["num_evals"], res[1][1]["fp32_evals"], res[0][1]["seconds"],
+        res[1][1]["seconds"]))
+    assert abs(diffs["total"]) < 1e-5
+    assert res[1][1]["fp32_evals"] > 0
+
+
+def test_config5_wgc_mg2000_forces(pa):
+    if not R.available():
+        pytest.skip("reference RKSUITE table builder not built")
+    from profess_amd import inputs
+    _say("config 5: hcp Mg 2000 atoms, 256^3, WGC")
+    c = inputs.mg_hcp_case((10, 10, 10), (256, 256, 256), sigma=0.1, seed_ion=53)
+    mg, vals = c["recpot"]
+    ne = c["num_electrons"]
+    s = pa.OFDSystem(c["lattice"], c["grid"], ne, kinetic=5, gamma=2.7)
+    s.build_ion_potential([(mg, vals, c["frac"])])
+    p = s.params
+    table = R.wgc_eta_table(p["alpha"], p["beta"], p["gamma"])
+    s.set_wgc_table(*table)
+    rho0 = np.full(s.shape, ne / s.vol)
+    rho, et, oc, st = pa.ProjNormRhoMinimization(s, rho0, conv_check="POT", pot_tol=1e-5)
+    _say("config 5 solve: outcome %d, %d iterations, %d evaluations, %.2f s, potnorm %.2e" % (
+        oc, st["iterations"], st["num_evals"], st["seconds"], st["final_pot_norm"]))
+    assert oc == 0
+    assert abs(np.sum(rho) * s.dV - ne) < 1e-8 * ne
+    forces = pa.IonElectronForcesSpline(s, rho)
+    pot, et_gpu = pa.CalculatePotentialPlus(s, rho)
+    vion = s.ion_potential()
+    s.close()
+    # the oracle at the converged density
+    cell = O.Cell(c["lattice"], *c["grid"])
+    f = O.Functional(kinetic=5, gamma=2.7).resolve(ne, cell.vol)
+    sysm = O.System(cell, f, vion)
+    sysm.wgc_kernels = O.wgc_kernels_from_table(sysm.qtable, *table, f)
+    _, et_ref = O.calculate_potential_plus(rho, sysm, True, True)
+    for k in range(9):
+        assert abs(et_gpu[k] - et_ref[k]) <= 1e-10 * abs(et_ref[k]) + 1e-12, (NAMES[k], et_gpu[k], et_ref[k])
+    f_ref = O.ion_forces(cell, [(O.make_psp(mg, vals), c["frac"])], rho)
+    err = float(np.max(np.abs(forces - f_ref)))
+    _say("config 5 forces: max |F_gpu - F_oracle| = %.2e Ha/bohr (max |F| %.2e)" % (err, np.max(np.abs(f_ref))))
+    assert err < 1e-6
