@@ -258,12 +258,22 @@ def read_traffic():
     return None
 
 
-def serving_sample(pa, nreq=24, seed=5):
-    """Small continuous-batching sample: fcc Al cells of 4-32 atoms on 32^3/48^3
-    grids through profess_amd.serve (FP32-then-FP64 NTN to 1e-5)."""
+def serving_sample(pa, nreq=128, seed=4):
+    """Batched serving on the mixed-grid workload (fcc Al / hcp Mg cells of
+    4-500 atoms on 32^3 ... 128^3 grids, Poisson arrivals at 1000/s, each a
+    full NTN solve to a potential norm of 1e-5): the FP32-then-FP64 pool and,
+    on the same requests, the plain FP64 drop-in path (ProjNormRhoMinimization
+    one request at a time). Reports solves/s, p50/p99 latency and the ratio."""
     from profess_amd import serve
-    return serve.run_workload(serve.make_requests(nreq, seed=seed, grids=(32, 48), max_atoms=32, rate=200.0),
-                              mixed=True, verbose=False)
+    grids = (32, 48, 64, 80, 96, 120, 128)
+    mk = lambda: serve.make_requests(nreq, seed=seed, grids=grids, max_atoms=500, rate=1000.0)   # noqa: E731
+    pool = serve.run_workload(mk(), mixed=True)
+    plain = serve.run_sequential(mk())
+    out = {k: pool[k] for k in ("requests", "converged", "solves_per_s", "p50_ms", "p99_ms", "grids")}
+    out["mixed_precision"] = True
+    out["plain_fp64_sequential"] = {k: plain[k] for k in ("converged", "solves_per_s", "p50_ms", "p99_ms")}
+    out["speedup_vs_plain_fp64"] = pool["solves_per_s"] / plain["solves_per_s"]
+    return out
 
 
 def main():
