@@ -297,6 +297,7 @@ struct ofd_ctx {
   double* sums = nullptr;    // partials
   double* fin = nullptr;     // kNSum finalized sums
   double* wgck = nullptr;    // 4 x nc
+  double* wtk = nullptr;     // nc: WT kernel coef * LindG / N
   ReqDesc* desc = nullptr;   // one descriptor
   ReqDesc last_desc{};       // what `desc` holds after an enqueue
   ReqDesc* many_desc = nullptr;
@@ -325,6 +326,7 @@ struct ofd_ctx {
     D.vion = vion;
     D.sums = sums;
     D.wgck = wgck;
+    D.wtk = wtk;
     D.work = work;
     D.fin = fin;
     for (int i = 0; i < 9; ++i) D.rb[i] = rb[i];
@@ -429,6 +431,12 @@ extern "C" int ofd_create(const ofd_grid* g, const ofd_functional* f, double num
     c->sums = dalloc<double>((size_t)nrows * kNSum);
     c->fin = dalloc<double>(kNSum);
     if (c->kind == kWGC8) c->wgck = dalloc<double>(4 * nc);
+    if (c->kind == kWT3 || c->kind == kWT4) {
+      c->wtk = dalloc<double>(nc);
+      if (build_wt_kernel(c->sp, c->rb, c->coef, c->tkf, c->lam, c->mu, c->wtk, c->st))
+        throw std::runtime_error("WT kernel build failed");
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
     c->desc = dalloc<ReqDesc>(1);
     c->dot_part = dalloc<double>(512 * 4);
     c->dot_out = dalloc<double>(4);
@@ -464,7 +472,7 @@ extern "C" void ofd_destroy(ofd_ctx* c) {
   if (c->st) hipStreamSynchronize(c->st);
   free_opt(c);
   hipFree(c->vion); hipFree(c->rho); hipFree(c->pot); hipFree(c->work); hipFree(c->sums); hipFree(c->fin);
-  hipFree(c->wgck); hipFree(c->desc); hipFree(c->dot_part); hipFree(c->dot_out);
+  hipFree(c->wgck); hipFree(c->wtk); hipFree(c->desc); hipFree(c->dot_part); hipFree(c->dot_out);
   hipFree(c->many_desc); hipFree(c->many_sums);
   graphs_clear(c->graphs);
   hipFree(c->d_frac); hipFree(c->d_psp_vqs); hipFree(c->d_psp_dd);
@@ -1682,6 +1690,7 @@ extern "C" long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int 
   b += 8 * 8 * nr;                     // optimizer vectors
   b += 8 * (long long)n2 * n3 * kNSum; // partial sums
   if (kinetic == 5) b += 8 * 4 * nc;   // WGC kernels
+  else b += 8 * nc;                    // WT kernel
   (void)precision;
   return b;
 }

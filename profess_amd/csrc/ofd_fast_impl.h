@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
       double sc;
       if (f == 0) sc = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
       else if (f == NF - 1) sc = q2 * invn;
-      else sc = D.coef * lind_g(sqrt(q2) / D.tkf, D.lam, D.mu) * invn;
+      else sc = D.wtk[(size_t)i3 * es + col];
       x[m].x = (R)((double)x[m].x * sc);
       x[m].y = (R)((double)x[m].y * sc);
     }

@@ -49,6 +49,7 @@ struct ReqDesc {
   double* out;          // potential / dEdphi / Ap
   double* sums;         // kNSum partials per epilogue workgroup
   const double* wgck;   // WGC kernels K1..K4 (4 x Nc doubles), kWGC8 only
+  const double* wtk;    // WT kernel coef * LindG / N (Nc doubles), WT kinds
   double* fin;          // kNSum finalized sums of this evaluation (device)
   void* work;           // complex work buffer, NF fields x Nc
   double ca, cb;        // phi_t = ca*u + cb*v
@@ -159,6 +160,9 @@ constexpr int kDotBlocksBatched = 256;
 void vec_dots_batched(const DotDesc* d_dd, int ndots, long long n, double* d_part, double* d_res, hipStream_t st);
 
 // Per-geometry tables.
+// coef * LindG(|G|/tkf; lam, mu) / N on the half grid (WT kinds)
+int build_wt_kernel(const ShapePlan& sp, const double* rb, double coef, double tkf, double lam, double mu,
+                    double* d_out, hipStream_t st);
 int build_wgc_kernels(const ShapePlan& sp, const double* rb, const double* d_eta, const double* d_w,
                       const double* d_w1, const double* d_w2, const double* d_pp0, const double* d_pp1,
                       const double* d_pp2, int nknots, double tkfs, double gamma, double rhoS, double* d_out,

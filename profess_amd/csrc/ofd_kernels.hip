@@ -201,7 +201,7 @@ __device__ __forceinline__ void gop(const ReqDesc& D, const double* __restrict__
   // x[f][re/im] in, out overwrite (NI == NF for all kinds)
   const double hart = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
   if (KIND == kWT3 || KIND == kWT4) {
-    const double K = D.coef * lind_g(sqrt(q2) / D.tkf, D.lam, D.mu) * invn;
+    const double K = D.wtk[gidx];
     const double vw = q2 * invn;
     x[0][0] *= hart; x[0][1] *= hart;
     x[1][0] *= K; x[1][1] *= K;

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "ofd_internal.h"
+#include "ofd_point.h"
 
 namespace ofd {
 
@@ -83,6 +84,31 @@ __global__ void k_wgc_fill(GridDims d, RB rb, const double* __restrict__ eta, co
   out[d.nc + idx] = -s1 / (6.0 * rhoS);
   out[2 * d.nc + idx] = (s2 + (7 - gamma) * s1) / (36.0 * rhoS * rhoS);
   out[3 * d.nc + idx] = (s2 + (1 + gamma) * s1) / (36.0 * rhoS * rhoS);
+}
+
+// The Wang-Teter kernel on the half grid, coef * LindG(|G| / 2kF) / N
+// (FillWT_ReciprocalSpace, KEDF_WTkernel.f90:81-208, with the forward
+// transform's 1/N folded in), in the layout of the half spectra.
+__global__ void k_wt_fill(GridDims d, RB rb, double coef, double tkf, double lam, double mu, double* __restrict__ out) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= d.nc) return;
+  const int k1 = (int)(idx % d.K1);
+  const long long rest = idx / d.K1;
+  const int i2 = (int)(rest % d.n2), i3 = (int)(rest / d.n2);
+  const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
+  const int m3 = i3 > d.n3 / 2 ? i3 - d.n3 : i3;
+  const double q2 = gnorm2(rb.v, k1, m2, m3);
+  out[idx] = coef * lind_g(sqrt(q2) / tkf, lam, mu) * (1.0 / (double)d.nr);
+}
+
+int build_wt_kernel(const ShapePlan& sp, const double* rb, double coef, double tkf, double lam, double mu,
+                    double* d_out, hipStream_t st) {
+  RB r;
+  for (int i = 0; i < 9; ++i) r.v[i] = rb[i];
+  const long long n = sp.d.nc;
+  hipLaunchKernelGGL(k_wt_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sp.d, r, coef, tkf, lam, mu,
+                     d_out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int build_wgc_kernels(const ShapePlan& sp, const double* rb, const double* d_eta, const double* d_w,
