@@ -4,8 +4,9 @@
 //   density XC_LDA.f90:51-196 (the constants below are the formula's);
 //   LindG, MathFunctions.f90:440-528 (including its 15 Taylor terms for
 //   eta > 3.65); the rho powers of the Wang-Teter terms.
-// rho^(1/3), rho^(-1/3), rs and sqrt(rs) are computed once per point and
-// shared between the kinetic and the exchange-correlation terms.
+// rho^(1/3), rho^(-1/3), rs and sqrt(rs) are computed once per point, all
+// from one rho^(-1/6), and shared between the kinetic and the
+// exchange-correlation terms.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,28 +21,44 @@ struct PtVals {
   double exc;          // LDA energy density times rho
 };
 
+// rho^(-1/6) to FP64 accuracy: an FP32 estimate (relative error below 1e-6)
+// refined by two Newton steps on y^-6 = rho, y <- y (7 - rho y^6) / 6, whose
+// error maps e -> -3.5 e^2. Positive, normal rho (as densities are).
+__device__ __forceinline__ double rho_m16(double rho) {
+  // hardware log2 / exp2 (v_log_f32, v_exp_f32) give the seed
+  const float g = __builtin_amdgcn_exp2f(-__builtin_amdgcn_logf((float)rho) * (1.0f / 6.0f));
+  double y = (double)g;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double y2 = y * y;
+    const double y6 = y2 * y2 * y2;
+    y = y * (7.0 - rho * y6) * (1.0 / 6.0);
+  }
+  return y;
+}
+
 __device__ __forceinline__ void point_vals(double rho, double alpha, double beta, PtVals& p) {
   constexpr double cC = 0.62035049089940009;       // (3/(4 pi))^(1/3)
   constexpr double sqrt_cC = 0.78762331789974330;  // sqrt(cC)
   constexpr double cXp = -0.98474502184269641, cXe = -0.73855876638202234;
-  p.phi = sqrt(rho);
-  p.r13 = cbrt(rho);
-  const double i13 = 1.0 / p.r13;
+  // every power from one rho^(-1/6): rho^(-1/3) = y^2, rho^(1/3) = rho y^4,
+  // rho^(1/2) = rho y^3, rho^(5/6) = rho y
+  const double i16 = rho_m16(rho);
+  const double i13 = i16 * i16;
+  p.r13 = rho * (i13 * i13);
+  p.phi = rho * (i13 * i16);
   const double rs = cC * i13;
-  double sq;
+  const double sq = sqrt_cC * i16;   // sqrt(rs)
   if (alpha == 5.0 / 6.0 && beta == 5.0 / 6.0) {
-    const double i16 = sqrt(i13);   // rho^(-1/6)
     p.ram1 = i16;
-    p.ra = p.phi * p.r13;           // rho^(5/6) = rho^(1/2) rho^(1/3)
+    p.ra = rho * i16;
     p.rb = p.ra;
     p.rbm1 = i16;
-    sq = sqrt_cC * i16;
   } else {
     p.ra = pow(rho, alpha);
     p.ram1 = pow(rho, alpha - 1.0);
     p.rb = pow(rho, beta);
     p.rbm1 = pow(rho, beta - 1.0);
-    sq = sqrt(rs);
   }
   double ec;
   if (rs < 1.0) {
@@ -59,7 +76,7 @@ __device__ __forceinline__ void point_vals(double rho, double alpha, double beta
 // rho^alpha and rho^beta for the forward fields
 __device__ __forceinline__ void fwd_pows(double rho, double alpha, double beta, double& ra, double& rb) {
   if (alpha == 5.0 / 6.0 && beta == 5.0 / 6.0) {
-    ra = sqrt(rho) * cbrt(rho);
+    ra = rho * rho_m16(rho);
     rb = ra;
   } else {
     ra = pow(rho, alpha);
