@@ -692,6 +692,8 @@ struct NtnSolver {
   // fused_dots: |p|^2, r.p and r_new.p_old carried by the CG recurrences
   bool fused = false;
   double pp = 0, rp = 0, rp_after = 0;
+  bool pending_cg = false;   // fused: dir and r updates of the last step not queued yet
+  double alpha_cg = 0;
   // getNextGradient / line search
   bool switched = false;
   double norm_phi_dir = 0;
@@ -709,12 +711,16 @@ struct NtnSolver {
 
   // ---- queued vector operations
   void lincomb(double* z, const double* x, const double* y, double a, double b) {   // z = fma(b, y, a x)
-    ops.push_back(VecOp{kOpLincomb, z, x, y, a, b});
+    ops.push_back(VecOp{kOpLincomb, z, x, y, a, b, nullptr, nullptr, nullptr});
   }
   void axpby(double* y, const double* x, double a, double b) {   // y = fma(a, x, b y)
-    ops.push_back(VecOp{kOpAxpby, y, x, y, a, b});
+    ops.push_back(VecOp{kOpAxpby, y, x, y, a, b, nullptr, nullptr, nullptr});
   }
-  void zero(double* z) { ops.push_back(VecOp{kOpZero, z, nullptr, nullptr, 0.0, 0.0}); }
+  void zero(double* z) { ops.push_back(VecOp{kOpZero, z, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr, nullptr}); }
+  // dir += a p; r -= a Ap; p = r + b p, in one pass (bitwise the three axpby)
+  void cg_step(double a, double b) {
+    ops.push_back(VecOp{kOpCG, c->dir, nullptr, nullptr, a, b, c->p, c->r, c->ap});
+  }
 
   void need_dots(int np, const double* a0, const double* b0, const double* a1 = nullptr, const double* b1 = nullptr) {
     need = Need();
@@ -789,7 +795,12 @@ struct NtnSolver {
       return;
     }
     const double beta = ry / ry_last;
-    axpby(c->p, c->r, 1.0, beta);
+    if (pending_cg) {
+      cg_step(alpha_cg, beta);
+      pending_cg = false;
+    } else {
+      axpby(c->p, c->r, 1.0, beta);
+    }
     if (fused) {
       // p' = r + beta p: |p'|^2 = rr + 2 beta r.p + beta^2 |p|^2, r.p' = rr + beta r.p
       const double pp_new = rr + 2.0 * beta * rp_after + beta * beta * pp;
@@ -827,6 +838,10 @@ struct NtnSolver {
     }
   }
   void newton_done() {
+    if (pending_cg) {   // the step's direction update (r is reset by the next Newton start)
+      axpby(c->dir, c->p, alpha_cg, 1.0);
+      pending_cg = false;
+    }
     stats.newton_iters_total += nit;
     switched = false;
     line_prep1();
@@ -994,18 +1009,21 @@ struct NtnSolver {
           return;
         }
         const double alpha = ry / pap;
-        axpby(c->dir, c->p, alpha, 1.0);
-        axpby(c->r, c->ap, -alpha, 1.0);
         rr_last = rr;
         if (fused) {
           // r' = r - alpha Ap: |r'|^2 = rr - 2 alpha r.Ap + alpha^2 |Ap|^2, r'.p = r.p - alpha pAp
           const double rr_new = rr - 2.0 * alpha * v[7] + alpha * alpha * v[8];
           rp_after = rp - alpha * pap;
           if (rr_new > 1e-6 * rr) {
+            // the dir and r updates go out with the next p update (or alone at exit)
+            pending_cg = true;
+            alpha_cg = alpha;
             newton_after_rr(rr_new);
             return;
           }
         }
+        axpby(c->dir, c->p, alpha, 1.0);
+        axpby(c->r, c->ap, -alpha, 1.0);
         st = ST_NEWTON_RR;
         if (fused) need_dots(2, c->r, c->r, c->r, c->p);
         else need_dots(1, c->r, c->r);

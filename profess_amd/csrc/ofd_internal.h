@@ -140,15 +140,20 @@ std::vector<int> fast_tile_candidates(const ShapePlan& sp, int kind, int fp32, i
 bool fast_plane_available(const ShapePlan& sp, int kind, int fp32);
 
 // Batched vector operations over many requests of one grid size.
-enum VecOpKind : int { kOpLincomb = 0, kOpAxpby = 1, kOpZero = 2 };
+enum VecOpKind : int { kOpLincomb = 0, kOpAxpby = 1, kOpZero = 2, kOpCG = 3 };
 struct VecOp {
   int kind;
   double* z;
   const double* x;
   const double* y;
   double a, b;
+  double* x2;        // kOpCG: p (read and written)
+  double* y2;        // kOpCG: r (read and written)
+  const double* w;   // kOpCG: Ap
 };
-// kOpLincomb: z = fma(b, y, a*x); kOpAxpby: z = fma(a, x, b*z); kOpZero: z = 0
+// kOpLincomb: z = fma(b, y, a*x); kOpAxpby: z = fma(a, x, b*z); kOpZero: z = 0;
+// kOpCG (one CG step, the three axpby updates fused):
+//   dir = fma(a, p, dir); r = fma(-a, Ap, r); p = fma(1, r, b*p)
 void vec_ops_batched(const VecOp* d_ops, int nops, long long n, hipStream_t st);
 struct DotDesc {
   const double* a;

@@ -596,9 +596,19 @@ __global__ void __launch_bounds__(256) k_dots_fin(const double* partial, int nbl
 __global__ void __launch_bounds__(256) k_vecops(const VecOp* __restrict__ ops, long long n) {
   const VecOp o = ops[blockIdx.y];
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    if (o.kind == kOpLincomb) o.z[i] = fma(o.b, o.y[i], o.a * o.x[i]);
-    else if (o.kind == kOpAxpby) o.z[i] = fma(o.a, o.x[i], o.b * o.z[i]);
-    else o.z[i] = 0.0;
+    if (o.kind == kOpLincomb) {
+      o.z[i] = fma(o.b, o.y[i], o.a * o.x[i]);
+    } else if (o.kind == kOpAxpby) {
+      o.z[i] = fma(o.a, o.x[i], o.b * o.z[i]);
+    } else if (o.kind == kOpCG) {
+      const double p = o.x2[i];
+      o.z[i] = fma(o.a, p, 1.0 * o.z[i]);
+      const double r = fma(-o.a, o.w[i], 1.0 * o.y2[i]);
+      o.y2[i] = r;
+      o.x2[i] = fma(1.0, r, o.b * p);
+    } else {
+      o.z[i] = 0.0;
+    }
   }
 }
 __global__ void __launch_bounds__(256) k_dotsb(const DotDesc* __restrict__ dd, long long n, double* part) {
