@@ -180,6 +180,32 @@ int ofd_pool_collect(ofd_pool* pool, int slot, double* rho_out, double* energy9,
 /* Device bytes one request of this grid needs (admission by HBM footprint). */
 long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int precision);
 
+/* ---- progress server (no reference counterpart): a host thread that drives
+ *      pools asynchronously. Each pool keeps one phase (vector updates, the
+ *      batched inner products and evaluations its requests wait on) in flight
+ *      on its own stream; when it completes, the thread hands the results to
+ *      the requests and launches the pool's next phase at once, so pools of
+ *      different grids never wait for each other and the caller's admissions
+ *      and collections overlap the device work. Per-request results are
+ *      identical to ofd_pool_step's (each request runs the same operations in
+ *      the same order). ofd_pool_admit, ofd_pool_finished and
+ *      ofd_pool_collect stay usable on a driven pool; ofd_pool_step and
+ *      ofd_pools_step refuse it. ---- */
+typedef struct ofd_server ofd_server;
+int ofd_server_create(ofd_server** out);
+/* From now on the server drives `pool` (a pool belongs to at most one
+ * server; ofd_pool_destroy takes it out of its server first). */
+int ofd_server_add_pool(ofd_server* server, ofd_pool* pool);
+/* Blocks until a request of any of the server's pools has finished since the
+ * previous call, or `timeout_s` has passed. Returns the number of requests
+ * finished since the previous call (0 on timeout), negative on a device or
+ * launch error (message from ofd_server_last_error; the failing pool stops). */
+int ofd_server_wait(ofd_server* server, double timeout_s);
+const char* ofd_server_last_error(const ofd_server* server);
+/* Stops the thread after completing the phases in flight; the pools remain
+ * with the caller and can be stepped or destroyed. */
+void ofd_server_destroy(ofd_server* server);
+
 /* ---- autotuner (no reference counterpart: FFTW chooses the reference's
  *      plans inside FFTW_PLAN_DFT_R2C_3D, Fourier_NEW.f90) ---- */
 typedef struct ofd_tune_result {

@@ -50,7 +50,8 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_calpot", "ofd_calpot_device", "ofd_rho_opt", "ofd_ion_forces", "ofd_pool_create",
            "ofd_pool_destroy", "ofd_pool_last_error", "ofd_pool_admit", "ofd_pool_step", "ofd_pool_collect",
            "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
-           "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished"]
+           "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished", "ofd_server_create",
+           "ofd_server_add_pool", "ofd_server_wait", "ofd_server_last_error", "ofd_server_destroy"]
 
 
 def lib():
@@ -97,6 +98,13 @@ def lib():
     L.ofd_autotune.argtypes = [vp, C.c_int, C.POINTER(ofd_tune_result)]
     L.ofd_pools_step.argtypes = [C.POINTER(vp), C.c_int]
     L.ofd_pool_finished.argtypes = [vp, ip, C.c_int]
+    L.ofd_server_create.argtypes = [C.POINTER(vp)]
+    L.ofd_server_add_pool.argtypes = [vp, vp]
+    L.ofd_server_wait.argtypes = [vp, C.c_double]
+    L.ofd_server_last_error.argtypes = [vp]
+    L.ofd_server_last_error.restype = C.c_char_p
+    L.ofd_server_destroy.argtypes = [vp]
+    L.ofd_server_destroy.restype = None
     _lib = L
     return L
 

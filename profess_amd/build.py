@@ -17,7 +17,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["ofd_kernels.hip", "ofd_fast.hip", "ofd_fast_d3.hip", "ofd_fast_d4.hip", "ofd_fast_f3.hip",
            "ofd_fast_f4.hip", "ofd_tables.hip", "ofd_plan.cpp", "ofd_api.cpp"]
-COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wno-unused-result", "-Wno-unused-value"]
+COMMON = ["-std=c++17", "-O3", "-fPIC", "-pthread", "-Wno-unused-result", "-Wno-unused-value"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
@@ -47,7 +47,7 @@ def build(verbose=False, force=True):
     with cf.ThreadPoolExecutor(max_workers=8) as ex:
         objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "-pthread", "--offload-arch=gfx950", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -22,6 +23,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ofd.h"
@@ -230,13 +232,13 @@ static bool graphs_enabled() {
 // Runs run_eval + run_finalize for `nb` descriptors at `dd` on `st`,
 // through a cached graph when possible. Returns 0 on success.
 static int launch_eval(GraphCache& gc, const ShapePlan& sp, int kind, int inmode, int outmode, int fp32,
-                       const ReqDesc* dd, int nb, double* fin_out, hipStream_t st) {
+                       const ReqDesc* dd, int nb, double* fin_out, hipStream_t st, bool capture = true) {
   const GraphKey key{kind, inmode, outmode, fp32, nb, dd};
   auto direct = [&]() -> int {
     if (run_eval(sp, kind, inmode, outmode, fp32, dd, nb, st)) return -1;
     return run_finalize(sp, kind, fp32, dd, nb, fin_out, st);
   };
-  if (!graphs_enabled()) return direct();
+  if (!capture || !graphs_enabled()) return direct();
   auto it = gc.exec.find(key);
   if (it == gc.exec.end()) {
     if (gc.seen[key]++ == 0) return direct();   // first use: eager
@@ -1684,7 +1686,8 @@ struct ofd_pool {
   // captured graphs; a request's context may be destroyed at collection)
   ShapePlan sp;
   struct Slot {
-    bool used = false;
+    bool used = false;    // reserved by an admission
+    bool ready = false;   // prepared and handed to the pool's stream
     bool done = false;
     ofd_ctx* c = nullptr;
     hipStream_t saved = nullptr;
@@ -1698,10 +1701,21 @@ struct ofd_pool {
   double* h_sums = nullptr;    // pinned
   BatchScratch bs;
   GraphCache graphs;
-  // per step
+  // one phase (queued vector updates, then the dot products and the
+  // evaluations the solvers wait on) is launched at a time on `st`
   std::vector<NtnSolver*> act, dots;
-  std::vector<int> where;
+  std::vector<int> where;      // slot -> row of h_sums in the phase, -1 if none
   int nsum = 0;
+  bool inflight = false;
+  hipEvent_t ev_phase = nullptr;
+  // guards slots and the staging buffers; held by admission, collection and
+  // whoever drives the pool (ofd_pool_step or a server's progress thread)
+  mutable std::mutex mu;
+  // admission prepares a request on the request's own stream with this scratch
+  std::mutex admit_mu;
+  BatchScratch bs_admit;
+  ofd_server* server = nullptr;
+  bool failed = false;
 };
 
 extern "C" long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int precision) {
@@ -1723,7 +1737,7 @@ extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic
   ofd_pool* p = new ofd_pool();
   p->n1 = n1; p->n2 = n2; p->n3 = n3; p->capacity = capacity; p->kinetic = kinetic; p->precision = precision;
   p->slots.resize(capacity);
-  p->where.resize(capacity);
+  p->where.assign(capacity, -1);
   if (hipHostMalloc((void**)&p->h_desc, sizeof(ReqDesc) * capacity, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&p->h_sums, sizeof(double) * kNSum * capacity, hipHostMallocDefault) != hipSuccess) {
     p->err = "pinned host allocation failed";
@@ -1745,6 +1759,7 @@ extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic
     }
   }
   if (hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev_phase, hipEventDisableTiming) != hipSuccess ||
       hipMalloc(&p->d_desc, sizeof(ReqDesc) * capacity) != hipSuccess ||
       hipMalloc(&p->d_sums, sizeof(double) * kNSum * capacity) != hipSuccess) {
     p->err = "pool allocation failed";
@@ -1755,27 +1770,38 @@ extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic
   return 0;
 }
 
+static void server_remove_pool(ofd_server* s, ofd_pool* p);
+
 extern "C" void ofd_pool_destroy(ofd_pool* p) {
   if (!p) return;
+  if (p->server) server_remove_pool(p->server, p);
   hipStreamSynchronize(p->st);
   for (auto& s : p->slots)
     if (s.used && s.c) {
-      s.c->st = s.saved;
+      if (s.ready) s.c->st = s.saved;
       ofd_destroy(s.c);
     }
   p->bs.free_all();
+  p->bs_admit.free_all();
   graphs_clear(p->graphs);
   if (p->sp.tw_x) shape_plan_free(&p->sp);
   hipHostFree(p->h_desc);
   hipHostFree(p->h_sums);
   hipFree(p->d_desc);
   hipFree(p->d_sums);
+  if (p->ev_phase) hipEventDestroy(p->ev_phase);
   hipStreamDestroy(p->st);
   delete p;
 }
 
 extern "C" const char* ofd_pool_last_error(const ofd_pool* p) { return p ? p->err.c_str() : "null pool"; }
 
+static void server_kick(ofd_server* s);
+
+// Admission: reserves a slot, then prepares the request (density upload,
+// phi = sqrt|rho|, the electron count) on the request's own stream without
+// holding the pool, so a server keeps driving the pool meanwhile; the slot
+// joins the pool's phases once it is ready.
 extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const ofd_opt_params* prm) {
   if (!p || !c || !rho) return -1;
   if (c->sp.d.n1 != p->n1 || c->sp.d.n2 != p->n2 || c->sp.d.n3 != p->n3 || c->kinetic != p->kinetic) {
@@ -1787,74 +1813,67 @@ extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const 
     return -1;
   }
   int slot = -1;
-  for (int i = 0; i < p->capacity; ++i)
-    if (!p->slots[i].used) { slot = i; break; }
-  if (slot < 0) return -2;
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    for (int i = 0; i < p->capacity; ++i)
+      if (!p->slots[i].used) { slot = i; break; }
+    if (slot < 0) return -2;
+    p->slots[slot] = ofd_pool::Slot();
+    p->slots[slot].used = true;
+    p->slots[slot].c = c;
+  }
   auto& s = p->slots[slot];
   try {
+    std::lock_guard<std::mutex> ak(p->admit_mu);
     HIPCHK(hipStreamSynchronize(c->st));
-    s = ofd_pool::Slot();
-    s.used = true;
-    s.c = c;
-    s.saved = c->st;
-    c->st = p->st;   // every request of the pool runs on the pool's stream
-    s.t0 = std::chrono::steady_clock::now();
     alloc_opt(c);
-    HIPCHK(hipMemcpyAsync(c->rho, rho, c->nr() * sizeof(double), hipMemcpyHostToDevice, p->st));
-    solver_prepare(s.S, c, prm, p->bs, p->st);
+    HIPCHK(hipMemcpyAsync(c->rho, rho, c->nr() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    solver_prepare(s.S, c, prm, p->bs_admit, c->st);
+    HIPCHK(hipStreamSynchronize(c->st));
   } catch (const std::exception& e) {
+    std::lock_guard<std::mutex> lk(p->mu);
     p->err = e.what();
-    s.used = false;
-    c->st = s.saved;
+    s = ofd_pool::Slot();
     return -1;
   }
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    s.t0 = std::chrono::steady_clock::now();
+    s.saved = c->st;
+    c->st = p->st;   // every request of the pool runs on the pool's stream
+    s.ready = true;
+  }
+  if (p->server) server_kick(p->server);
   return slot;
 }
 
-// Advances every active request of every pool by one evaluation. Dot rounds
-// first: each round launches, on every pool's own stream, the batched vector
-// updates and the inner products of all requests that need them, then waits
-// once for all pools; rounds repeat until every request waits on an
-// evaluation. Then the evaluations, batched per (functional kind, output
-// mode, precision) within a pool, launched for all pools before one wait.
-// Pools on different streams run concurrently on the device.
-static int pools_step(ofd_pool* const* pools, int npool) {
-  for (int ip = 0; ip < npool; ++ip) {
-    ofd_pool* p = pools[ip];
-    p->act.clear();
-    for (auto& s : p->slots)
-      if (s.used && !s.done) p->act.push_back(&s.S);
+// Launches one phase of pool `p` on its stream (the caller holds p->mu):
+// the queued vector updates of every ready request, then, for the requests
+// that wait on inner products (when with_dots), one batched dot-product
+// launch, and for those that wait on an evaluation (when with_evals), one
+// batched launch set per (functional kind, output mode, precision); the
+// results are copied to pinned memory and ev_phase is recorded after them.
+// Returns false (nothing launched) when no request waits on either.
+static bool launch_phase(ofd_pool* p, bool with_dots, bool with_evals) {
+  const long long n = (long long)p->n1 * p->n2 * p->n3;
+  p->act.clear();
+  for (auto& s : p->slots)
+    if (s.used && s.ready && !s.done) p->act.push_back(&s.S);
+  std::fill(p->where.begin(), p->where.end(), -1);
+  p->dots.clear();
+  p->nsum = 0;
+  if (p->act.empty()) return false;
+  flush_ops(p->act, n, p->bs, p->st);
+  if (with_dots) {
+    for (auto* s : p->act)
+      if (s->need.kind == NEED_DOTS) p->dots.push_back(s);
+    if (!p->dots.empty()) launch_dots(p->dots, n, p->bs, p->st);
   }
-  for (int guard = 0; guard < 64; ++guard) {
-    bool any = false;
-    for (int ip = 0; ip < npool; ++ip) {
-      ofd_pool* p = pools[ip];
-      const long long n = (long long)p->n1 * p->n2 * p->n3;
-      flush_ops(p->act, n, p->bs, p->st);
-      p->dots.clear();
-      for (auto* s : p->act)
-        if (s->need.kind == NEED_DOTS) p->dots.push_back(s);
-      if (!p->dots.empty()) {
-        launch_dots(p->dots, n, p->bs, p->st);
-        any = true;
-      }
-    }
-    if (!any) break;
-    for (int ip = 0; ip < npool; ++ip)
-      if (!pools[ip]->dots.empty()) HIPCHK(hipStreamSynchronize(pools[ip]->st));
-    for (int ip = 0; ip < npool; ++ip)
-      if (!pools[ip]->dots.empty()) deliver_dots(pools[ip]->dots, pools[ip]->bs);
-  }
-  for (int ip = 0; ip < npool; ++ip) {
-    ofd_pool* p = pools[ip];
-    const long long n = (long long)p->n1 * p->n2 * p->n3;
-    flush_ops(p->act, n, p->bs, p->st);
-    // one batched launch set per (functional kind, output mode, precision)
+  if (with_evals) {
     std::vector<int> groups[3][3][2];
     for (int i = 0; i < p->capacity; ++i) {
       auto& s = p->slots[i];
-      p->where[i] = -1;
-      if (!s.used || s.done) continue;
+      if (!s.used || !s.ready || s.done) continue;
       if (s.S.need.kind == NEED_EVAL) groups[s.c->kind][s.S.need.ev.outmode][s.S.fp32 ? 1 : 0].push_back(i);
     }
     int off = 0;
@@ -1870,8 +1889,10 @@ static int pools_step(ofd_pool* const* pools, int npool) {
           }
           HIPCHK(hipMemcpyAsync(p->d_desc + off, p->h_desc + off, sizeof(ReqDesc) * g.size(),
                                 hipMemcpyHostToDevice, p->st));
+          // no capture while a server drives the pool: another thread's
+          // allocations and frees would run during the capture
           if (launch_eval(p->graphs, p->sp, kd, kInPhi, om, pr, p->d_desc + off, (int)g.size(),
-                          p->d_sums + (size_t)off * kNSum, p->st))
+                          p->d_sums + (size_t)off * kNSum, p->st, p->server == nullptr))
             throw std::runtime_error("batched evaluation launch failed");
           off += (int)g.size();
         }
@@ -1879,17 +1900,71 @@ static int pools_step(ofd_pool* const* pools, int npool) {
     if (off > 0)
       HIPCHK(hipMemcpyAsync(p->h_sums, p->d_sums, sizeof(double) * kNSum * off, hipMemcpyDeviceToHost, p->st));
   }
+  if (p->dots.empty() && p->nsum == 0) return false;
+  HIPCHK(hipEventRecord(p->ev_phase, p->st));
+  p->inflight = true;
+  return true;
+}
+
+// Hands the results of the completed phase to its requests (the caller holds
+// p->mu and has seen ev_phase complete). Returns how many requests finished.
+static int complete_phase(ofd_pool* p) {
+  if (!p->dots.empty()) deliver_dots(p->dots, p->bs);
+  p->dots.clear();
+  int fin = 0;
+  for (int i = 0; i < p->capacity; ++i) {
+    auto& s = p->slots[i];
+    if (!s.used || !s.ready || s.done) continue;
+    if (p->where[i] >= 0) s.S.deliver(&p->h_sums[(size_t)p->where[i] * kNSum]);
+    if (s.S.need.kind == NEED_NONE) {
+      s.done = true;
+      ++fin;
+    }
+  }
+  std::fill(p->where.begin(), p->where.end(), -1);
+  p->nsum = 0;
+  p->inflight = false;
+  return fin;
+}
+
+static int count_active(const ofd_pool* p) {
+  int k = 0;
+  for (auto& s : p->slots)
+    if (s.used && s.ready && !s.done) ++k;
+  return k;
+}
+
+// Advances every active request of every pool by one evaluation. Dot rounds
+// first: each round launches, on every pool's own stream, the batched vector
+// updates and the inner products of all requests that need them, then waits
+// once for all pools; rounds repeat until every request waits on an
+// evaluation. Then the evaluations, batched per (functional kind, output
+// mode, precision) within a pool, launched for all pools before one wait.
+// Pools on different streams run concurrently on the device.
+static int pools_step(ofd_pool* const* pools, int npool) {
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (int ip = 0; ip < npool; ++ip) locks.emplace_back(pools[ip]->mu);
+  for (int ip = 0; ip < npool; ++ip)
+    if (pools[ip]->server) throw std::runtime_error("the pool is driven by a server");
+  for (int guard = 0; guard < 64; ++guard) {
+    bool any = false;
+    for (int ip = 0; ip < npool; ++ip) any = launch_phase(pools[ip], true, false) || any;
+    if (!any) break;
+    for (int ip = 0; ip < npool; ++ip)
+      if (pools[ip]->inflight) {
+        HIPCHK(hipEventSynchronize(pools[ip]->ev_phase));
+        complete_phase(pools[ip]);
+      }
+  }
+  for (int ip = 0; ip < npool; ++ip) launch_phase(pools[ip], false, true);
   int still = 0;
   for (int ip = 0; ip < npool; ++ip) {
     ofd_pool* p = pools[ip];
-    if (p->nsum > 0) HIPCHK(hipStreamSynchronize(p->st));
-    for (int i = 0; i < p->capacity; ++i) {
-      auto& s = p->slots[i];
-      if (!s.used) continue;
-      if (p->where[i] >= 0) s.S.deliver(&p->h_sums[(size_t)p->where[i] * kNSum]);
-      if (s.S.need.kind == NEED_NONE) s.done = true;
-      if (!s.done) ++still;
+    if (p->inflight) {
+      HIPCHK(hipEventSynchronize(p->ev_phase));
+      complete_phase(p);
     }
+    still += count_active(p);
   }
   return still;
 }
@@ -1918,6 +1993,7 @@ extern "C" int ofd_pools_step(ofd_pool* const* pools, int npool) {
 
 extern "C" int ofd_pool_finished(const ofd_pool* p, int* slots, int max_slots) {
   if (!p || (!slots && max_slots > 0)) return -1;
+  std::lock_guard<std::mutex> lk(p->mu);
   int k = 0;
   for (int i = 0; i < p->capacity; ++i)
     if (p->slots[i].used && p->slots[i].done) {
@@ -1930,6 +2006,7 @@ extern "C" int ofd_pool_finished(const ofd_pool* p, int* slots, int max_slots) {
 extern "C" int ofd_pool_collect(ofd_pool* p, int slot, double* rho_out, double* energy9, int* outcome,
                                 ofd_opt_stats* stats, ofd_ctx** ctx_out) {
   if (!p || slot < 0 || slot >= p->capacity) return -1;
+  std::unique_lock<std::mutex> lk(p->mu);
   auto& s = p->slots[slot];
   if (!s.used) return -1;
   if (!s.done) return 0;
@@ -1954,8 +2031,179 @@ extern "C" int ofd_pool_collect(ofd_pool* p, int slot, double* rho_out, double* 
   }
   c->st = s.saved;
   if (!s.S.err.empty()) c->err = s.S.err;
+  s = ofd_pool::Slot();
+  lk.unlock();
   if (ctx_out) *ctx_out = c;
   else ofd_destroy(c);
-  s = ofd_pool::Slot();
   return 1;
+}
+
+// ====================================================================== server
+// A progress thread that drives its pools without a global step: each pool
+// has at most one phase in flight on its own stream; when the phase's event
+// has fired, the thread hands the results to the requests and at once
+// launches the pool's next phase. Pools of different grids thus never wait
+// for each other, and the caller's admissions and collections (which take a
+// pool's mutex only briefly) run while the device works.
+struct ofd_server {
+  std::mutex mu;                  // pools, counters, error
+  std::mutex sweep_mu;            // held by the thread for one pass over the pools
+  std::condition_variable cv_done;
+  std::condition_variable cv_work;
+  std::vector<ofd_pool*> pools;
+  std::thread th;
+  bool stop = false;
+  bool kick = false;
+  long long finished = 0, reported = 0;
+  std::string err;
+};
+
+static void server_kick(ofd_server* s) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->kick = true;
+  }
+  s->cv_work.notify_one();
+}
+
+static void server_loop(ofd_server* s) {
+  std::vector<ofd_pool*> ps;
+  for (;;) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->stop) return;
+      ps = s->pools;
+    }
+    bool progressed = false, busy = false;
+    int newly = 0;
+    std::string failure;
+    {
+      std::lock_guard<std::mutex> sw(s->sweep_mu);
+      for (ofd_pool* p : ps) {
+        std::unique_lock<std::mutex> lk(p->mu, std::try_to_lock);
+        if (!lk.owns_lock()) {
+          busy = true;
+          continue;
+        }
+        if (p->failed) continue;
+        try {
+          if (p->inflight) {
+            const hipError_t q = hipEventQuery(p->ev_phase);
+            if (q == hipErrorNotReady) {
+              busy = true;
+              continue;
+            }
+            if (q != hipSuccess) throw std::runtime_error(std::string("phase failed: ") + hipGetErrorString(q));
+            newly += complete_phase(p);
+            progressed = true;
+          }
+          if (launch_phase(p, true, true)) {
+            busy = true;
+            progressed = true;
+          }
+        } catch (const std::exception& e) {
+          p->failed = true;
+          p->err = e.what();
+          failure = e.what();
+        }
+      }
+    }
+    if (newly > 0 || !failure.empty()) {
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->finished += newly;
+        if (!failure.empty() && s->err.empty()) s->err = failure;
+      }
+      s->cv_done.notify_all();
+    }
+    if (progressed) continue;
+    if (busy) {
+      std::this_thread::yield();
+    } else {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait_for(lk, std::chrono::milliseconds(1), [&] { return s->stop || s->kick; });
+      s->kick = false;
+    }
+  }
+}
+
+static void server_remove_pool(ofd_server* s, ofd_pool* p) {
+  std::lock_guard<std::mutex> sw(s->sweep_mu);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->pools.erase(std::remove(s->pools.begin(), s->pools.end(), p), s->pools.end());
+  }
+  std::lock_guard<std::mutex> pk(p->mu);
+  if (p->inflight) {
+    if (hipEventSynchronize(p->ev_phase) == hipSuccess) {
+      try {
+        complete_phase(p);
+      } catch (const std::exception& e) {
+        p->err = e.what();
+        p->failed = true;
+      }
+    }
+    p->inflight = false;
+  }
+  p->server = nullptr;
+}
+
+extern "C" int ofd_server_create(ofd_server** out) {
+  if (!out) return -1;
+  ofd_server* s = new ofd_server();
+  try {
+    s->th = std::thread(server_loop, s);
+  } catch (const std::exception&) {
+    delete s;
+    *out = nullptr;
+    return -1;
+  }
+  *out = s;
+  return 0;
+}
+
+extern "C" int ofd_server_add_pool(ofd_server* s, ofd_pool* p) {
+  if (!s || !p) return -1;
+  {
+    std::lock_guard<std::mutex> pk(p->mu);
+    if (p->server) return p->server == s ? 0 : -1;
+    p->server = s;
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->pools.push_back(p);
+  }
+  server_kick(s);
+  return 0;
+}
+
+extern "C" int ofd_server_wait(ofd_server* s, double timeout_s) {
+  if (!s) return -1;
+  std::unique_lock<std::mutex> lk(s->mu);
+  const auto dt = std::chrono::duration<double>(std::max(0.0, timeout_s));
+  s->cv_done.wait_for(lk, std::chrono::duration_cast<std::chrono::microseconds>(dt),
+                      [&] { return s->finished != s->reported || !s->err.empty(); });
+  if (!s->err.empty()) return -1;
+  const long long k = s->finished - s->reported;
+  s->reported = s->finished;
+  return (int)k;
+}
+
+extern "C" const char* ofd_server_last_error(const ofd_server* s) { return s ? s->err.c_str() : "null server"; }
+
+extern "C" void ofd_server_destroy(ofd_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  std::vector<ofd_pool*> ps;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    ps = s->pools;
+  }
+  for (ofd_pool* p : ps) server_remove_pool(s, p);
+  delete s;
 }

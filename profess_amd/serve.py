@@ -4,11 +4,12 @@ Requests (fcc Al / hcp Mg supercells, synthetic V_loc, uniform start density)
 arrive over time. A request is admitted as soon as (a) it has arrived, (b) the
 pool for its grid shape has a free slot and (c) its device footprint
 (``ofd_request_bytes``) fits in the HBM budget; its rho, ion potential, WGC
-kernels and optimizer state then live on the device until it converges. Every
-server step advances every active request of every pool by one evaluation:
-one batched launch set per (grid shape, optimizer phase, precision), so
-requests that arrive while others run share those launches. Latency is
-arrival -> converged result.
+kernels and optimizer state then live on the device until it converges. A
+native progress thread (``Server``) drives every pool on its own stream: one
+phase at a time per pool, each phase one batched launch set per (optimizer
+phase, precision) over all of the pool's active requests, so requests that
+arrive while others run share those launches, and pools of different grids
+never wait for each other. Latency is arrival -> converged result.
 """
 from __future__ import annotations
 
@@ -155,6 +156,35 @@ def step_pools(pools):
     return n
 
 
+class Server:
+    """A native progress thread (ofd_server) that drives the pools added to it
+    asynchronously: every pool keeps one phase in flight on its own stream and
+    launches the next as soon as it completes, while this thread admits and
+    collects requests."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        if lib().ofd_server_create(C.byref(h)) != 0:
+            raise OFDError("server create failed")
+        self.h = h
+
+    def add(self, pool):
+        if lib().ofd_server_add_pool(self.h, pool.h) != 0:
+            raise OFDError("pool cannot join the server")
+
+    def wait(self, timeout):
+        """Requests finished since the previous call (0 after `timeout` s)."""
+        k = lib().ofd_server_wait(self.h, float(timeout))
+        if k < 0:
+            raise OFDError(lib().ofd_server_last_error(self.h).decode())
+        return k
+
+    def close(self):
+        if self.h is not None:
+            lib().ofd_server_destroy(self.h)
+            self.h = None
+
+
 def capacity_for(grid):
     """Slots of a pool: enough concurrent requests that one batched launch
     set fills the device even at 32^3; memory is bounded separately by the
@@ -163,11 +193,13 @@ def capacity_for(grid):
 
 
 def run_workload(requests, mixed=True, hbm_budget=160e9, pot_tol=1e-5, mixed_switch=1e-3, verbose=False,
-                 keep_rho=False, fused_dots=True, tune=True):
+                 keep_rho=False, fused_dots=True, tune=True, threaded=True):
     """Serve `requests` (arrival times in seconds from the start) to completion.
     Returns throughput and latency statistics. fused_dots: the optimizer's
     inner products from the evaluation's sums and the CG recurrences (one
-    host round trip per inner iteration)."""
+    host round trip per inner iteration). threaded: the pools are driven by
+    a native progress thread (Server); otherwise by lock-step
+    ofd_pools_step calls from this loop."""
     precision = 1 if mixed else 0
     params = opt_params(pot_tol=pot_tol, conv_check="POT", mixed_switch=mixed_switch if mixed else 0.0,
                         fused_dots=fused_dots)
@@ -179,52 +211,65 @@ def run_workload(requests, mixed=True, hbm_budget=160e9, pot_tol=1e-5, mixed_swi
     finished = []
     qi = 0
     waiting = []
-    while len(finished) < len(requests):
-        t = now()
-        while qi < len(pending) and pending[qi].arrival <= t:
-            waiting.append(pending[qi])
-            qi += 1
-        # admission: FIFO among arrived requests that fit
-        still = []
-        for req in waiting:
-            g = req.grid
-            pool = pools.get(g)
-            s = None
-            if pool is None:
-                # a new grid shape: tune its pass tiles on this request's own
-                # system before the pool (which copies the tuned tiles) exists
-                s, rho = build_system(req, precision)
-                if tune:
-                    s.autotune(reps=3)
-                pool = pools[g] = Pool(g, capacity_for(g), precision)
-            need = float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
-            if len(pool.slots) < pool.capacity and in_use + need <= hbm_budget:
-                if s is None:
+    server = Server() if threaded else None
+    try:
+        while len(finished) < len(requests):
+            t = now()
+            while qi < len(pending) and pending[qi].arrival <= t:
+                waiting.append(pending[qi])
+                qi += 1
+            # admission: FIFO among arrived requests that fit
+            still = []
+            for req in waiting:
+                g = req.grid
+                pool = pools.get(g)
+                s = None
+                if pool is None:
+                    # a new grid shape: tune its pass tiles on this request's own
+                    # system before the pool (which copies the tuned tiles) exists
                     s, rho = build_system(req, precision)
-                if pool.admit(req, s, rho, params) >= 0:
-                    req.admitted = now()
-                    in_use += need
-                    continue
-            if s is not None:
-                s.close()
-            still.append(req)
-        waiting = still
-        busy = [pool for pool in pools.values() if pool.slots]
-        active = step_pools(busy) if busy else 0
-        for g, pool in pools.items():
-            for req in pool.collect(now, want_rho=keep_rho):
-                in_use -= float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
-                if not keep_rho:
-                    req.rho = None
-                finished.append(req)
-                if verbose:
-                    print("req %d grid %s done in %.3f s (E = %.8f Ha, outcome %d)"
-                          % (req.rid, g, req.finished - req.arrival, req.energy[0], req.outcome), flush=True)
-        if active == 0 and not waiting and qi < len(pending):
-            time.sleep(max(0.0, min(0.01, pending[qi].arrival - now())))
-    makespan = now()
-    for p in pools.values():
-        p.close()
+                    if tune:
+                        s.autotune(reps=3)
+                    pool = pools[g] = Pool(g, capacity_for(g), precision)
+                    if server is not None:
+                        server.add(pool)
+                need = float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
+                if len(pool.slots) < pool.capacity and in_use + need <= hbm_budget:
+                    if s is None:
+                        s, rho = build_system(req, precision)
+                    if pool.admit(req, s, rho, params) >= 0:
+                        req.admitted = now()
+                        in_use += need
+                        continue
+                if s is not None:
+                    s.close()
+                still.append(req)
+            waiting = still
+            if server is not None:
+                # wake on a finished request, or in time for the next arrival
+                nxt = pending[qi].arrival - now() if qi < len(pending) else 0.005
+                server.wait(max(0.0, min(0.005, nxt)))
+                active = sum(len(p.slots) for p in pools.values())
+            else:
+                busy = [pool for pool in pools.values() if pool.slots]
+                active = step_pools(busy) if busy else 0
+            for g, pool in pools.items():
+                for req in pool.collect(now, want_rho=keep_rho):
+                    in_use -= float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
+                    if not keep_rho:
+                        req.rho = None
+                    finished.append(req)
+                    if verbose:
+                        print("req %d grid %s done in %.3f s (E = %.8f Ha, outcome %d)"
+                              % (req.rid, g, req.finished - req.arrival, req.energy[0], req.outcome), flush=True)
+            if server is None and active == 0 and not waiting and qi < len(pending):
+                time.sleep(max(0.0, min(0.01, pending[qi].arrival - now())))
+        makespan = now()
+    finally:
+        if server is not None:
+            server.close()
+        for p in pools.values():
+            p.close()
     lat = np.array([r.finished - r.arrival for r in finished])
     ok = sum(1 for r in finished if r.outcome == 0)
     return {"requests": len(requests), "converged": ok, "makespan_s": makespan,

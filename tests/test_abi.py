@@ -41,8 +41,27 @@ def test_request_bytes_is_monotone_and_host_only():
     b128 = request_bytes((128, 128, 128))
     bw = request_bytes((128, 128, 128), kinetic=5)
     assert 0 < b32 < b128 < bw
-    # about 25 real grids at 128^3 (SURVEY footprint estimate)
+    # about 25 real grids at 128^3
     assert 0.2e9 < b128 < 0.6e9
+
+
+def test_server_thread_starts_waits_and_stops_without_pools():
+    """The progress server's thread and its wait/stop protocol are host-only
+    (no pool, no device call): a wait with nothing to drive times out with 0."""
+    import ctypes as C
+    import time
+    from profess_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.ofd_server_create(C.byref(h)) == 0 and h.value
+    t = time.perf_counter()
+    assert L.ofd_server_wait(h, 0.05) == 0
+    assert time.perf_counter() - t >= 0.04
+    assert L.ofd_server_last_error(h) == b""
+    assert L.ofd_server_add_pool(h, None) < 0
+    L.ofd_server_destroy(h)
 
 
 def test_input_generators():

@@ -139,3 +139,24 @@ def test_pool_replays_after_requests_leave(pa):
         one = serve.make_requests(6, seed=9, grids=(32,), max_atoms=8, rate=1e6)[r.rid]
         serve.run_workload([one], mixed=False)
         assert np.array_equal(one.energy, r.energy), (one.energy, r.energy)
+
+
+def test_server_thread_matches_lock_step(pa):
+    """Pools driven by the native progress thread (one phase in flight per
+    pool, no global step) give every request the same operations in the same
+    order as lock-step ofd_pools_step: energies, densities and evaluation
+    counts are bitwise equal, in FP64 and in the FP32-then-FP64 mode."""
+    from profess_amd import serve
+    for mixed in (False, True):
+        res = {}
+        for threaded in (False, True):
+            reqs = serve.make_requests(8, seed=21, grids=(32, 48, 64), max_atoms=16, rate=2000.0)
+            out = serve.run_workload(reqs, mixed=mixed, keep_rho=True, threaded=threaded, tune=False)
+            assert out["converged"] == len(reqs)
+            res[threaded] = {r.rid: (r.energy.copy(), r.rho.copy(), r.stats["num_evals"],
+                                     r.stats["fp32_evals"]) for r in reqs}
+        for rid, (e0, rho0, n0, f0) in res[False].items():
+            e1, rho1, n1, f1 = res[True][rid]
+            assert (n0, f0) == (n1, f1), (rid, n0, n1, f0, f1)
+            assert np.array_equal(e0, e1), (rid, e0, e1)
+            assert np.array_equal(rho0, rho1), rid
