@@ -85,6 +85,9 @@ void ofd_destroy(ofd_ctx* ctx);
 const char* ofd_last_error(const ofd_ctx* ctx);
 /* resolved functional parameters: alpha, beta, gamma, rho0, rhoS, lambda, mu */
 int ofd_get_params(const ofd_ctx* ctx, double* out7);
+/* the cell volume (bohr^3) as computed from the lattice (cell%vol,
+ * CellInfo.f90:247-330); the uniform density is num_electrons / volume */
+int ofd_get_volume(const ofd_ctx* ctx, double* vol);
 
 /* ionPotReal and ionIonEnergy as RefreshIonTerms leaves them
  * (RefreshIons.f90:196,221). */
@@ -158,8 +161,9 @@ int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic, int preci
 void ofd_pool_destroy(ofd_pool* pool);
 const char* ofd_pool_last_error(const ofd_pool* pool);
 /* Admit a request: takes ownership of a context (same grid, ion terms set)
- * and the starting density; returns the slot id (>= 0) or a negative status
- * when the pool is full. */
+ * and the starting density (rho_init NULL: the uniform density
+ * num_electrons / volume); returns the slot id (>= 0), -2 when the pool is
+ * full, another negative status on error. */
 int ofd_pool_admit(ofd_pool* pool, ofd_ctx* ctx, const double* rho_init, const ofd_opt_params* params);
 /* Advance every active request by one evaluation (one batched launch set).
  * Returns the number of requests still active. */
@@ -173,8 +177,10 @@ int ofd_pools_step(ofd_pool* const* pools, int npool);
  * max_slots of them and returns how many there are. */
 int ofd_pool_finished(const ofd_pool* pool, int* slots, int max_slots);
 /* Collect a finished request: returns 1 and fills outputs when slot `slot`
- * has finished, 0 while it is running. The slot is freed on collection and
- * the context is handed back to the caller. */
+ * has finished, 0 while it is running. The slot is freed on collection; the
+ * context is handed back through ctx_out, or destroyed when ctx_out is NULL.
+ * With rho_out NULL the final density is not formed (the context's device
+ * density is then stale) and the call does not wait on the device. */
 int ofd_pool_collect(ofd_pool* pool, int slot, double* rho_out, double* energy9, int* rho_outcome,
                      ofd_opt_stats* stats, ofd_ctx** ctx_out);
 /* Device bytes one request of this grid needs (admission by HBM footprint). */

@@ -51,7 +51,8 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_pool_destroy", "ofd_pool_last_error", "ofd_pool_admit", "ofd_pool_step", "ofd_pool_collect",
            "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
            "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished", "ofd_server_create",
-           "ofd_server_add_pool", "ofd_server_wait", "ofd_server_last_error", "ofd_server_destroy"]
+           "ofd_server_add_pool", "ofd_server_wait", "ofd_server_last_error", "ofd_server_destroy",
+           "ofd_get_volume"]
 
 
 def lib():
@@ -70,6 +71,7 @@ def lib():
     L.ofd_last_error.argtypes = [vp]
     L.ofd_last_error.restype = C.c_char_p
     L.ofd_get_params.argtypes = [vp, dp]
+    L.ofd_get_volume.argtypes = [vp, dp]
     L.ofd_set_ion_terms.argtypes = [vp, dp, C.c_double]
     L.ofd_build_ion_potential.argtypes = [vp, C.c_int, ip, dp, dp, ip, ip, dp, C.c_int, C.c_double]
     L.ofd_get_ion_potential.argtypes = [vp, dp]

@@ -42,7 +42,9 @@ class OFDSystem:
             msg = L.ofd_last_error(h).decode() if h.value else "ofd_create failed"
             self.close()
             raise OFDError(msg)
-        self.vol = abs(np.linalg.det(self.lattice))
+        v = C.c_double(0.0)
+        L.ofd_get_volume(h, C.byref(v))
+        self.vol = v.value   # the library's cell%vol: uniform densities agree bit for bit
         self.dV = self.vol / (self.n1 * self.n2 * self.n3)
 
     # -- lifetime

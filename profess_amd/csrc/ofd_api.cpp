@@ -44,12 +44,39 @@ constexpr double kCTF = 2.87123400018819;
     }                                                                      \
   } while (0)
 
+// device buffers of contexts and scratch come from the process-wide cache
 template <typename T>
 T* dalloc(size_t n) {
-  void* p = nullptr;
-  if (n == 0) n = 1;
-  HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  void* p = dev_alloc((n ? n : 1) * sizeof(T));
+  if (!p) throw std::runtime_error("device allocation failed");
   return (T*)p;
+}
+
+// Streams of contexts are recycled: creating and destroying a stream per
+// request costs more than the request's own launches on a busy device.
+std::mutex g_stream_mu;
+std::vector<hipStream_t> g_streams;
+hipStream_t stream_get() {
+  {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    if (!g_streams.empty()) {
+      hipStream_t s = g_streams.back();
+      g_streams.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+void stream_put(hipStream_t s) {
+  if (!s) return;
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    hipStreamDestroy(s);
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_stream_mu);
+  g_streams.push_back(s);
 }
 
 // ------------------------------------------------------------------ Moré–Thuente
@@ -421,11 +448,11 @@ extern "C" int ofd_create(const ofd_grid* g, const ofd_functional* f, double num
       *out = c;
       return -1;
     }
-    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    c->st = stream_get();
     const long long nr = c->sp.d.nr, nc = c->sp.d.nc;
     const int nf = func_nf(c->kind);
     c->vion = dalloc<double>(nr);
-    HIPCHK(hipMemset(c->vion, 0, nr * sizeof(double)));
+    HIPCHK(hipMemsetAsync(c->vion, 0, nr * sizeof(double), c->st));
     c->rho = dalloc<double>(nr);
     c->pot = dalloc<double>(nr);
     c->work = dalloc<double2>((size_t)nf * nc);
@@ -442,6 +469,7 @@ extern "C" int ofd_create(const ofd_grid* g, const ofd_functional* f, double num
     c->desc = dalloc<ReqDesc>(1);
     c->dot_part = dalloc<double>(512 * 4);
     c->dot_out = dalloc<double>(4);
+    HIPCHK(hipStreamSynchronize(c->st));   // later calls may copy on other streams
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tuned.find(TuneKey{g->n1, g->n2, g->n3, c->kind, c->precision == 1 ? 1 : 0});
@@ -464,7 +492,7 @@ extern "C" int ofd_create(const ofd_grid* g, const ofd_functional* f, double num
 static void free_opt(ofd_ctx* c) {
   double** v[] = {&c->phi, &c->dir, &c->dedphi, &c->dldphi, &c->r, &c->p, &c->ap, &c->tmp};
   for (auto pp : v) {
-    hipFree(*pp);
+    dev_free(*pp);
     *pp = nullptr;
   }
 }
@@ -473,13 +501,14 @@ extern "C" void ofd_destroy(ofd_ctx* c) {
   if (!c) return;
   if (c->st) hipStreamSynchronize(c->st);
   free_opt(c);
-  hipFree(c->vion); hipFree(c->rho); hipFree(c->pot); hipFree(c->work); hipFree(c->sums); hipFree(c->fin);
-  hipFree(c->wgck); hipFree(c->wtk); hipFree(c->desc); hipFree(c->dot_part); hipFree(c->dot_out);
-  hipFree(c->many_desc); hipFree(c->many_sums);
+  for (void* b : {(void*)c->vion, (void*)c->rho, (void*)c->pot, c->work, (void*)c->sums, (void*)c->fin,
+                  (void*)c->wgck, (void*)c->wtk, (void*)c->desc, (void*)c->dot_part, (void*)c->dot_out,
+                  (void*)c->many_desc, (void*)c->many_sums, (void*)c->d_frac, (void*)c->d_psp_vqs,
+                  (void*)c->d_psp_dd})
+    dev_free(b);
   graphs_clear(c->graphs);
-  hipFree(c->d_frac); hipFree(c->d_psp_vqs); hipFree(c->d_psp_dd);
   if (c->sp.tw_x) shape_plan_free(&c->sp);
-  if (c->st) hipStreamDestroy(c->st);
+  stream_put(c->st);
   delete c;
 }
 
@@ -488,6 +517,12 @@ extern "C" const char* ofd_last_error(const ofd_ctx* c) { return c ? c->err.c_st
 extern "C" int ofd_get_params(const ofd_ctx* c, double* o) {
   if (!c || !o) return -1;
   o[0] = c->alpha; o[1] = c->beta; o[2] = c->gamma; o[3] = c->rho0; o[4] = c->rhoS; o[5] = c->lam; o[6] = c->mu;
+  return 0;
+}
+
+extern "C" int ofd_get_volume(const ofd_ctx* c, double* vol) {
+  if (!c || !vol) return -1;
+  *vol = c->vol;
   return 0;
 }
 
@@ -599,8 +634,8 @@ extern "C" int ofd_calpot_many(ofd_ctx* const* cs, int nb, const double* const* 
     if ((int)c0->many_cap < nb) {
       HIPCHK(hipStreamSynchronize(c0->st));
       graphs_clear(c0->graphs);
-      hipFree(c0->many_desc);
-      hipFree(c0->many_sums);
+      dev_free(c0->many_desc);
+      dev_free(c0->many_sums);
       c0->many_desc = dalloc<ReqDesc>(nb);
       c0->many_sums = dalloc<double>((size_t)nb * kNSum);
       c0->many_cap = nb;
@@ -1153,7 +1188,7 @@ struct BatchScratch {
   int ndots = 0;   // results pending in h_res
   hipEvent_t ev_ops = nullptr, ev_dots = nullptr;
   void free_all() {
-    hipFree(d_ops); hipFree(d_dots); hipFree(d_part); hipFree(d_res);
+    dev_free(d_ops); dev_free(d_dots); dev_free(d_part); dev_free(d_res);
     hipHostFree(h_ops); hipHostFree(h_dots); hipHostFree(h_res);
     if (ev_ops) hipEventDestroy(ev_ops);
     if (ev_dots) hipEventDestroy(ev_dots);
@@ -1184,7 +1219,8 @@ void flush_ops(std::vector<NtnSolver*>& ss, long long n, BatchScratch& bs, hipSt
   bs.events();
   HIPCHK(hipEventSynchronize(bs.ev_ops));   // the previous upload has left the staging buffer
   if ((int)total > bs.ops_cap) {
-    hipFree(bs.d_ops);
+    HIPCHK(hipStreamSynchronize(st));   // earlier vector ops may still read d_ops
+    dev_free(bs.d_ops);
     hipHostFree(bs.h_ops);
     bs.ops_cap = (int)total * 2;
     bs.d_ops = dalloc<VecOp>(bs.ops_cap);
@@ -1215,7 +1251,8 @@ void launch_dots(std::vector<NtnSolver*>& ss, long long n, BatchScratch& bs, hip
   bs.events();
   HIPCHK(hipEventSynchronize(bs.ev_dots));
   if (cnt > bs.dots_cap) {
-    hipFree(bs.d_dots); hipFree(bs.d_part); hipFree(bs.d_res);
+    HIPCHK(hipStreamSynchronize(st));   // earlier dot launches may still use them
+    dev_free(bs.d_dots); dev_free(bs.d_part); dev_free(bs.d_res);
     hipHostFree(bs.h_dots); hipHostFree(bs.h_res);
     bs.dots_cap = cnt * 2;
     bs.d_dots = dalloc<DotDesc>(bs.dots_cap);
@@ -1381,7 +1418,7 @@ extern "C" int ofd_set_wgc_table(ofd_ctx* c, int nk, const double* eta, const do
     int rc = build_wgc_kernels(c->sp, c->rb, d, d + nk, d + 2 * (size_t)nk, d + 3 * (size_t)nk, d + 4 * (size_t)nk,
                                d + 5 * (size_t)nk, d + 6 * (size_t)nk, nk, tkfs, c->gamma, c->rhoS, c->wgck, c->st);
     HIPCHK(hipStreamSynchronize(c->st));
-    HIPCHK(hipFree(d));
+    dev_free(d);
     if (rc) return fail(c, "WGC kernel fill failed");
     c->have_wgc = true;
   } catch (const std::exception& e) {
@@ -1519,19 +1556,24 @@ extern "C" int ofd_build_ion_potential(ofd_ctx* c, int nspecies, const int* type
       c->psp_dd.insert(c->psp_dd.end(), dd.begin(), dd.end());
     }
     c->spline_order = order;
-    hipFree(c->d_frac); hipFree(c->d_psp_vqs); hipFree(c->d_psp_dd);
+    HIPCHK(hipStreamSynchronize(c->st));
+    dev_free(c->d_frac); dev_free(c->d_psp_vqs); dev_free(c->d_psp_dd);
     c->d_frac = dalloc<double>(3 * (size_t)nion);
     c->d_psp_vqs = dalloc<double>(c->psp_vqs.size());
     c->d_psp_dd = dalloc<double>(c->psp_dd.size());
-    HIPCHK(hipMemcpy(c->d_frac, c->frac.data(), sizeof(double) * c->frac.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_psp_vqs, c->psp_vqs.data(), sizeof(double) * c->psp_vqs.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_psp_dd, c->psp_dd.data(), sizeof(double) * c->psp_dd.size(), hipMemcpyHostToDevice));
+    // the host vectors outlive the copies: build_ion_potential synchronizes the stream
+    HIPCHK(hipMemcpyAsync(c->d_frac, c->frac.data(), sizeof(double) * c->frac.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_psp_vqs, c->psp_vqs.data(), sizeof(double) * c->psp_vqs.size(),
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_psp_dd, c->psp_dd.data(), sizeof(double) * c->psp_dd.size(), hipMemcpyHostToDevice,
+                          c->st));
     const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
     void* sc = dalloc<char>(scratch);
     int rc = build_ion_potential(c->sp, c->rb, c->vol, nspecies, c->type_start.data(), c->d_frac, nion, c->d_psp_vqs,
                                  c->d_psp_dd, c->psp_off.data(), c->psp_npts.data(), c->psp_maxg.data(),
                                  c->psp_v0.data(), c->psp_charge.data(), order, c->vion, sc, c->st);
-    hipFree(sc);
+    hipStreamSynchronize(c->st);
+    dev_free(sc);
     if (rc) return fail(c, "ion potential build failed");
     c->e_ionion = eii;
     c->have_vion = true;
@@ -1557,8 +1599,9 @@ extern "C" int ofd_ion_forces(ofd_ctx* c, const double* rho, double* forces) {
                                c->d_psp_dd, c->psp_off.data(), c->psp_npts.data(), c->psp_maxg.data(),
                                c->psp_v0.data(), c->psp_charge.data(), c->spline_order, c->rho, df, sc, c->st);
     HIPCHK(hipMemcpy(forces, df, sizeof(double) * 3 * nion, hipMemcpyDeviceToHost));
-    hipFree(sc);
-    hipFree(df);
+    hipStreamSynchronize(c->st);
+    dev_free(sc);
+    dev_free(df);
     if (rc) return fail(c, "force kernels failed");
   } catch (const std::exception& e) {
     return fail(c, e.what());
@@ -1803,7 +1846,7 @@ static void server_kick(ofd_server* s);
 // holding the pool, so a server keeps driving the pool meanwhile; the slot
 // joins the pool's phases once it is ready.
 extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const ofd_opt_params* prm) {
-  if (!p || !c || !rho) return -1;
+  if (!p || !c) return -1;
   if (c->sp.d.n1 != p->n1 || c->sp.d.n2 != p->n2 || c->sp.d.n3 != p->n3 || c->kinetic != p->kinetic) {
     p->err = "request grid or functional does not match the pool";
     return -1;
@@ -1827,7 +1870,8 @@ extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const 
     std::lock_guard<std::mutex> ak(p->admit_mu);
     HIPCHK(hipStreamSynchronize(c->st));
     alloc_opt(c);
-    HIPCHK(hipMemcpyAsync(c->rho, rho, c->nr() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (rho) HIPCHK(hipMemcpyAsync(c->rho, rho, c->nr() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    else vec_fill(c->rho, c->num_ele / c->vol, c->nr(), c->st);   // the uniform density
     solver_prepare(s.S, c, prm, p->bs_admit, c->st);
     HIPCHK(hipStreamSynchronize(c->st));
   } catch (const std::exception& e) {
@@ -2011,16 +2055,21 @@ extern "C" int ofd_pool_collect(ofd_pool* p, int slot, double* rho_out, double* 
   if (!s.used) return -1;
   if (!s.done) return 0;
   ofd_ctx* c = s.c;
-  try {
-    std::vector<NtnSolver*> one{&s.S};
-    flush_ops(one, c->nr(), p->bs, p->st);
-    const long long n = c->nr();
-    vec_square(c->rho, c->phi, n, p->st);
-    if (rho_out) HIPCHK(hipMemcpyAsync(rho_out, c->rho, n * sizeof(double), hipMemcpyDeviceToHost, p->st));
-    HIPCHK(hipStreamSynchronize(p->st));
-  } catch (const std::exception& e) {
-    p->err = e.what();
-    return -1;
+  // Without rho_out the final density is not formed: the request's last
+  // vector updates are dropped and nothing waits on the pool's stream (no
+  // launched work of a finished request is still pending).
+  if (rho_out) {
+    try {
+      std::vector<NtnSolver*> one{&s.S};
+      flush_ops(one, c->nr(), p->bs, p->st);
+      const long long n = c->nr();
+      vec_square(c->rho, c->phi, n, p->st);
+      HIPCHK(hipMemcpyAsync(rho_out, c->rho, n * sizeof(double), hipMemcpyDeviceToHost, p->st));
+      HIPCHK(hipStreamSynchronize(p->st));
+    } catch (const std::exception& e) {
+      p->err = e.what();
+      return -1;
+    }
   }
   if (energy9) std::memcpy(energy9, s.S.energy, sizeof(s.S.energy));
   if (outcome) *outcome = s.S.err.empty() ? s.S.outcome : 1;

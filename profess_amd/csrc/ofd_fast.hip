@@ -1,4 +1,4 @@
-// Dispatch of the register-resident fast path (power-of-two grids, Wang-Teter
+// Dispatch of the register-resident fast path (two-stage line lengths, Wang-Teter
 // functionals). The kernels live in ofd_fast_impl.h; each (precision,
 // functional) pair is instantiated in its own translation unit
 // (ofd_fast_d3.hip, ofd_fast_d4.hip, ofd_fast_f3.hip, ofd_fast_f4.hip).

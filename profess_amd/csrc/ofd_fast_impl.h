@@ -1,4 +1,6 @@
-// Register-resident two-stage FFT passes for power-of-two lines (n = 16..256).
+// Register-resident two-stage FFT passes for lines of n = R1*R2 points with
+// R2 dividing R1: the powers of two 16..256 and the mixed lengths 24, 40, 48,
+// 80, 96, 192 (radix-12, -20 and -24 first stages).
 //
 // A line of n = R1*R2 complex points is owned by T = R2 threads; thread t
 // holds the R1 points t + T*m (m = 0..R1-1) in registers. Stage 1 is one
@@ -34,9 +36,15 @@ constexpr double kCTF = 2.87123400018819;
 
 template <int N> struct P2;
 template <> struct P2<16> { static constexpr int R1 = 4, R2 = 4; };
+template <> struct P2<24> { static constexpr int R1 = 12, R2 = 2; };
 template <> struct P2<32> { static constexpr int R1 = 8, R2 = 4; };
+template <> struct P2<40> { static constexpr int R1 = 20, R2 = 2; };
+template <> struct P2<48> { static constexpr int R1 = 12, R2 = 4; };
 template <> struct P2<64> { static constexpr int R1 = 8, R2 = 8; };
+template <> struct P2<80> { static constexpr int R1 = 20, R2 = 4; };
+template <> struct P2<96> { static constexpr int R1 = 24, R2 = 4; };
 template <> struct P2<128> { static constexpr int R1 = 16, R2 = 8; };
+template <> struct P2<192> { static constexpr int R1 = 24, R2 = 8; };
 template <> struct P2<256> { static constexpr int R1 = 16, R2 = 16; };
 
 template <int N> __device__ __forceinline__ constexpr int padded(int j) { return j + j / P2<N>::R1; }
@@ -609,19 +617,46 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
 }  // namespace fast
 
 // ------------------------------------------------------------ dispatch
-static inline bool is_fast_len(int n) { return n == 32 || n == 64 || n == 128 || n == 256; }
+// line lengths with a two-stage plan: y and z columns, and n1/2 along x
+static inline bool is_fast_len(int n) {
+  switch (n) {
+    case 24: case 32: case 40: case 48: case 64: case 80: case 96: case 128: case 192: case 256: return true;
+    default: return false;
+  }
+}
+static inline bool is_fast_half(int m) {
+  switch (m) {
+    case 16: case 24: case 32: case 40: case 48: case 64: case 96: case 128: case 256: return true;
+    default: return false;
+  }
+}
 
 inline bool fast_path_ok_impl(const ShapePlan& sp, int kind) {
-  return (kind == kWT3 || kind == kWT4) && (sp.d.M == 16 || is_fast_len(sp.d.M)) && is_fast_len(sp.d.n2) &&
-         is_fast_len(sp.d.n3);
+  return (kind == kWT3 || kind == kWT4) && is_fast_half(sp.d.M) && is_fast_len(sp.d.n2) && is_fast_len(sp.d.n3);
 }
 
 struct FastTiles {
   int rpb, kby, kbz, ntile;
 };
 
-static inline int p2_T(int n) { return n == 16 ? 4 : n == 32 ? 4 : n == 64 ? 8 : n == 128 ? 8 : 16; }
-static inline int p2_R1(int n) { return n == 16 ? 4 : n == 32 ? 8 : n == 64 ? 8 : 16; }
+template <typename F> static inline int p2_of(int n, F get) {
+  using namespace fast;
+  switch (n) {
+    case 16: return get(P2<16>());
+    case 24: return get(P2<24>());
+    case 32: return get(P2<32>());
+    case 40: return get(P2<40>());
+    case 48: return get(P2<48>());
+    case 64: return get(P2<64>());
+    case 80: return get(P2<80>());
+    case 96: return get(P2<96>());
+    case 128: return get(P2<128>());
+    case 192: return get(P2<192>());
+    default: return get(P2<256>());
+  }
+}
+static inline int p2_T(int n) { return p2_of(n, [](auto p) { return decltype(p)::R2; }); }
+static inline int p2_R1(int n) { return p2_of(n, [](auto p) { return decltype(p)::R1; }); }
 static inline int p2_ll(int n) { return n + n / p2_R1(n); }
 static inline int p2_cs(int n) { return p2_ll(n) | 1; }   // in real components (column passes)
 
@@ -648,6 +683,16 @@ static inline FastTiles fast_tiles(const ShapePlan& sp, int nf, int cbytes) {
   return ft;
 }
 
+// Raises kernel K's dynamic LDS limit (once per size increase) when a launch
+// needs more than 64 KiB.
+template <auto K> static inline void allow_lds(size_t lds) {
+  static size_t granted = 65536;
+  if (lds > granted) {
+    hipFuncSetAttribute((const void*)K, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    granted = lds;
+  }
+}
+
 template <typename R, int M, int KIND, int INMODE, int OUTMODE>
 static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd, int nb, hipStream_t st, bool fwd,
                      const typename V2<R>::T* tw, const typename V2<R>::T* tws) {
@@ -660,6 +705,7 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
     const long long nrows = (long long)sp.d.n2 * sp.d.n3;
     const int ntile = (int)((nrows + RPB - 1) / RPB);
     size_t lds = (size_t)RPB * NF * line_lds<M>() * sizeof(C);
+    allow_lds<k_fx_fwd<M, KIND, INMODE, R>>(lds);
     hipLaunchKernelGGL((k_fx_fwd<M, KIND, INMODE, R>), dim3(ntile, nb), dim3(RPB * NF * T), lds, st, dd, sp.d, tw,
                        tws);
   } else {
@@ -667,6 +713,7 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
     const long long nrows = (long long)sp.d.n2 * sp.d.n3;
     const int ntile = (int)((nrows + RPB - 1) / RPB);
     size_t lds = (size_t)RPB * NF * line_lds<M>() * sizeof(C);
+    allow_lds<k_fx_inv<M, KIND, INMODE, OUTMODE, R>>(lds);
     hipLaunchKernelGGL((k_fx_inv<M, KIND, INMODE, OUTMODE, R>), dim3(ntile, nb), dim3(RPB * NF * T), lds, st, dd,
                        sp.d, tw, tws);
   }
@@ -680,6 +727,7 @@ static void launch_y(const ShapePlan& sp, int kb, int nf, const ReqDesc* dd, int
   const long long ncol = (long long)sp.d.K1 * sp.d.n3;
   const int nblk = (int)((ncol + kb - 1) / kb);
   size_t lds = (size_t)kb * col_stride<N>() * sizeof(R);
+  allow_lds<k_fy<N, S, R>>(lds);
   hipLaunchKernelGGL((k_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb);
 }
 
@@ -691,6 +739,7 @@ static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hip
   const long long ncol = (long long)sp.d.K1 * sp.d.n2;
   const int nblk = (int)((ncol + kb - 1) / kb);
   size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
+  allow_lds<k_fz_gop<N, KIND, R>>(lds);
   hipLaunchKernelGGL((k_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d,
                      tw, kb);
 }
@@ -731,9 +780,15 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
   auto ylaunch = [&](auto sgn) {
     constexpr int S = decltype(sgn)::value;
     switch (sp.d.n2) {
+      case 24: launch_y<R, 24, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 32: launch_y<R, 32, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 40: launch_y<R, 40, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 48: launch_y<R, 48, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 64: launch_y<R, 64, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 80: launch_y<R, 80, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 96: launch_y<R, 96, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 128: launch_y<R, 128, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 192: launch_y<R, 192, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       default: launch_y<R, 256, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
     }
   };
@@ -742,9 +797,15 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
     ylaunch(std::integral_constant<int, -1>());
   }
   switch (sp.d.n3) {
+    case 24: launch_z<R, 24, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 40: launch_z<R, 40, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 48: launch_z<R, 48, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 64: launch_z<R, 64, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 80: launch_z<R, 80, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 96: launch_z<R, 96, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 128: launch_z<R, 128, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 192: launch_z<R, 192, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     default: launch_z<R, 256, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
   }
   ylaunch(std::integral_constant<int, +1>());
@@ -764,7 +825,11 @@ static int fast_eval_t(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream
   (void)f32;
   switch (sp.d.M) {
     case 16: fast_chain<R, KIND, INMODE, OUTMODE, 16>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    case 24: fast_chain<R, KIND, INMODE, OUTMODE, 24>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 32: fast_chain<R, KIND, INMODE, OUTMODE, 32>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    case 40: fast_chain<R, KIND, INMODE, OUTMODE, 40>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    case 48: fast_chain<R, KIND, INMODE, OUTMODE, 48>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    case 96: fast_chain<R, KIND, INMODE, OUTMODE, 96>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 64: fast_chain<R, KIND, INMODE, OUTMODE, 64>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 128: fast_chain<R, KIND, INMODE, OUTMODE, 128>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     default: fast_chain<R, KIND, INMODE, OUTMODE, 256>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;

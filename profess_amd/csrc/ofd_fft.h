@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace ofd {
 
 constexpr int kMaxStages = 12;
@@ -205,6 +207,104 @@ template <int P, int S, typename C> struct BflyPrime {
 template <int S, typename C> struct Bfly<3, S, C> { __device__ static void run(C* x) { BflyPrime<3, S, C>::run(x); } };
 template <int S, typename C> struct Bfly<5, S, C> { __device__ static void run(C* x) { BflyPrime<5, S, C>::run(x); } };
 template <int S, typename C> struct Bfly<7, S, C> { __device__ static void run(C* x) { BflyPrime<7, S, C>::run(x); } };
+
+// ---------------------------------------------------------- composite radices
+// cos and sin of 2 pi j / n evaluated by the compiler: quadrant reduction to
+// an angle in [0, pi/4], then Taylor series (terms to x^25, far below one
+// ulp there). Used only in constant expressions.
+constexpr double ct_sin_series(double x) {
+  double t = x, s = x;
+  for (int k = 1; k <= 12; ++k) {
+    t *= -x * x / (double)((2 * k) * (2 * k + 1));
+    s += t;
+  }
+  return s;
+}
+constexpr double ct_cos_series(double x) {
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k <= 12; ++k) {
+    t *= -x * x / (double)((2 * k - 1) * (2 * k));
+    s += t;
+  }
+  return s;
+}
+// out: c = cos(2 pi j/n), s = sin(2 pi j/n)
+constexpr void ct_sincos(int j, int n, double& c, double& s) {
+  const double half_pi = 1.5707963267948966192313216916398;
+  j = ((j % n) + n) % n;
+  const int q = (4 * j) / n;        // quadrant
+  const int r = 4 * j - q * n;      // angle within it = (pi/2) r/n
+  double cb = 0.0, sb = 0.0;
+  if (2 * r <= n) {
+    const double b = half_pi * (double)r / (double)n;
+    cb = ct_cos_series(b);
+    sb = ct_sin_series(b);
+  } else {
+    const double b = half_pi * (double)(n - r) / (double)n;
+    cb = ct_sin_series(b);
+    sb = ct_cos_series(b);
+  }
+  if (r == 0) { cb = 1.0; sb = 0.0; }
+  if (q == 0) { c = cb; s = sb; }
+  else if (q == 1) { c = -sb; s = cb; }
+  else if (q == 2) { c = -cb; s = -sb; }
+  else { c = sb; s = -cb; }
+}
+constexpr double ct_cos(int j, int n) { double c = 0, s = 0; ct_sincos(j, n, c, s); return c; }
+constexpr double ct_sin(int j, int n) { double c = 0, s = 0; ct_sincos(j, n, c, s); return s; }
+template <int J, int N> struct CtW {
+  static constexpr double c = ct_cos(J, N);
+  static constexpr double s = ct_sin(J, N);
+};
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [I0, E)
+template <int I0, int E> struct SFor {
+  template <typename F> __device__ __forceinline__ static void run(F&& f) {
+    if constexpr (I0 < E) {
+      f(std::integral_constant<int, I0>{});
+      SFor<I0 + 1, E>::run(f);
+    }
+  }
+};
+
+// Radix A*B in registers: B radix-A butterflies over x[B*j + m] (j < A),
+// twiddles exp(S 2 pi i m k / (A B)) as compile-time constants, then A
+// radix-B butterflies; the result is in natural order, x[k + A*q].
+template <int A, int B, int S, typename C> struct BflyAB {
+  __device__ static void run(C* x) {
+    using R = decltype(x[0].x);
+    constexpr int N = A * B;
+    C a[B][A];
+#pragma unroll
+    for (int m = 0; m < B; ++m) {
+#pragma unroll
+      for (int j = 0; j < A; ++j) a[m][j] = x[B * j + m];
+      Bfly<A, S, C>::run(a[m]);
+    }
+    SFor<1, B>::run([&](auto mi) {
+      constexpr int m = decltype(mi)::value;
+      SFor<1, A>::run([&](auto ki) {
+        constexpr int k = decltype(ki)::value;
+        C w;
+        w.x = (R)CtW<m * k, N>::c;
+        w.y = (R)((double)S * CtW<m * k, N>::s);
+        a[m][k] = cmul(a[m][k], w);
+      });
+    });
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      C t[B];
+#pragma unroll
+      for (int m = 0; m < B; ++m) t[m] = a[m][k];
+      Bfly<B, S, C>::run(t);
+#pragma unroll
+      for (int q = 0; q < B; ++q) x[k + A * q] = t[q];
+    }
+  }
+};
+template <int S, typename C> struct Bfly<12, S, C> { __device__ static void run(C* x) { BflyAB<4, 3, S, C>::run(x); } };
+template <int S, typename C> struct Bfly<20, S, C> { __device__ static void run(C* x) { BflyAB<4, 5, S, C>::run(x); } };
+template <int S, typename C> struct Bfly<24, S, C> { __device__ static void run(C* x) { BflyAB<8, 3, S, C>::run(x); } };
 
 // ------------------------------------------------------------------ stages
 // One Stockham stage of radix RAD over `nlines` lines of length n.

@@ -99,6 +99,11 @@ struct ShapePlan {
 struct Tiles {
   int rows, kby, kbz, ntile;
 };
+
+// Device memory through a process-wide cache of freed blocks (ofd_plan.cpp).
+// dev_free may only be called once no queued device work uses the block.
+void* dev_alloc(size_t bytes);   // nullptr on failure
+void dev_free(void* p);
 // Tiles for NF fields with complex elements of `cbytes` bytes.
 Tiles tiles_for(const ShapePlan& sp, int nf, int cbytes);
 
@@ -127,6 +132,7 @@ void vec_axpby(double* y, const double* x, double a, double b, long long n, hipS
 void vec_lincomb(double* z, const double* x, const double* y, double a, double b, long long n, hipStream_t st);  // z = fma(b, y, a*x)
 void vec_sqrt_abs(double* y, const double* x, long long n, hipStream_t st);                   // y = sqrt(|x|)
 void vec_square(double* y, const double* x, long long n, hipStream_t st);                     // y = x*x
+void vec_fill(double* y, double v, long long n, hipStream_t st);                               // y = v
 // dot products of up to 4 pairs into out[4] (device), deterministic order
 void vec_dots(const double* const* a, const double* const* b, int npairs, long long n, double* d_partial,
               double* d_out, hipStream_t st);

@@ -562,6 +562,10 @@ __global__ void k_sqrt_abs(double* y, const double* x, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = sqrt(fabs(x[i]));
 }
+__global__ void k_fill(double* y, double v, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = v;
+}
 __global__ void k_square(double* y, const double* x, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = x[i] * x[i];
@@ -652,6 +656,9 @@ void vec_lincomb(double* z, const double* x, const double* y, double a, double b
 }
 void vec_sqrt_abs(double* y, const double* x, long long n, hipStream_t st) {
   hipLaunchKernelGGL(k_sqrt_abs, dim3(nblocks(n)), dim3(256), 0, st, y, x, n);
+}
+void vec_fill(double* y, double v, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(256), 0, st, y, v, n);
 }
 void vec_square(double* y, const double* x, long long n, hipStream_t st) {
   hipLaunchKernelGGL(k_square, dim3(nblocks(n)), dim3(256), 0, st, y, x, n);

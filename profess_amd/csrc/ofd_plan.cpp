@@ -5,6 +5,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "ofd_internal.h"
@@ -33,7 +36,90 @@ bool make_plan1d(int n, Plan1D* p) {
   return true;
 }
 
+// ------------------------------------------------------------ device memory cache
+// Blocks released by dev_free wait in per-size free lists and are handed out
+// again by dev_alloc, so a server that creates and destroys a context per
+// request reaches a steady state with no hipMalloc or hipFree (hipFree waits
+// for the whole device). Sizes round up to 4 KiB; requests of one grid shape
+// ask for identical sizes. Free blocks beyond kCacheCap bytes go back to HIP.
+namespace {
+constexpr size_t kCacheCap = 48ull << 30;
+std::mutex g_mem_mu;
+std::unordered_map<void*, size_t> g_live;
+std::multimap<size_t, void*> g_free;
+size_t g_free_bytes = 0;
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+  const size_t b = ((bytes ? bytes : 1) + 4095) / 4096 * 4096;
+  {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto it = g_free.find(b);
+    if (it != g_free.end()) {
+      void* p = it->second;
+      g_free.erase(it);
+      g_free_bytes -= b;
+      g_live[p] = b;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, b) != hipSuccess) {
+    (void)hipGetLastError();
+    // release the cached blocks and retry once
+    std::multimap<size_t, void*> drop;
+    {
+      std::lock_guard<std::mutex> lk(g_mem_mu);
+      drop.swap(g_free);
+      g_free_bytes = 0;
+    }
+    for (auto& kv : drop) hipFree(kv.second);
+    p = nullptr;
+    if (hipMalloc(&p, b) != hipSuccess) return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  g_live[p] = b;
+  return p;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto it = g_live.find(p);
+    if (it != g_live.end()) {
+      const size_t b = it->second;
+      g_live.erase(it);
+      if (g_free_bytes + b <= kCacheCap) {
+        g_free.emplace(b, p);
+        g_free_bytes += b;
+        return;
+      }
+    }
+  }
+  hipFree(p);
+}
+
+// ------------------------------------------------------------ twiddle tables
+// exp(-2 pi i j / n), j < count, in FP64 and FP32, uploaded once per (n, count)
+// and shared by every plan of the process (never freed).
+namespace {
+struct TwTables {
+  double2* d64 = nullptr;
+  float2* d32 = nullptr;
+};
+std::mutex g_tw_mu;
+std::map<std::pair<int, int>, TwTables> g_tw;
+}  // namespace
+
 static int upload_tw(int n, int count, double2** d64, float2** d32) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto it = g_tw.find({n, count});
+  if (it != g_tw.end()) {
+    *d64 = it->second.d64;
+    *d32 = it->second.d32;
+    return 0;
+  }
   std::vector<double2> h(count);
   std::vector<float2> hf(count);
   const long double two_pi = 6.283185307179586476925286766559005768L;
@@ -44,10 +130,14 @@ static int upload_tw(int n, int count, double2** d64, float2** d32) {
     hf[j].x = (float)cosl(a);
     hf[j].y = (float)sinl(a);
   }
-  if (hipMalloc(d64, sizeof(double2) * count) != hipSuccess) return -1;
-  if (hipMalloc(d32, sizeof(float2) * count) != hipSuccess) return -1;
-  if (hipMemcpy(*d64, h.data(), sizeof(double2) * count, hipMemcpyHostToDevice) != hipSuccess) return -1;
-  if (hipMemcpy(*d32, hf.data(), sizeof(float2) * count, hipMemcpyHostToDevice) != hipSuccess) return -1;
+  TwTables t;
+  if (hipMalloc(&t.d64, sizeof(double2) * count) != hipSuccess) return -1;
+  if (hipMalloc(&t.d32, sizeof(float2) * count) != hipSuccess) return -1;
+  if (hipMemcpy(t.d64, h.data(), sizeof(double2) * count, hipMemcpyHostToDevice) != hipSuccess) return -1;
+  if (hipMemcpy(t.d32, hf.data(), sizeof(float2) * count, hipMemcpyHostToDevice) != hipSuccess) return -1;
+  g_tw[{n, count}] = t;
+  *d64 = t.d64;
+  *d32 = t.d32;
   return 0;
 }
 
@@ -118,10 +208,7 @@ Tiles tiles_for(const ShapePlan& sp, int nf, int cbytes) {
   return t;
 }
 
-void shape_plan_free(ShapePlan* sp) {
-  hipFree(sp->tw_x); hipFree(sp->tw_y); hipFree(sp->tw_z); hipFree(sp->tw_s);
-  hipFree(sp->tw_x32); hipFree(sp->tw_y32); hipFree(sp->tw_z32); hipFree(sp->tw_s32);
-  *sp = ShapePlan();
-}
+// The twiddle tables are shared process-wide (upload_tw): nothing to free.
+void shape_plan_free(ShapePlan* sp) { *sp = ShapePlan(); }
 
 }  // namespace ofd

@@ -325,18 +325,19 @@ struct BTables {
     fill_b_host(order, d.K1, d.n1, h1);
     fill_b_host(order, d.n2, d.n2, h2);
     fill_b_host(order, d.n3, d.n3, h3);
-    hipMalloc(&b1, sizeof(double2) * h1.size());
-    hipMalloc(&b2, sizeof(double2) * h2.size());
-    hipMalloc(&b3, sizeof(double2) * h3.size());
+    b1 = (double2*)dev_alloc(sizeof(double2) * h1.size());
+    b2 = (double2*)dev_alloc(sizeof(double2) * h2.size());
+    b3 = (double2*)dev_alloc(sizeof(double2) * h3.size());
     hipMemcpyAsync(b1, h1.data(), sizeof(double2) * h1.size(), hipMemcpyHostToDevice, st);
     hipMemcpyAsync(b2, h2.data(), sizeof(double2) * h2.size(), hipMemcpyHostToDevice, st);
     hipMemcpyAsync(b3, h3.data(), sizeof(double2) * h3.size(), hipMemcpyHostToDevice, st);
     hipStreamSynchronize(st);
   }
+  // released after the caller has synchronized the stream
   ~BTables() {
-    hipFree(b1);
-    hipFree(b2);
-    hipFree(b3);
+    dev_free(b1);
+    dev_free(b2);
+    dev_free(b3);
   }
 };
 
@@ -369,7 +370,8 @@ int build_ion_potential(const ShapePlan& sp, const double* rb, double vol, int n
   ReqDesc D;
   std::memset(&D, 0, sizeof(D));
   ReqDesc* dd;
-  if (hipMalloc(&dd, sizeof(ReqDesc)) != hipSuccess) return -1;
+  dd = (ReqDesc*)dev_alloc(sizeof(ReqDesc));
+  if (!dd) return -1;
   const unsigned gb = (unsigned)((d.nc + 255) / 256);
   for (int s = 0; s < ntypes; ++s) {
     const int i0 = h_type_start[s], n_s = h_type_start[s + 1] - i0;
@@ -393,7 +395,7 @@ int build_ion_potential(const ShapePlan& sp, const double* rb, double vol, int n
   hipMemcpyAsync(dd, &D, sizeof(D), hipMemcpyHostToDevice, st);
   run_fft3d_plain(sp, 0, dd, 1, st);
   hipError_t e = hipStreamSynchronize(st);
-  hipFree(dd);
+  dev_free(dd);
   return e == hipSuccess ? 0 : -1;
 }
 
@@ -415,7 +417,8 @@ int ion_forces_spline(const ShapePlan& sp, const double* rb, const double* cell_
   ReqDesc D;
   std::memset(&D, 0, sizeof(D));
   ReqDesc* dd;
-  if (hipMalloc(&dd, sizeof(ReqDesc)) != hipSuccess) return -1;
+  dd = (ReqDesc*)dev_alloc(sizeof(ReqDesc));
+  if (!dd) return -1;
   D.u = d_rho;
   D.work = rg;
   hipMemcpyAsync(dd, &D, sizeof(D), hipMemcpyHostToDevice, st);
@@ -435,7 +438,7 @@ int ion_forces_spline(const ShapePlan& sp, const double* rb, const double* cell_
       hipLaunchKernelGGL(k_force_gather, dim3(n_s), dim3(256), 0, st, d, d_frac, i0, order, ci, factor, d_forces);
   }
   hipError_t e = hipStreamSynchronize(st);
-  hipFree(dd);
+  dev_free(dd);
   return e == hipSuccess ? 0 : -1;
 }
 

@@ -71,7 +71,9 @@ def make_requests(n, seed=0, grids=(32, 48, 64, 80, 96, 120, 128), max_atoms=500
     return out
 
 
-def build_system(req: Request, precision=1):
+def build_system(req: Request, precision=1, with_rho=True):
+    """The request's system (cell, ions, ionic potential on the device) and,
+    with_rho, its uniform starting density on the host."""
     if req.species == "Al":
         lat, frac = inputs.fcc_supercell(inputs.AL_A, req.reps)
         Z, rc = inputs.AL_Z, inputs.AL_RC
@@ -83,7 +85,7 @@ def build_system(req: Request, precision=1):
     s = OFDSystem(lat, req.grid, ne, kinetic=4, precision=precision)
     mg, vals = inputs.synthetic_recpot(Z, rc)
     s.build_ion_potential([(mg, vals, frac)])
-    rho = np.full(s.shape, ne / s.vol)
+    rho = np.full(s.shape, ne / s.vol) if with_rho else None
     return s, rho
 
 
@@ -104,14 +106,23 @@ class Pool:
             self.h = None
 
     def admit(self, req, system, rho, params):
-        L = lib()
-        r = np.ascontiguousarray(rho)
-        ctx = system._h
-        slot = L.ofd_pool_admit(self.h, ctx, dptr(r), C.byref(params))
-        if slot < 0:
+        """Admit `req` with its system and starting density (None: uniform).
+        Returns the slot, or -1 when the pool did not take it."""
+        slot = self.admit_native(system, rho, params)
+        if slot >= 0:
+            self.slots[slot] = req
+        return slot
+
+    def admit_native(self, system, rho, params):
+        """The native admission alone (safe from a worker thread); the caller
+        records the slot in `slots`. Returns -1 when the pool is full."""
+        r = None if rho is None else np.ascontiguousarray(rho)
+        slot = lib().ofd_pool_admit(self.h, system._h, dptr(r), C.byref(params))
+        if slot == -2:
             return -1
+        if slot < 0:
+            raise OFDError("admission failed: " + lib().ofd_pool_last_error(self.h).decode())
         system.detach()
-        self.slots[slot] = req
         return slot
 
     def step(self):
@@ -128,7 +139,9 @@ class Pool:
         if nfin <= 0:
             return done
         for slot in list(fin[:nfin]):
-            req = self.slots[slot]
+            req = self.slots.get(slot)
+            if req is None:   # admitted by a worker, not registered yet
+                continue
             n = self.grid[0] * self.grid[1] * self.grid[2]
             rho = np.empty(n) if want_rho else None
             et = np.zeros(9)
@@ -193,90 +206,188 @@ def capacity_for(grid):
 
 
 def run_workload(requests, mixed=True, hbm_budget=160e9, pot_tol=1e-5, mixed_switch=1e-3, verbose=False,
-                 keep_rho=False, fused_dots=True, tune=True, threaded=True):
+                 keep_rho=False, fused_dots=True, tune=True, threaded=True, builders=8):
     """Serve `requests` (arrival times in seconds from the start) to completion.
     Returns throughput and latency statistics. fused_dots: the optimizer's
     inner products from the evaluation's sums and the CG recurrences (one
     host round trip per inner iteration). threaded: the pools are driven by
-    a native progress thread (Server); otherwise by lock-step
-    ofd_pools_step calls from this loop."""
+    a native progress thread (Server) and `builders` worker threads build and
+    admit arriving requests (system, ionic potential, uniform start density
+    on the device) while the device works; otherwise this loop builds, admits
+    and steps the pools (ofd_pools_step) in lock step."""
     precision = 1 if mixed else 0
     params = opt_params(pot_tol=pot_tol, conv_check="POT", mixed_switch=mixed_switch if mixed else 0.0,
                         fused_dots=fused_dots)
-    pools = {}
-    pending = sorted(requests, key=lambda r: r.arrival)
-    in_use = 0.0
+    run = _serve_threaded if threaded else _serve_lockstep
     t0 = time.perf_counter()
-    now = lambda: time.perf_counter() - t0   # noqa: E731
-    finished = []
-    qi = 0
-    waiting = []
-    server = Server() if threaded else None
-    try:
-        while len(finished) < len(requests):
-            t = now()
-            while qi < len(pending) and pending[qi].arrival <= t:
-                waiting.append(pending[qi])
-                qi += 1
-            # admission: FIFO among arrived requests that fit
-            still = []
-            for req in waiting:
-                g = req.grid
-                pool = pools.get(g)
-                s = None
-                if pool is None:
-                    # a new grid shape: tune its pass tiles on this request's own
-                    # system before the pool (which copies the tuned tiles) exists
-                    s, rho = build_system(req, precision)
-                    if tune:
-                        s.autotune(reps=3)
-                    pool = pools[g] = Pool(g, capacity_for(g), precision)
-                    if server is not None:
-                        server.add(pool)
-                need = float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
-                if len(pool.slots) < pool.capacity and in_use + need <= hbm_budget:
-                    if s is None:
-                        s, rho = build_system(req, precision)
-                    if pool.admit(req, s, rho, params) >= 0:
-                        req.admitted = now()
-                        in_use += need
-                        continue
-                if s is not None:
-                    s.close()
-                still.append(req)
-            waiting = still
-            if server is not None:
-                # wake on a finished request, or in time for the next arrival
-                nxt = pending[qi].arrival - now() if qi < len(pending) else 0.005
-                server.wait(max(0.0, min(0.005, nxt)))
-                active = sum(len(p.slots) for p in pools.values())
-            else:
-                busy = [pool for pool in pools.values() if pool.slots]
-                active = step_pools(busy) if busy else 0
-            for g, pool in pools.items():
-                for req in pool.collect(now, want_rho=keep_rho):
-                    in_use -= float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
-                    if not keep_rho:
-                        req.rho = None
-                    finished.append(req)
-                    if verbose:
-                        print("req %d grid %s done in %.3f s (E = %.8f Ha, outcome %d)"
-                              % (req.rid, g, req.finished - req.arrival, req.energy[0], req.outcome), flush=True)
-            if server is None and active == 0 and not waiting and qi < len(pending):
-                time.sleep(max(0.0, min(0.01, pending[qi].arrival - now())))
-        makespan = now()
-    finally:
-        if server is not None:
-            server.close()
-        for p in pools.values():
-            p.close()
+    finished, host = run(requests, precision, params, hbm_budget, keep_rho, tune, verbose, t0, builders)
+    makespan = max((r.finished for r in finished), default=0.0)
     lat = np.array([r.finished - r.arrival for r in finished])
     ok = sum(1 for r in finished if r.outcome == 0)
     return {"requests": len(requests), "converged": ok, "makespan_s": makespan,
             "solves_per_s": len(requests) / makespan,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mixed_precision": bool(mixed),
-            "grids": sorted({r.grid[0] for r in requests})}
+            "grids": sorted({r.grid[0] for r in requests}),
+            "host_s": {k: round(v, 4) for k, v in host.items()}}
+
+
+def _new_pool(req, precision, tune):
+    """A pool for the grid of `req`: its pass tiles are tuned first on the
+    request's own system (the pool copies the tuned tiles)."""
+    s, _ = build_system(req, precision, with_rho=False)
+    if tune:
+        s.autotune(reps=3)
+    s.close()
+    return Pool(req.grid, capacity_for(req.grid), precision)
+
+
+def _finish(req, keep_rho, verbose, finished):
+    if not keep_rho:
+        req.rho = None
+    finished.append(req)
+    if verbose:
+        print("req %d grid %s done in %.3f s (E = %.8f Ha, outcome %d)"
+              % (req.rid, req.grid, req.finished - req.arrival, req.energy[0], req.outcome), flush=True)
+
+
+def _serve_threaded(requests, precision, params, hbm_budget, keep_rho, tune, verbose, t0, builders):
+    from concurrent.futures import ThreadPoolExecutor
+    now = lambda: time.perf_counter() - t0   # noqa: E731
+    tic = time.perf_counter
+    host = {"pool_setup_s": 0.0, "submit_s": 0.0, "wait_s": 0.0, "collect_s": 0.0}
+    pending = sorted(requests, key=lambda r: r.arrival)
+    pools, reserved = {}, {}
+    in_use = 0.0
+    finished, waiting = [], []
+    building = {}   # future -> (req, pool, need)
+    qi = 0
+
+    def build_admit(req, pool):
+        s, _ = build_system(req, precision, with_rho=False)
+        try:
+            slot = pool.admit_native(s, None, params)
+        except Exception:
+            s.close()
+            raise
+        if slot < 0:
+            s.close()
+        else:
+            req.admitted = now()
+        return slot
+
+    server = Server()
+    ex = ThreadPoolExecutor(max_workers=builders)
+    try:
+        while len(finished) < len(requests):
+            t = now()
+            while qi < len(pending) and pending[qi].arrival <= t:
+                waiting.append(pending[qi])
+                qi += 1
+            t1 = tic()
+            still = []
+            for req in waiting:   # FIFO among arrived requests that fit
+                g = req.grid
+                if g not in pools:
+                    t2 = tic()
+                    pools[g] = _new_pool(req, precision, tune)
+                    reserved[g] = 0
+                    server.add(pools[g])
+                    host["pool_setup_s"] += tic() - t2
+                pool = pools[g]
+                need = float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
+                if len(pool.slots) + reserved[g] < pool.capacity and in_use + need <= hbm_budget:
+                    reserved[g] += 1
+                    in_use += need
+                    building[ex.submit(build_admit, req, pool)] = (req, pool, need)
+                else:
+                    still.append(req)
+            waiting = still
+            for fut in [f for f in building if f.done()]:
+                req, pool, need = building.pop(fut)
+                reserved[req.grid] -= 1
+                slot = fut.result()
+                if slot >= 0:
+                    pool.slots[slot] = req
+                else:
+                    in_use -= need
+                    waiting.insert(0, req)
+            host["submit_s"] += tic() - t1
+            # wake on a finished request, in time for the next arrival, or soon
+            # when admissions are under way
+            nxt = pending[qi].arrival - now() if qi < len(pending) else 0.005
+            tmo = 0.0005 if building else max(0.0, min(0.005, nxt))
+            t1 = tic()
+            server.wait(tmo)
+            host["wait_s"] += tic() - t1
+            t1 = tic()
+            for g, pool in pools.items():
+                for req in pool.collect(now, want_rho=keep_rho):
+                    in_use -= float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
+                    _finish(req, keep_rho, verbose, finished)
+            host["collect_s"] += tic() - t1
+    finally:
+        ex.shutdown(wait=True)
+        server.close()
+        for p in pools.values():
+            p.close()
+    return finished, host
+
+
+def _serve_lockstep(requests, precision, params, hbm_budget, keep_rho, tune, verbose, t0, builders):
+    now = lambda: time.perf_counter() - t0   # noqa: E731
+    tic = time.perf_counter
+    host = {"pool_setup_s": 0.0, "build_s": 0.0, "admit_s": 0.0, "step_s": 0.0, "collect_s": 0.0}
+    pending = sorted(requests, key=lambda r: r.arrival)
+    pools = {}
+    in_use = 0.0
+    finished, waiting = [], []
+    qi = 0
+    try:
+        while len(finished) < len(requests):
+            t = now()
+            while qi < len(pending) and pending[qi].arrival <= t:
+                waiting.append(pending[qi])
+                qi += 1
+            still = []
+            for req in waiting:   # FIFO among arrived requests that fit
+                g = req.grid
+                if g not in pools:
+                    t1 = tic()
+                    pools[g] = _new_pool(req, precision, tune)
+                    host["pool_setup_s"] += tic() - t1
+                pool = pools[g]
+                need = float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
+                if len(pool.slots) < pool.capacity and in_use + need <= hbm_budget:
+                    t1 = tic()
+                    s, rho = build_system(req, precision)
+                    t2 = tic()
+                    ok = pool.admit(req, s, rho, params) >= 0
+                    host["build_s"] += t2 - t1
+                    host["admit_s"] += tic() - t2
+                    if ok:
+                        req.admitted = now()
+                        in_use += need
+                        continue
+                    s.close()
+                still.append(req)
+            waiting = still
+            busy = [pool for pool in pools.values() if pool.slots]
+            t1 = tic()
+            active = step_pools(busy) if busy else 0
+            host["step_s"] += tic() - t1
+            t1 = tic()
+            for g, pool in pools.items():
+                for req in pool.collect(now, want_rho=keep_rho):
+                    in_use -= float(lib().ofd_request_bytes(g[0], g[1], g[2], 4, precision))
+                    _finish(req, keep_rho, verbose, finished)
+            host["collect_s"] += tic() - t1
+            if active == 0 and not waiting and qi < len(pending):
+                time.sleep(max(0.0, min(0.01, pending[qi].arrival - now())))
+    finally:
+        for p in pools.values():
+            p.close()
+    return finished, host
 
 
 def run_sequential(requests, pot_tol=1e-5, precision=0, mixed_switch=0.0, keep_rho=False):

@@ -61,9 +61,12 @@ def _case(grid, lattice, kinetic=4, alpha=-100.0, beta=-100.0, seed=0, nion=3):
 
 
 # generic LDS path (mixed radices, odd lengths) and register-resident fast path
-# (n1/2 in {16, 32, 64, 128, 256}, n2 and n3 in {32, 64, 128, 256})
+# (n1/2 in {16, 24, 32, 40, 48, 64, 96, 128, 256}, n2 and n3 in {24, 32, 40,
+# 48, 64, 80, 96, 128, 192, 256}; the mixed lengths use radix-12/20/24 first
+# stages)
 GRIDS = [(16, 16, 16), (24, 20, 18), (14, 21, 10), (32, 30, 28), (6, 5, 4),
          (64, 32, 64), (128, 64, 32), (64, 128, 256), (512, 32, 32), (32, 64, 128),
+         (48, 48, 48), (96, 80, 40), (80, 192, 24), (192, 48, 96),
          (128, 128, 32), (64, 64, 32), (32, 32, 32)]   # the last three take the fused x+y plane kernel
 
 
@@ -84,7 +87,7 @@ def test_calpot_wt_matches_oracle(pa, grid):
     assert et2 is None and np.max(np.abs(pot2 - pot_ref)) <= POT_RTOL * np.max(np.abs(pot_ref))
 
 
-@pytest.mark.parametrize("grid", [(16, 18, 20), (64, 64, 32)])
+@pytest.mark.parametrize("grid", [(16, 18, 20), (64, 64, 32), (48, 40, 24)])
 def test_calpot_wt_alpha_ne_beta(pa, grid):
     cell, frac, vion, ne, f, rho = _case(grid, _tri_lattice(), alpha=0.9, beta=0.7)
     sysm = O.System(cell, f, vion)
@@ -212,6 +215,29 @@ def test_plane_kernel_matches_separate_passes(pa, monkeypatch):
     (p0, e0), (p1, e1) = res
     assert np.max(np.abs(p0 - p1)) <= 1e-13 * np.max(np.abs(p1))
     _check_terms(e0, e1, rtol=1e-13)
+
+
+@pytest.mark.parametrize("grid", [(96, 96, 96), (80, 80, 80)])
+def test_fast_mixed_lengths_match_generic_path(pa, monkeypatch, grid):
+    """The two-stage register FFT on mixed lengths (radix-24 x 4, 20 x 4, ...)
+    and the generic LDS Stockham passes agree to rounding, in FP64; in the
+    FP32 passes both stay within FP32 rounding of the FP64 result."""
+    cell, frac, vion, ne, f, rho = _case(grid, _tri_lattice(), seed=8)
+    res = {}
+    for prec in (0, 1):
+        for flag in ("0", "1"):
+            monkeypatch.setenv("OFD_GENERIC", flag)
+            s = pa.OFDSystem(cell.lattice, grid, ne, kinetic=4, precision=prec)
+            s.set_ion_terms(vion, 0.0)
+            res[prec, flag] = pa.CalculatePotentialPlus(s, rho)
+            s.close()
+    (p0, e0), (p1, e1) = res[0, "0"], res[0, "1"]
+    assert np.max(np.abs(p0 - p1)) <= 1e-13 * np.max(np.abs(p1))
+    _check_terms(e0, e1, rtol=1e-13)
+    for flag in ("0", "1"):
+        p32, e32 = res[1, flag]
+        assert np.max(np.abs(p32 - p1)) <= 1e-5 * np.max(np.abs(p1))
+        _check_terms(e32, e1, rtol=1e-5, floor=1e-8)
 
 
 def test_ntn_fused_dots_matches_reference_products(pa):
