@@ -79,6 +79,14 @@ void stream_put(hipStream_t s) {
   g_streams.push_back(s);
 }
 
+// A blocking copy ordered on `st` (never on the legacy null stream, which
+// HIP refuses while another thread captures a graph).
+hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
 // ------------------------------------------------------------------ Moré–Thuente
 // Restates MINPACK-2 dcstep/dcsrch (MathDCstep.f, MathDCsrch.f) as used by
 // LineSearchTN (RhoLineSearch.f90:369-419).
@@ -235,7 +243,9 @@ struct LineSearch {
 // from the descriptor array in device memory, so a replay only needs the
 // descriptors uploaded first. The first use of a key runs eagerly (so that
 // one-time function attributes are set outside any capture). No memset or
-// copy is ever recorded. OFD_NO_GRAPH=1 disables the capture.
+// copy is ever recorded. Capture is thread-local: a server's progress thread
+// captures while other threads admit and collect requests. OFD_NO_GRAPH=1
+// disables the capture.
 struct GraphKey {
   int kind, inmode, outmode, fp32, nb;
   const void* desc;
@@ -259,13 +269,13 @@ static bool graphs_enabled() {
 // Runs run_eval + run_finalize for `nb` descriptors at `dd` on `st`,
 // through a cached graph when possible. Returns 0 on success.
 static int launch_eval(GraphCache& gc, const ShapePlan& sp, int kind, int inmode, int outmode, int fp32,
-                       const ReqDesc* dd, int nb, double* fin_out, hipStream_t st, bool capture = true) {
+                       const ReqDesc* dd, int nb, double* fin_out, hipStream_t st) {
   const GraphKey key{kind, inmode, outmode, fp32, nb, dd};
   auto direct = [&]() -> int {
     if (run_eval(sp, kind, inmode, outmode, fp32, dd, nb, st)) return -1;
     return run_finalize(sp, kind, fp32, dd, nb, fin_out, st);
   };
-  if (!capture || !graphs_enabled()) return direct();
+  if (!graphs_enabled()) return direct();
   auto it = gc.exec.find(key);
   if (it == gc.exec.end()) {
     if (gc.seen[key]++ == 0) return direct();   // first use: eager
@@ -528,7 +538,7 @@ extern "C" int ofd_get_volume(const ofd_ctx* c, double* vol) {
 
 extern "C" int ofd_set_ion_terms(ofd_ctx* c, const double* vion, double eii) {
   if (!c || !vion) return fail(c, "null argument");
-  if (hipMemcpy(c->vion, vion, c->nr() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  if (copy_sync(c->vion, vion, c->nr() * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess)
     return fail(c, "copy of the ion potential failed");
   c->e_ionion = eii;
   c->have_vion = true;
@@ -537,7 +547,7 @@ extern "C" int ofd_set_ion_terms(ofd_ctx* c, const double* vion, double eii) {
 
 extern "C" int ofd_get_ion_potential(ofd_ctx* c, double* vion) {
   if (!c || !vion) return fail(c, "null argument");
-  if (hipMemcpy(vion, c->vion, c->nr() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+  if (copy_sync(vion, c->vion, c->nr() * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess)
     return fail(c, "copy of the ion potential failed");
   return 0;
 }
@@ -560,9 +570,11 @@ static void etable_from_sums(const ofd_ctx* c, const double* s, double* et) {
 // One evaluation on the context's own descriptor; sums (kNSum) copied back.
 static int eval_one(ofd_ctx* c, const ReqDesc& D, int inmode, int outmode, int fp32, double* h_sums) {
   if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set (ofd_set_wgc_table)");
-  if (hipMemcpy(c->desc, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess)
-    return fail(c, "descriptor upload failed");
+  // stream-ordered upload from the context's own copy, which stays unchanged
+  // until the synchronization below
   c->last_desc = D;
+  if (hipMemcpyAsync(c->desc, &c->last_desc, sizeof(D), hipMemcpyHostToDevice, c->st) != hipSuccess)
+    return fail(c, "descriptor upload failed");
   if (launch_eval(c->graphs, c->sp, c->kind, inmode, outmode, fp32, c->desc, 1, c->fin, c->st))
     return fail(c, "kernel launch failed");
   if (h_sums) {
@@ -593,13 +605,13 @@ extern "C" int ofd_calpot_enqueue(ofd_ctx* c, const double* rho, int opt_sqrt, d
   D.out = pot;
   D.opt_sqrt = opt_sqrt ? 1 : 0;
   if (sums_dev) D.fin = sums_dev;
-  // the descriptor is uploaded only when it changes (blocking copy, so the
-  // host copy may go out of scope at once)
+  // the descriptor is uploaded only when it changes, from the context's own
+  // copy (the stream is drained first, so no earlier upload still reads it)
   if (std::memcmp(&D, &c->last_desc, sizeof(D)) != 0) {
     if (hipStreamSynchronize(c->st) != hipSuccess) return fail(c, "stream synchronize failed");
-    if (hipMemcpy(c->desc, &D, sizeof(D), hipMemcpyHostToDevice) != hipSuccess)
-      return fail(c, "descriptor upload failed");
     c->last_desc = D;
+    if (hipMemcpyAsync(c->desc, &c->last_desc, sizeof(D), hipMemcpyHostToDevice, c->st) != hipSuccess)
+      return fail(c, "descriptor upload failed");
   }
   const int fp32 = c->precision == 1;
   if (launch_eval(c->graphs, c->sp, c->kind, kInRho, kOutPot, fp32, c->desc, 1, sums_dev ? sums_dev : c->fin,
@@ -651,8 +663,9 @@ extern "C" int ofd_calpot_many(ofd_ctx* const* cs, int nb, const double* const* 
     if (c0->many_last.size() != (size_t)nb ||
         std::memcmp(h.data(), c0->many_last.data(), sizeof(ReqDesc) * nb) != 0) {
       HIPCHK(hipStreamSynchronize(c0->st));
-      HIPCHK(hipMemcpy(c0->many_desc, h.data(), sizeof(ReqDesc) * nb, hipMemcpyHostToDevice));
       c0->many_last = h;
+      HIPCHK(hipMemcpyAsync(c0->many_desc, c0->many_last.data(), sizeof(ReqDesc) * nb, hipMemcpyHostToDevice,
+                            c0->st));
     }
     const int fp32 = c0->precision == 1;
     if (launch_eval(c0->graphs, c0->sp, c0->kind, kInRho, kOutPot, fp32, c0->many_desc, nb, c0->many_sums, c0->st))
@@ -674,7 +687,7 @@ extern "C" int ofd_calpot(ofd_ctx* c, const double* rho, int opt_sqrt, double* p
   const size_t b = c->nr() * sizeof(double);
   if (hipMemcpyAsync(c->rho, rho, b, hipMemcpyHostToDevice, c->st) != hipSuccess) return fail(c, "copy failed");
   if (ofd_calpot_device(c, c->rho, opt_sqrt, c->pot, et)) return -1;
-  if (hipMemcpy(pot, c->pot, b, hipMemcpyDeviceToHost) != hipSuccess) return fail(c, "copy failed");
+  if (copy_sync(pot, c->pot, b, hipMemcpyDeviceToHost, c->st) != hipSuccess) return fail(c, "copy failed");
   return 0;
 }
 
@@ -1598,7 +1611,7 @@ extern "C" int ofd_ion_forces(ofd_ctx* c, const double* rho, double* forces) {
     int rc = ion_forces_spline(c->sp, c->rb, cinv, ns, c->type_start.data(), c->d_frac, nion, c->d_psp_vqs,
                                c->d_psp_dd, c->psp_off.data(), c->psp_npts.data(), c->psp_maxg.data(),
                                c->psp_v0.data(), c->psp_charge.data(), c->spline_order, c->rho, df, sc, c->st);
-    HIPCHK(hipMemcpy(forces, df, sizeof(double) * 3 * nion, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(forces, df, sizeof(double) * 3 * nion, hipMemcpyDeviceToHost, c->st));
     hipStreamSynchronize(c->st);
     dev_free(sc);
     dev_free(df);
@@ -1622,7 +1635,7 @@ static double time_current(ofd_ctx* c, int reps) {
   D.out = c->pot;
   D.opt_sqrt = 1;
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(c->desc, &D, sizeof(D), hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c->desc, &D, sizeof(D), hipMemcpyHostToDevice, c->st));
   c->last_desc = D;
   const int fp32 = c->precision == 1;
   auto once = [&]() {
@@ -1651,7 +1664,7 @@ extern "C" int ofd_autotune(ofd_ctx* c, int reps, ofd_tune_result* res) {
   try {
     const long long n = c->nr();
     std::vector<double> h(n, c->num_ele / c->vol);
-    HIPCHK(hipMemcpy(c->rho, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(c->rho, h.data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
     graphs_clear(c->graphs);
     const bool fast = !c->sp.force_generic && fast_path_ok(c->sp, c->kind);
     const int fp32 = c->precision == 1;
@@ -1738,6 +1751,11 @@ struct ofd_pool {
     std::chrono::steady_clock::time_point t0;
   };
   std::vector<Slot> slots;
+  // descriptors and sums of one phase: launch group g (functional kind,
+  // output mode, precision) owns rows [g * capacity, (g + 1) * capacity), so
+  // a group's launch always reads the same descriptor array and its captured
+  // graph is keyed by the batch width alone
+  static constexpr int kGroups = 3 * 3 * 2;
   ReqDesc* d_desc = nullptr;
   double* d_sums = nullptr;
   ReqDesc* h_desc = nullptr;   // pinned
@@ -1781,8 +1799,9 @@ extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic
   p->n1 = n1; p->n2 = n2; p->n3 = n3; p->capacity = capacity; p->kinetic = kinetic; p->precision = precision;
   p->slots.resize(capacity);
   p->where.assign(capacity, -1);
-  if (hipHostMalloc((void**)&p->h_desc, sizeof(ReqDesc) * capacity, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&p->h_sums, sizeof(double) * kNSum * capacity, hipHostMallocDefault) != hipSuccess) {
+  const size_t rows = (size_t)ofd_pool::kGroups * capacity;
+  if (hipHostMalloc((void**)&p->h_desc, sizeof(ReqDesc) * rows, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&p->h_sums, sizeof(double) * kNSum * rows, hipHostMallocDefault) != hipSuccess) {
     p->err = "pinned host allocation failed";
     *out = p;
     return -1;
@@ -1803,8 +1822,8 @@ extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic
   }
   if (hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&p->ev_phase, hipEventDisableTiming) != hipSuccess ||
-      hipMalloc(&p->d_desc, sizeof(ReqDesc) * capacity) != hipSuccess ||
-      hipMalloc(&p->d_sums, sizeof(double) * kNSum * capacity) != hipSuccess) {
+      hipMalloc(&p->d_desc, sizeof(ReqDesc) * rows) != hipSuccess ||
+      hipMalloc(&p->d_sums, sizeof(double) * kNSum * rows) != hipSuccess) {
     p->err = "pool allocation failed";
     *out = p;
     return -1;
@@ -1920,12 +1939,13 @@ static bool launch_phase(ofd_pool* p, bool with_dots, bool with_evals) {
       if (!s.used || !s.ready || s.done) continue;
       if (s.S.need.kind == NEED_EVAL) groups[s.c->kind][s.S.need.ev.outmode][s.S.fp32 ? 1 : 0].push_back(i);
     }
-    int off = 0;
+    int nev = 0;
     for (int kd = 0; kd < 3; ++kd)
       for (int om = 0; om < 3; ++om)
         for (int pr = 0; pr < 2; ++pr) {
           auto& g = groups[kd][om][pr];
           if (g.empty()) continue;
+          const int off = ((kd * 3 + om) * 2 + pr) * p->capacity;
           for (size_t k = 0; k < g.size(); ++k) {
             p->h_desc[off + k] = desc_for(p->slots[g[k]].c, p->slots[g[k]].S.need.ev);
             p->h_desc[off + k].fin = p->d_sums + (size_t)(off + k) * kNSum;
@@ -1933,16 +1953,14 @@ static bool launch_phase(ofd_pool* p, bool with_dots, bool with_evals) {
           }
           HIPCHK(hipMemcpyAsync(p->d_desc + off, p->h_desc + off, sizeof(ReqDesc) * g.size(),
                                 hipMemcpyHostToDevice, p->st));
-          // no capture while a server drives the pool: another thread's
-          // allocations and frees would run during the capture
           if (launch_eval(p->graphs, p->sp, kd, kInPhi, om, pr, p->d_desc + off, (int)g.size(),
-                          p->d_sums + (size_t)off * kNSum, p->st, p->server == nullptr))
+                          p->d_sums + (size_t)off * kNSum, p->st))
             throw std::runtime_error("batched evaluation launch failed");
-          off += (int)g.size();
+          HIPCHK(hipMemcpyAsync(p->h_sums + (size_t)off * kNSum, p->d_sums + (size_t)off * kNSum,
+                                sizeof(double) * kNSum * g.size(), hipMemcpyDeviceToHost, p->st));
+          nev += (int)g.size();
         }
-    p->nsum = off;
-    if (off > 0)
-      HIPCHK(hipMemcpyAsync(p->h_sums, p->d_sums, sizeof(double) * kNSum * off, hipMemcpyDeviceToHost, p->st));
+    p->nsum = nev;
   }
   if (p->dots.empty() && p->nsum == 0) return false;
   HIPCHK(hipEventRecord(p->ev_phase, p->st));

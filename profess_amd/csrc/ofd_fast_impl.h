@@ -284,7 +284,9 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
 }
 
 // ------------------------------------------------------------ fused x + y forward (one plane)
-// One workgroup per (z-plane i3, field f, request). The plane's half
+// One workgroup per (z-plane i3, field f, request); the NF workgroups of a
+// plane are dispatched one after another on the same XCD, so the plane's rho
+// is read from HBM once and from that XCD's L2 by the other fields. The plane's half
 // spectrum (K1 x n2) of the field stays in LDS between the x R2C and the y
 // FFT: rows of stride S >= max(M + M/R1x, M + 1) complex, n2 + n2/R1y rows so
 // that the y transform can use the padded column layout as its scratch.
@@ -307,9 +309,10 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
   constexpr int R1X = P2<M>::R1, TX = G::TX, R1Y = P2<N2>::R1, TY = G::TY, S = G::S;
   extern __shared__ __align__(16) unsigned char smem[];
   C* P = reinterpret_cast<C*>(smem);
-  const ReqDesc& D = descs[blockIdx.z];
-  const int f = blockIdx.y;
-  const int i3 = blockIdx.x;
+  const ReqDesc& D = descs[blockIdx.y];
+  constexpr int NFK = NF_<KIND>::v;
+  const int pf = xcd_block(blockIdx.x, gridDim.x);
+  const int i3 = pf / NFK, f = pf - i3 * NFK;
   // ---- x step: thread (row i2, lane t)
   const int row = threadIdx.x / TX, tx = threadIdx.x % TX;
   const bool ax = row < N2;
@@ -758,7 +761,7 @@ static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStre
     hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
-  hipLaunchKernelGGL(kern, dim3(sp.d.n3, NF_<KIND>::v, nb), dim3(G::THREADS), lds, st, dd, sp.d, twx, tws, twy);
+  hipLaunchKernelGGL(kern, dim3(sp.d.n3 * NF_<KIND>::v, nb), dim3(G::THREADS), lds, st, dd, sp.d, twx, tws, twy);
   return true;
 }
 

@@ -133,8 +133,15 @@ static int upload_tw(int n, int count, double2** d64, float2** d32) {
   TwTables t;
   if (hipMalloc(&t.d64, sizeof(double2) * count) != hipSuccess) return -1;
   if (hipMalloc(&t.d32, sizeof(float2) * count) != hipSuccess) return -1;
-  if (hipMemcpy(t.d64, h.data(), sizeof(double2) * count, hipMemcpyHostToDevice) != hipSuccess) return -1;
-  if (hipMemcpy(t.d32, hf.data(), sizeof(float2) * count, hipMemcpyHostToDevice) != hipSuccess) return -1;
+  // copies on a stream of their own: the legacy null stream is refused while
+  // another thread captures a graph
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return -1;
+  const bool ok = hipMemcpyAsync(t.d64, h.data(), sizeof(double2) * count, hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(t.d32, hf.data(), sizeof(float2) * count, hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipStreamSynchronize(st) == hipSuccess;
+  hipStreamDestroy(st);
+  if (!ok) return -1;
   g_tw[{n, count}] = t;
   *d64 = t.d64;
   *d32 = t.d32;

@@ -199,10 +199,11 @@ class Server:
 
 
 def capacity_for(grid):
-    """Slots of a pool: enough concurrent requests that one batched launch
-    set fills the device even at 32^3; memory is bounded separately by the
-    HBM budget at admission."""
-    return 32
+    """Slots of a pool: 64 concurrent requests, so one batched launch set
+    fills the device even at 32^3 and a burst of arrivals on one grid runs in
+    few generations; memory is bounded separately by the HBM budget at
+    admission."""
+    return 64
 
 
 def run_workload(requests, mixed=True, hbm_budget=160e9, pot_tol=1e-5, mixed_switch=1e-3, verbose=False,
