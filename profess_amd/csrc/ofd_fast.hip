@@ -42,11 +42,11 @@ int fast_eval(const ShapePlan& sp, int kind, int inmode, int outmode, int fp32, 
 
 std::vector<int> fast_tile_candidates(const ShapePlan& sp, int kind, int fp32, int which) {
   const int n = which == 1 ? sp.d.n2 : sp.d.n3;
-  const int T = p2_T(n), w = 64 / T;
+  const int T = len_T(n), w = 64 / T;
   const int cb = fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
   std::vector<int> out;
   for (int kb = w; kb * T <= 256; kb += w)
-    if ((long long)kb * p2_cs(n) * (which == 1 ? cb / 2 : cb) <= sp.lds_cap) out.push_back(kb);
+    if (kb * col_line_bytes(n, cb, which) <= sp.lds_cap) out.push_back(kb);
   (void)kind;
   return out;
 }
@@ -64,13 +64,9 @@ bool fast_plane_available(const ShapePlan& sp, int kind, int fp32) {
   return false;
 }
 
-int fast_ntile(const ShapePlan& sp, int kind, int fp32) {
-  return fast_tiles(sp, func_nf(kind), fp32 ? (int)sizeof(float2) : (int)sizeof(double2)).ntile;
-}
-
 // Sums the x-inverse pass's per-workgroup partials into each request's fin.
 int fast_finalize(const ShapePlan& sp, int kind, int, const ReqDesc* dd, int nb, double*, hipStream_t st) {
-  const int rpb = fast::inv_rpb(func_nf(kind), p2_T(sp.d.M));
+  const int rpb = fast::inv_rpb(func_nf(kind), len_T(sp.d.M));
   const long long nrows = (long long)sp.d.n2 * sp.d.n3;
   const int nwg = (int)((nrows + rpb - 1) / rpb);
   hipLaunchKernelGGL(fast::k_fin, dim3(kNSum, nb), dim3(256), 0, st, dd, nwg);

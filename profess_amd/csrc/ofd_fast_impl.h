@@ -617,6 +617,392 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
   block_sum_strided<kNSum>(acc, D.sums + blk, gridDim.x);
 }
 
+// ------------------------------------------------------------ general two-stage lengths
+// n = R1*R2 where no split has R2 dividing R1 (120 = 8*15, 60 = 4*15). A line
+// is owned by T threads: thread t holds the stage-1 columns c = t + T*j
+// (j < J1, c < R2), i.e. the points c + R2*m (m < R1), and after the forward
+// transform the stage-2 rows k1 = t + T*q (q < J2, k1 < R1), i.e. the
+// outputs k1 + R1*k2. The forward transform is decimation in time (columns
+// -> rows), the inverse decimation in frequency (rows -> columns), so a
+// pointwise G-space operator between the two and the loads and stores
+// around them need no reordering; each direction crosses lanes once
+// through the line's LDS block (R2 rows of LDR complex, LDR odd).
+template <int N> struct GP;
+template <> struct GP<60> { static constexpr int R1 = 4, R2 = 15, T = 4; };
+template <> struct GP<120> { static constexpr int R1 = 8, R2 = 15, T = 8; };
+template <int N> struct IsGP { static constexpr bool v = false; };
+template <> struct IsGP<60> { static constexpr bool v = true; };
+template <> struct IsGP<120> { static constexpr bool v = true; };
+template <int N> struct GPD {
+  static constexpr int R1 = GP<N>::R1, R2 = GP<N>::R2, T = GP<N>::T;
+  static constexpr int J1 = (R2 + T - 1) / T, J2 = (R1 + T - 1) / T;
+  static constexpr int LDR = R1 | 1;
+  static constexpr int LINE = (R2 * LDR) | 1;   // complex elements per line block
+  static constexpr int NC = J1 * R1, NS = J2 * R2;
+  static_assert(LINE >= N, "a line block holds the whole line in natural order");
+  __device__ static constexpr bool col_ok(int t, int j) { return J1 * T == R2 || t + T * j < R2; }
+  __device__ static constexpr bool row_ok(int t, int q) { return J2 * T == R1 || t + T * q < R1; }
+};
+
+// forward, decimation in time: xc (columns) -> xs (rows); two barriers
+template <int N, int S, typename C>
+__device__ __forceinline__ void gfwd(C (&xc)[GPD<N>::NC], C (&xs)[GPD<N>::NS], C* __restrict__ lds, int t,
+                                     const C* __restrict__ tw, bool active) {
+  using G = GPD<N>;
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < G::J1; ++j) {
+      if (!G::col_ok(t, j)) continue;
+      const int c = t + G::T * j;
+      Bfly<G::R1, S, C>::run(&xc[j * G::R1]);
+#pragma unroll
+      for (int m = 0; m < G::R1; ++m) lds[c * G::LDR + m] = xc[j * G::R1 + m];
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < G::J2; ++q) {
+      if (!G::row_ok(t, q)) continue;
+      const int k1 = t + G::T * q;
+      C y[G::R2];
+#pragma unroll
+      for (int c = 0; c < G::R2; ++c) y[c] = lds[c * G::LDR + k1];
+#pragma unroll
+      for (int c = 1; c < G::R2; ++c) {
+        C w = tw[c * k1];
+        if (S > 0) w.y = -w.y;
+        y[c] = cmul(y[c], w);
+      }
+      Bfly<G::R2, S, C>::run(y);
+#pragma unroll
+      for (int k2 = 0; k2 < G::R2; ++k2) xs[q * G::R2 + k2] = y[k2];
+    }
+  }
+  __syncthreads();
+}
+
+// inverse, decimation in frequency: xs (rows) -> xc (columns); two barriers
+template <int N, int S, typename C>
+__device__ __forceinline__ void ginv(C (&xs)[GPD<N>::NS], C (&xc)[GPD<N>::NC], C* __restrict__ lds, int t,
+                                     const C* __restrict__ tw, bool active) {
+  using G = GPD<N>;
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < G::J2; ++q) {
+      if (!G::row_ok(t, q)) continue;
+      const int k1 = t + G::T * q;
+      Bfly<G::R2, S, C>::run(&xs[q * G::R2]);
+      lds[k1] = xs[q * G::R2];
+#pragma unroll
+      for (int c = 1; c < G::R2; ++c) {
+        C w = tw[c * k1];
+        if (S > 0) w.y = -w.y;
+        lds[c * G::LDR + k1] = cmul(xs[q * G::R2 + c], w);
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < G::J1; ++j) {
+      if (!G::col_ok(t, j)) continue;
+      const int c = t + G::T * j;
+#pragma unroll
+      for (int k1 = 0; k1 < G::R1; ++k1) xc[j * G::R1 + k1] = lds[c * G::LDR + k1];
+      Bfly<G::R1, S, C>::run(&xc[j * G::R1]);
+    }
+  }
+  __syncthreads();
+}
+
+// y columns (flattened over (i3, k1)), one field per blockIdx.y: S = -1
+// loads columns and stores rows, S = +1 the reverse.
+template <int N, int S, typename R>
+__global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, GridDims d,
+                                            const typename V2<R>::T* __restrict__ tw, int kb) {
+  using C = typename V2<R>::T;
+  using G = GPD<N>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.z];
+  const int cl = threadIdx.x % kb, t = threadIdx.x / kb;
+  const long long col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + cl;
+  const long long ncol = (long long)d.K1 * d.n3;
+  const bool active = col < ncol;
+  const long long i3 = active ? col / d.K1 : 0, k1c = active ? col - i3 * d.K1 : 0;
+  C* base = reinterpret_cast<C*>(D.work) + (size_t)blockIdx.y * d.nc + (size_t)i3 * d.n2 * d.K1 + k1c;
+  C xc[G::NC], xs[G::NS];
+  if (S < 0) {
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < G::J1; ++j)
+        if (G::col_ok(t, j)) {
+#pragma unroll
+          for (int m = 0; m < G::R1; ++m) xc[j * G::R1 + m] = base[(size_t)(t + G::T * j + G::R2 * m) * d.K1];
+        }
+    }
+    gfwd<N, S>(xc, xs, lds + (size_t)cl * G::LINE, t, tw, active);
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < G::J2; ++q)
+        if (G::row_ok(t, q)) {
+#pragma unroll
+          for (int k2 = 0; k2 < G::R2; ++k2) base[(size_t)(t + G::T * q + G::R1 * k2) * d.K1] = xs[q * G::R2 + k2];
+        }
+    }
+  } else {
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < G::J2; ++q)
+        if (G::row_ok(t, q)) {
+#pragma unroll
+          for (int k2 = 0; k2 < G::R2; ++k2) xs[q * G::R2 + k2] = base[(size_t)(t + G::T * q + G::R1 * k2) * d.K1];
+        }
+    }
+    ginv<N, S>(xs, xc, lds + (size_t)cl * G::LINE, t, tw, active);
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < G::J1; ++j)
+        if (G::col_ok(t, j)) {
+#pragma unroll
+          for (int m = 0; m < G::R1; ++m) base[(size_t)(t + G::T * j + G::R2 * m) * d.K1] = xc[j * G::R1 + m];
+        }
+    }
+  }
+}
+
+// z columns + the diagonal G-space operator, one field per blockIdx.y
+template <int N, int KIND, typename R>
+__global__ void __launch_bounds__(256) g_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
+                                                const typename V2<R>::T* __restrict__ tw, int kb) {
+  using C = typename V2<R>::T;
+  using G = GPD<N>;
+  constexpr int NF = NF_<KIND>::v;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.z];
+  const int f = blockIdx.y;
+  const int cl = threadIdx.x % kb, t = threadIdx.x / kb;
+  const long long col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + cl;
+  const long long ncol = (long long)d.K1 * d.n2;
+  const bool active = col < ncol;
+  const int i2 = active ? (int)(col / d.K1) : 0;
+  const int m1 = active ? (int)(col - (long long)i2 * d.K1) : 0;
+  const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
+  const size_t es = (size_t)d.n2 * d.K1;
+  const double invn = 1.0 / (double)d.nr;
+  C* base = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)col;
+  C xc[G::NC], xs[G::NS];
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < G::J1; ++j)
+      if (G::col_ok(t, j)) {
+#pragma unroll
+        for (int m = 0; m < G::R1; ++m) xc[j * G::R1 + m] = base[(size_t)(t + G::T * j + G::R2 * m) * es];
+      }
+  }
+  C* mine = lds + (size_t)cl * G::LINE;
+  gfwd<N, -1>(xc, xs, mine, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < G::J2; ++q) {
+      if (!G::row_ok(t, q)) continue;
+#pragma unroll
+      for (int k2 = 0; k2 < G::R2; ++k2) {
+        const int i3 = t + G::T * q + G::R1 * k2;
+        const int m3 = i3 > N / 2 ? i3 - N : i3;
+        const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
+        const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
+        const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
+        const double q2 = gx * gx + gy * gy + gz * gz;
+        double sc;
+        if (f == 0) sc = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
+        else if (f == NF - 1) sc = q2 * invn;
+        else sc = D.wtk[(size_t)i3 * es + col];
+        C& v = xs[q * G::R2 + k2];
+        v.x = (R)((double)v.x * sc);
+        v.y = (R)((double)v.y * sc);
+      }
+    }
+  }
+  ginv<N, +1>(xs, xc, mine, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < G::J1; ++j)
+      if (G::col_ok(t, j)) {
+#pragma unroll
+        for (int m = 0; m < G::R1; ++m) base[(size_t)(t + G::T * j + G::R2 * m) * es] = xc[j * G::R1 + m];
+      }
+  }
+}
+
+// x rows, forward: fields, half-length FFT, R2C split (lines = (row, field))
+template <int M, int KIND, int INMODE, typename R>
+__global__ void __launch_bounds__(256) g_fx_fwd(const ReqDesc* __restrict__ descs, GridDims d,
+                                                const typename V2<R>::T* __restrict__ tw,
+                                                const typename V2<R>::T* __restrict__ tws) {
+  using C = typename V2<R>::T;
+  using G = GPD<M>;
+  constexpr int NF = NF_<KIND>::v, T = G::T;
+  constexpr int RPB = inv_rpb(NF, T);
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.y];
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int r = line / NF, f = line - r * NF;
+  const long long nrows = (long long)d.n2 * d.n3;
+  const long long row = (long long)xcd_block(blockIdx.x, gridDim.x) * RPB + r;
+  const bool active = row < nrows;
+  C xc[G::NC], xs[G::NS];
+  if (active) {
+    const size_t base = (size_t)row * (2 * M);
+#pragma unroll
+    for (int j = 0; j < G::J1; ++j) {
+      if (!G::col_ok(t, j)) continue;
+#pragma unroll
+      for (int m = 0; m < G::R1; ++m) {
+        const int k = t + T * j + G::R2 * m;
+        const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
+        double a, b;
+        if (INMODE == kInRho) {
+          a = uu.x;
+          b = uu.y;
+        } else {
+          double ta = D.ca * uu.x, tb = D.ca * uu.y;
+          if (D.cb != 0.0) {
+            const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
+            ta = fma(D.cb, vv.x, ta);
+            tb = fma(D.cb, vv.y, tb);
+          }
+          a = ta * ta;
+          b = tb * tb;
+        }
+        xc[j * G::R1 + m].x = (R)field_of<KIND>(D, f, a);
+        xc[j * G::R1 + m].y = (R)field_of<KIND>(D, f, b);
+      }
+    }
+  }
+  C* mine = lds + (size_t)line * G::LINE;
+  gfwd<M, -1>(xc, xs, mine, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < G::J2; ++q)
+      if (G::row_ok(t, q)) {
+#pragma unroll
+        for (int k2 = 0; k2 < G::R2; ++k2) mine[t + T * q + G::R1 * k2] = xs[q * G::R2 + k2];
+      }
+  }
+  __syncthreads();
+  if (active) {
+    C* out = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)row * (M + 1);
+    for (int k = t; k <= M; k += T) {
+      const C zk = mine[k == M ? 0 : k];
+      const C zm = mine[k == 0 ? 0 : M - k];
+      const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
+      const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
+      const C w = tws[k];
+      C x;
+      x.x = er + (w.x * orr - w.y * oi);
+      x.y = ei + (w.x * oi + w.y * orr);
+      out[k] = x;
+    }
+  }
+}
+
+// x rows, inverse: C2R merge, half-length inverse FFT, epilogue and sums
+template <int M, int KIND, int INMODE, int OUTMODE, typename R>
+__global__ void __launch_bounds__(256) g_fx_inv(const ReqDesc* __restrict__ descs, GridDims d,
+                                                const typename V2<R>::T* __restrict__ tw,
+                                                const typename V2<R>::T* __restrict__ tws) {
+  using C = typename V2<R>::T;
+  using G = GPD<M>;
+  constexpr int NF = NF_<KIND>::v, T = G::T;
+  constexpr int RPB = inv_rpb(NF, T);
+  constexpr int LL = G::LINE;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.y];
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int r = line / NF, f = line - r * NF;
+  const long long nrows = (long long)d.n2 * d.n3;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const long long row0 = (long long)blk * RPB;
+  const long long row = row0 + r;
+  const bool active = row < nrows;
+  C* mine = lds + (size_t)line * LL;
+  C xc[G::NC], xs[G::NS];
+  if (active) {
+    const C* X = reinterpret_cast<const C*>(D.work) + (size_t)f * d.nc + (size_t)row * (M + 1);
+#pragma unroll
+    for (int q = 0; q < G::J2; ++q) {
+      if (!G::row_ok(t, q)) continue;
+#pragma unroll
+      for (int k2 = 0; k2 < G::R2; ++k2) {
+        const int k = t + T * q + G::R1 * k2;
+        C xk = X[k], xm = X[M - k];
+        if (k == 0) { xk.y = (R)0; xm.y = (R)0; }
+        const R er = xk.x + xm.x, ei = xk.y - xm.y;
+        const R dr = xk.x - xm.x, di = xk.y + xm.y;
+        const C w = tws[k];
+        const R orr = w.x * dr + w.y * di, oi = w.x * di - w.y * dr;
+        xs[q * G::R2 + k2].x = er - oi;
+        xs[q * G::R2 + k2].y = ei + orr;
+      }
+    }
+  }
+  ginv<M, +1>(xs, xc, mine, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < G::J1; ++j)
+      if (G::col_ok(t, j)) {
+#pragma unroll
+        for (int m = 0; m < G::R1; ++m) mine[t + T * j + G::R2 * m] = xc[j * G::R1 + m];
+      }
+  }
+  __syncthreads();
+  double acc[kNSum];
+#pragma unroll
+  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
+  const int nrow = (int)min((long long)RPB, nrows - row0);
+  const int npts = nrow * 2 * M;
+  const size_t gbase = (size_t)row0 * 2 * M;
+  const R* lr = reinterpret_cast<const R*>(lds);
+  for (int e = threadIdx.x; e < npts; e += blockDim.x) {
+    const int rr = e / (2 * M), i = e - rr * (2 * M);
+    const size_t g = gbase + e;
+    double o[NF];
+#pragma unroll
+    for (int j = 0; j < NF; ++j) o[j] = (double)lr[2 * (size_t)(rr * NF + j) * LL + i];
+    double tphi;
+    const double rho = load_rho<INMODE>(D, g, tphi);
+    const double pot = epi<KIND>(D, rho, D.vion[g], o, acc);
+    if (OUTMODE == kOutPot) {
+      D.out[g] = pot;
+    } else if (OUTMODE == kOutSigned) {
+      const double de = pot * fsign(tphi);
+      D.out[g] = de;
+      const double dir = D.s1 * D.u[g] + (D.v != nullptr ? D.s2 * D.v[g] : 0.0);
+      acc[6] += de * dir;
+      const double dl = de - 2.0 * tphi * D.mu0;
+      acc[7] += dl * dl;
+      acc[8] += de * tphi;
+      acc[9] += de * de;
+      acc[10] += tphi * tphi;
+    } else {
+      const double de = pot * fsign(tphi);
+      const double ap = ((de - D.mu0 * 2.0 * tphi) - D.bvec[g]) / D.eps;
+      D.out[g] = ap;
+      acc[6] += D.v[g] * ap;
+      if (D.w != nullptr) {
+        acc[7] += D.w[g] * ap;
+        acc[8] += ap * ap;
+      }
+    }
+  }
+  block_sum_strided<kNSum>(acc, D.sums + blk, gridDim.x);
+}
+
 }  // namespace fast
 
 // ------------------------------------------------------------ dispatch
@@ -634,12 +1020,17 @@ static inline bool is_fast_half(int m) {
   }
 }
 
+// lengths of the general two-stage form (decimation in time forward, in
+// frequency inverse)
+static inline bool is_gp_len(int n) { return n == 60 || n == 120; }
+
 inline bool fast_path_ok_impl(const ShapePlan& sp, int kind) {
-  return (kind == kWT3 || kind == kWT4) && is_fast_half(sp.d.M) && is_fast_len(sp.d.n2) && is_fast_len(sp.d.n3);
+  return (kind == kWT3 || kind == kWT4) && (is_fast_half(sp.d.M) || is_gp_len(sp.d.M)) &&
+         (is_fast_len(sp.d.n2) || is_gp_len(sp.d.n2)) && (is_fast_len(sp.d.n3) || is_gp_len(sp.d.n3));
 }
 
 struct FastTiles {
-  int rpb, kby, kbz, ntile;
+  int kby, kbz;
 };
 
 template <typename F> static inline int p2_of(int n, F get) {
@@ -663,26 +1054,36 @@ static inline int p2_R1(int n) { return p2_of(n, [](auto p) { return decltype(p)
 static inline int p2_ll(int n) { return n + n / p2_R1(n); }
 static inline int p2_cs(int n) { return p2_ll(n) | 1; }   // in real components (column passes)
 
-// Tiles of the fast passes: at most 256 threads per workgroup and at most
-// lds_target bytes of LDS (the tuner may override rows_x/kb_y/kb_z).
+template <typename F> static inline int gp_of(int n, F get) {
+  using namespace fast;
+  return n == 60 ? get(GPD<60>()) : get(GPD<120>());
+}
+// threads per line of the two-stage plan of length n (either form)
+static inline int len_T(int n) {
+  return is_gp_len(n) ? gp_of(n, [](auto g) { return decltype(g)::T; }) : p2_T(n);
+}
+// LDS bytes per line of a column pass (y: which = 1, exchanging one real
+// component at a time in the power-of-two form; z: which = 2)
+static inline long long col_line_bytes(int n, int cbytes, int which) {
+  if (is_gp_len(n)) return (long long)gp_of(n, [](auto g) { return decltype(g)::LINE; }) * cbytes;
+  return (long long)p2_cs(n) * (which == 1 ? cbytes / 2 : cbytes);
+}
+
+// Column tiles of the y and z passes: whole wavefronts, at most 256 threads
+// and at most lds_target bytes of LDS per workgroup (a tuned kb_y/kb_z is
+// bound by the LDS capacity instead).
 static inline FastTiles fast_tiles(const ShapePlan& sp, int nf, int cbytes) {
+  (void)nf;
   FastTiles ft;
   const long long tgt = sp.lds_target;
-  const int M = sp.d.M;
-  const int llx = std::max(p2_ll(M), M + 1);
-  // whole wavefronts only: lines per workgroup are multiples of 64/T
-  const int wx = 64 / p2_T(M), wy = 64 / p2_T(sp.d.n2), wz = 64 / p2_T(sp.d.n3);
-  ft.rpb = sp.rows_x > 0 ? sp.rows_x : 256 / p2_T(M);
-  while (ft.rpb > wx && ((long long)ft.rpb * nf * llx * cbytes > tgt || ft.rpb * p2_T(M) > 256)) ft.rpb -= wx;
-  // a tuned value (sp.kb_y/kb_z > 0) is bound by the LDS capacity, a derived
-  // one by the LDS target
+  const int Ty = len_T(sp.d.n2), Tz = len_T(sp.d.n3);
+  const int wy = 64 / Ty, wz = 64 / Tz;
+  const long long ly = col_line_bytes(sp.d.n2, cbytes, 1), lz = col_line_bytes(sp.d.n3, cbytes, 2);
   const long long ty = sp.kb_y > 0 ? sp.lds_cap : tgt, tz = sp.kb_z > 0 ? sp.lds_cap : tgt;
-  ft.kby = sp.kb_y > 0 ? std::max(wy, sp.kb_y / wy * wy) : 256 / p2_T(sp.d.n2);
-  while (ft.kby > wy && ((long long)ft.kby * p2_cs(sp.d.n2) * (cbytes / 2) > ty || ft.kby * p2_T(sp.d.n2) > 256)) ft.kby -= wy;
-  ft.kbz = sp.kb_z > 0 ? std::max(wz, sp.kb_z / wz * wz) : 256 / p2_T(sp.d.n3);
-  while (ft.kbz > wz && ((long long)ft.kbz * p2_cs(sp.d.n3) * cbytes > tz || ft.kbz * p2_T(sp.d.n3) > 256)) ft.kbz -= wz;
-  const long long nrows = (long long)sp.d.n2 * sp.d.n3;
-  ft.ntile = (int)((nrows + ft.rpb - 1) / ft.rpb);
+  ft.kby = sp.kb_y > 0 ? std::max(wy, sp.kb_y / wy * wy) : 256 / Ty;
+  while (ft.kby > wy && (ft.kby * ly > ty || ft.kby * Ty > 256)) ft.kby -= wy;
+  ft.kbz = sp.kb_z > 0 ? std::max(wz, sp.kb_z / wz * wz) : 256 / Tz;
+  while (ft.kbz > wz && (ft.kbz * lz > tz || ft.kbz * Tz > 256)) ft.kbz -= wz;
   return ft;
 }
 
@@ -701,8 +1102,24 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
                      const typename V2<R>::T* tw, const typename V2<R>::T* tws) {
   using namespace fast;
   using C = typename V2<R>::T;
-  constexpr int T = P2<M>::R2, NF = NF_<KIND>::v;
+  constexpr int NF = NF_<KIND>::v;
   (void)ft;
+  if constexpr (IsGP<M>::v) {
+    constexpr int RPB = fast::inv_rpb(NF, GPD<M>::T);
+    const long long nrows = (long long)sp.d.n2 * sp.d.n3;
+    const int ntile = (int)((nrows + RPB - 1) / RPB);
+    const size_t lds = (size_t)RPB * NF * GPD<M>::LINE * sizeof(C);
+    const dim3 grid(ntile, nb), block(RPB * NF * GPD<M>::T);
+    if (fwd) {
+      allow_lds<g_fx_fwd<M, KIND, INMODE, R>>(lds);
+      hipLaunchKernelGGL((g_fx_fwd<M, KIND, INMODE, R>), grid, block, lds, st, dd, sp.d, tw, tws);
+    } else {
+      allow_lds<g_fx_inv<M, KIND, INMODE, OUTMODE, R>>(lds);
+      hipLaunchKernelGGL((g_fx_inv<M, KIND, INMODE, OUTMODE, R>), grid, block, lds, st, dd, sp.d, tw, tws);
+    }
+    return;
+  } else {
+  constexpr int T = P2<M>::R2;
   if (fwd) {
     constexpr int RPB = fast::inv_rpb(NF, T);
     const long long nrows = (long long)sp.d.n2 * sp.d.n3;
@@ -720,6 +1137,7 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
     hipLaunchKernelGGL((k_fx_inv<M, KIND, INMODE, OUTMODE, R>), dim3(ntile, nb), dim3(RPB * NF * T), lds, st, dd,
                        sp.d, tw, tws);
   }
+  }
 }
 
 template <typename R, int N, int S>
@@ -729,9 +1147,15 @@ static void launch_y(const ShapePlan& sp, int kb, int nf, const ReqDesc* dd, int
   using C = typename V2<R>::T;
   const long long ncol = (long long)sp.d.K1 * sp.d.n3;
   const int nblk = (int)((ncol + kb - 1) / kb);
-  size_t lds = (size_t)kb * col_stride<N>() * sizeof(R);
-  allow_lds<k_fy<N, S, R>>(lds);
-  hipLaunchKernelGGL((k_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb);
+  if constexpr (IsGP<N>::v) {
+    const size_t lds = (size_t)kb * GPD<N>::LINE * sizeof(C);
+    allow_lds<g_fy<N, S, R>>(lds);
+    hipLaunchKernelGGL((g_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * GPD<N>::T), lds, st, dd, sp.d, tw, kb);
+  } else {
+    const size_t lds = (size_t)kb * col_stride<N>() * sizeof(R);
+    allow_lds<k_fy<N, S, R>>(lds);
+    hipLaunchKernelGGL((k_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb);
+  }
 }
 
 template <typename R, int N, int KIND>
@@ -741,10 +1165,17 @@ static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hip
   using C = typename V2<R>::T;
   const long long ncol = (long long)sp.d.K1 * sp.d.n2;
   const int nblk = (int)((ncol + kb - 1) / kb);
-  size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
-  allow_lds<k_fz_gop<N, KIND, R>>(lds);
-  hipLaunchKernelGGL((k_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d,
-                     tw, kb);
+  if constexpr (IsGP<N>::v) {
+    const size_t lds = (size_t)kb * GPD<N>::LINE * sizeof(C);
+    allow_lds<g_fz_gop<N, KIND, R>>(lds);
+    hipLaunchKernelGGL((g_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * GPD<N>::T), lds, st, dd,
+                       sp.d, tw, kb);
+  } else {
+    const size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
+    allow_lds<k_fz_gop<N, KIND, R>>(lds);
+    hipLaunchKernelGGL((k_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * P2<N>::R2), lds, st, dd,
+                       sp.d, tw, kb);
+  }
 }
 
 template <typename R, int M, int N2, int KIND, int INMODE>
@@ -787,9 +1218,11 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
       case 32: launch_y<R, 32, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 40: launch_y<R, 40, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 48: launch_y<R, 48, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 60: launch_y<R, 60, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 64: launch_y<R, 64, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 80: launch_y<R, 80, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 96: launch_y<R, 96, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 120: launch_y<R, 120, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 128: launch_y<R, 128, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       case 192: launch_y<R, 192, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
       default: launch_y<R, 256, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
@@ -804,9 +1237,11 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
     case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 40: launch_z<R, 40, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 48: launch_z<R, 48, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 60: launch_z<R, 60, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 64: launch_z<R, 64, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 80: launch_z<R, 80, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 96: launch_z<R, 96, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 120: launch_z<R, 120, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 128: launch_z<R, 128, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     case 192: launch_z<R, 192, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     default: launch_z<R, 256, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
@@ -832,7 +1267,9 @@ static int fast_eval_t(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream
     case 32: fast_chain<R, KIND, INMODE, OUTMODE, 32>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 40: fast_chain<R, KIND, INMODE, OUTMODE, 40>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 48: fast_chain<R, KIND, INMODE, OUTMODE, 48>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    case 60: fast_chain<R, KIND, INMODE, OUTMODE, 60>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 96: fast_chain<R, KIND, INMODE, OUTMODE, 96>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
+    case 120: fast_chain<R, KIND, INMODE, OUTMODE, 120>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 64: fast_chain<R, KIND, INMODE, OUTMODE, 64>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     case 128: fast_chain<R, KIND, INMODE, OUTMODE, 128>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;
     default: fast_chain<R, KIND, INMODE, OUTMODE, 256>(sp, ft, dd, nb, st, twx, tws, twy, twz); break;

@@ -305,6 +305,7 @@ template <int A, int B, int S, typename C> struct BflyAB {
 template <int S, typename C> struct Bfly<12, S, C> { __device__ static void run(C* x) { BflyAB<4, 3, S, C>::run(x); } };
 template <int S, typename C> struct Bfly<20, S, C> { __device__ static void run(C* x) { BflyAB<4, 5, S, C>::run(x); } };
 template <int S, typename C> struct Bfly<24, S, C> { __device__ static void run(C* x) { BflyAB<8, 3, S, C>::run(x); } };
+template <int S, typename C> struct Bfly<15, S, C> { __device__ static void run(C* x) { BflyAB<5, 3, S, C>::run(x); } };
 
 // ------------------------------------------------------------------ stages
 // One Stockham stage of radix RAD over `nlines` lines of length n.

@@ -61,12 +61,13 @@ def _case(grid, lattice, kinetic=4, alpha=-100.0, beta=-100.0, seed=0, nion=3):
 
 
 # generic LDS path (mixed radices, odd lengths) and register-resident fast path
-# (n1/2 in {16, 24, 32, 40, 48, 64, 96, 128, 256}, n2 and n3 in {24, 32, 40,
-# 48, 64, 80, 96, 128, 192, 256}; the mixed lengths use radix-12/20/24 first
-# stages)
+# (n1/2 in {16, 24, 32, 40, 48, 60, 64, 96, 120, 128, 256}, n2 and n3 in {24,
+# 32, 40, 48, 60, 64, 80, 96, 120, 128, 192, 256}; the mixed lengths use
+# radix-12/15/20/24 stages)
 GRIDS = [(16, 16, 16), (24, 20, 18), (14, 21, 10), (32, 30, 28), (6, 5, 4),
          (64, 32, 64), (128, 64, 32), (64, 128, 256), (512, 32, 32), (32, 64, 128),
          (48, 48, 48), (96, 80, 40), (80, 192, 24), (192, 48, 96),
+         (120, 120, 120), (120, 48, 120), (240, 24, 60),   # 60 = 4 x 15, 120 = 8 x 15 (uneven lanes)
          (128, 128, 32), (64, 64, 32), (32, 32, 32)]   # the last three take the fused x+y plane kernel
 
 
@@ -217,7 +218,7 @@ def test_plane_kernel_matches_separate_passes(pa, monkeypatch):
     _check_terms(e0, e1, rtol=1e-13)
 
 
-@pytest.mark.parametrize("grid", [(96, 96, 96), (80, 80, 80)])
+@pytest.mark.parametrize("grid", [(96, 96, 96), (80, 80, 80), (120, 120, 120)])
 def test_fast_mixed_lengths_match_generic_path(pa, monkeypatch, grid):
     """The two-stage register FFT on mixed lengths (radix-24 x 4, 20 x 4, ...)
     and the generic LDS Stockham passes agree to rounding, in FP64; in the
