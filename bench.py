@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--serve", type=int, default=0, metavar="NREQ",
+                    help="only the mixed-grid serving workload with NREQ requests (pool alone, no plain run)")
     ap.add_argument("--fp32", action="store_true", help="FP32 FFT passes (mixed-precision form), reported apart")
     ap.add_argument("--pmc-summary", nargs="+", default=None)
     args = ap.parse_args()
@@ -305,6 +307,15 @@ def main():
         probe(pa, hip, n, args.steps)
         return
     case = make_case(n)
+    if args.serve:
+        from profess_amd import serve
+        reqs = serve.make_requests(args.serve, seed=4, grids=(32, 48, 64, 80, 96, 120, 128), max_atoms=500,
+                                   rate=1000.0)
+        out = serve.run_workload(reqs, mixed=True)
+        out["evals_mean"] = sum(q.stats["num_evals"] for q in reqs) / len(reqs)
+        out["fp32_evals_mean"] = sum(q.stats["fp32_evals"] for q in reqs) / len(reqs)
+        print(json.dumps(out), flush=True)
+        return
     if args.fp32:
         _, ms32, et32, _, _ = time_single(pa, hip, case, args.steps, args.warmup, precision=1)
         print(json.dumps({"fp32_passes": True, "grid": n, "us_per_eval": ms32 * 1e3, "evals_per_s": 1000.0 / ms32,

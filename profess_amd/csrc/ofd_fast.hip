@@ -61,6 +61,9 @@ bool fast_plane_available(const ShapePlan& sp, int kind, int fp32) {
   if (sp.d.M == 64 && sp.d.n2 == 128) return fits(64, 128);
   if (sp.d.M == 32 && sp.d.n2 == 64) return fits(32, 64);
   if (sp.d.M == 16 && sp.d.n2 == 32) return fits(16, 32);
+  if (sp.d.M == 48 && sp.d.n2 == 96) return fits(48, 96);
+  if (sp.d.M == 40 && sp.d.n2 == 80) return fits(40, 80);
+  if (sp.d.M == 24 && sp.d.n2 == 48) return fits(24, 48);
   return false;
 }
 

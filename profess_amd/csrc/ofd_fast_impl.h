@@ -1203,6 +1203,9 @@ static bool try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_
   if (sp.d.M == 64 && sp.d.n2 == 128) return launch_plane<R, 64, 128, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   if (sp.d.M == 32 && sp.d.n2 == 64) return launch_plane<R, 32, 64, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   if (sp.d.M == 16 && sp.d.n2 == 32) return launch_plane<R, 16, 32, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  if (sp.d.M == 48 && sp.d.n2 == 96) return launch_plane<R, 48, 96, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  if (sp.d.M == 40 && sp.d.n2 == 80) return launch_plane<R, 40, 80, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  if (sp.d.M == 24 && sp.d.n2 == 48) return launch_plane<R, 24, 48, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   return false;
 }
 
