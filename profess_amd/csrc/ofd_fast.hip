@@ -6,13 +6,19 @@
 
 namespace ofd {
 
+// instantiated in their own translation units (not again here)
+extern template int fast_eval_k<double, kWT3>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+extern template int fast_eval_k<double, kWT4>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+extern template int fast_eval_k<float, kWT3>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+extern template int fast_eval_k<float, kWT4>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+
 namespace fast {
 // One workgroup per (slot, request): fin[slot] = sum over the workgroups'
 // partials, fixed order.
 __global__ void __launch_bounds__(256) k_fin(const ReqDesc* __restrict__ descs, int nwg) {
   const ReqDesc& D = descs[blockIdx.y];
   const int k = blockIdx.x;
-  const double* src = D.sums + (size_t)k * nwg;
+  gptr<const double> src = glb(D.sums) + (size_t)k * nwg;
   double acc[1] = {0.0};
   // eight independent loads in flight per thread, summed in a fixed order
   for (int base = 0; base < nwg; base += 8 * 256) {
@@ -24,7 +30,7 @@ __global__ void __launch_bounds__(256) k_fin(const ReqDesc* __restrict__ descs, 
     }
     acc[0] += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
   }
-  block_sum_strided<1>(acc, D.fin + k, 1);
+  block_sum_strided<1>(acc, glb(D.fin) + k, 1);
 }
 
 }  // namespace fast

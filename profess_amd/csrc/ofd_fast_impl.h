@@ -158,20 +158,10 @@ __device__ __forceinline__ void fields(const ReqDesc& D, double rho, double* f) 
   }
 }
 
-template <int INMODE>
-__device__ __forceinline__ double load_rho(const ReqDesc& D, size_t g, double& tphi) {
-  if (INMODE == kInRho) {
-    tphi = 0.0;
-    return D.u[g];
-  }
-  tphi = D.cb != 0.0 ? fma(D.cb, D.v[g], D.ca * D.u[g]) : D.ca * D.u[g];
-  return tphi * tphi;
-}
-
 // Fixed-order workgroup sum: wave butterflies, then the waves in order.
 // out[k * stride] = sum over the workgroup of acc[k].
 template <int N>
-__device__ __forceinline__ void block_sum_strided(double (&acc)[N], double* out, int stride) {
+__device__ __forceinline__ void block_sum_strided(double (&acc)[N], gptr<double> out, int stride) {
   __shared__ double red[16][N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
@@ -238,7 +228,7 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
       const int k = t + T * m;
-      const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
+      const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + base + 2 * k)));
       double a, b;
       if (INMODE == kInRho) {
         a = uu.x;
@@ -246,7 +236,7 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
       } else {
         double ta = D.ca * uu.x, tb = D.ca * uu.y;
         if (D.cb != 0.0) {
-          const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
+          const double2 vv = ldc(glb(reinterpret_cast<const double2*>(D.v + base + 2 * k)));
           ta = fma(D.cb, vv.x, ta);
           tb = fma(D.cb, vv.y, tb);
         }
@@ -265,7 +255,7 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
   }
   __syncthreads();
   if (active) {
-    C* out = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)row * (M + 1);
+    gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)row * (M + 1);
 #pragma unroll
     for (int m = 0; m <= R1; ++m) {
       const int k = m < R1 ? t + T * m : M;
@@ -278,7 +268,7 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
       C x;
       x.x = er + (w.x * orr - w.y * oi);
       x.y = ei + (w.x * oi + w.y * orr);
-      out[k] = x;
+      stc(out + k, x);
     }
   }
 }
@@ -322,7 +312,7 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
 #pragma unroll
     for (int m = 0; m < R1X; ++m) {
       const int k = t_plus(tx, TX, m);
-      const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
+      const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + base + 2 * k)));
       double a, b;
       if (INMODE == kInRho) {
         a = uu.x;
@@ -330,7 +320,7 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
       } else {
         double ta = D.ca * uu.x, tb = D.ca * uu.y;
         if (D.cb != 0.0) {
-          const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
+          const double2 vv = ldc(glb(reinterpret_cast<const double2*>(D.v + base + 2 * k)));
           ta = fma(D.cb, vv.x, ta);
           tb = fma(D.cb, vv.y, tb);
         }
@@ -410,9 +400,9 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
     }
   }
   if (ay) {
-    C* out = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)i3 * N2 * (M + 1) + c;
+    gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)i3 * N2 * (M + 1) + c;
 #pragma unroll
-    for (int m = 0; m < R1Y; ++m) out[(size_t)(ty + TY * m) * (M + 1)] = y[m];
+    for (int m = 0; m < R1Y; ++m) stc(out + (size_t)(ty + TY * m) * (M + 1), y[m]);
   }
 }
 
@@ -432,16 +422,16 @@ __global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, G
   const long long ncol = (long long)d.K1 * d.n3;
   const bool active = col < ncol;
   const long long i3 = active ? col / d.K1 : 0, k1 = active ? col - i3 * d.K1 : 0;
-  C* base = reinterpret_cast<C*>(D.work) + (size_t)blockIdx.y * d.nc + (size_t)i3 * d.n2 * d.K1 + k1;
+  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)blockIdx.y * d.nc + (size_t)i3 * d.n2 * d.K1 + k1;
   C x[R1];
   if (active) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m) x[m] = base[(size_t)(t + T * m) * d.K1];
+    for (int m = 0; m < R1; ++m) x[m] = ldc(base + (size_t)(t + T * m) * d.K1);
   }
   fft2s<N, S>(x, lds + (size_t)c * LL, t, tw, active);
   if (active) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m) base[(size_t)(t + T * m) * d.K1] = x[m];
+    for (int m = 0; m < R1; ++m) stc(base + (size_t)(t + T * m) * d.K1, x[m]);
   }
 }
 
@@ -468,11 +458,11 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
   const size_t es = (size_t)d.n2 * d.K1;
   const double invn = 1.0 / (double)d.nr;
-  C* base = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)col;
+  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)col;
   C x[R1];
   if (active) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m) x[m] = base[(size_t)(t + T * m) * es];
+    for (int m = 0; m < R1; ++m) x[m] = ldc(base + (size_t)(t + T * m) * es);
   }
   fft2<N, -1>(x, lds + (size_t)c * LL, t, tw, active);
   if (active) {
@@ -487,7 +477,7 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
       double sc;
       if (f == 0) sc = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
       else if (f == NF - 1) sc = q2 * invn;
-      else sc = D.wtk[(size_t)i3 * es + col];
+      else sc = glb(D.wtk)[(size_t)i3 * es + col];
       x[m].x = (R)((double)x[m].x * sc);
       x[m].y = (R)((double)x[m].y * sc);
     }
@@ -495,7 +485,7 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   fft2<N, +1>(x, lds + (size_t)c * LL, t, tw, active);
   if (active) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m) base[(size_t)(t + T * m) * es] = x[m];
+    for (int m = 0; m < R1; ++m) stc(base + (size_t)(t + T * m) * es, x[m]);
   }
 }
 
@@ -527,6 +517,106 @@ __device__ __forceinline__ double epi(const ReqDesc& D, double rho, double vion,
   return loc + D.mu * t / (2.0 * p.phi);
 }
 
+// Epilogue of the x-inverse passes over the workgroup's RPB rows, whose NF
+// inverse-transformed fields sit in LDS (line rr*NF + j of stride LL complex;
+// real point i of a row is real component i of its line). Consecutive
+// threads take consecutive points, EPT points per thread, in chunks of PF:
+// the global inputs of a whole chunk are loaded before its first point is
+// computed, so they are in flight together instead of one point at a time,
+// and the first chunk's loads are issued before the workgroup barrier that
+// completes the lines in LDS (this function starts with that barrier).
+template <int M, int KIND, int INMODE, int OUTMODE, int RPB, int NT, int LL, typename R>
+__device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict__ lr, long long row0,
+                                           long long nrows, int blk) {
+  constexpr int NF = NF_<KIND>::v;
+  constexpr int EPT = (RPB * 2 * M + NT - 1) / NT;
+  // global loads per point: u and vion, v (trial points and the optimizer
+  // forms), b and w (Newton form); at most 16 doubles in flight per thread
+  constexpr int NL = 2 + (INMODE == kInPhi || OUTMODE != kOutPot ? 1 : 0) + (OUTMODE == kOutNewton ? 2 : 0);
+  constexpr int PF = EPT < 16 / NL ? EPT : 16 / NL;
+  const int nrow = (int)min((long long)RPB, nrows - row0);
+  const int npts = nrow * 2 * M;
+  const size_t gbase = (size_t)row0 * 2 * M;
+  // the second vector: the phi direction of a trial point (kInPhi), the
+  // search direction of the signed form, the CG direction of the Newton form
+  const bool want_v = D.v != nullptr && (INMODE == kInPhi || OUTMODE != kOutPot);
+  double acc[kNSum];
+#pragma unroll
+  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int c0 = 0; c0 < EPT; c0 += PF) {
+    double pu[PF], pv[PF], pion[PF], pb[PF], pw[PF];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      // points past the workgroup's last read its last point (in bounds, no
+      // branch between the loads) and are not computed
+      const size_t g = gbase + min((int)threadIdx.x + (c0 + q) * NT, npts - 1);
+      pv[q] = pb[q] = pw[q] = 0.0;
+      pu[q] = glb(D.u)[g];
+      if (want_v) pv[q] = glb(D.v)[g];
+      pion[q] = glb(D.vion)[g];
+      if (OUTMODE == kOutNewton) {
+        pb[q] = glb(D.bvec)[g];
+        if (D.w != nullptr) pw[q] = glb(D.w)[g];
+      }
+    }
+    if (c0 == 0) __syncthreads();
+    // the empty asm makes the chunk's loads complete here, together, and
+    // keeps the compiler from sinking each load into its point's block
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      asm volatile("" : "+v"(pu[q]), "+v"(pion[q]));
+      if (want_v) asm volatile("" : "+v"(pv[q]));
+      if (OUTMODE == kOutNewton) asm volatile("" : "+v"(pb[q]), "+v"(pw[q]));
+    }
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int e = threadIdx.x + (c0 + q) * NT;
+      if (c0 + q >= EPT || e >= npts) break;
+      const int rr = e / (2 * M), i = e - rr * (2 * M);
+      const size_t g = gbase + e;
+      double o[NF];
+#pragma unroll
+      for (int j = 0; j < NF; ++j) o[j] = (double)lr[2 * (size_t)(rr * NF + j) * LL + i];
+      double tphi, rho;
+      if (INMODE == kInRho) {
+        tphi = 0.0;
+        rho = pu[q];
+      } else {
+        tphi = D.cb != 0.0 ? fma(D.cb, pv[q], D.ca * pu[q]) : D.ca * pu[q];
+        rho = tphi * tphi;
+      }
+      const double pot = epi<KIND>(D, rho, pion[q], o, acc);
+      if (OUTMODE == kOutPot) {
+        glb(D.out)[g] = pot;
+      } else if (OUTMODE == kOutSigned) {
+        const double de = pot * fsign(tphi);
+        glb(D.out)[g] = de;
+        const double dir = D.s1 * pu[q] + (D.v != nullptr ? D.s2 * pv[q] : 0.0);
+        acc[6] += de * dir;
+        const double dl = de - 2.0 * tphi * D.mu0;
+        acc[7] += dl * dl;
+        acc[8] += de * tphi;
+        acc[9] += de * de;
+        acc[10] += tphi * tphi;
+      } else {
+        const double de = pot * fsign(tphi);
+        const double ap = ((de - D.mu0 * 2.0 * tphi) - pb[q]) / D.eps;
+        glb(D.out)[g] = ap;
+        acc[6] += pv[q] * ap;
+        if (D.w != nullptr) {
+          acc[7] += pw[q] * ap;
+          acc[8] += ap * ap;
+        }
+      }
+    }
+  }
+  // partials laid out [slot][workgroup]; the finalize kernel sums each slot
+  // in a fixed order (no device-scope fence or atomic here: on a multi-XCD
+  // part a release fence per workgroup writes back the whole L2)
+  block_sum_strided<kNSum>(acc, glb(D.sums) + blk, gridDim.x);
+}
+
 // One workgroup = RPB rows x NF fields; thread (line = row*NF + field, lane t).
 // Every line is merged (C2R), transformed and left in LDS (complex k holds the
 // reals 2k, 2k+1), then all threads run the epilogue over the workgroup's
@@ -552,11 +642,11 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
   C* mine = lds + (size_t)line * LL;
   C z[R1];
   if (active) {
-    const C* X = reinterpret_cast<const C*>(D.work) + (size_t)f * d.nc + (size_t)row * (M + 1);
+    gptr<const C> X = glb(reinterpret_cast<const C*>(D.work)) + (size_t)f * d.nc + (size_t)row * (M + 1);
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
       const int k = t + T * m;
-      C xk = X[k], xm = X[M - k];
+      C xk = ldc(X + k), xm = ldc(X + (M - k));
       if (k == 0) { xk.y = (R)0; xm.y = (R)0; }
       const R er = xk.x + xm.x, ei = xk.y - xm.y;
       const R dr = xk.x - xm.x, di = xk.y + xm.y;
@@ -571,50 +661,7 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
 #pragma unroll
     for (int m = 0; m < R1; ++m) mine[t + T * m] = z[m];
   }
-  __syncthreads();
-  double acc[kNSum];
-#pragma unroll
-  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
-  const int nrow = (int)min((long long)RPB, nrows - row0);
-  const int npts = nrow * 2 * M;
-  const size_t gbase = (size_t)row0 * 2 * M;
-  const R* lr = reinterpret_cast<const R*>(lds);
-  for (int e = threadIdx.x; e < npts; e += blockDim.x) {
-    const int rr = e / (2 * M), i = e - rr * (2 * M);
-    const size_t g = gbase + e;
-    double o[NF];
-#pragma unroll
-    for (int j = 0; j < NF; ++j) o[j] = (double)lr[2 * (size_t)(rr * NF + j) * LL + i];
-    double tphi;
-    const double rho = load_rho<INMODE>(D, g, tphi);
-    const double pot = epi<KIND>(D, rho, D.vion[g], o, acc);
-    if (OUTMODE == kOutPot) {
-      D.out[g] = pot;
-    } else if (OUTMODE == kOutSigned) {
-      const double de = pot * fsign(tphi);
-      D.out[g] = de;
-      const double dir = D.s1 * D.u[g] + (D.v != nullptr ? D.s2 * D.v[g] : 0.0);
-      acc[6] += de * dir;
-      const double dl = de - 2.0 * tphi * D.mu0;
-      acc[7] += dl * dl;
-      acc[8] += de * tphi;
-      acc[9] += de * de;
-      acc[10] += tphi * tphi;
-    } else {
-      const double de = pot * fsign(tphi);
-      const double ap = ((de - D.mu0 * 2.0 * tphi) - D.bvec[g]) / D.eps;
-      D.out[g] = ap;
-      acc[6] += D.v[g] * ap;
-      if (D.w != nullptr) {
-        acc[7] += D.w[g] * ap;
-        acc[8] += ap * ap;
-      }
-    }
-  }
-  // partials laid out [slot][workgroup]; the finalize kernel sums each slot
-  // in a fixed order (no device-scope fence or atomic here: on a multi-XCD
-  // part a release fence per workgroup writes back the whole L2)
-  block_sum_strided<kNSum>(acc, D.sums + blk, gridDim.x);
+  x_epilogue<M, KIND, INMODE, OUTMODE, RPB, RPB * NF * T, LL>(D, reinterpret_cast<const R*>(lds), row0, nrows, blk);
 }
 
 // ------------------------------------------------------------ general two-stage lengths
@@ -731,7 +778,7 @@ __global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, G
   const long long ncol = (long long)d.K1 * d.n3;
   const bool active = col < ncol;
   const long long i3 = active ? col / d.K1 : 0, k1c = active ? col - i3 * d.K1 : 0;
-  C* base = reinterpret_cast<C*>(D.work) + (size_t)blockIdx.y * d.nc + (size_t)i3 * d.n2 * d.K1 + k1c;
+  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)blockIdx.y * d.nc + (size_t)i3 * d.n2 * d.K1 + k1c;
   C xc[G::NC], xs[G::NS];
   if (S < 0) {
     if (active) {
@@ -739,7 +786,7 @@ __global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, G
       for (int j = 0; j < G::J1; ++j)
         if (G::col_ok(t, j)) {
 #pragma unroll
-          for (int m = 0; m < G::R1; ++m) xc[j * G::R1 + m] = base[(size_t)(t + G::T * j + G::R2 * m) * d.K1];
+          for (int m = 0; m < G::R1; ++m) xc[j * G::R1 + m] = ldc(base + (size_t)(t + G::T * j + G::R2 * m) * d.K1);
         }
     }
     gfwd<N, S>(xc, xs, lds + (size_t)cl * G::LINE, t, tw, active);
@@ -748,7 +795,7 @@ __global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, G
       for (int q = 0; q < G::J2; ++q)
         if (G::row_ok(t, q)) {
 #pragma unroll
-          for (int k2 = 0; k2 < G::R2; ++k2) base[(size_t)(t + G::T * q + G::R1 * k2) * d.K1] = xs[q * G::R2 + k2];
+          for (int k2 = 0; k2 < G::R2; ++k2) stc(base + (size_t)(t + G::T * q + G::R1 * k2) * d.K1, xs[q * G::R2 + k2]);
         }
     }
   } else {
@@ -757,7 +804,7 @@ __global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, G
       for (int q = 0; q < G::J2; ++q)
         if (G::row_ok(t, q)) {
 #pragma unroll
-          for (int k2 = 0; k2 < G::R2; ++k2) xs[q * G::R2 + k2] = base[(size_t)(t + G::T * q + G::R1 * k2) * d.K1];
+          for (int k2 = 0; k2 < G::R2; ++k2) xs[q * G::R2 + k2] = ldc(base + (size_t)(t + G::T * q + G::R1 * k2) * d.K1);
         }
     }
     ginv<N, S>(xs, xc, lds + (size_t)cl * G::LINE, t, tw, active);
@@ -766,7 +813,7 @@ __global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, G
       for (int j = 0; j < G::J1; ++j)
         if (G::col_ok(t, j)) {
 #pragma unroll
-          for (int m = 0; m < G::R1; ++m) base[(size_t)(t + G::T * j + G::R2 * m) * d.K1] = xc[j * G::R1 + m];
+          for (int m = 0; m < G::R1; ++m) stc(base + (size_t)(t + G::T * j + G::R2 * m) * d.K1, xc[j * G::R1 + m]);
         }
     }
   }
@@ -792,14 +839,14 @@ __global__ void __launch_bounds__(256) g_fz_gop(const ReqDesc* __restrict__ desc
   const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
   const size_t es = (size_t)d.n2 * d.K1;
   const double invn = 1.0 / (double)d.nr;
-  C* base = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)col;
+  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)col;
   C xc[G::NC], xs[G::NS];
   if (active) {
 #pragma unroll
     for (int j = 0; j < G::J1; ++j)
       if (G::col_ok(t, j)) {
 #pragma unroll
-        for (int m = 0; m < G::R1; ++m) xc[j * G::R1 + m] = base[(size_t)(t + G::T * j + G::R2 * m) * es];
+        for (int m = 0; m < G::R1; ++m) xc[j * G::R1 + m] = ldc(base + (size_t)(t + G::T * j + G::R2 * m) * es);
       }
   }
   C* mine = lds + (size_t)cl * G::LINE;
@@ -819,7 +866,7 @@ __global__ void __launch_bounds__(256) g_fz_gop(const ReqDesc* __restrict__ desc
         double sc;
         if (f == 0) sc = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
         else if (f == NF - 1) sc = q2 * invn;
-        else sc = D.wtk[(size_t)i3 * es + col];
+        else sc = glb(D.wtk)[(size_t)i3 * es + col];
         C& v = xs[q * G::R2 + k2];
         v.x = (R)((double)v.x * sc);
         v.y = (R)((double)v.y * sc);
@@ -832,7 +879,7 @@ __global__ void __launch_bounds__(256) g_fz_gop(const ReqDesc* __restrict__ desc
     for (int j = 0; j < G::J1; ++j)
       if (G::col_ok(t, j)) {
 #pragma unroll
-        for (int m = 0; m < G::R1; ++m) base[(size_t)(t + G::T * j + G::R2 * m) * es] = xc[j * G::R1 + m];
+        for (int m = 0; m < G::R1; ++m) stc(base + (size_t)(t + G::T * j + G::R2 * m) * es, xc[j * G::R1 + m]);
       }
   }
 }
@@ -863,7 +910,7 @@ __global__ void __launch_bounds__(256) g_fx_fwd(const ReqDesc* __restrict__ desc
 #pragma unroll
       for (int m = 0; m < G::R1; ++m) {
         const int k = t + T * j + G::R2 * m;
-        const double2 uu = *reinterpret_cast<const double2*>(D.u + base + 2 * k);
+        const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + base + 2 * k)));
         double a, b;
         if (INMODE == kInRho) {
           a = uu.x;
@@ -871,7 +918,7 @@ __global__ void __launch_bounds__(256) g_fx_fwd(const ReqDesc* __restrict__ desc
         } else {
           double ta = D.ca * uu.x, tb = D.ca * uu.y;
           if (D.cb != 0.0) {
-            const double2 vv = *reinterpret_cast<const double2*>(D.v + base + 2 * k);
+            const double2 vv = ldc(glb(reinterpret_cast<const double2*>(D.v + base + 2 * k)));
             ta = fma(D.cb, vv.x, ta);
             tb = fma(D.cb, vv.y, tb);
           }
@@ -895,7 +942,7 @@ __global__ void __launch_bounds__(256) g_fx_fwd(const ReqDesc* __restrict__ desc
   }
   __syncthreads();
   if (active) {
-    C* out = reinterpret_cast<C*>(D.work) + (size_t)f * d.nc + (size_t)row * (M + 1);
+    gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)row * (M + 1);
     for (int k = t; k <= M; k += T) {
       const C zk = mine[k == M ? 0 : k];
       const C zm = mine[k == 0 ? 0 : M - k];
@@ -905,7 +952,7 @@ __global__ void __launch_bounds__(256) g_fx_fwd(const ReqDesc* __restrict__ desc
       C x;
       x.x = er + (w.x * orr - w.y * oi);
       x.y = ei + (w.x * oi + w.y * orr);
-      out[k] = x;
+      stc(out + k, x);
     }
   }
 }
@@ -933,14 +980,14 @@ __global__ void __launch_bounds__(256) g_fx_inv(const ReqDesc* __restrict__ desc
   C* mine = lds + (size_t)line * LL;
   C xc[G::NC], xs[G::NS];
   if (active) {
-    const C* X = reinterpret_cast<const C*>(D.work) + (size_t)f * d.nc + (size_t)row * (M + 1);
+    gptr<const C> X = glb(reinterpret_cast<const C*>(D.work)) + (size_t)f * d.nc + (size_t)row * (M + 1);
 #pragma unroll
     for (int q = 0; q < G::J2; ++q) {
       if (!G::row_ok(t, q)) continue;
 #pragma unroll
       for (int k2 = 0; k2 < G::R2; ++k2) {
         const int k = t + T * q + G::R1 * k2;
-        C xk = X[k], xm = X[M - k];
+        C xk = ldc(X + k), xm = ldc(X + (M - k));
         if (k == 0) { xk.y = (R)0; xm.y = (R)0; }
         const R er = xk.x + xm.x, ei = xk.y - xm.y;
         const R dr = xk.x - xm.x, di = xk.y + xm.y;
@@ -960,47 +1007,7 @@ __global__ void __launch_bounds__(256) g_fx_inv(const ReqDesc* __restrict__ desc
         for (int m = 0; m < G::R1; ++m) mine[t + T * j + G::R2 * m] = xc[j * G::R1 + m];
       }
   }
-  __syncthreads();
-  double acc[kNSum];
-#pragma unroll
-  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
-  const int nrow = (int)min((long long)RPB, nrows - row0);
-  const int npts = nrow * 2 * M;
-  const size_t gbase = (size_t)row0 * 2 * M;
-  const R* lr = reinterpret_cast<const R*>(lds);
-  for (int e = threadIdx.x; e < npts; e += blockDim.x) {
-    const int rr = e / (2 * M), i = e - rr * (2 * M);
-    const size_t g = gbase + e;
-    double o[NF];
-#pragma unroll
-    for (int j = 0; j < NF; ++j) o[j] = (double)lr[2 * (size_t)(rr * NF + j) * LL + i];
-    double tphi;
-    const double rho = load_rho<INMODE>(D, g, tphi);
-    const double pot = epi<KIND>(D, rho, D.vion[g], o, acc);
-    if (OUTMODE == kOutPot) {
-      D.out[g] = pot;
-    } else if (OUTMODE == kOutSigned) {
-      const double de = pot * fsign(tphi);
-      D.out[g] = de;
-      const double dir = D.s1 * D.u[g] + (D.v != nullptr ? D.s2 * D.v[g] : 0.0);
-      acc[6] += de * dir;
-      const double dl = de - 2.0 * tphi * D.mu0;
-      acc[7] += dl * dl;
-      acc[8] += de * tphi;
-      acc[9] += de * de;
-      acc[10] += tphi * tphi;
-    } else {
-      const double de = pot * fsign(tphi);
-      const double ap = ((de - D.mu0 * 2.0 * tphi) - D.bvec[g]) / D.eps;
-      D.out[g] = ap;
-      acc[6] += D.v[g] * ap;
-      if (D.w != nullptr) {
-        acc[7] += D.w[g] * ap;
-        acc[8] += ap * ap;
-      }
-    }
-  }
-  block_sum_strided<kNSum>(acc, D.sums + blk, gridDim.x);
+  x_epilogue<M, KIND, INMODE, OUTMODE, RPB, RPB * NF * T, LL>(D, reinterpret_cast<const R*>(lds), row0, nrows, blk);
 }
 
 }  // namespace fast

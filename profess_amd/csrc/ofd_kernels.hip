@@ -597,33 +597,50 @@ __global__ void __launch_bounds__(256) k_dots_fin(const double* partial, int nbl
   block_sum_store<4>(acc, out);
 }
 
+// One block row per operation (blockIdx.y); the kind is uniform per block row,
+// so each loop is a plain streaming loop with global loads and stores.
 __global__ void __launch_bounds__(256) k_vecops(const VecOp* __restrict__ ops, long long n) {
-  const VecOp o = ops[blockIdx.y];
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    if (o.kind == kOpLincomb) {
-      o.z[i] = fma(o.b, o.y[i], o.a * o.x[i]);
-    } else if (o.kind == kOpAxpby) {
-      o.z[i] = fma(o.a, o.x[i], o.b * o.z[i]);
-    } else if (o.kind == kOpCG) {
-      const double p = o.x2[i];
-      o.z[i] = fma(o.a, p, 1.0 * o.z[i]);
-      const double r = fma(-o.a, o.w[i], 1.0 * o.y2[i]);
-      o.y2[i] = r;
-      o.x2[i] = fma(1.0, r, o.b * p);
-    } else {
-      o.z[i] = 0.0;
+  const VecOp& o = ops[blockIdx.y];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const gptr<double> z = glb(o.z);
+  if (o.kind == kOpLincomb) {
+    const gptr<const double> x = glb(o.x), y = glb(o.y);
+    const double a = o.a, b = o.b;
+#pragma unroll 4
+    for (; i < n; i += stride) z[i] = fma(b, y[i], a * x[i]);
+  } else if (o.kind == kOpAxpby) {
+    const gptr<const double> x = glb(o.x);
+    const double a = o.a, b = o.b;
+#pragma unroll 4
+    for (; i < n; i += stride) z[i] = fma(a, x[i], b * z[i]);
+  } else if (o.kind == kOpCG) {
+    const gptr<double> x2 = glb(o.x2), y2 = glb(o.y2);
+    const gptr<const double> w = glb(o.w);
+    const double a = o.a, b = o.b;
+#pragma unroll 4
+    for (; i < n; i += stride) {
+      const double p = x2[i];
+      z[i] = fma(a, p, 1.0 * z[i]);
+      const double r = fma(-a, w[i], 1.0 * y2[i]);
+      y2[i] = r;
+      x2[i] = fma(1.0, r, b * p);
     }
+  } else {
+#pragma unroll 4
+    for (; i < n; i += stride) z[i] = 0.0;
   }
 }
 __global__ void __launch_bounds__(256) k_dotsb(const DotDesc* __restrict__ dd, long long n, double* part) {
-  const DotDesc q = dd[blockIdx.y];
+  const DotDesc& q = dd[blockIdx.y];
+  const long long stride = (long long)gridDim.x * blockDim.x;
   double acc[1] = {0.0};
+  const gptr<const double> a = glb(q.a);
   if (q.b != nullptr) {
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-      acc[0] += q.a[i] * q.b[i];
+    const gptr<const double> b = glb(q.b);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) acc[0] += a[i] * b[i];
   } else {
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-      acc[0] += q.a[i];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) acc[0] += a[i];
   }
   block_sum_store<1>(acc, part + (size_t)blockIdx.y * kDotBlocksBatched + blockIdx.x);
 }

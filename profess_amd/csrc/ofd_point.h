@@ -12,6 +12,33 @@
 
 namespace ofd {
 
+// Global-memory pointers. The buffers of a request reach the kernels through
+// the descriptor array, so the compiler cannot tell their address space and
+// emits flat loads and stores, which also count against the LDS wait counter
+// (every wait for an LDS exchange then waits for the outstanding HBM loads
+// too). Accesses through gptr compile to global_load / global_store.
+template <class T> using gptr = T __attribute__((address_space(1)))*;
+template <class T> __device__ __forceinline__ gptr<T> glb(T* p) { return (gptr<T>)p; }
+// Complex (double2 / float2) elements through a gptr, member by member: the
+// HIP vector types cannot be copy-constructed from an address-space-1
+// reference, and the two members still merge into one 8- or 16-byte access.
+template <class C> __device__ __forceinline__ C ldc(gptr<const C> p) {
+  C r;
+  r.x = p->x;
+  r.y = p->y;
+  return r;
+}
+template <class C> __device__ __forceinline__ C ldc(gptr<C> p) {
+  C r;
+  r.x = p->x;
+  r.y = p->y;
+  return r;
+}
+template <class C> __device__ __forceinline__ void stc(gptr<C> p, const C& v) {
+  p->x = v.x;
+  p->y = v.y;
+}
+
 struct PtVals {
   double phi;          // rho^(1/2)
   double r13;          // rho^(1/3)
