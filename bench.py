@@ -258,7 +258,7 @@ def read_traffic():
     return None
 
 
-def serving_sample(pa, nreq=128, seed=4):
+def serving_sample(pa, nreq=512, seed=4):
     """Batched serving on the mixed-grid workload (fcc Al / hcp Mg cells of
     4-500 atoms on 32^3 ... 128^3 grids, Poisson arrivals at 1000/s, each a
     full NTN solve to a potential norm of 1e-5): the FP32-then-FP64 pool and,

@@ -153,6 +153,19 @@ int ofd_rho_opt(ofd_ctx* ctx, double* rhoR, int dir_type, const ofd_opt_params* 
  * given to ofd_build_ion_potential. */
 int ofd_ion_forces(ofd_ctx* ctx, const double* rho, double* forces);
 
+/* Ion-ion Ewald energy (Hartree) and, when forces is not NULL, forces
+ * (Ha/bohr, 3 per ion) of the ions given to ofd_build_ion_potential, with the
+ * species' valence charges from their .recpot data: EwaldSetup (Ewald.f90:
+ * 208-460; the particle-mesh grid is the density grid), EwaldEnergy
+ * (:465-596) with the spline reciprocal sum EwaldRecipSpaceSpline
+ * (:993-1147), EwaldForces (:1196-1310) with EwaldRecipSpaceForcesSpline
+ * (:1649-1851), at the spline order given to ofd_build_ion_potential. The
+ * energy becomes the context's ion-ion energy, eTable(6) of every later
+ * evaluation, as RefreshIonTerms sets ionIonEnergy (RefreshIons.f90:196).
+ * info5 (may be NULL): eta, real-space cutoff (bohr), reciprocal cutoff
+ * (1/bohr), and the two error bounds (Hartree). */
+int ofd_ewald(ofd_ctx* ctx, double* energy, double* forces, double* info5);
+
 /* ---- batched serving (no reference counterpart: many density
  *      optimizations of one grid shape advanced together) ---- */
 typedef struct ofd_pool ofd_pool;

@@ -659,35 +659,44 @@ def ion_forces(cell: Cell, species, rho, order=10):
     Returns forces (nion_total, 3) in Ha/bohr, species order."""
     _, qt, _ = g_tables(cell)
     rr = fft_forward(rho)
-    cell_inv = np.linalg.inv(cell.lattice).T
     b1, b2, b3 = fill_b(order, cell.n1, cell.n2, cell.n3)
     out = []
-    n3, n2, n1 = cell.shape
     for psp, frac in species:
         prec = psp_lookup(psp, qt)
         prec[0, 0, 0] = 0.0
         tmp = rr * prec * b1[None, None, :] * b2[None, :, None] * b3[:, None, None]
         factor = fft_backward(tmp, cell.shape)
-        for s in np.asarray(frac, dtype=np.float64).reshape(-1, 3):
-            g = [s[0] * n1, s[1] * n2, s[2] * n3]
-            fl = [math.floor(v) for v in g]
-            M = [np.array(cardinal_bspline(order, g[d] - fl[d])) for d in range(3)]
-            D = [np.array(cardinal_bspline(order - 1, g[d] - fl[d]) + [0.0]) for d in range(3)]
-            i = np.arange(order)
-            l1 = np.mod(fl[0] - i, n1)
-            l2 = np.mod(fl[1] - i, n2)
-            l3 = np.mod(fl[2] - i, n3)
-            fac = factor[l3[:, None, None], l2[None, :, None], l1[None, None, :]]
-            m1, m2, m3 = M[0][1:order + 1], M[1][1:order + 1], M[2][1:order + 1]
-            d1 = D[0][1:order + 1] - D[0][0:order]
-            d2 = D[1][1:order + 1] - D[1][0:order]
-            d3 = D[2][1:order + 1] - D[2][0:order]
-            q1 = (m2[None, :, None] * m3[:, None, None] * float(n1) * d1[None, None, :])
-            q2 = (m1[None, None, :] * m3[:, None, None] * float(n2) * d2[None, :, None])
-            q3 = (m1[None, None, :] * m2[None, :, None] * float(n3) * d3[:, None, None])
-            qd = np.array([np.sum(q1 * fac), np.sum(q2 * fac), np.sum(q3 * fac)])
-            out.append(-(cell_inv @ qd))
+        out.extend(spline_force_gather(cell, frac, factor, order))
     return np.array(out)
+
+
+def spline_force_gather(cell: Cell, frac, factor, order=10):
+    """CalculateSplineForces (IonElectronSpline.f90:642-753): per ion,
+    -cellInv . sum over the order^3 stencil of the B-spline derivative
+    weights times factor (a real grid). Returns (nion, 3)."""
+    cell_inv = np.linalg.inv(cell.lattice).T
+    n3, n2, n1 = cell.shape
+    out = []
+    for s in np.asarray(frac, dtype=np.float64).reshape(-1, 3):
+        g = [s[0] * n1, s[1] * n2, s[2] * n3]
+        fl = [math.floor(v) for v in g]
+        M = [np.array(cardinal_bspline(order, g[d] - fl[d])) for d in range(3)]
+        D = [np.array(cardinal_bspline(order - 1, g[d] - fl[d]) + [0.0]) for d in range(3)]
+        i = np.arange(order)
+        l1 = np.mod(fl[0] - i, n1)
+        l2 = np.mod(fl[1] - i, n2)
+        l3 = np.mod(fl[2] - i, n3)
+        fac = factor[l3[:, None, None], l2[None, :, None], l1[None, None, :]]
+        m1, m2, m3 = M[0][1:order + 1], M[1][1:order + 1], M[2][1:order + 1]
+        d1 = D[0][1:order + 1] - D[0][0:order]
+        d2 = D[1][1:order + 1] - D[1][0:order]
+        d3 = D[2][1:order + 1] - D[2][0:order]
+        q1 = (m2[None, :, None] * m3[:, None, None] * float(n1) * d1[None, None, :])
+        q2 = (m1[None, None, :] * m3[:, None, None] * float(n2) * d2[None, :, None])
+        q3 = (m1[None, None, :] * m2[None, :, None] * float(n3) * d3[:, None, None])
+        qd = np.array([np.sum(q1 * fac), np.sum(q2 * fac), np.sum(q3 * fac)])
+        out.append(-(cell_inv @ qd))
+    return out
 
 
 def ion_energy_exact_recip(cell: Cell, species):
@@ -706,6 +715,200 @@ def ion_energy_exact_recip(cell: Cell, species):
             sf += ph
         acc += psp_lookup(psp, qt) * sf
     return acc / cell.vol
+
+
+# --------------------------------------------------------------------------
+# Ion-ion Ewald sum (Ewald.f90)
+# --------------------------------------------------------------------------
+EWALD_ERROR_TOL = 1.0e-10 / HARTREE_TO_EV   # errorTolerance, Ewald.f90:112
+EWALD_ETA_STEP = 0.01 * BOHR                # etaIncrement, Ewald.f90:121
+EWALD_RCUT_STEP = 0.01 * BOHR               # realCutoffIncrement, Ewald.f90:128
+
+
+def _anint(x):
+    """Fortran ANINT: round half away from zero."""
+    return np.where(x >= 0.0, np.floor(x + 0.5), -np.floor(-x + 0.5))
+
+
+@dataclass
+class EwaldParams:
+    eta: float
+    real_cutoff: float
+    recip_cutoff: float
+    real_error: float
+    recip_error: float
+
+
+def ewald_setup(cell: Cell, charges):
+    """EwaldSetup (Ewald.f90:208-460) with the spline (particle-mesh) grid equal
+    to the density grid: recipCutoff from the grid (:409-411), eta lowered from
+    1.2 until the reciprocal error bound is met (:419-427), realCutoff raised
+    from 6 until the real-space bound is met (:431-439)."""
+    q = np.asarray(charges, dtype=np.float64)
+    nion = q.size
+    avg_q2 = float(np.sum(q * q)) / nion
+    a = cell.lattice
+    norms = [math.sqrt(float(a[0, j] ** 2 + a[1, j] ** 2 + a[2, j] ** 2)) for j in range(3)]
+    kc = min((cell.n1 - 1) / norms[0], (cell.n2 - 1) / norms[1], (cell.n3 - 1) / norms[2]) * PI
+    sqrt_pi = math.sqrt(PI)
+    eta = 1.2
+    while True:
+        recip_err = float(nion) ** 2 * avg_q2 / sqrt_pi * eta * math.erfc(kc / (2.0 * eta))
+        if recip_err < EWALD_ERROR_TOL:
+            break
+        eta -= EWALD_ETA_STEP
+    rc = 6.0
+    while True:
+        real_err = PI * float(nion) ** 2 * avg_q2 / (cell.vol * eta ** 2) * math.erfc(rc * eta)
+        if real_err < EWALD_ERROR_TOL:
+            break
+        rc += EWALD_RCUT_STEP
+    return EwaldParams(eta, rc, kc, real_err, recip_err)
+
+
+def _ewald_images(cell: Cell, fr, rc):
+    """The shell loop of EwaldRealSpace (Ewald.f90:681-760): image offsets
+    (i_a, i_b, i_c) of one wrapped fractional separation fr, shell nr = 1, 2,
+    ... until a shell adds no image within the cutoff. Yields (r (3,) frac,
+    dist^2) of every image with 1e-40 < dist^2 <= rc^2."""
+    rmet = cell.lattice.T @ cell.lattice   # Metric3d (MathFunctions.f90:745-749)
+    rc2 = rc * rc
+    nr = 1
+    while True:
+        newr = False
+        rng = np.arange(-nr, nr + 1)
+        A, B, Cc = np.meshgrid(rng, rng, rng, indexing="ij")
+        A, B, Cc = A.ravel(), B.ravel(), Cc.ravel()
+        if nr > 1:
+            on = (np.abs(A) == nr) | (np.abs(B) == nr) | (np.abs(Cc) == nr)
+            A, B, Cc = A[on], B[on], Cc[on]
+        r = np.stack([A + fr[0], B + fr[1], Cc + fr[2]], axis=1)
+        d2 = (rmet[0, 0] * r[:, 0] ** 2 + rmet[1, 1] * r[:, 1] ** 2 + rmet[2, 2] * r[:, 2] ** 2
+              + 2.0 * (rmet[1, 0] * r[:, 1] * r[:, 0] + rmet[2, 1] * r[:, 2] * r[:, 1] + rmet[2, 0] * r[:, 2] * r[:, 0]))
+        keep = (d2 <= rc2) & (d2 > 1e-40)
+        if np.any(keep):
+            newr = True
+            yield r[keep], d2[keep]
+        if not newr:
+            return
+        nr += 1
+
+
+def ewald_real_space(cell: Cell, charges, frac, eta, rc):
+    """EwaldRealSpace (Ewald.f90:599-775): 1/2 sum_ij q_i q_j erfc(eta d)/d."""
+    q = np.asarray(charges, dtype=np.float64)
+    fr_all = np.asarray(frac, dtype=np.float64).reshape(-1, 3)
+    e = 0.0
+    for i in range(q.size):
+        for j in range(q.size):
+            fr = fr_all[i] - fr_all[j]
+            fr = fr - _anint(fr)
+            for _, d2 in _ewald_images(cell, fr, rc):
+                d = np.sqrt(d2)
+                e += q[i] * q[j] * float(np.sum(np.array([math.erfc(eta * x) for x in d]) / d))
+    return e / 2.0
+
+
+def ewald_recip_spline(cell: Cell, charges, frac, eta, kc, order=10):
+    """EwaldRecipSpaceSpline (Ewald.f90:993-1147): charge-weighted B-spline
+    table, R2C (1/N), b-spline moduli, |.|^2, half-space sum (x = 0 plane
+    once, every other plane twice) of exp(-G^2/4eta^2)/G^2 for 0 < G^2 <=
+    kc^2, times 2 pi / vol * N^2."""
+    q = np.asarray(charges, dtype=np.float64)
+    fr = np.asarray(frac, dtype=np.float64).reshape(-1, 3)
+    table = np.zeros(cell.shape)
+    for z in np.unique(q):
+        table += fill_qion_table(fr[q == z], cell.shape, order) * z
+    rec = fft_forward(table)
+    b1, b2, b3 = fill_b(order, cell.n1, cell.n2, cell.n3)
+    cc = rec * b1[None, None, :] * b2[None, :, None] * b3[:, None, None]
+    s2 = (cc * np.conj(cc)).real
+    _, qt, _ = g_tables(cell)
+    m2 = qt * qt
+    ok = (m2 <= kc * kc) & (m2 > 0.0)
+    w = np.full(qt.shape, 2.0)
+    w[:, :, 0] = 1.0
+    term = np.zeros(qt.shape)
+    term[ok] = w[ok] * np.exp(-m2[ok] / eta ** 2 / 4.0) / m2[ok] * s2[ok]
+    return float(np.sum(term)) * 2.0 * PI / cell.vol * float(cell.nr) ** 2
+
+
+def ewald_recip_exact(cell: Cell, charges, frac, eta, kc):
+    """EwaldRecipSpace (Ewald.f90:815-990): the exact structure factor over
+    the half space of lattice vectors with |G| <= kc, times 4 pi / vol."""
+    q = np.asarray(charges, dtype=np.float64)
+    fr = np.asarray(frac, dtype=np.float64).reshape(-1, 3)
+    a = cell.lattice
+    nmax = [int(math.ceil(kc * math.sqrt(float(a[0, j] ** 2 + a[1, j] ** 2 + a[2, j] ** 2)) / PI / 2)) for j in range(3)]
+    e = 0.0
+    for ia in range(0, nmax[0] + 1):
+        for ib in range(0 if ia == 0 else -nmax[1], nmax[1] + 1):
+            for ic in range(1 if (ia == 0 and ib == 0) else -nmax[2], nmax[2] + 1):
+                m = ia * cell.recip[:, 0] + ib * cell.recip[:, 1] + ic * cell.recip[:, 2]
+                if math.sqrt(float(m @ m)) > kc:
+                    continue
+                ph = 2.0 * PI * (fr @ np.array([ia, ib, ic], dtype=np.float64))
+                ss = float(np.sum(q * np.sin(ph)))
+                sc = float(np.sum(q * np.cos(ph)))
+                m2 = float(m @ m)
+                e += math.exp(-(m2 / eta ** 2 / 4.0)) / m2 * (ss * ss + sc * sc)
+    return e * 4.0 * PI / cell.vol
+
+
+def ewald_energy(cell: Cell, charges, frac, order=10, spline=True, params: EwaldParams = None):
+    """EwaldEnergy (Ewald.f90:465-596): real space + self term + reciprocal
+    (particle-mesh spline by default) + average-potential term, in Hartree.
+    Returns (total, [real, self, recip, average], params)."""
+    p = params if params is not None else ewald_setup(cell, charges)
+    q = np.asarray(charges, dtype=np.float64)
+    real = ewald_real_space(cell, q, frac, p.eta, p.real_cutoff)
+    self_term = -p.eta / math.sqrt(PI) * float(np.sum(q * q))              # :778-812
+    recip = (ewald_recip_spline(cell, q, frac, p.eta, p.recip_cutoff, order) if spline
+             else ewald_recip_exact(cell, q, frac, p.eta, p.recip_cutoff))
+    avg = -0.5 * PI / p.eta ** 2 / cell.vol * float(np.sum(q)) ** 2        # :1150-1190
+    comps = [real, self_term, recip, avg]
+    return float(sum(comps)), comps, p
+
+
+def ewald_forces(cell: Cell, charges, frac, order=10, params: EwaldParams = None):
+    """EwaldForces (Ewald.f90:1196-1310) with EwaldRealSpaceForces (:1313-1390)
+    and EwaldRecipSpaceForcesSpline (:1649-1851). Returns (nion, 3) Ha/bohr."""
+    p = params if params is not None else ewald_setup(cell, charges)
+    q = np.asarray(charges, dtype=np.float64)
+    fr_all = np.asarray(frac, dtype=np.float64).reshape(-1, 3)
+    eta = p.eta
+    sqrt_pi = math.sqrt(PI)
+    real = np.zeros((q.size, 3))
+    for i in range(q.size):
+        acc = np.zeros(3)
+        for j in range(q.size):
+            fr = fr_all[i] - fr_all[j]
+            fr = fr - _anint(fr)
+            for r, d2 in _ewald_images(cell, fr, p.real_cutoff):
+                dvec = r @ cell.lattice.T            # cellReal . r
+                ed = eta * np.sqrt(d2)
+                f = np.array([math.erfc(x) for x in ed]) / ed ** 3 + 2.0 / sqrt_pi * np.exp(-(ed ** 2)) / ed ** 2
+                acc += q[j] * np.sum(dvec * f[:, None], axis=0)
+        real[i] = acc * q[i] * eta ** 3
+    # reciprocal part: conj(S) |b|^2 exp(-G^2/4eta^2)/G^2, C2R, spline gather
+    table = np.zeros(cell.shape)
+    for z in np.unique(q):
+        table += fill_qion_table(fr_all[q == z], cell.shape, order) * z
+    rec = fft_forward(table)
+    b1, b2, b3 = fill_b(order, cell.n1, cell.n2, cell.n3)
+    bb = ((b1 * np.conj(b1))[None, None, :] * (b2 * np.conj(b2))[None, :, None] * (b3 * np.conj(b3))[:, None, None])
+    cc = np.conj(rec) * bb
+    _, qt, _ = g_tables(cell)
+    m2 = qt * qt
+    ok = (m2 <= p.recip_cutoff ** 2) & (m2 > 0.0)
+    cc = np.where(ok, cc * np.exp(-np.where(ok, m2, 1.0) / eta ** 2 / 4.0) / np.where(ok, m2, 1.0), 0.0)
+    grid = fft_backward(cc, cell.shape)
+    recip = np.zeros((q.size, 3))
+    for z in np.unique(q):
+        idx = np.nonzero(q == z)[0]
+        recip[idx] = np.array(spline_force_gather(cell, fr_all[idx], grid * z, order))
+    recip *= 4.0 * PI / cell.vol * float(cell.nr)
+    return real + recip
 
 
 # --------------------------------------------------------------------------
@@ -1130,3 +1333,223 @@ def make_evaluators(sysm: System):
     def pot_only(rho):
         return calculate_potential_plus(rho, sysm, True, False)[0]
     return full, pot_only
+
+
+# --------------------------------------------------------------------------
+# Ion relaxation: CG2 with LnSrch (IonOptCG2.f90, MathLNsrch.f90, Neighbor.f90)
+# --------------------------------------------------------------------------
+def _f32(x):
+    """A default-kind (single precision) Fortran literal widened to DP."""
+    return float(np.float32(x))
+
+
+class LnSrch:
+    """LnSrch (MathLNsrch.f90:52-296) with its module state points(2,3) and
+    dsave. ``step(alp, f, g, task, c1, c2, stpmin, stpmax)`` returns
+    (alp, task) as the Fortran leaves them."""
+
+    def __init__(self):
+        self.points = np.full((2, 3), -1.0)
+        self.dsave = np.zeros(13)
+
+    @staticmethod
+    def _new_alp(x1, y1, x2, y2, stpmin, stpmax):
+        slope = (y1 - y2) / (x1 - x2)
+        b = y1 - x1 * slope
+        alp_t = -b / slope
+        if alp_t >= stpmax or alp_t <= stpmin:
+            return -1.0
+        return alp_t
+
+    def step(self, alp, f, g, task, c1, c2, stpmin, stpmax):
+        P = self.points
+        na = lambda: self._new_alp(P[0, 0], P[0, 2], P[1, 0], P[1, 2], stpmin, stpmax)   # noqa: E731
+        if task[:5] == "START":
+            if g > 0.0:
+                return alp, "ERROR: Initial G>0"
+            P[:] = -1.0
+            P[0] = (0.0, f, g)
+            self.dsave[0], self.dsave[1] = f, g
+            P[1, 0] = -1.0
+            return alp, "FG" + task[2:]
+        if task[:2] == "FG" and P[1, 0] > 0.0:
+            if f <= self.dsave[0] + c1 * alp * self.dsave[1] and abs(g) <= c2 * abs(self.dsave[1]):
+                return alp, "CONV"
+        if P[1, 0] < 0.0:
+            if g < P[0, 2]:
+                alp = alp * 5.0 if f < P[0, 1] else (P[0, 0] + alp) / 2.0
+                return alp, "WARNING: Need a better second point."
+            P[1] = (alp, f, g)
+            alp = na()
+            if alp < 0.0 or alp < P[0, 0]:
+                if P[1, 1] < P[0, 1]:
+                    return 5.0 * P[1, 0], "WARNING: Interpolation error, code:-1, energy decreases, step further"
+                return alp, "ERROR: Interpolation error, code:-1"
+            return alp, "FG"
+        if g > 0:
+            if P[1, 2] > 0.0:
+                if P[1, 0] > alp:
+                    P[1] = (alp, f, g)
+                    alp = na()
+                    if alp < P[0, 0]:
+                        return (P[0, 0] + P[1, 0]) / 2.0, "WARNING: Order flips, code:0, energy decreases, step further"
+                    return alp, "FG"
+                if f < P[1, 1]:
+                    return alp * 5.0, "WARNINIG: Order flipped, code:1"
+                return (P[0, 0] + P[1, 0]) / 2.0, "WARNING: Order flipped, code:2"
+            if P[1, 2] < 0.0:
+                P[0] = P[1]
+                P[1] = (alp, f, g)
+                alp = na()
+                if alp < P[0, 0] or alp > P[1, 0]:
+                    if P[0, 1] > P[1, 1]:
+                        return P[1, 0] * 5.0, "WARNING: Interpolation error, code:1, energy decreases, step further"
+                    return (P[0, 0] + P[1, 0]) / 2.0, "WARNING: Interpolation error, code:1"
+                return alp, "FG"
+        if g < 0.0:
+            if P[1, 2] > 0.0:
+                P[0] = (alp, f, g)
+                alp = na()
+                if alp < P[0, 0] or alp > P[1, 0]:
+                    if P[0, 1] > P[1, 1]:
+                        return 5.0 * P[1, 0], "WARNING: Interpolation, code:2, f decreases, step further"
+                    return (P[0, 0] + P[1, 0]) / 2.0, "WARNING: Interpolation, code:4, brackets the minimum by half."
+                return alp, "FG"
+            if P[1, 2] < 0.0:
+                P[0] = P[1]
+                P[1] = (alp, f, g)
+                alp = na()
+                if alp < P[1, 0]:
+                    if P[0, 1] > P[1, 1]:
+                        return P[1, 0] * 5.0, "WARNING: Interpolation error: code:3, f decrease, step further"
+                    return alp, "ERROR: Interpolation error: code:3"
+                return alp, "FG"
+        return alp, task
+
+
+def nn_min_distance(cell: Cell, frac):
+    """NNDist_bin (Neighbor.f90:68-420) for a cell of one bin (every lattice
+    vector shorter than the 20 bohr bin spacing) and, in general, the minimum
+    over all pairs and image shifts in {-1,0,1}^3 of the wrapped positions,
+    which the binned search returns whenever the nearest distance is below the
+    bin spacing."""
+    fr = np.mod(np.asarray(frac, dtype=np.float64).reshape(-1, 3), 1.0)
+    xyz = fr @ cell.lattice.T
+    n = xyz.shape[0]
+    best = 1e6
+    shifts = [(i, j, k) for k in (-1, 0, 1) for j in (-1, 0, 1) for i in (-1, 0, 1)]
+    for a in range(n):
+        for b in range(n):
+            for sh in shifts:
+                if a == b and sh == (0, 0, 0):
+                    continue
+                d = xyz[a] - (xyz[b] + cell.lattice @ np.array(sh, dtype=np.float64))
+                best = min(best, float(math.sqrt(float(d @ d))))
+    return best
+
+
+@dataclass
+class IonRelaxResult:
+    frac: np.ndarray
+    energy: float
+    forces: np.ndarray
+    steps: int
+    energies: list          # total energy at every geometry evaluated, in order
+    max_force: float
+
+
+def ion_relax_cg2(cell: Cell, frac, evaluate, cg_type=2, force_cutoff=_f32(5e-5), max_steps=200, nn_dist=2.0,
+                  max_line=10):
+    """ConjugateGradient (IonOptCG2.f90:60-520). ``evaluate(frac)`` does what
+    RefreshIonTerms + RhoOptimizer + CalculateForces do for the positions
+    (fractional, wrapped) and returns (total energy in Ha, forces (nion, 3)
+    in Ha/bohr); it is called once for the starting geometry and once per
+    line-search trial."""
+    ftol, gtol = _f32(1e-4), _f32(1e-1)
+    frac = np.mod(np.asarray(frac, dtype=np.float64).reshape(-1, 3), 1.0)
+    frac2cart = cell.lattice
+    cart2frac = np.linalg.inv(cell.lattice)
+    energy, forces = evaluate(frac)
+    energies = [energy]
+    nion = frac.shape[0]
+    old_grad = np.zeros((nion, 3))
+    dir_old = np.zeros((nion, 3))
+    alpha = 1.0
+    restart = False
+    step = 0
+    ls = LnSrch()
+    nline = 0
+    max_force = float(np.max(np.sqrt(np.sum(forces ** 2, axis=1))))
+    while True:
+        step += 1
+        if step > max_steps:
+            break
+        max_force = float(np.max(np.sqrt(np.sum(forces ** 2, axis=1))))
+        if max_force < force_cutoff:
+            break
+        old_cart = frac @ frac2cart.T
+        new_grad = forces.copy()
+        if step == 1 or restart or step % 30 == 0:
+            gam = 0.0
+            restart = False
+        elif cg_type == 1:
+            gam = max(0.0, float(np.sum((new_grad - old_grad) * new_grad) / np.sum(old_grad ** 2)))
+        elif cg_type == 2:
+            y = -new_grad + old_grad
+            norm_y = math.sqrt(float(np.sum(y ** 2)))
+            dy = float(np.sum(dir_old * y))
+            t1 = y - 2.0 * dir_old * norm_y ** 2 / dy
+            t2 = -new_grad / dy
+            gam = float(np.sum(t1 * t2))
+            cg_eta = 0.01
+            norm_d = math.sqrt(float(np.sum(dir_old ** 2)))
+            norm_g = math.sqrt(float(np.sum(old_grad ** 2)))
+            gam = max(gam, -1.0 / (norm_d * min(cg_eta, norm_g)))
+        else:
+            y = -new_grad + old_grad
+            gam = max(0.0, float(np.sum(new_grad ** 2) / np.sum(dir_old * y)))
+        d = new_grad + gam * dir_old
+        dir_old = d
+        old_grad = new_grad
+        grad_alp = -float(np.sum(forces * d))
+        if nn_dist > 0:
+            stpmax = abs((nn_min_distance(cell, frac) - nn_dist) / float(np.max(d)))
+        else:
+            stpmax = abs(1.0 / float(np.max(d)))
+        stpmin = 0.0
+        alpha = min(alpha, stpmax)
+        task = "START"
+        nline = 0
+        relax_done = False
+        while True:
+            alpha, task = ls.step(alpha, energy, grad_alp, task, ftol, gtol, stpmin, stpmax)
+            P = ls.points
+            if nline >= 4 and abs(P[0, 1] - P[1, 1]) < _f32(1e-6):
+                break
+            if P[0, 0] > 0.0:
+                bc = (P[0, 0] - P[1, 0]) * d
+                if float(np.max(np.sqrt(np.sum(bc ** 2, axis=1)))) < _f32(1e-5):
+                    break
+            if task[:4] == "CONV":
+                break
+            if task[:5] == "ERROR":
+                restart = True
+                alpha = 1.0
+                break
+            if task[:2] == "FG" or task[:4] == "WARN":
+                frac = np.mod((old_cart + alpha * d) @ cart2frac.T, 1.0)
+                energy, forces = evaluate(frac)
+                energies.append(energy)
+                grad_alp = -float(np.sum(forces * d))
+                max_force = float(np.max(np.sqrt(np.sum(forces ** 2, axis=1))))
+                if max_force < force_cutoff:
+                    relax_done = True
+                    break
+                nline += 1
+                if nline > max_line:
+                    restart = True
+                    alpha = 1.0
+                    break
+        if relax_done:
+            break
+    return IonRelaxResult(frac, energy, forces, step, energies, max_force)

@@ -1,11 +1,15 @@
 ! Test infrastructure only: C-callable driver around the reference's own
 ! self-contained Fortran files, compiled from their sources in the reference
 ! tree (Constants.f90, MathSplines.f90, MathDCsrch.f, MathDCstep.f,
-! MathRKsuite.f) into oracle/_ref/libprofess_ref.so by oracle/ref/Makefile.
+! MathRKsuite.f, MathLNsrch.f90 with the serial OutputFiles.f90 and
+! MPI_Functions.f90 it uses) into oracle/_ref/libprofess_ref.so by
+! oracle/ref/Makefile.
 !
 ! Exports (BIND(C)):
 !   ref_spline_set / ref_spline_val  -> MathSplines spline_cubic_set/_val
 !   ref_dcsrch                       -> MathDCsrch dcsrch (task as an int code)
+!   ref_lnsrch                       -> MathLNsrch LnSrch (task as 70 characters,
+!                                       the module's points(2,3) copied out)
 !   ref_wgc_eta_table                -> restatement of makeKernel2 + the WGC ODE
 !                                       right-hand side (KEDF_KernelODE.f90:254-537)
 !                                       integrated by the reference's RKSUITE
@@ -173,3 +177,27 @@ FUNCTION ref_wgc_eta_table(alpha, beta, gamma, nmax, eta_out, w_out, w1_out, w2_
     END IF
   END DO
 END FUNCTION ref_wgc_eta_table
+
+
+SUBROUTINE ref_lnsrch(alp, f, g, task_c, c1, c2, xtol, stpmin, stpmax, points_out) BIND(C, name="ref_lnsrch")
+  USE ISO_C_BINDING
+  USE LnSrch_MOD, ONLY: LnSrch, points
+  IMPLICIT NONE
+  REAL(C_DOUBLE) :: alp
+  REAL(C_DOUBLE), VALUE :: f, g, c1, c2, xtol, stpmin, stpmax
+  CHARACTER(KIND=C_CHAR) :: task_c(70)
+  REAL(C_DOUBLE) :: points_out(6)
+  CHARACTER(LEN=70) :: task
+  REAL(KIND=8) :: a1, a2
+  INTEGER :: i
+  DO i = 1, 70
+    task(i:i) = task_c(i)
+  END DO
+  a1 = c1
+  a2 = c2
+  CALL LnSrch(alp, f, g, task, a1, a2, xtol, stpmin, stpmax)
+  DO i = 1, 70
+    task_c(i) = task(i:i)
+  END DO
+  points_out(1:6) = RESHAPE(points, [6])
+END SUBROUTINE ref_lnsrch

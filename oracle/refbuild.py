@@ -2,7 +2,9 @@
 
 The library holds the reference's own self-contained Fortran files compiled
 from their sources by oracle/ref/Makefile (MathSplines.f90, MathDCsrch.f,
-MathDCstep.f, MathRKsuite.f, Constants.f90) plus oracle/ref/ref_driver.f90.
+MathDCstep.f, MathRKsuite.f, MathLNsrch.f90, Constants.f90, and the serial
+OutputFiles.f90 / MPI_Functions.f90 that MathLNsrch uses) plus
+oracle/ref/ref_driver.f90.
 ``available()`` is False when it was not built (no /root/reference here).
 """
 from __future__ import annotations
@@ -48,6 +50,8 @@ def lib():
         _lib.ref_wgc_eta_table.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_int, d, d, d, d]
         _lib.ref_wgc_eta_table.restype = ctypes.c_int
+        _lib.ref_lnsrch.argtypes = [d, ctypes.c_double, ctypes.c_double, ctypes.c_char_p, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, d]
     return _lib
 
 
@@ -89,6 +93,22 @@ class RefDcsrch:
                          ctypes.byref(self.code), self.stpmin, self.stpmax, self.isave, self.dsave)
         task = {1: "FG", 2: "CONVERGENCE", 3: "WARNING", 4: "ERROR"}[self.code.value]
         return s.value, task
+
+
+class RefLnSrch:
+    """Same interface as profess_oracle.LnSrch, backed by the Fortran LnSrch
+    (its module state lives in the library: one instance at a time)."""
+
+    def __init__(self):
+        self.points = np.full((2, 3), -1.0)
+
+    def step(self, alp, f, g, task, c1, c2, stpmin, stpmax):
+        a = ctypes.c_double(alp)
+        buf = ctypes.create_string_buffer(task.ljust(70).encode()[:70], 70)
+        pts = np.zeros(6)
+        lib().ref_lnsrch(ctypes.byref(a), f, g, buf, c1, c2, 1e-12, stpmin, stpmax, _p(pts))
+        self.points = pts.reshape(3, 2).T.copy()    # Fortran points(2,3), column-major
+        return a.value, buf.raw.decode().rstrip()
 
 
 def wgc_eta_table(alpha, beta, gamma):

@@ -52,7 +52,7 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
            "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished", "ofd_server_create",
            "ofd_server_add_pool", "ofd_server_wait", "ofd_server_last_error", "ofd_server_destroy",
-           "ofd_get_volume"]
+           "ofd_get_volume", "ofd_ewald"]
 
 
 def lib():
@@ -81,6 +81,7 @@ def lib():
     L.ofd_calpot_device.argtypes = [vp, C.c_void_p, C.c_int, C.c_void_p, dp]
     L.ofd_rho_opt.argtypes = [vp, dp, C.c_int, C.POINTER(ofd_opt_params), dp, ip, C.POINTER(ofd_opt_stats)]
     L.ofd_ion_forces.argtypes = [vp, dp, dp]
+    L.ofd_ewald.argtypes = [vp, dp, dp, dp]
     L.ofd_pool_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L.ofd_pool_destroy.argtypes = [vp]
     L.ofd_pool_destroy.restype = None

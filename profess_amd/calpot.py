@@ -176,6 +176,19 @@ def IonElectronForcesSpline(system: OFDSystem, rho):
     return out
 
 
+def EwaldEnergy(system: OFDSystem, forces=False):
+    """Ion-ion Ewald energy (Hartree) of the system's ions with the particle-mesh
+    reciprocal sum (Ewald.f90:465-596), and with forces=True also the Ewald
+    forces (Ha/bohr, EwaldForces Ewald.f90:1196-1310). The energy becomes the
+    system's ion-ion energy (eTable(6)) for later evaluations. Returns
+    (energy, forces or None, info) with info the Ewald parameters."""
+    e = C.c_double()
+    out = np.zeros((system.nion, 3)) if forces else None
+    info = np.zeros(5)
+    system._check(lib().ofd_ewald(system._h, C.byref(e), dptr(out), dptr(info)))
+    return e.value, out, dict(zip(("eta", "real_cutoff", "recip_cutoff", "real_error", "recip_error"), info))
+
+
 def request_bytes(grid, kinetic=4, precision=0):
     return int(lib().ofd_request_bytes(int(grid[0]), int(grid[1]), int(grid[2]), int(kinetic), int(precision)))
 

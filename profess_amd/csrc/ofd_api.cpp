@@ -1622,6 +1622,85 @@ extern "C" int ofd_ion_forces(ofd_ctx* c, const double* rho, double* forces) {
   return 0;
 }
 
+// ====================================================================== ion-ion Ewald sum
+// EwaldSetup (Ewald.f90:208-460) with the particle-mesh grid equal to the
+// density grid: recipCutoff from the grid dimensions, eta lowered from 1.2
+// in steps of 0.01 A until the reciprocal error bound is below 1e-10 eV, the
+// real cutoff raised from 6 bohr in steps of 0.01 A until the real-space
+// bound is (errorTolerance, etaIncrement, realCutoffIncrement: Ewald.f90:112-128).
+static void ewald_setup(const double* lat, double vol, int n1, int n2, int n3, const std::vector<double>& q,
+                        double* eta_out, double* rc_out, double* kc_out, double* real_err, double* recip_err) {
+  const double bohr = 0.529177208607388;
+  const double qElec = 1.602176462e-19, light = 299792458.0, mu0 = 4.0 * kPi * 1.0e-7;
+  const double eps0 = 1.0 / (mu0 * light * light);
+  const double h2ev = qElec * 1.0e10 / (4.0 * kPi * eps0 * bohr);
+  const double tol = 1.0e-10 / h2ev, step = 0.01 * bohr;
+  const int nion = (int)q.size();
+  double sq = 0.0;
+  for (double v : q) sq += v * v;
+  const double avg = sq / (double)nion;
+  auto norm = [&](int j) { return std::sqrt(lat[3 * j] * lat[3 * j] + lat[3 * j + 1] * lat[3 * j + 1] + lat[3 * j + 2] * lat[3 * j + 2]); };
+  const double kc = std::min(std::min((n1 - 1) / norm(0), (n2 - 1) / norm(1)), (n3 - 1) / norm(2)) * kPi;
+  const double sqrt_pi = std::sqrt(kPi);
+  double eta = 1.2, re = 0.0, ke = 0.0;
+  for (;;) {
+    ke = (double)nion * (double)nion * avg / sqrt_pi * eta * std::erfc(kc / (2.0 * eta));
+    if (ke < tol) break;
+    eta -= step;
+  }
+  double rc = 6.0;
+  for (;;) {
+    re = kPi * (double)nion * (double)nion * avg / (vol * eta * eta) * std::erfc(rc * eta);
+    if (re < tol) break;
+    rc += step;
+  }
+  *eta_out = eta;
+  *rc_out = rc;
+  *kc_out = kc;
+  *real_err = re;
+  *recip_err = ke;
+}
+
+extern "C" int ofd_ewald(ofd_ctx* c, double* energy, double* forces, double* info5) {
+  if (!c || !energy) return fail(c, "null argument");
+  if (c->type_start.empty()) return fail(c, "ions not set (ofd_build_ion_potential)");
+  try {
+    const int ns = (int)c->type_start.size() - 1;
+    const int nion = c->type_start[ns];
+    std::vector<double> charge(ns), q(nion);
+    for (int s = 0; s < ns; ++s) {
+      charge[s] = (double)c->psp_charge[s];
+      for (int k = c->type_start[s]; k < c->type_start[s + 1]; ++k) q[k] = charge[s];
+    }
+    double eta, rc, kc, re, ke;
+    ewald_setup(c->lat, c->vol, c->sp.d.n1, c->sp.d.n2, c->sp.d.n3, q, &eta, &rc, &kc, &re, &ke);
+    double cinv[9], rb[9], vol;
+    cell_setup(c->lat, rb, &vol, cinv);
+    const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
+    void* sc = dalloc<char>(scratch);
+    double comps[4];
+    const int rcode = ewald_device(c->sp, c->rb, c->lat, c->vol, ns, c->type_start.data(), charge.data(), c->d_frac,
+                                   nion, c->spline_order, eta, rc, kc, cinv, comps, forces, sc, c->st);
+    hipStreamSynchronize(c->st);
+    dev_free(sc);
+    if (rcode) return fail(c, "Ewald kernels failed");
+    // EwaldEnergy = SUM(ewaldComponents) (Ewald.f90:553-580), the context's
+    // ionIonEnergy from now on (RefreshIons.f90:196)
+    *energy = comps[0] + comps[1] + comps[2] + comps[3];
+    c->e_ionion = *energy;
+    if (info5) {
+      info5[0] = eta;
+      info5[1] = rc;
+      info5[2] = kc;
+      info5[3] = re;
+      info5[4] = ke;
+    }
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+  return 0;
+}
+
 // ====================================================================== autotuner
 // Times one evaluation (uniform density at the context's electron count) per
 // candidate tile shape on the context's stream and keeps the fastest, by

@@ -189,4 +189,13 @@ int ion_forces_spline(const ShapePlan& sp, const double* rb, const double* cell_
                       const double* h_psp_maxg, const double* h_psp_v0, const int* h_psp_charge, int order,
                       const double* d_rho, double* d_forces, void* d_scratch, hipStream_t st);
 
+// Ion-ion Ewald sum on the device (ofd_tables.hip): real-space pair sums and
+// forces, particle-mesh reciprocal energy and forces on the density grid.
+// comps4: real, self, reciprocal, average-potential terms (Hartree); h_forces
+// (host, 3 per ion) may be NULL. Scratch as for build_ion_potential.
+int ewald_device(const ShapePlan& sp, const double* rb, const double* lat, double vol, int ntypes,
+                 const int* h_type_start, const double* h_charge, const double* d_frac, int nion, int order,
+                 double eta, double rc, double kc, const double* cell_inv_t, double* comps4, double* h_forces,
+                 void* d_scratch, hipStream_t st);
+
 }  // namespace ofd

@@ -91,6 +91,14 @@ MODULE ofd_c
       TYPE(C_PTR), VALUE :: ctx, rho, forces
       INTEGER(C_INT) :: ofd_ion_forces
     END FUNCTION
+    ! EwaldEnergy / EwaldForces (Ewald.f90:465-596, 1196-1310); forces and
+    ! info5 may be C_NULL_PTR
+    FUNCTION ofd_ewald(ctx, energy, forces, info5) BIND(C, name="ofd_ewald")
+      IMPORT :: C_PTR, C_INT, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx, forces, info5
+      REAL(C_DOUBLE), INTENT(OUT) :: energy
+      INTEGER(C_INT) :: ofd_ewald
+    END FUNCTION
     FUNCTION c_strlen(s) BIND(C, name="strlen")
       IMPORT :: C_PTR, C_SIZE_T
       TYPE(C_PTR), VALUE :: s

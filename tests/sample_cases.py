@@ -11,6 +11,20 @@ OPTION_FRAC = np.array([[0.1, 0.0, 0.0], [0.5, 0.55, 0.99], [0.0, 0.5, 0.5], [0.
 OPTION_STEP0_EV = -2.199483598873e+02
 OPTION_FINAL_EV = -2.311425327622e+02
 OPTION_EII_EV = -2.986846473280e+02
+# displaced.out:934 (ion-ion at the final geometry of displaced.final.geom)
+OPTION_EII_FINAL_EV = -3.011568598115e+02
+# the CG2 relaxation of displaced.inpt: TOTAL ENERGY of every geometry evaluated
+# (displaced.out:123,192,...,938, eV), the NTN steps of each density solve
+# (:103,172,...,918) and the final positions (displaced.final.geom)
+OPTION_RELAX_EV = [-2.311425327622e+02, -2.312770513588e+02, -2.316147260670e+02, -2.316471183146e+02,
+                   -2.316860136007e+02, -2.316996685913e+02, -2.317083720964e+02, -2.317111562797e+02,
+                   -2.317116104934e+02, -2.317118205898e+02, -2.317118261145e+02, -2.317118465883e+02,
+                   -2.317118498940e+02]
+OPTION_RELAX_NTN_STEPS = [7, 5, 6, 5, 5, 5, 5, 5, 3, 4, 3, 3, 3]
+OPTION_FINAL_FRAC = np.array([[0.026242363836079, 0.012513411405787, 0.997499799127206],
+                              [0.526213499121448, 0.512582285849498, 0.997494511543261],
+                              [0.026254939350756, 0.512439721648613, 0.497479346361002],
+                              [0.526294902673969, 0.012517237611835, 0.497501509902651]])
 OPTION_ITERS = 7
 # standard.out:62-68 (Ha), per NTN iteration
 OPTION_ITER_HA = [-8.4856061397, -8.4936397786, -8.4942793002, -8.4943223357,

@@ -260,3 +260,149 @@ def test_reference_sample_optDen_wgc():
     assert [h["newton_iter"] for h in res.history[1:]] == S.OPTDEN_NEWTON_ITERS
     for h, ref in zip(res.history, S.OPTDEN_STEP_EV):
         assert abs(h["energy"] * O.HARTREE_TO_EV - ref) / 4 < 5e-6
+
+
+# ---------------------------------------------------------------- Ewald
+def _ewald_cell(lat_a, n=16):
+    return O.Cell(np.asarray(lat_a).T / O.BOHR, n, n, n)
+
+
+def test_ewald_reference_samples():
+    """The oracle's particle-mesh Ewald energy against the reference's recorded
+    ion-ion energies (test/optIon first and final geometry, test/optDen) and
+    the printed Ewald parameters."""
+    fin = np.array([[0.026242363836079, 0.012513411405787, 0.997499799127206],
+                    [0.526213499121448, 0.512582285849498, 0.997494511543261],
+                    [0.026254939350756, 0.512439721648613, 0.497479346361002],
+                    [0.526294902673969, 0.012517237611835, 0.497501509902651]])
+    cases = [(S.OPTION_LATTICE_A, S.OPTION_FRAC, S.OPTION_EII_EV, (0.5862, 8.582, 6.234)),
+             (S.OPTION_LATTICE_A, fin, S.OPTION_EII_FINAL_EV, (0.5862, 8.582, 6.234)),
+             (S.OPTDEN_LATTICE_A, S.OPTDEN_FRAC, S.OPTDEN_EII_EV, (0.5914, 8.498, 6.283))]
+    for lat, frac, ref_ev, printed in cases:
+        cell = _ewald_cell(lat)
+        e, comps, p = O.ewald_energy(cell, np.full(4, 3.0), frac)
+        assert abs(e * O.HARTREE_TO_EV - ref_ev) <= 1e-11 * abs(ref_ev)
+        # G12.4 / F formats of EwaldSetup's report (Ewald.f90:451-455)
+        assert round(p.eta, 4) == printed[0]
+        assert round(p.real_cutoff, 3) == printed[1]
+        assert round(p.recip_cutoff, 3) == printed[2]
+
+
+def test_ewald_madelung_and_eta_independence():
+    """fcc lattice of point charges Z in a neutralizing background: energy per
+    ion = -0.8958736 Z^2 / r_ws (the fcc Madelung constant referred to the
+    Wigner-Seitz radius); the exact-sum energy does not depend on eta; the
+    particle-mesh sum agrees with the exact one."""
+    cell = O.Cell(np.eye(3) * 7.65, 24, 24, 24)
+    fcc = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]], dtype=float)
+    q = np.full(4, 3.0)
+    rws = (3.0 * cell.vol / (4.0 * np.pi * 4)) ** (1.0 / 3.0)
+    e, _, _ = O.ewald_energy(cell, q, fcc, spline=False)
+    assert abs(e / 4 * rws / 9.0 - (-0.8958736)) < 2e-7
+    rng = np.random.default_rng(3)
+    fr = rng.random((5, 3))
+    qq = np.array([3.0, 3.0, 2.0, 2.0, 3.0])
+    tri = O.Cell(np.array([[7.3, 0.2, 0.1], [0.3, 7.5, 0.0], [-0.2, 0.4, 7.6]]).T, 24, 24, 24)
+    es = []
+    for eta in (0.9, 0.7):
+        p = O.EwaldParams(eta, 6.5 / eta, 13.0 * eta, 0.0, 0.0)
+        es.append(O.ewald_energy(tri, qq, fr, spline=False, params=p)[0])
+    assert abs(es[0] - es[1]) < 1e-10 * abs(es[0])
+    ep, _, _ = O.ewald_energy(tri, qq, fr, spline=True)
+    ex, _, _ = O.ewald_energy(tri, qq, fr, spline=False)
+    assert abs(ep - ex) < 1e-7 * abs(ex)
+
+
+def test_ewald_forces_are_minus_energy_gradient():
+    cell = _ewald_cell(S.OPTION_LATTICE_A)
+    q = np.full(4, 3.0)
+    fr = S.OPTION_FRAC.copy()
+    p = O.ewald_setup(cell, q)
+    f = O.ewald_forces(cell, q, fr, params=p)
+    h = 1e-5
+    for i, k in [(0, 0), (1, 1), (3, 2), (2, 0)]:
+        dfr = np.linalg.solve(cell.lattice, np.eye(3)[k]) * h
+        fp, fm = fr.copy(), fr.copy()
+        fp[i] += dfr
+        fm[i] -= dfr
+        de = (O.ewald_energy(cell, q, fp, params=p)[0] - O.ewald_energy(cell, q, fm, params=p)[0]) / (2 * h)
+        assert abs(-de - f[i, k]) < 1e-8
+
+
+# ---------------------------------------------------------------- ion relaxation
+@needs_ref
+def test_lnsrch_step_for_step_against_reference_fortran():
+    """The oracle's LnSrch against the reference's MathLNsrch.f90 on line
+    functions that take every branch kind (bracketing from the left and the
+    right, order flips, interpolation failures): same alpha, same task, same
+    bracket points after every call."""
+    fns = [lambda a: ((a - 0.7) ** 2 - 1.0, 2 * (a - 0.7)),
+           lambda a: (np.cos(3 * a) + 0.1 * a, -3 * np.sin(3 * a) + 0.1),
+           lambda a: (-a / (a * a + 2.0), -(2.0 - a * a) / (a * a + 2.0) ** 2),
+           lambda a: (a ** 4 - 3 * a ** 2 + a, 4 * a ** 3 - 6 * a + 1),
+           lambda a: (np.exp(-a) * 0.3 - 0.01 * a, -0.3 * np.exp(-a) - 0.01)]
+    c1, c2 = O._f32(1e-4), O._f32(1e-1)
+    for fn in fns:
+        for a0, smax in ((1.0, 5.0), (0.05, 3.0), (2.5, 10.0), (0.3, 0.5)):
+            ours, theirs = O.LnSrch(), R.RefLnSrch()
+            f, g = fn(0.0)
+            if g > 0:
+                continue
+            t1 = t2 = "START"
+            a1 = a2 = a0
+            for _ in range(25):
+                a1, t1 = ours.step(a1, f, g, t1, c1, c2, 0.0, smax)
+                a2, t2 = theirs.step(a2, f, g, t2, c1, c2, 0.0, smax)
+                assert a1 == a2 and t1.strip() == t2.strip(), (t1, t2, a1, a2)
+                assert np.array_equal(ours.points, theirs.points)
+                if not (t1[:2] == "FG" or t1[:4] == "WARN"):
+                    break
+                f, g = fn(a1)
+
+
+def test_nn_min_distance():
+    cell = O.Cell(np.eye(3) * 7.65, 16, 16, 16)
+    fcc = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]], dtype=float)
+    assert abs(O.nn_min_distance(cell, fcc) - 7.65 / np.sqrt(2.0)) < 1e-12
+    one = O.Cell(np.diag([5.0, 6.0, 7.0]), 8, 8, 8)
+    assert abs(O.nn_min_distance(one, [[0.1, 0.2, 0.3], [0.9, 0.2, 0.3]]) - 1.0) < 1e-12
+
+
+def _option_relax_evaluator(state):
+    psp = _al_psp()
+    cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
+    f = O.Functional(kinetic=4).resolve(12.0, cell.vol)
+    q = np.full(4, 3.0)
+
+    def evaluate(frac):
+        vion = O.ion_potential(cell, [(psp, frac)])
+        eii, _, _ = O.ewald_energy(cell, q, frac)
+        full, pot = O.make_evaluators(O.System(cell, f, vion, eii))
+        res = O.rho_minimization(state["rho"], full, pot, cell.dV)   # warm start (trashPreviousRho = F)
+        state["rho"] = res.rho
+        state["iters"].append(res.iters)
+        return res.energy[0], O.ion_forces(cell, [(psp, frac)], res.rho) + O.ewald_forces(cell, q, frac)
+    return cell, evaluate
+
+
+def test_reference_sample_optIon_relaxation():
+    """test/optIon CG2 relaxation (displaced.inpt: MINI ion, meth ion cg2,
+    method ntn): the same 13 geometries with the same density step counts
+    but one, the final energy within 1e-6 eV/atom of displaced.out and the
+    final geometry within 1e-3 bohr of displaced.final.geom. The intermediate
+    energies agree to 5e-5 eV/atom: forces are first order in the density's
+    residual error (the solve stops at a potential norm of 1e-5), so the
+    first forces already differ from the reference's by about 1e-5 Ha/bohr
+    (its recorded dE/dalpha -5.43963e-3 against -5.44135e-3 here), and the
+    line-search trial points move with them."""
+    cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
+    state = {"rho": np.full(cell.shape, 12.0 / cell.vol), "iters": []}
+    cell, evaluate = _option_relax_evaluator(state)
+    r = O.ion_relax_cg2(cell, S.OPTION_FRAC, evaluate)
+    ev = np.array(r.energies) * O.HARTREE_TO_EV
+    assert len(ev) == len(S.OPTION_RELAX_EV)
+    assert np.max(np.abs(ev - np.array(S.OPTION_RELAX_EV))) / 4 < 5e-5
+    assert abs(ev[-1] - S.OPTION_RELAX_EV[-1]) / 4 < 1e-6
+    assert sum(a != b for a, b in zip(state["iters"], S.OPTION_RELAX_NTN_STEPS)) <= 1
+    d = np.mod(r.frac - S.OPTION_FINAL_FRAC + 0.5, 1.0) - 0.5
+    assert np.max(np.abs(d @ cell.lattice.T)) < 1e-3
