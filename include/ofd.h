@@ -166,6 +166,34 @@ int ofd_ion_forces(ofd_ctx* ctx, const double* rho, double* forces);
  * (1/bohr), and the two error bounds (Hartree). */
 int ofd_ewald(ofd_ctx* ctx, double* energy, double* forces, double* info5);
 
+/* Ion relaxation by conjugate gradients, ConjugateGradient of IonOptCG2
+ * (IonOptCG2.f90:60-520) with its LnSrch line search (MathLNsrch.f90:52-296),
+ * called as OptimizeIons does with RhoOptimizer = ProjNormRhoMinimization
+ * (dir_type 1) and CalculateForces = Ewald + ion-electron spline forces
+ * (CalForce.f90:134-194). Every geometry refreshes the ion terms (ionic
+ * potential and Ewald energy, RefreshIons.f90:125-238), optimizes the density
+ * from the previous one (trashPreviousRho = .FALSE.) and computes forces.
+ * rhoR (n1*n2*n3, in/out) and frac_xyz (3 per ion, in/out, wrapped into
+ * [0,1)) are updated to the final geometry; energy9 and forces (3 per ion,
+ * Ha/bohr) are those of the final geometry (either may be NULL); energies
+ * (may be NULL) receives the total energy of every geometry evaluated, up
+ * to max_energies of them. */
+typedef struct {
+  int cg_type;          /* 1 Polak-Ribiere, 2 Hager-Zhang, 3 Dai-Yuan; <= 0: 2 (IonOptimizers.f90:36) */
+  double force_cutoff;  /* stop when max |F| falls below it (Ha/bohr); <= 0: 5e-5 (:29) */
+  int max_steps;        /* <= 0: 200 (:41) */
+  double nn_dist;       /* nearest-ion distance guard (bohr); < 0: 2.0 (Neighbor.f90:33); 0: off */
+} ofd_ion_params;
+typedef struct {
+  int steps;            /* CG iterations */
+  int geometries;       /* geometries evaluated (density solves) */
+  double max_force;     /* at the final geometry */
+  double seconds;
+} ofd_ion_stats;
+int ofd_ion_relax(ofd_ctx* ctx, double* rhoR, double* frac_xyz, const ofd_opt_params* rho_params,
+                  const ofd_ion_params* ion, double* energy9, double* forces, double* energies, int max_energies,
+                  ofd_ion_stats* stats);
+
 /* ---- batched serving (no reference counterpart: many density
  *      optimizations of one grid shape advanced together) ---- */
 typedef struct ofd_pool ofd_pool;

@@ -37,6 +37,14 @@ class ofd_tune_result(C.Structure):
                 ("configs_tried", C.c_int), ("us_default", C.c_double), ("us_best", C.c_double)]
 
 
+class ofd_ion_params(C.Structure):
+    _fields_ = [("cg_type", C.c_int), ("force_cutoff", C.c_double), ("max_steps", C.c_int), ("nn_dist", C.c_double)]
+
+
+class ofd_ion_stats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("geometries", C.c_int), ("max_force", C.c_double), ("seconds", C.c_double)]
+
+
 class ofd_opt_stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("num_evals", C.c_int), ("newton_iters_total", C.c_int),
                 ("fp32_evals", C.c_int), ("final_pot_norm", C.c_double), ("seconds", C.c_double)]
@@ -52,7 +60,7 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
            "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished", "ofd_server_create",
            "ofd_server_add_pool", "ofd_server_wait", "ofd_server_last_error", "ofd_server_destroy",
-           "ofd_get_volume", "ofd_ewald"]
+           "ofd_get_volume", "ofd_ewald", "ofd_ion_relax"]
 
 
 def lib():
@@ -82,6 +90,8 @@ def lib():
     L.ofd_rho_opt.argtypes = [vp, dp, C.c_int, C.POINTER(ofd_opt_params), dp, ip, C.POINTER(ofd_opt_stats)]
     L.ofd_ion_forces.argtypes = [vp, dp, dp]
     L.ofd_ewald.argtypes = [vp, dp, dp, dp]
+    L.ofd_ion_relax.argtypes = [vp, dp, dp, C.POINTER(ofd_opt_params), C.POINTER(ofd_ion_params), dp, dp, dp,
+                                C.c_int, C.POINTER(ofd_ion_stats)]
     L.ofd_pool_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L.ofd_pool_destroy.argtypes = [vp]
     L.ofd_pool_destroy.restype = None

@@ -189,6 +189,29 @@ def EwaldEnergy(system: OFDSystem, forces=False):
     return e.value, out, dict(zip(("eta", "real_cutoff", "recip_cutoff", "real_error", "recip_error"), info))
 
 
+def ConjugateGradient(system: OFDSystem, rhoR, frac, cg_type=2, force_cutoff=-1.0, max_steps=-1, nn_dist=-1.0,
+                      max_geometries=1000, **rho_kw):
+    """Ion relaxation (IonOptCG2::ConjugateGradient, IonOptCG2.f90:60-520)
+    from fractional positions `frac` (ions in the order and species grouping
+    given to build_ion_potential) and the starting density rhoR, with the NTN
+    density optimizer at every geometry (rho_kw as for
+    ProjNormRhoMinimization). Returns (rho, frac, energy9, forces, energies of
+    every geometry, stats)."""
+    from ._lib import ofd_ion_params, ofd_ion_stats
+    r = np.ascontiguousarray(rhoR, dtype=np.float64).copy()
+    fr = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1, 3).copy()
+    ip = ofd_ion_params(int(cg_type), float(force_cutoff), int(max_steps), float(nn_dist))
+    p = opt_params(**rho_kw)
+    et = np.zeros(9)
+    forces = np.zeros_like(fr)
+    en = np.zeros(max_geometries)
+    st = ofd_ion_stats()
+    system._check(lib().ofd_ion_relax(system._h, dptr(r), dptr(fr), C.byref(p), C.byref(ip), dptr(et), dptr(forces),
+                                      dptr(en), int(max_geometries), C.byref(st)))
+    stats = {k: getattr(st, k) for k, _ in st._fields_}
+    return r, fr, et, forces, en[:st.geometries].copy(), stats
+
+
 def request_bytes(grid, kinetic=4, precision=0):
     return int(lib().ofd_request_bytes(int(grid[0]), int(grid[1]), int(grid[2]), int(kinetic), int(precision)))
 

@@ -1701,6 +1701,368 @@ extern "C" int ofd_ewald(ofd_ctx* c, double* energy, double* forces, double* inf
   return 0;
 }
 
+// ====================================================================== ion relaxation (CG2)
+// RefreshIonTerms (RefreshIons.f90:125-238) for positions already stored in
+// c->frac: the ionic potential from the species tables kept by
+// ofd_build_ion_potential, on the device.
+static int refresh_ion_potential(ofd_ctx* c) {
+  const int ns = (int)c->type_start.size() - 1;
+  const int nion = c->type_start[ns];
+  HIPCHK(hipMemcpyAsync(c->d_frac, c->frac.data(), sizeof(double) * c->frac.size(), hipMemcpyHostToDevice, c->st));
+  const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
+  void* sc = dalloc<char>(scratch);
+  const int rc = build_ion_potential(c->sp, c->rb, c->vol, ns, c->type_start.data(), c->d_frac, nion, c->d_psp_vqs,
+                                     c->d_psp_dd, c->psp_off.data(), c->psp_npts.data(), c->psp_maxg.data(),
+                                     c->psp_v0.data(), c->psp_charge.data(), c->spline_order, c->vion, sc, c->st);
+  hipStreamSynchronize(c->st);
+  dev_free(sc);
+  return rc;
+}
+
+// NNDist_bin (Neighbor.f90:68-420) for positions wrapped into the cell: the
+// minimum over ion pairs and image shifts in {-1,0,1}^3 (the one-bin form;
+// the binned search returns the same whenever the nearest distance is below
+// its 20 bohr bin spacing).
+static double nn_min_distance(const double* lat, const std::vector<double>& frac) {
+  const int n = (int)frac.size() / 3;
+  std::vector<double> xyz(3 * (size_t)n);
+  for (int a = 0; a < n; ++a) {
+    double f[3];
+    for (int k = 0; k < 3; ++k) {
+      f[k] = std::fmod(frac[3 * a + k], 1.0);
+      if (f[k] < 0) f[k] += 1.0;
+    }
+    for (int i = 0; i < 3; ++i) xyz[3 * a + i] = lat[i] * f[0] + lat[3 + i] * f[1] + lat[6 + i] * f[2];
+  }
+  double best = 1e6;
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b)
+      for (int sk = -1; sk <= 1; ++sk)
+        for (int sj = -1; sj <= 1; ++sj)
+          for (int si = -1; si <= 1; ++si) {
+            if (a == b && si == 0 && sj == 0 && sk == 0) continue;
+            double d2 = 0.0;
+            for (int i = 0; i < 3; ++i) {
+              const double d = xyz[3 * a + i] - (xyz[3 * b + i] + lat[i] * si + lat[3 + i] * sj + lat[6 + i] * sk);
+              d2 += d * d;
+            }
+            best = std::min(best, std::sqrt(d2));
+          }
+  return best;
+}
+
+// LnSrch (MathLNsrch.f90:52-296) with its module state points(2,3) and dsave:
+// two bracket points (alpha, E, dE/dalpha), secant steps on dE/dalpha.
+struct IonLnSrch {
+  double P[2][3];
+  double d0 = 0, d1 = 0;   // dsave(1:2)
+  static double new_alp(double x1, double y1, double x2, double y2, double stpmin, double stpmax) {
+    const double slope = (y1 - y2) / (x1 - x2);
+    const double b = y1 - x1 * slope;
+    const double a = -b / slope;
+    return (a >= stpmax || a <= stpmin) ? -1.0 : a;
+  }
+  double na(double stpmin, double stpmax) const { return new_alp(P[0][0], P[0][2], P[1][0], P[1][2], stpmin, stpmax); }
+  void set(int k, double a, double f, double g) { P[k][0] = a; P[k][1] = f; P[k][2] = g; }
+  // task in/out: the reference's CHARACTER task, here as a string
+  void step(double& alp, double f, double g, std::string& task, double c1, double c2, double stpmin, double stpmax) {
+    if (task.compare(0, 5, "START") == 0) {
+      if (g > 0.0) { task = "ERROR: Initial G>0"; return; }
+      for (auto& r : P) r[0] = r[1] = r[2] = -1.0;
+      set(0, 0.0, f, g);
+      d0 = f;
+      d1 = g;
+      P[1][0] = -1.0;
+      task = "FG" + task.substr(2);
+      return;
+    }
+    if (task.compare(0, 2, "FG") == 0 && P[1][0] > 0.0) {
+      if (f <= d0 + c1 * alp * d1 && std::fabs(g) <= c2 * std::fabs(d1)) { task = "CONV"; return; }
+    }
+    if (P[1][0] < 0.0) {
+      if (g < P[0][2]) {
+        alp = f < P[0][1] ? alp * 5.0 : (P[0][0] + alp) / 2.0;
+        task = "WARNING: Need a better second point.";
+        return;
+      }
+      set(1, alp, f, g);
+      alp = na(stpmin, stpmax);
+      if (alp < 0.0 || alp < P[0][0]) {
+        if (P[1][1] < P[0][1]) {
+          task = "WARNING: Interpolation error, code:-1, energy decreases, step further";
+          alp = 5.0 * P[1][0];
+        } else {
+          task = "ERROR: Interpolation error, code:-1";
+        }
+        return;
+      }
+      task = "FG";
+      return;
+    }
+    if (g > 0) {
+      if (P[1][2] > 0.0) {
+        if (P[1][0] > alp) {
+          set(1, alp, f, g);
+          alp = na(stpmin, stpmax);
+          if (alp < P[0][0]) {
+            alp = (P[0][0] + P[1][0]) / 2.0;
+            task = "WARNING: Order flips, code:0, energy decreases, step further";
+          } else {
+            task = "FG";
+          }
+          return;
+        }
+        if (f < P[1][1]) {
+          task = "WARNINIG: Order flipped, code:1";
+          alp = alp * 5.0;
+        } else {
+          alp = (P[0][0] + P[1][0]) / 2.0;
+          task = "WARNING: Order flipped, code:2";
+        }
+        return;
+      }
+      if (P[1][2] < 0.0) {
+        std::memcpy(P[0], P[1], sizeof(P[0]));
+        set(1, alp, f, g);
+        alp = na(stpmin, stpmax);
+        if (alp < P[0][0] || alp > P[1][0]) {
+          if (P[0][1] > P[1][1]) {
+            alp = P[1][0] * 5.0;
+            task = "WARNING: Interpolation error, code:1, energy decreases, step further";
+          } else {
+            alp = (P[0][0] + P[1][0]) / 2.0;
+            task = "WARNING: Interpolation error, code:1";
+          }
+        } else {
+          task = "FG";
+        }
+        return;
+      }
+    }
+    if (g < 0.0) {
+      if (P[1][2] > 0.0) {
+        set(0, alp, f, g);
+        alp = na(stpmin, stpmax);
+        if (alp < P[0][0] || alp > P[1][0]) {
+          if (P[0][1] > P[1][1]) {
+            task = "WARNING: Interpolation, code:2, f decreases, step further";
+            alp = 5.0 * P[1][0];
+          } else {
+            alp = (P[0][0] + P[1][0]) / 2.0;
+            task = "WARNING: Interpolation, code:4, brackets the minimum by half.";
+          }
+        } else {
+          task = "FG";
+        }
+        return;
+      }
+      if (P[1][2] < 0.0) {
+        std::memcpy(P[0], P[1], sizeof(P[0]));
+        set(1, alp, f, g);
+        alp = na(stpmin, stpmax);
+        if (alp < P[1][0]) {
+          if (P[0][1] > P[1][1]) {
+            alp = P[1][0] * 5.0;
+            task = "WARNING: Interpolation error: code:3, f decrease, step further";
+          } else {
+            task = "ERROR: Interpolation error: code:3";
+          }
+        } else {
+          task = "FG";
+        }
+        return;
+      }
+    }
+  }
+};
+
+extern "C" int ofd_ion_relax(ofd_ctx* c, double* rhoR, double* frac_xyz, const ofd_opt_params* rho_params,
+                             const ofd_ion_params* ion, double* energy9, double* forces, double* energies,
+                             int max_energies, ofd_ion_stats* stats) {
+  if (!c || !rhoR || !frac_xyz || !rho_params) return fail(c, "null argument");
+  if (c->type_start.empty()) return fail(c, "ions not set (ofd_build_ion_potential)");
+  auto t0 = std::chrono::steady_clock::now();
+  // IonOptimizers.f90:29,36,41 and Neighbor.f90:33 defaults; forceCutoff is
+  // a default-real literal (5e-5 in single precision)
+  const int cg_type = (ion && ion->cg_type > 0) ? ion->cg_type : 2;
+  const double fcut = (ion && ion->force_cutoff > 0) ? ion->force_cutoff : (double)5e-5f;
+  const int max_steps = (ion && ion->max_steps > 0) ? ion->max_steps : 200;
+  const double nn_dist = (ion && ion->nn_dist >= 0) ? ion->nn_dist : 2.0;
+  const int max_line = 10;
+  const char* tr = std::getenv("OFD_TRACE");
+  const bool trace = tr && std::atoi(tr) != 0;
+  // IonOptCG2.f90:134-139: ftol, gtol default-real literals, xtol 1e-12_DP
+  const double ftol = (double)1e-4f, gtol = (double)1e-1f;
+  if (cg_type < 1 || cg_type > 3) return fail(c, "cg_type must be 1, 2 or 3");
+  const int ns = (int)c->type_start.size() - 1;
+  const int nion = c->type_start[ns];
+  const long long n = c->nr();
+  std::vector<double> frac(frac_xyz, frac_xyz + 3 * (size_t)nion);
+  for (double& v : frac) {
+    v = std::fmod(v, 1.0);
+    if (v < 0) v += 1.0;
+  }
+  double et[9] = {0};
+  std::vector<double> F(3 * (size_t)nion), fie(3 * (size_t)nion), fii(3 * (size_t)nion);
+  int ngeo = 0, step = 0;
+  double max_force = 0.0;
+  std::string err;
+  // one geometry: RefreshIonTerms, RhoOptimizer (warm start from rhoR),
+  // CalculateForces = Ewald + ion-electron (CalForce.f90:134-194)
+  auto evaluate = [&]() -> bool {
+    c->frac = frac;
+    if (nn_dist > 0 && nn_min_distance(c->lat, frac) < nn_dist) {
+      err = "nearest ion distance below the allowed threshold (CheckNearestDistanceAtoms)";
+      return false;
+    }
+    // the device positions first: the Ewald sum reads them too
+    if (refresh_ion_potential(c)) { err = "ion potential build failed"; return false; }
+    double eii = 0.0;
+    if (ofd_ewald(c, &eii, fii.data(), nullptr)) return false;
+    int outcome = 0;
+    ofd_opt_stats st;
+    if (ofd_rho_opt(c, rhoR, 1, rho_params, et, &outcome, &st)) return false;
+    if (ofd_ion_forces(c, rhoR, fie.data())) return false;
+    for (size_t k = 0; k < F.size(); ++k) F[k] = fii[k] + fie[k];
+    if (energies && ngeo < max_energies) energies[ngeo] = et[0];
+    if (trace) {
+      double mf = 0.0;
+      for (int a = 0; a < nion; ++a)
+        mf = std::max(mf, std::sqrt(F[3 * a] * F[3 * a] + F[3 * a + 1] * F[3 * a + 1] + F[3 * a + 2] * F[3 * a + 2]));
+      std::fprintf(stderr, "ofd ion-relax geometry %d: E = %.12e Ha, ion-ion %.12e, max |F| = %.6e, NTN steps %d\n",
+                   ngeo, et[0], eii, mf, st.iterations);
+    }
+    ++ngeo;
+    return true;
+  };
+  auto maxf = [&]() {
+    double m = 0.0;
+    for (int a = 0; a < nion; ++a)
+      m = std::max(m, std::sqrt(F[3 * a] * F[3 * a] + F[3 * a + 1] * F[3 * a + 1] + F[3 * a + 2] * F[3 * a + 2]));
+    return m;
+  };
+  auto dot = [](const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0.0;
+    for (size_t k = 0; k < a.size(); ++k) s += a[k] * b[k];
+    return s;
+  };
+  try {
+    if (!evaluate()) return err.empty() ? -1 : fail(c, err);
+    // frac <-> cart with cellReal (columns = lattice vectors) and its inverse
+    double cinv[9], rb[9], vol;
+    cell_setup(c->lat, rb, &vol, cinv);   // cinv[3c+j] = inv(cellReal)(j, c)
+    const size_t m = F.size();
+    std::vector<double> old_grad(m, 0.0), dir_old(m, 0.0), new_grad(m), d(m), old_cart(m);
+    double alpha = 1.0;
+    bool restart = false;
+    IonLnSrch ls;
+    for (;;) {
+      ++step;
+      if (step > max_steps) break;
+      max_force = maxf();
+      if (max_force < fcut) break;
+      for (int a = 0; a < nion; ++a)
+        for (int i = 0; i < 3; ++i)
+          old_cart[3 * a + i] = c->lat[i] * frac[3 * a] + c->lat[3 + i] * frac[3 * a + 1] + c->lat[6 + i] * frac[3 * a + 2];
+      new_grad = F;
+      double gam = 0.0;
+      if (step == 1 || restart || step % 30 == 0) {
+        gam = 0.0;
+        restart = false;
+      } else if (cg_type == 1) {
+        double num = 0.0;
+        for (size_t k = 0; k < m; ++k) num += (new_grad[k] - old_grad[k]) * new_grad[k];
+        gam = std::max(0.0, num / dot(old_grad, old_grad));
+      } else if (cg_type == 2) {
+        // Hager-Zhang (IonOptCG2.f90:203-214)
+        std::vector<double> y(m);
+        for (size_t k = 0; k < m; ++k) y[k] = -new_grad[k] + old_grad[k];
+        const double ny = std::sqrt(dot(y, y)), dy = dot(dir_old, y);
+        double g2 = 0.0;
+        for (size_t k = 0; k < m; ++k) g2 += (y[k] - 2.0 * dir_old[k] * ny * ny / dy) * (-new_grad[k] / dy);
+        const double nd = std::sqrt(dot(dir_old, dir_old)), ng = std::sqrt(dot(old_grad, old_grad));
+        gam = std::max(g2, -1.0 / (nd * std::min(0.01, ng)));
+      } else {
+        std::vector<double> y(m);
+        for (size_t k = 0; k < m; ++k) y[k] = -new_grad[k] + old_grad[k];
+        gam = std::max(0.0, dot(new_grad, new_grad) / dot(dir_old, y));
+      }
+      for (size_t k = 0; k < m; ++k) d[k] = new_grad[k] + gam * dir_old[k];
+      dir_old = d;
+      old_grad = new_grad;
+      double grad_alp = -dot(F, d);
+      double dmax = d[0];
+      for (double v : d) dmax = std::max(dmax, v);
+      const double stpmin = 0.0;
+      const double stpmax = nn_dist > 0 ? std::fabs((nn_min_distance(c->lat, frac) - nn_dist) / dmax)
+                                        : std::fabs(1.0 / dmax);
+      alpha = std::min(alpha, stpmax);
+      std::string task = "START";
+      int nline = 0;
+      bool done = false;
+      for (;;) {
+        ls.step(alpha, et[0], grad_alp, task, ftol, gtol, stpmin, stpmax);
+        if (nline >= 4 && std::fabs(ls.P[0][1] - ls.P[1][1]) < (double)1.e-6f) break;
+        if (ls.P[0][0] > 0.0) {
+          double bmax = 0.0;
+          for (int a = 0; a < nion; ++a) {
+            double s2 = 0.0;
+            for (int i = 0; i < 3; ++i) {
+              const double b = (ls.P[0][0] - ls.P[1][0]) * d[3 * a + i];
+              s2 += b * b;
+            }
+            bmax = std::max(bmax, std::sqrt(s2));
+          }
+          if (bmax < (double)1e-5f) break;
+        }
+        if (task.compare(0, 4, "CONV") == 0) break;
+        if (task.compare(0, 5, "ERROR") == 0) {
+          restart = true;
+          alpha = 1.0;
+          break;
+        }
+        if (task.compare(0, 2, "FG") == 0 || task.compare(0, 4, "WARN") == 0) {
+          // coord = MODULO(cart2frac (oldCart + alpha dir), 1)
+          for (int a = 0; a < nion; ++a)
+            for (int j = 0; j < 3; ++j) {
+              double v = 0.0;
+              for (int i = 0; i < 3; ++i) v += cinv[3 * i + j] * (old_cart[3 * a + i] + alpha * d[3 * a + i]);
+              v = std::fmod(v, 1.0);
+              if (v < 0) v += 1.0;
+              frac[3 * a + j] = v;
+            }
+          if (!evaluate()) return err.empty() ? -1 : fail(c, err);
+          grad_alp = -dot(F, d);
+          max_force = maxf();
+          if (max_force < fcut) {
+            done = true;
+            break;
+          }
+          if (++nline > max_line) {
+            restart = true;
+            alpha = 1.0;
+            break;
+          }
+        }
+      }
+      if (done) break;
+    }
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+  std::copy(frac.begin(), frac.end(), frac_xyz);
+  if (energy9) std::memcpy(energy9, et, sizeof(et));
+  if (forces) std::copy(F.begin(), F.end(), forces);
+  if (stats) {
+    stats->steps = step;
+    stats->geometries = ngeo;
+    stats->max_force = max_force;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  (void)n;
+  return 0;
+}
+
 // ====================================================================== autotuner
 // Times one evaluation (uniform density at the context's electron count) per
 // candidate tile shape on the context's stream and keeps the fastest, by

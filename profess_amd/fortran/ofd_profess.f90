@@ -99,6 +99,16 @@ MODULE ofd_c
       REAL(C_DOUBLE), INTENT(OUT) :: energy
       INTEGER(C_INT) :: ofd_ewald
     END FUNCTION
+    ! IonOptCG2::ConjugateGradient (IonOptCG2.f90:60-520); ion and stats are
+    ! ofd_ion_params / ofd_ion_stats (C_PTR, may be C_NULL_PTR for ion)
+    FUNCTION ofd_ion_relax(ctx, rhoR, frac, params, ion, energy, forces, energies, max_energies, stats) &
+        BIND(C, name="ofd_ion_relax")
+      IMPORT :: C_PTR, C_INT, ofd_opt_params_t
+      TYPE(C_PTR), VALUE :: ctx, rhoR, frac, ion, energy, forces, energies, stats
+      TYPE(ofd_opt_params_t), INTENT(IN) :: params
+      INTEGER(C_INT), VALUE :: max_energies
+      INTEGER(C_INT) :: ofd_ion_relax
+    END FUNCTION
     FUNCTION c_strlen(s) BIND(C, name="strlen")
       IMPORT :: C_PTR, C_SIZE_T
       TYPE(C_PTR), VALUE :: s
