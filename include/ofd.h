@@ -194,6 +194,44 @@ int ofd_ion_relax(ofd_ctx* ctx, double* rhoR, double* frac_xyz, const ofd_opt_pa
                   const ofd_ion_params* ion, double* energy9, double* forces, double* energies, int max_energies,
                   ofd_ion_stats* stats);
 
+/* ---- input and density files (host only) ---- */
+typedef struct { char s[16]; } ofd_name;
+typedef struct { char s[256]; } ofd_path;
+/* A geometry as ReadGeometry leaves it (ReadIonFile.f90:80-705): cell in
+ * bohr (cell[3*j + i] = component i of lattice vector j, the columns of
+ * cellReal; LATTICE_CART in Angstrom or LATTICE_ABC), ions grouped by species
+ * in order of first appearance (type_start: nspecies + 1 offsets, as
+ * ofd_build_ion_potential takes them), fractional positions wrapped into
+ * [0,1) (POSITIONS_FRAC, POSITIONS_CART in Angstrom, POSITIONS_BOHR),
+ * file_index[k] = the ion's position in the file, and per species its name
+ * and SPECIES_POT file. Arrays are allocated by the library. */
+typedef struct {
+  double cell[9];
+  int nion, nspecies;
+  int* type_start;
+  double* frac;
+  int* file_index;
+  ofd_name* species;
+  ofd_path* pseudo_file;
+} ofd_geometry;
+int ofd_read_geometry(const char* path, ofd_geometry* out, char* err, int errlen);
+void ofd_free_geometry(ofd_geometry* g);
+/* A .recpot table as ReadPseudo reads it (ReadIonFile.f90:708-930): maxG in
+ * 1/A and the values in eV*A^3, as ofd_build_ion_potential takes them.
+ * values NULL: only *npts is set (the count to allocate). */
+int ofd_read_recpot(const char* path, double* maxg_invA, double* values, int capacity, int* npts, char* err,
+                    int errlen);
+/* Density files in the reference's format (27-byte records: four A14,I12
+ * header records, then E26.20 values, x fastest): Print3DArray without
+ * rescaling (Output.f90:617-873) and ReadDensity (ReadIonFile.f90:1054-1314),
+ * which copies a file of the same grid (optionally repeated repeat_x/y/z
+ * times) and otherwise interpolates trilinearly, periodically, at the cell
+ * centres of the new grid; with cell (bohr) and num_electrons > 0 the result
+ * is scaled to hold num_electrons. One spin channel. */
+int ofd_write_density(const char* path, int n1, int n2, int n3, int nspin, const double* rho, char* err, int errlen);
+int ofd_read_density(const char* path, int n1, int n2, int n3, const double* cell, double num_electrons,
+                     int repeat_x, int repeat_y, int repeat_z, double* rho, char* err, int errlen);
+
 /* ---- batched serving (no reference counterpart: many density
  *      optimizations of one grid shape advanced together) ---- */
 typedef struct ofd_pool ofd_pool;

@@ -37,6 +37,21 @@ class ofd_tune_result(C.Structure):
                 ("configs_tried", C.c_int), ("us_default", C.c_double), ("us_best", C.c_double)]
 
 
+class ofd_name(C.Structure):
+    _fields_ = [("s", C.c_char * 16)]
+
+
+class ofd_path(C.Structure):
+    _fields_ = [("s", C.c_char * 256)]
+
+
+class ofd_geometry(C.Structure):
+    _fields_ = [("cell", C.c_double * 9), ("nion", C.c_int), ("nspecies", C.c_int),
+                ("type_start", C.POINTER(C.c_int)), ("frac", C.POINTER(C.c_double)),
+                ("file_index", C.POINTER(C.c_int)), ("species", C.POINTER(ofd_name)),
+                ("pseudo_file", C.POINTER(ofd_path))]
+
+
 class ofd_ion_params(C.Structure):
     _fields_ = [("cg_type", C.c_int), ("force_cutoff", C.c_double), ("max_steps", C.c_int), ("nn_dist", C.c_double)]
 
@@ -60,7 +75,8 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
            "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished", "ofd_server_create",
            "ofd_server_add_pool", "ofd_server_wait", "ofd_server_last_error", "ofd_server_destroy",
-           "ofd_get_volume", "ofd_ewald", "ofd_ion_relax"]
+           "ofd_get_volume", "ofd_ewald", "ofd_ion_relax", "ofd_read_geometry", "ofd_free_geometry",
+           "ofd_read_recpot", "ofd_write_density", "ofd_read_density"]
 
 
 def lib():
@@ -90,6 +106,13 @@ def lib():
     L.ofd_rho_opt.argtypes = [vp, dp, C.c_int, C.POINTER(ofd_opt_params), dp, ip, C.POINTER(ofd_opt_stats)]
     L.ofd_ion_forces.argtypes = [vp, dp, dp]
     L.ofd_ewald.argtypes = [vp, dp, dp, dp]
+    L.ofd_read_geometry.argtypes = [C.c_char_p, C.POINTER(ofd_geometry), C.c_char_p, C.c_int]
+    L.ofd_free_geometry.argtypes = [C.POINTER(ofd_geometry)]
+    L.ofd_free_geometry.restype = None
+    L.ofd_read_recpot.argtypes = [C.c_char_p, dp, dp, C.c_int, ip, C.c_char_p, C.c_int]
+    L.ofd_write_density.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_char_p, C.c_int]
+    L.ofd_read_density.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, dp, C.c_double, C.c_int, C.c_int,
+                                   C.c_int, dp, C.c_char_p, C.c_int]
     L.ofd_ion_relax.argtypes = [vp, dp, dp, C.POINTER(ofd_opt_params), C.POINTER(ofd_ion_params), dp, dp, dp,
                                 C.c_int, C.POINTER(ofd_ion_stats)]
     L.ofd_pool_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]

@@ -130,3 +130,107 @@ def mg_hcp_case(reps, grid, sigma=0.1, seed_ion=3, seed_rho=4):
     rho = cosine_density(shape, ne, vol, seed_rho)
     return dict(lattice=lat, frac=frac, num_electrons=ne, grid=tuple(grid), rho=rho,
                 recpot=synthetic_recpot(MG_Z, MG_RC), Z=MG_Z)
+
+
+# ---------------------------------------------------------------- files
+def _errbuf():
+    import ctypes as C
+    return C.create_string_buffer(512)
+
+
+def read_geometry(path):
+    """A .ion file through the library's reader (ofd_read_geometry,
+    ReadGeometry ReadIonFile.f90:80-705). Returns a dict: lattice (columns,
+    bohr), frac (ions grouped by species), type_start, file_index, species,
+    pseudo_files."""
+    import ctypes as C
+    from ._lib import OFDError, lib, ofd_geometry
+    g = ofd_geometry()
+    e = _errbuf()
+    if lib().ofd_read_geometry(str(path).encode(), C.byref(g), e, 512) != 0:
+        raise OFDError(e.value.decode())
+    try:
+        n, ns = g.nion, g.nspecies
+        out = {"lattice": np.array(g.cell[:]).reshape(3, 3).T.copy(),
+               "frac": np.array(g.frac[:3 * n]).reshape(n, 3),
+               "type_start": [g.type_start[k] for k in range(ns + 1)],
+               "file_index": [g.file_index[k] for k in range(n)],
+               "species": [g.species[k].s.decode() for k in range(ns)],
+               "pseudo_files": [g.pseudo_file[k].s.decode() for k in range(ns)]}
+    finally:
+        lib().ofd_free_geometry(C.byref(g))
+    return out
+
+
+def read_recpot_file(path):
+    """(maxG in 1/A, values in eV A^3) of a .recpot file through the
+    library's reader (ofd_read_recpot, ReadPseudo ReadIonFile.f90:708-930)."""
+    import ctypes as C
+    from ._lib import OFDError, dptr, lib
+    mg = C.c_double()
+    n = C.c_int()
+    e = _errbuf()
+    if lib().ofd_read_recpot(str(path).encode(), C.byref(mg), None, 0, C.byref(n), e, 512) != 0:
+        raise OFDError(e.value.decode())
+    vals = np.zeros(n.value)
+    if lib().ofd_read_recpot(str(path).encode(), C.byref(mg), dptr(vals), vals.size, C.byref(n), e, 512) != 0:
+        raise OFDError(e.value.decode())
+    return mg.value, vals
+
+
+def write_density(path, rho):
+    """rho (n3, n2, n1) in the reference's density file format (Print3DArray)."""
+    from ._lib import OFDError, dptr, lib
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    n3, n2, n1 = r.shape
+    e = _errbuf()
+    if lib().ofd_write_density(str(path).encode(), n1, n2, n3, 1, dptr(r), e, 512) != 0:
+        raise OFDError(e.value.decode())
+
+
+def read_density(path, grid, lattice=None, num_electrons=0.0, repeat=(1, 1, 1)):
+    """A density file on grid (n1, n2, n3) (ReadDensity): copied when the
+    file's grid (times repeat) matches, interpolated otherwise; scaled to
+    num_electrons when lattice (columns, bohr) and num_electrons > 0 are given.
+    Returns (n3, n2, n1)."""
+    from ._lib import OFDError, dptr, lib
+    n1, n2, n3 = grid
+    out = np.zeros((n3, n2, n1))
+    cell = None if lattice is None else np.ascontiguousarray(np.asarray(lattice, dtype=np.float64).T).ravel()
+    e = _errbuf()
+    if lib().ofd_read_density(str(path).encode(), n1, n2, n3, dptr(cell), float(num_electrons), int(repeat[0]),
+                              int(repeat[1]), int(repeat[2]), dptr(out), e, 512) != 0:
+        raise OFDError(e.value.decode())
+    return out
+
+
+def recpot_charge(maxg_invA, values_ev_a3):
+    """The valence charge ReadPseudo derives from a .recpot table
+    (ReadIonFile.f90:895-900): NINT(-(V(2) - V(1)) (maxG / (n-1))^2 / 4 pi)
+    with V in Ha bohr^3 and maxG in 1/bohr."""
+    v = np.asarray(values_ev_a3, dtype=np.float64) / (BOHR ** 3 * HARTREE_TO_EV)
+    mg = maxg_invA * BOHR
+    return int(round(-(v[1] - v[0]) * (mg / (v.size - 1)) ** 2 / (4.0 * math.pi)))
+
+
+def system_from_files(ion_path, grid, pseudo_dir=None, kinetic=4, precision=0, spline_order=10, ewald=True):
+    """An OFDSystem of a PROFESS .ion file on `grid` (n1, n2, n3): geometry and
+    species from the file, each species' .recpot from its SPECIES_POT entry
+    (relative to pseudo_dir, default the .ion file's folder), electron count
+    from the tables' charges, ionic potential built on the device and, with
+    ewald, the ion-ion energy from the Ewald sum. Returns (system, geometry)."""
+    import os
+    from .calpot import EwaldEnergy, OFDSystem
+    g = read_geometry(ion_path)
+    base = pseudo_dir if pseudo_dir is not None else os.path.dirname(os.path.abspath(str(ion_path)))
+    species, ne = [], 0.0
+    for k, fname in enumerate(g["pseudo_files"]):
+        mg, vals = read_recpot_file(os.path.join(base, fname))
+        lo, hi = g["type_start"][k], g["type_start"][k + 1]
+        species.append((mg, vals, g["frac"][lo:hi]))
+        ne += recpot_charge(mg, vals) * (hi - lo)
+    s = OFDSystem(g["lattice"], tuple(grid), ne, kinetic=kinetic, precision=precision)
+    s.build_ion_potential(species, spline_order=spline_order)
+    if ewald:
+        EwaldEnergy(s)
+    return s, g

@@ -80,3 +80,17 @@ def test_ewald_two_species_triclinic(pa):
     assert info["eta"] == p.eta
     assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
     assert np.max(np.abs(f - f_ref)) <= 1e-10 * np.max(np.abs(f_ref))
+
+
+def test_system_from_reference_files(pa):
+    """The reference's own input files (test/optIon/displaced.ion and its
+    al_HC.lda.recpot) through the library's readers, the device ionic
+    potential and the Ewald sum: the uniform-density energy of
+    displaced.out:92, ion-ion term included."""
+    _say("system from files")
+    from profess_amd import inputs
+    s, g = inputs.system_from_files(os.path.join(GOLDEN, "displaced.ion"), (16, 16, 16))
+    rho = np.full(s.shape, 12.0 / s.vol)
+    _, et = pa.CalculatePotentialPlus(s, rho)
+    assert abs(et[0] * O.HARTREE_TO_EV - S.OPTION_STEP0_EV) <= 1e-10 * abs(S.OPTION_STEP0_EV)
+    assert abs(et[5] * O.HARTREE_TO_EV - S.OPTION_EII_EV) <= 1e-10 * abs(S.OPTION_EII_EV)
