@@ -57,7 +57,10 @@ typedef struct {
  * Newton step's |p|^2 and the next |dL/dphi|^2, |phi|^2 from sums the
  * evaluation already produces and from the CG recurrences (one host round
  * trip per inner iteration instead of three; direct products whenever the
- * recurrence would lose accuracy). */
+ * recurrence would lose accuracy).
+ * cg_alg: the NCG formula of dir_type 2, as the input keyword DENC
+ * (ReadInputFile.f90:517-533): 1 Polak-Ribiere, 2 (or 0, the default)
+ * Hager-Zhang (RhoDirCG.f90:26). */
 typedef struct {
   int max_iter;
   double pot_tol;
@@ -66,6 +69,7 @@ typedef struct {
   int max_line;
   double mixed_switch;
   int fused_dots;
+  int cg_alg;
 } ofd_opt_params;
 
 typedef struct {
@@ -142,7 +146,8 @@ int ofd_calpot_many(ofd_ctx* const* ctxs, int nb, const double* const* rho_devic
                     int opt_sqrt, double* etables);
 
 /* ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60-850) with
- * dir_type = 1 (NTN) and numSpin = 1; rhoR in/out (SYS::rhoR), energy9 out,
+ * dir_type = 1 (NTN, RhoDirNew.f90) or 2 (NCG, RhoDirCG.f90) and
+ * numSpin = 1; rhoR in/out (SYS::rhoR), energy9 out,
  * rho_outcome as RhoOptimizers::rhoOutcome (0 converged, 1 failed). */
 int ofd_rho_opt(ofd_ctx* ctx, double* rhoR, int dir_type, const ofd_opt_params* params, double* energy9,
                 int* rho_outcome, ofd_opt_stats* stats);

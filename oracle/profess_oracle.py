@@ -1213,16 +1213,39 @@ def newton_direction(phi, b, mu0, evaluate, dV):
     return direction, it, flag, nevals
 
 
-def rho_minimization(rho_init, evaluate_full, evaluate_pot, dV, params: OptParams = OptParams()):
-    """ProjNormRhoMinimization with dir_type = 1 (NTN), numSpin = 1
-    (RhoOptN.f90:60-850, DensityOptimization :550-641, CheckExit :853-977).
-    evaluate_full(rho) -> (dE/dphi, eTable); evaluate_pot(rho) -> dE/dphi."""
+def cg_direction(it, dldphi, dldphi_old, dir_old, dV, cg_alg="HZ"):
+    """CGDirection (RhoDirCG.f90:32-106), numSpin = 1, with InnerProd =
+    SUM(a b) dV (:109-142): restart every 100 steps, Polak-Ribiere or
+    Hager-Zhang gamma. Returns (direction, new dL/dphi_old)."""
+    ip = lambda a, b: float(np.sum(a * b)) * dV   # noqa: E731
+    if it == 1 or it % 100 == 1:
+        gam = 0.0
+    elif cg_alg[:2] == "PR":
+        gam = max(0.0, ip(dldphi, dldphi - dldphi_old) / ip(dldphi_old, dldphi_old))
+    else:
+        y = dldphi - dldphi_old
+        dty = ip(dir_old, y)
+        norm_y = math.sqrt(ip(y, y) / dV)
+        gam = 1.0 / dty * ip(y - 2.0 * dir_old * norm_y ** 2 / dty, dldphi)
+        norm_dk = math.sqrt(ip(dir_old, dir_old) / dV)
+        norm_gk = math.sqrt(ip(dldphi_old, dldphi_old) / dV)
+        gam = max(gam, -1.0 / (norm_dk * min(0.01, norm_gk)))
+    return -dldphi + gam * dir_old, dldphi.copy()
+
+
+def rho_minimization(rho_init, evaluate_full, evaluate_pot, dV, params: OptParams = OptParams(), dir_type=1,
+                     cg_alg="HZ"):
+    """ProjNormRhoMinimization with dir_type = 1 (NTN) or 2 (NCG), numSpin = 1
+    (RhoOptN.f90:60-850, DensityOptimization :550-641, CheckExit :853-977,
+    getNextDirection :655-687). evaluate_full(rho) -> (dE/dphi, eTable);
+    evaluate_pot(rho) -> dE/dphi."""
     nr = rho_init.size
     num_points = float(nr)
-    # ftol = 1e-4, gtol = 2e-1 are default-real (single precision) literals
-    # assigned to DP variables (RhoOptN.f90:255-258); xtol is 1.E-12_DP (:193)
+    # ftol = 1e-4, gtol = 2e-1 (NTN) or 1e-2 (NCG) are default-real (single
+    # precision) literals assigned to DP variables (RhoOptN.f90:255-264);
+    # xtol is 1.E-12_DP (:193)
     ftol = float(np.float32(1e-4))
-    gtol = float(np.float32(2e-1))
+    gtol = float(np.float32(2e-1 if dir_type == 1 else 1e-2))
     xtol = 1e-12
     tot_ele = np.sum(rho_init) * dV
     theta = 0.2
@@ -1239,11 +1262,17 @@ def rho_minimization(rho_init, evaluate_full, evaluate_pot, dV, params: OptParam
     success = 1
     it = 1
     direction = np.zeros_like(phi)
+    dir_old = np.zeros_like(phi)
+    dldphi_old = np.zeros_like(phi)
     temp_phi = phi
+    newton_iter, ntflag = 0, 0
     while True:
         switch_stp = False
-        direction, newton_iter, ntflag, ne = newton_direction(phi, dldphi, mu0, evaluate_pot, dV)
-        nevals += ne
+        if dir_type == 1:
+            direction, newton_iter, ntflag, ne = newton_direction(phi, dldphi, mu0, evaluate_pot, dV)
+            nevals += ne
+        else:
+            direction, dldphi_old = cg_direction(it, dldphi, dldphi_old, dir_old, dV, cg_alg)
         while True:
             # getNextGradient (RhoLineSearch.f90:40-139), numSpin = 1
             pd = phi + direction
@@ -1264,6 +1293,7 @@ def rho_minimization(rho_init, evaluate_full, evaluate_pot, dV, params: OptParam
                 direction = -dldphi
                 continue
             break
+        dir_old = direction.copy()     # dirNext_old = dirNext (RhoOptN.f90:566)
         if outcome == 1:
             break
         # LineSearchTN (RhoLineSearch.f90:312-444)

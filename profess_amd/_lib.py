@@ -29,7 +29,8 @@ class ofd_functional(C.Structure):
 
 class ofd_opt_params(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("pot_tol", C.c_double), ("tole", C.c_double), ("conv_check", C.c_int),
-                ("max_line", C.c_int), ("mixed_switch", C.c_double), ("fused_dots", C.c_int)]
+                ("max_line", C.c_int), ("mixed_switch", C.c_double), ("fused_dots", C.c_int),
+                ("cg_alg", C.c_int)]
 
 
 class ofd_tune_result(C.Structure):

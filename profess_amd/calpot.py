@@ -149,13 +149,14 @@ def CalculatePotentialPlusDevice(system: OFDSystem, rho_ptr, pot_ptr, optSqrt=Tr
 
 
 def opt_params(max_iter=-1, pot_tol=-1.0, tole=-1.0, conv_check="ENE", max_line=-1, mixed_switch=0.0,
-               fused_dots=False):
+               fused_dots=False, cg_alg="HZ"):
     return ofd_opt_params(int(max_iter), float(pot_tol), float(tole), CONV[conv_check.upper()], int(max_line),
-                          float(mixed_switch), int(bool(fused_dots)))
+                          float(mixed_switch), int(bool(fused_dots)), {"PR": 1, "HZ": 2}[cg_alg.upper()[:2]])
 
 
 def ProjNormRhoMinimization(system: OFDSystem, rhoR, dir_type=1, **kw):
-    """ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60) with NTN.
+    """ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60) with
+    dir_type 1 (NTN) or 2 (NCG; cg_alg="HZ" or "PR" as DENC 2 or 1).
     Returns (rho, energy(9), rhoOutcome, stats dict)."""
     r = np.array(rhoR, dtype=np.float64, order="C", copy=True)
     et = np.zeros(9)

@@ -722,7 +722,7 @@ constexpr int kFp32MaxLine = 20;
 
 enum SState {
   ST_INIT_EVAL, ST_INIT_DOTS, ST_NEWTON_PP, ST_NEWTON_EVAL, ST_NEWTON_RR, ST_LP1, ST_LP2, ST_LP3, ST_LINE_EVAL,
-  ST_ACC_DOTS, ST_DONE
+  ST_ACC_DOTS, ST_CG_DOTS1, ST_CG_DOTS2, ST_DONE
 };
 
 struct NtnSolver {
@@ -739,6 +739,12 @@ struct NtnSolver {
   // NewtonDirection (RhoDirNew.f90:54-75)
   int nit = 0, ntflag = 0;
   double rr = 0, rr_last = 0, criteria = 0, ry = 0, ry_last = 0, ss1 = 0, eps = 0;
+  // direction type (RhoOptN.f90:655-687): 1 NTN, 2 NCG; NCG keeps dL/dphi_old
+  // in r and the previous direction in p (the Newton-only vectors)
+  int dir_type = 1;
+  bool cg_pr = false;          // cg_alg 'PR' (default 'HZ', RhoDirCG.f90:26)
+  bool cg_restart = false;
+  double cg_dots[6] = {0};
   // fused_dots: |p|^2, r.p and r_new.p_old carried by the CG recurrences
   bool fused = false;
   double pp = 0, rp = 0, rp_after = 0;
@@ -772,12 +778,16 @@ struct NtnSolver {
     ops.push_back(VecOp{kOpCG, c->dir, nullptr, nullptr, a, b, c->p, c->r, c->ap});
   }
 
-  void need_dots(int np, const double* a0, const double* b0, const double* a1 = nullptr, const double* b1 = nullptr) {
+  void need_dots(int np, const double* a0, const double* b0, const double* a1 = nullptr, const double* b1 = nullptr,
+                 const double* a2 = nullptr, const double* b2 = nullptr, const double* a3 = nullptr,
+                 const double* b3 = nullptr) {
     need = Need();
     need.kind = NEED_DOTS;
     need.np = np;
     need.a[0] = a0; need.b[0] = b0;
     need.a[1] = a1; need.b[1] = b1;
+    need.a[2] = a2; need.b[2] = b2;
+    need.a[3] = a3; need.b[3] = b3;
   }
   void need_eval(const EvalReq& q) {
     need = Need();
@@ -796,6 +806,7 @@ struct NtnSolver {
   void fall_back_fp64() {
     force64 = true;
     fp32 = false;
+    cg_restart = true;
     mach = 2.220446049250313e-16;
     e_hist[0] = e_hist[1] = e_hist[2] = 0.0;
     success = 1;
@@ -820,6 +831,37 @@ struct NtnSolver {
     need_eval(EvalReq{kOutSigned, c->phi, nullptr, nullptr, c->dedphi, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, true});
   }
 
+  // getNextDirection (RhoOptN.f90:655-687)
+  void direction_begin(double rr0, double s1) {
+    if (dir_type == 2) cg_begin();
+    else newton_begin(rr0, s1);
+  }
+  // CGDirection (RhoDirCG.f90:32-106), InnerProd = SUM(a b) dV (:109-142)
+  void cg_begin() {
+    // restart on the reference's schedule, and after leaving an FP32 phase
+    // (dL_old and the old direction belong to the FP32 iterates)
+    if (it == 1 || it % 100 == 1 || cg_restart) {
+      cg_restart = false;
+      cg_finish(0.0);
+      return;
+    }
+    st = ST_CG_DOTS1;
+    if (cg_pr) {
+      lincomb(c->ap, c->dldphi, c->r, 1.0, -1.0);   // dL - dL_old
+      need_dots(2, c->dldphi, c->ap, c->r, c->r);
+      return;
+    }
+    lincomb(c->ap, c->dldphi, c->r, 1.0, -1.0);     // y = dL - dL_old
+    need_dots(4, c->p, c->ap, c->ap, c->ap, c->ap, c->dldphi, c->p, c->dldphi);
+  }
+  void cg_finish(double gam) {
+    // dirNext = -dL + gamma dirNext_old; a restart never reads dirNext_old,
+    // which holds nothing defined before the first line search
+    lincomb(c->dir, c->dldphi, gam == 0.0 ? c->dldphi : c->p, -1.0, gam);
+    lincomb(c->r, c->dldphi, c->dldphi, 1.0, 0.0);   // dL_old = dL
+    switched = false;
+    line_prep1();
+  }
   // NewtonDirection start (RhoDirNew.f90:95-115); rr0 = sum r^2 = sum b^2, s1 = sum phi^2
   void newton_begin(double rr0, double s1) {
     lincomb(c->r, c->dldphi, c->dldphi, -1.0, 0.0);   // r = -b
@@ -905,7 +947,7 @@ struct NtnSolver {
   void line_start(double gradth) {
     // LineSearchTN start (RhoLineSearch.f90:369-374); ftol/gtol are
     // default-real literals (RhoOptN.f90:255-258)
-    const float ftol32 = 1e-4f, gtol32 = 2e-1f;
+    const float ftol32 = 1e-4f, gtol32 = dir_type == 2 ? 1e-2f : 2e-1f;
     ls = LineSearch();
     ls.ftol = (double)ftol32;
     ls.gtol = (double)gtol32;
@@ -1039,7 +1081,7 @@ struct NtnSolver {
         hist_potnorm.push_back(pot_norm);
         e_hist[0] = energy[0];
         set_precision_for(pot_norm);
-        newton_begin(v[0], v[1]);
+        direction_begin(v[0], v[1]);
         return;
       }
       case ST_NEWTON_PP:
@@ -1121,6 +1163,7 @@ struct NtnSolver {
           line_prep1();
           return;
         }
+        if (dir_type == 2) lincomb(c->p, c->dir, c->dir, 1.0, 0.0);   // dirNext_old = dirNext (RhoOptN.f90:566)
         line_start(gradth);
         return;
       }
@@ -1142,8 +1185,32 @@ struct NtnSolver {
         return;
       }
       case ST_ACC_DOTS:
-        newton_begin(v[0], v[1]);
+        direction_begin(v[0], v[1]);
         return;
+      case ST_CG_DOTS1: {
+        if (cg_pr) {
+          // gamma = MAX(0, <dL, dL - dL_old> / <dL_old, dL_old>)
+          cg_finish(std::max(0.0, (v[0] * c->dV) / (v[1] * c->dV)));
+          return;
+        }
+        for (int k = 0; k < 4; ++k) cg_dots[k] = v[k];   // d.y, y.y, y.dL, d.dL
+        st = ST_CG_DOTS2;
+        need_dots(2, c->p, c->p, c->r, c->r);
+        return;
+      }
+      case ST_CG_DOTS2: {
+        // Hager-Zhang (RhoDirCG.f90:79-88)
+        const double dV = c->dV;
+        const double dty = cg_dots[0] * dV;
+        const double norm_y = std::sqrt(cg_dots[1] * dV / dV);
+        // <y - 2 d |y|^2/dTy, dL> = <y, dL> - 2 |y|^2/dTy <d, dL>
+        double gam = 1.0 / dty * (cg_dots[2] * dV - 2.0 * norm_y * norm_y / dty * (cg_dots[3] * dV));
+        const double norm_dk = std::sqrt(v[0] * dV / dV);
+        const double norm_gk = std::sqrt(v[1] * dV / dV);
+        gam = std::max(gam, -1.0 / (norm_dk * std::min(0.01, norm_gk)));
+        cg_finish(gam);
+        return;
+      }
       default:
         return;
     }
@@ -1315,9 +1382,12 @@ double dsum(const double* x, long long n, BatchScratch& bs, hipStream_t st) {
 
 // Prepares phi = sqrt|rhoR| and totEleNum = sum(rhoR) dV (RhoOptN.f90:326-352)
 // from a density already on the device in c->rho.
-void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchScratch& bs, hipStream_t st) {
+void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchScratch& bs, hipStream_t st,
+                    int dir_type = 1) {
   S = NtnSolver();
   S.c = c;
+  S.dir_type = dir_type;
+  S.cg_pr = prm->cg_alg == 1;
   if (const char* e = std::getenv("OFD_TRACE")) S.trace = std::atoi(e) != 0;
   fill_params(prm, &S.prm);
   S.fused = S.prm.fused_dots != 0;
@@ -1332,7 +1402,7 @@ void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchSc
 extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt_params* prm, double* energy9,
                            int* rho_outcome, ofd_opt_stats* stats) {
   if (!c || !rhoR) return fail(c, "null argument");
-  if (dir_type != 1) return fail(c, "only dir_type 1 (NTN) is implemented");
+  if (dir_type != 1 && dir_type != 2) return fail(c, "dir_type 1 (NTN) or 2 (NCG)");
   if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set");
   auto t0 = std::chrono::steady_clock::now();
   BatchScratch bs;
@@ -1341,7 +1411,7 @@ extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt
     const long long n = c->nr();
     HIPCHK(hipMemcpyAsync(c->rho, rhoR, n * sizeof(double), hipMemcpyHostToDevice, c->st));
     NtnSolver S;
-    solver_prepare(S, c, prm, bs, c->st);
+    solver_prepare(S, c, prm, bs, c->st, dir_type);
     std::vector<NtnSolver*> one{&S};
     double s[kNSum];
     while (S.need.kind != NEED_NONE) {

@@ -32,6 +32,7 @@ MODULE ofd_c
     INTEGER(C_INT) :: conv_check, max_line
     REAL(C_DOUBLE) :: mixed_switch
     INTEGER(C_INT) :: fused_dots
+    INTEGER(C_INT) :: cg_alg
   END TYPE ofd_opt_params_t
 
   TYPE, BIND(C) :: ofd_opt_stats_t
@@ -151,6 +152,8 @@ MODULE OFD_Sys
   REAL(KIND=8), SAVE :: tole = 1.E-6
   CHARACTER(LEN=500), SAVE :: conv_check = "ENE"
   INTEGER, SAVE :: numEnergyTotal = 0, numPotentialTotal = 0
+  ! RhoDirCG::cg_alg (RhoDirCG.f90:26), 'PR' or 'HZ' (set by DENC)
+  CHARACTER(LEN=500), SAVE :: cg_alg = 'HZ'
 END MODULE OFD_Sys
 
 
@@ -195,7 +198,8 @@ MODULE RhoOptN
   IMPLICIT NONE
 CONTAINS
   ! ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60): optimizes
-  ! OFD_Sys::rhoR in place with the truncated-Newton method (dir_type = 1).
+  ! OFD_Sys::rhoR in place with the truncated-Newton method (dir_type = 1)
+  ! or nonlinear CG (dir_type = 2, formula from cg_alg).
   SUBROUTINE ProjNormRhoMinimization(energy, dir_type)
     REAL(KIND=8), INTENT(OUT) :: energy(:)
     INTEGER, INTENT(INOUT) :: dir_type
@@ -215,6 +219,8 @@ CONTAINS
     prm%max_line = 200
     prm%mixed_switch = 0.d0
     prm%fused_dots = 0
+    prm%cg_alg = 2
+    IF (cg_alg(1:2) == 'PR') prm%cg_alg = 1
     st = ofd_rho_opt(ofd_context, C_LOC(r), INT(dir_type, C_INT), prm, C_LOC(et), oc, stt)
     CALL ofd_check(ofd_context, st, 'ProjNormRhoMinimization')
     rhoR(:,:,:,1) = r

@@ -186,6 +186,35 @@ def test_ntn_optIon_sample_and_oracle(pa):
     assert abs(np.sum(rho) * s.dV - 12.0) < 1e-9
 
 
+@pytest.mark.parametrize("alg,tol", [("HZ", 1e-6), ("PR", 1e-3)])
+def test_ncg_optIon_matches_oracle(pa, alg, tol):
+    """dir_type 2 (NCG, RhoDirCG.f90) on the test/optIon input: the same
+    iteration and evaluation counts as the oracle's NCG on the same ion
+    potential, the same density and energy; HZ run to 1e-6 also reaches the
+    NTN minimum. PR converges slowly here (Dcsrch stalls below about 1e-5)
+    and over hundreds of iterations its recurrence amplifies last-bit
+    differences into a different iteration count (201 against 204 at 1e-5,
+    both converged), so it is compared over its first 63 iterations."""
+    _say(f"ncg optIon {alg}")
+    s = _option_gpu(pa)
+    rho0 = np.full(s.shape, 12.0 / s.vol)
+    rho, et, outcome, st = pa.ProjNormRhoMinimization(s, rho0, dir_type=2, conv_check="POT", pot_tol=tol,
+                                                      cg_alg=alg)
+    assert outcome == 0
+    cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
+    f = O.Functional(kinetic=4).resolve(12.0, cell.vol)
+    sysm = O.System(cell, f, s.ion_potential(), S.OPTION_EII_EV / O.HARTREE_TO_EV)
+    full, pot = O.make_evaluators(sysm)
+    res = O.rho_minimization(rho0, full, pot, cell.dV, O.OptParams(conv_check="POT", pot_tol=tol), dir_type=2,
+                             cg_alg=alg)
+    assert st["iterations"] == res.iters and st["num_evals"] == res.num_evals
+    assert abs(res.energy[0] - et[0]) * O.HARTREE_TO_EV / 4 < 1e-9
+    assert np.max(np.abs(rho - res.rho)) < 1e-7 * np.max(res.rho)
+    if alg == "HZ":
+        assert abs(et[0] * O.HARTREE_TO_EV - S.OPTION_FINAL_EV) / 4 < 1e-6
+    assert abs(np.sum(rho) * s.dV - 12.0) < 1e-9
+
+
 def test_ntn_optDen_wgc(pa):
     if not R.available():
         pytest.skip("reference RKSUITE table builder not built")
