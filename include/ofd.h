@@ -146,8 +146,9 @@ int ofd_calpot_many(ofd_ctx* const* ctxs, int nb, const double* const* rho_devic
                     int opt_sqrt, double* etables);
 
 /* ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60-850) with
- * dir_type = 1 (NTN, RhoDirNew.f90) or 2 (NCG, RhoDirCG.f90) and
- * numSpin = 1; rhoR in/out (SYS::rhoR), energy9 out,
+ * dir_type = 1 (NTN, RhoDirNew.f90), 2 (NCG, RhoDirCG.f90) or 3 (NBF,
+ * RhoDirBFGS.f90, 5 stored pairs) and numSpin = 1; rhoR in/out (SYS::rhoR),
+ * energy9 out,
  * rho_outcome as RhoOptimizers::rhoOutcome (0 converged, 1 failed). */
 int ofd_rho_opt(ofd_ctx* ctx, double* rhoR, int dir_type, const ofd_opt_params* params, double* energy9,
                 int* rho_outcome, ofd_opt_stats* stats);

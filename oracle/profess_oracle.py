@@ -1233,19 +1233,73 @@ def cg_direction(it, dldphi, dldphi_old, dir_old, dV, cg_alg="HZ"):
     return -dldphi + gam * dir_old, dldphi.copy()
 
 
+class BfgsMemory:
+    """The RhoDirBFGS module state (RhoDirBFGS.f90:26-42) after InitializeBFGS
+    (:45-92): MBFGS = 5 (s, y) pairs in a ring, the write pointer BFGSPT and
+    the call counter BFGSiter; numSpin = 1. Slots are 0-based here (slot k is
+    the reference's index k+1)."""
+
+    def __init__(self, shape, m=5):
+        self.m = m
+        self.s = np.zeros((m,) + tuple(shape))
+        self.y = np.zeros((m,) + tuple(shape))
+        self.rho = np.zeros(m)
+        self.alpha = np.zeros(m)
+        self.pt = 0
+        self.iter = 0
+
+    def direction(self, dldphi, dV):
+        """BFGSDirection (RhoDirBFGS.f90:120-218): -dL/dphi on the first call,
+        else the two-loop recursion (Nocedal, Math. Comp. 35 (1980) 773-782)
+        over the newest min(iter-1, MBFGS) pairs with H0 = (y.s / y.y) I from
+        the newest pair; InnerProd carries dV."""
+        ip = lambda a, b: float(np.sum(a * b)) * dV   # noqa: E731
+        m = self.m
+        self.iter += 1
+        if self.iter == 1:
+            return -dldphi
+        bound = min(self.iter - 1, m)
+        npt = (self.pt - 1) % m                        # the newest pair
+        ys = ip(self.s[npt], self.y[npt])
+        yy = ip(self.y[npt], self.y[npt])
+        diag = ys / yy
+        self.rho[npt] = 1.0 / ys
+        q = -dldphi
+        cp = self.pt
+        for _ in range(bound):                         # newest to oldest
+            cp = (cp - 1) % m
+            self.alpha[cp] = self.rho[cp] * ip(self.s[cp], q)
+            q = q - self.alpha[cp] * self.y[cp]
+        q = q * diag
+        for _ in range(bound):                         # oldest to newest
+            beta = self.alpha[cp] - self.rho[cp] * ip(self.y[cp], q)
+            q = q + beta * self.s[cp]
+            cp = (cp + 1) % m
+        return q
+
+    def update(self, phi, temp_phi, dedphi, dldphi, tot_ele, dV):
+        """UpdateBFGS (RhoDirBFGS.f90:221-290) with ChemicalPotential (:294-343):
+        s = tempPhi - phi, y = (dE/dphi - 2 mu(tempPhi) tempPhi) - dL/dphi."""
+        mu = dV * np.sum(dedphi * temp_phi) / (2.0 * tot_ele)
+        self.s[self.pt] = temp_phi - phi
+        self.y[self.pt] = (dedphi - 2.0 * mu * temp_phi) - dldphi
+        self.pt = (self.pt + 1) % self.m
+
+
 def rho_minimization(rho_init, evaluate_full, evaluate_pot, dV, params: OptParams = OptParams(), dir_type=1,
                      cg_alg="HZ"):
-    """ProjNormRhoMinimization with dir_type = 1 (NTN) or 2 (NCG), numSpin = 1
-    (RhoOptN.f90:60-850, DensityOptimization :550-641, CheckExit :853-977,
-    getNextDirection :655-687). evaluate_full(rho) -> (dE/dphi, eTable);
-    evaluate_pot(rho) -> dE/dphi."""
+    """ProjNormRhoMinimization with dir_type = 1 (NTN), 2 (NCG) or 3 (NBF),
+    numSpin = 1 (RhoOptN.f90:60-850, DensityOptimization :550-641, CheckExit
+    :853-977, getNextDirection :655-687). evaluate_full(rho) -> (dE/dphi,
+    eTable); evaluate_pot(rho) -> dE/dphi."""
     nr = rho_init.size
     num_points = float(nr)
-    # ftol = 1e-4, gtol = 2e-1 (NTN) or 1e-2 (NCG) are default-real (single
-    # precision) literals assigned to DP variables (RhoOptN.f90:255-264);
-    # xtol is 1.E-12_DP (:193)
+    # ftol = 1e-4, gtol = 2e-1 (NTN, NBF) or 1e-2 (NCG) are default-real
+    # (single precision) literals assigned to DP variables (RhoOptN.f90:255-
+    # 264); xtol is 1.E-12_DP (:193)
     ftol = float(np.float32(1e-4))
-    gtol = float(np.float32(2e-1 if dir_type == 1 else 1e-2))
+    gtol = float(np.float32(1e-2 if dir_type == 2 else 2e-1))
+    bfgs = BfgsMemory(rho_init.shape) if dir_type == 3 else None
     xtol = 1e-12
     tot_ele = np.sum(rho_init) * dV
     theta = 0.2
@@ -1271,8 +1325,10 @@ def rho_minimization(rho_init, evaluate_full, evaluate_pot, dV, params: OptParam
         if dir_type == 1:
             direction, newton_iter, ntflag, ne = newton_direction(phi, dldphi, mu0, evaluate_pot, dV)
             nevals += ne
-        else:
+        elif dir_type == 2:
             direction, dldphi_old = cg_direction(it, dldphi, dldphi_old, dir_old, dV, cg_alg)
+        else:
+            direction = bfgs.direction(dldphi, dV)
         while True:
             # getNextGradient (RhoLineSearch.f90:40-139), numSpin = 1
             pd = phi + direction
@@ -1344,6 +1400,8 @@ def rho_minimization(rho_init, evaluate_full, evaluate_pot, dV, params: OptParam
             exit_flag = True
         if it >= params.max_iter:
             outcome, exit_flag = 1, True
+        if dir_type == 3:
+            bfgs.update(phi, temp_phi, dedphi, dldphi, tot_ele, dV)   # RhoOptN.f90:603-610
         phi = temp_phi
         mu0 = dV * np.sum(dedphi * phi) / (2.0 * tot_ele)
         dldphi = dedphi - 2.0 * mu0 * phi

@@ -156,7 +156,7 @@ def opt_params(max_iter=-1, pot_tol=-1.0, tole=-1.0, conv_check="ENE", max_line=
 
 def ProjNormRhoMinimization(system: OFDSystem, rhoR, dir_type=1, **kw):
     """ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60) with
-    dir_type 1 (NTN) or 2 (NCG; cg_alg="HZ" or "PR" as DENC 2 or 1).
+    dir_type 1 (NTN), 2 (NCG; cg_alg="HZ" or "PR" as DENC 2 or 1) or 3 (NBF).
     Returns (rho, energy(9), rhoOutcome, stats dict)."""
     r = np.array(rhoR, dtype=np.float64, order="C", copy=True)
     et = np.zeros(9)

@@ -347,6 +347,11 @@ struct ofd_ctx {
   // optimizer vectors
   double *phi = nullptr, *dir = nullptr, *dedphi = nullptr, *dldphi = nullptr, *r = nullptr, *p = nullptr,
          *ap = nullptr, *tmp = nullptr;
+  // NBF memory: the (s, y) ring of RhoDirBFGS (BFGSs, BFGSy, MBFGS = 5 pairs),
+  // allocated on the first dir_type 3 solve
+  static constexpr int kBfgsM = 5;
+  double* bfgs_s[kBfgsM] = {};
+  double* bfgs_y[kBfgsM] = {};
   double* dot_part = nullptr;
   double* dot_out = nullptr;
   // ion data kept for forces
@@ -504,6 +509,11 @@ static void free_opt(ofd_ctx* c) {
   for (auto pp : v) {
     dev_free(*pp);
     *pp = nullptr;
+  }
+  for (int k = 0; k < ofd_ctx::kBfgsM; ++k) {
+    dev_free(c->bfgs_s[k]);
+    dev_free(c->bfgs_y[k]);
+    c->bfgs_s[k] = c->bfgs_y[k] = nullptr;
   }
 }
 
@@ -722,7 +732,7 @@ constexpr int kFp32MaxLine = 20;
 
 enum SState {
   ST_INIT_EVAL, ST_INIT_DOTS, ST_NEWTON_PP, ST_NEWTON_EVAL, ST_NEWTON_RR, ST_LP1, ST_LP2, ST_LP3, ST_LINE_EVAL,
-  ST_ACC_DOTS, ST_CG_DOTS1, ST_CG_DOTS2, ST_DONE
+  ST_ACC_DOTS, ST_CG_DOTS1, ST_CG_DOTS2, ST_BFGS_Q0, ST_BFGS_Q, ST_BFGS_R, ST_DONE
 };
 
 struct NtnSolver {
@@ -744,6 +754,11 @@ struct NtnSolver {
   int dir_type = 1;
   bool cg_pr = false;          // cg_alg 'PR' (default 'HZ', RhoDirCG.f90:26)
   bool cg_restart = false;
+  // NBF (RhoDirBFGS.f90:26-42): ring pointer BFGSPT, call counter BFGSiter,
+  // BFGSrho, BFGSalpha; the two-loop position (slot, step, bound) and H0
+  static constexpr int kM = ofd_ctx::kBfgsM;
+  int bfgs_pt = 0, bfgs_iter = 0, bfgs_cp = 0, bfgs_k = 0, bfgs_bound = 0;
+  double bfgs_rho[kM] = {0}, bfgs_alpha[kM] = {0}, bfgs_diag = 0.0;
   double cg_dots[6] = {0};
   // fused_dots: |p|^2, r.p and r_new.p_old carried by the CG recurrences
   bool fused = false;
@@ -807,6 +822,7 @@ struct NtnSolver {
     force64 = true;
     fp32 = false;
     cg_restart = true;
+    bfgs_iter = bfgs_pt = 0;   // the FP32 pairs do not describe the FP64 surface
     mach = 2.220446049250313e-16;
     e_hist[0] = e_hist[1] = e_hist[2] = 0.0;
     success = 1;
@@ -834,7 +850,54 @@ struct NtnSolver {
   // getNextDirection (RhoOptN.f90:655-687)
   void direction_begin(double rr0, double s1) {
     if (dir_type == 2) cg_begin();
+    else if (dir_type == 3) bfgs_begin();
     else newton_begin(rr0, s1);
+  }
+  // BFGSDirection (RhoDirBFGS.f90:120-218): -dL/dphi on the first call, else
+  // the two-loop recursion over the newest min(BFGSiter-1, MBFGS) pairs, built
+  // in place in dir; H0 = (y.s / y.y) I from the newest pair; InnerProd has dV
+  void bfgs_begin() {
+    ++bfgs_iter;
+    lincomb(c->dir, c->dldphi, c->dldphi, -1.0, 0.0);   // q = -dL/dphi
+    if (bfgs_iter == 1) {
+      switched = false;
+      line_prep1();
+      return;
+    }
+    bfgs_bound = std::min(bfgs_iter - 1, kM);
+    bfgs_cp = (bfgs_pt + kM - 1) % kM;   // the newest pair
+    bfgs_k = 0;
+    st = ST_BFGS_Q0;
+    need_dots(3, c->bfgs_s[bfgs_cp], c->bfgs_y[bfgs_cp], c->bfgs_y[bfgs_cp], c->bfgs_y[bfgs_cp],
+              c->bfgs_s[bfgs_cp], c->dir);
+  }
+  // first loop, newest to oldest: alpha = rho s.q, q -= alpha y
+  void bfgs_q_step(double sq) {
+    bfgs_alpha[bfgs_cp] = bfgs_rho[bfgs_cp] * (sq * c->dV);
+    axpby(c->dir, c->bfgs_y[bfgs_cp], -bfgs_alpha[bfgs_cp], 1.0);
+    ++bfgs_k;
+    if (bfgs_k < bfgs_bound) {
+      bfgs_cp = (bfgs_cp + kM - 1) % kM;
+      st = ST_BFGS_Q;
+      need_dots(1, c->bfgs_s[bfgs_cp], c->dir);
+      return;
+    }
+    // r0 = H0 q0, then the second loop from the oldest pair
+    axpby(c->dir, c->dir, 0.0, bfgs_diag);
+    bfgs_k = 0;
+    st = ST_BFGS_R;
+    need_dots(1, c->bfgs_y[bfgs_cp], c->dir);
+  }
+  // UpdateBFGS (RhoDirBFGS.f90:221-290) with phi_new in c->phi and phi_old in
+  // c->tmp: s = phi_new - phi_old; y = -dL/dphi_old here, += dL/dphi_new in
+  // bfgs_update_y once dL/dphi is recomputed with the new mu
+  void bfgs_update_s() {
+    lincomb(c->bfgs_s[bfgs_pt], c->phi, c->tmp, 1.0, -1.0);
+    lincomb(c->bfgs_y[bfgs_pt], c->dldphi, c->dldphi, -1.0, 0.0);
+  }
+  void bfgs_update_y() {
+    axpby(c->bfgs_y[bfgs_pt], c->dldphi, 1.0, 1.0);
+    bfgs_pt = (bfgs_pt + 1) % kM;
   }
   // CGDirection (RhoDirCG.f90:32-106), InnerProd = SUM(a b) dV (:109-142)
   void cg_begin() {
@@ -1011,8 +1074,11 @@ struct NtnSolver {
     const double ct = std::cos(theta), sn = std::sin(theta);
     lincomb(c->tmp, c->phi, c->dir, ct, sn);
     std::swap(c->phi, c->tmp);
+    const bool bfgs_pair = dir_type == 3 && !exit_flag;   // RhoOptN.f90:603-610
+    if (bfgs_pair) bfgs_update_s();
     mu0 = c->dV * last_sums[8] / (2.0 * tot_ele);
     lincomb(c->dldphi, c->dedphi, c->phi, 1.0, -2.0 * mu0);
+    if (bfgs_pair) bfgs_update_y();
     stats.iterations = it;
     if (exit_flag) {
       finish();
@@ -1040,7 +1106,7 @@ struct NtnSolver {
     st = ST_ACC_DOTS;
     double bb, ff;
     if (fused && fused_bb(last_sums, bb, ff)) {
-      newton_begin(bb, ff);
+      direction_begin(bb, ff);
       return;
     }
     need_dots(2, c->dldphi, c->dldphi, c->phi, c->phi);
@@ -1196,6 +1262,29 @@ struct NtnSolver {
         for (int k = 0; k < 4; ++k) cg_dots[k] = v[k];   // d.y, y.y, y.dL, d.dL
         st = ST_CG_DOTS2;
         need_dots(2, c->p, c->p, c->r, c->r);
+        return;
+      }
+      case ST_BFGS_Q0: {
+        const double ys = v[0] * c->dV, yy = v[1] * c->dV;
+        bfgs_diag = ys / yy;
+        bfgs_rho[bfgs_cp] = 1.0 / ys;
+        bfgs_q_step(v[2]);
+        return;
+      }
+      case ST_BFGS_Q:
+        bfgs_q_step(v[0]);
+        return;
+      case ST_BFGS_R: {
+        // beta = alpha - rho y.r; r += beta s, oldest to newest
+        const double beta = bfgs_alpha[bfgs_cp] - bfgs_rho[bfgs_cp] * (v[0] * c->dV);
+        axpby(c->dir, c->bfgs_s[bfgs_cp], beta, 1.0);
+        bfgs_cp = (bfgs_cp + 1) % kM;
+        if (++bfgs_k < bfgs_bound) {
+          need_dots(1, c->bfgs_y[bfgs_cp], c->dir);
+          return;
+        }
+        switched = false;
+        line_prep1();
         return;
       }
       case ST_CG_DOTS2: {
@@ -1388,6 +1477,13 @@ void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchSc
   S.c = c;
   S.dir_type = dir_type;
   S.cg_pr = prm->cg_alg == 1;
+  if (dir_type == 3 && !c->bfgs_s[0]) {
+    const long long n = c->nr();
+    for (int k = 0; k < ofd_ctx::kBfgsM; ++k) {
+      c->bfgs_s[k] = dalloc<double>(n);
+      c->bfgs_y[k] = dalloc<double>(n);
+    }
+  }
   if (const char* e = std::getenv("OFD_TRACE")) S.trace = std::atoi(e) != 0;
   fill_params(prm, &S.prm);
   S.fused = S.prm.fused_dots != 0;
@@ -1402,7 +1498,7 @@ void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchSc
 extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt_params* prm, double* energy9,
                            int* rho_outcome, ofd_opt_stats* stats) {
   if (!c || !rhoR) return fail(c, "null argument");
-  if (dir_type != 1 && dir_type != 2) return fail(c, "dir_type 1 (NTN) or 2 (NCG)");
+  if (dir_type < 1 || dir_type > 3) return fail(c, "dir_type 1 (NTN), 2 (NCG) or 3 (NBF)");
   if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set");
   auto t0 = std::chrono::steady_clock::now();
   BatchScratch bs;

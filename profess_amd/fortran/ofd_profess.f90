@@ -198,8 +198,8 @@ MODULE RhoOptN
   IMPLICIT NONE
 CONTAINS
   ! ProjNormRhoMinimization(energy, dir_type) (RhoOptN.f90:60): optimizes
-  ! OFD_Sys::rhoR in place with the truncated-Newton method (dir_type = 1)
-  ! or nonlinear CG (dir_type = 2, formula from cg_alg).
+  ! OFD_Sys::rhoR in place with the truncated-Newton method (dir_type = 1),
+  ! nonlinear CG (dir_type = 2, formula from cg_alg) or L-BFGS (dir_type = 3).
   SUBROUTINE ProjNormRhoMinimization(energy, dir_type)
     REAL(KIND=8), INTENT(OUT) :: energy(:)
     INTEGER, INTENT(INOUT) :: dir_type

@@ -304,8 +304,53 @@ def test_cg_direction_terminates_on_quadratic(alg, dV):
     assert np.linalg.norm(a @ x - b) < 1e-12 * np.linalg.norm(b)
 
 
+def _dense_bfgs_direction(pairs, g):
+    """-H g with H built explicitly by the BFGS inverse update
+    H <- (I - r s y^T) H (I - r y s^T) + r s s^T over the stored pairs, oldest
+    first, from H0 = (s.y / y.y) I of the newest pair (Nocedal & Wright,
+    Numerical Optimization, eq. 7.19)."""
+    s, y = pairs[-1]
+    n = g.size
+    h = np.dot(s, y) / np.dot(y, y) * np.eye(n)
+    for s, y in pairs:
+        r = 1.0 / np.dot(y, s)
+        v = np.eye(n) - r * np.outer(y, s)
+        h = v.T @ h @ v + r * np.outer(s, s)
+    return -h @ g
+
+
+def test_bfgs_two_loop_matches_dense_update():
+    """BFGSDirection's two-loop recursion over the MBFGS = 5 ring, called once
+    per step as the optimizer does (1/y.s is stored for the newest pair only,
+    RhoDirBFGS.f90:169-171), equals -H g of the dense inverse-BFGS matrix over
+    the newest five pairs, through the ring's wrap-around; dV cancels. Reversing
+    either loop, or taking H0 from the oldest pair, fails this by 20-80 %. The
+    first call returns -g; with g = y of the newest pair the result is -s
+    (secant condition)."""
+    rng = np.random.default_rng(2)
+    n = 9
+    mem = O.BfgsMemory((n,))
+    g0 = rng.standard_normal(n)
+    assert np.array_equal(mem.direction(g0, 0.37), -g0)
+    pairs = []
+    for k in range(12):
+        s = rng.standard_normal(n)
+        y = s + 0.5 * rng.standard_normal(n)
+        if np.dot(s, y) <= 0:
+            y = -y
+        mem.s[mem.pt], mem.y[mem.pt] = s, y
+        mem.pt = (mem.pt + 1) % mem.m
+        pairs.append((s, y))
+        g = rng.standard_normal(n)
+        d = mem.direction(g, 0.37)
+        ref = _dense_bfgs_direction(pairs[-5:], g)
+        assert np.max(np.abs(d - ref)) < 1e-13 * np.max(np.abs(ref))
+    mem.iter -= 1   # the same step again, now on g = y_newest
+    np.testing.assert_allclose(mem.direction(pairs[-1][1], 0.37), -pairs[-1][0], rtol=0, atol=1e-14)
+
+
 def test_ncg_reaches_ntn_minimum_optIon():
-    """ProjNormRhoMinimization with dir_type 2 (NCG, HZ and PR) on the
+    """ProjNormRhoMinimization with dir_type 2 (NCG, HZ and PR) and 3 (NBF) on the
     test/optIon input converges to the minimum NTN reaches, which the
     reference's recorded output pins (same energy to 1e-7 eV/atom), and no
     accepted step raises the energy. PR's line search stalls (Dcsrch
@@ -322,6 +367,10 @@ def test_ncg_reaches_ntn_minimum_optIon():
         assert abs(res.energy[0] - ntn.energy[0]) * O.HARTREE_TO_EV / 4 < 1e-7
         es = [h["energy"] for h in res.history]
         assert all(b < a + 1e-10 for a, b in zip(es, es[1:]))   # Hartree; rounding near the minimum
+    # dir_type 3 (NBF, RhoDirBFGS.f90), the same way
+    res = O.rho_minimization(rho, full, pot, sysm.cell.dV, O.OptParams(conv_check="POT", pot_tol=1e-6), dir_type=3)
+    assert res.outcome == 0 and res.iters > ntn.iters
+    assert abs(res.energy[0] - ntn.energy[0]) * O.HARTREE_TO_EV / 4 < 1e-7
 
 
 # ---------------------------------------------------------------- Ewald
