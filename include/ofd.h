@@ -233,7 +233,9 @@ int ofd_read_recpot(const char* path, double* maxg_invA, double* values, int cap
  * which copies a file of the same grid (optionally repeated repeat_x/y/z
  * times) and otherwise interpolates trilinearly, periodically, at the cell
  * centres of the new grid; with cell (bohr) and num_electrons > 0 the result
- * is scaled to hold num_electrons. One spin channel. */
+ * is scaled to hold num_electrons. One spin channel. The two-digit exponent
+ * of E26.20 bounds what can be written: magnitudes below 1e-99 are written as
+ * zero, non-finite values and magnitudes of 1e99 or more are an error. */
 int ofd_write_density(const char* path, int n1, int n2, int n3, int nspin, const double* rho, char* err, int errlen);
 int ofd_read_density(const char* path, int n1, int n2, int n3, const double* cell, double num_electrons,
                      int repeat_x, int repeat_y, int repeat_z, double* rho, char* err, int errlen);

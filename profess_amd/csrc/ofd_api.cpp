@@ -16,6 +16,7 @@
 #include <condition_variable>
 #include <map>
 #include <mutex>
+#include <new>
 #include <tuple>
 #include <cmath>
 #include <cstdio>
@@ -1687,11 +1688,15 @@ static int wgc_ode_table(double alpha, double beta, double gamma, std::vector<do
 extern "C" int ofd_build_wgc_table(ofd_ctx* c, int* nk_out) {
   if (!c) return -1;
   if (c->kind != kWGC8) return fail(c, "context is not a WGC functional");
-  std::vector<double> eta, w, w1, w2;
-  int n = wgc_ode_table(c->alpha, c->beta, c->gamma, eta, w, w1, w2);
-  if (n <= 0) return fail(c, "WGC ODE integration failed");
-  if (nk_out) *nk_out = n;
-  return ofd_set_wgc_table(c, n, eta.data(), w.data(), w1.data(), w2.data());
+  try {
+    std::vector<double> eta, w, w1, w2;
+    int n = wgc_ode_table(c->alpha, c->beta, c->gamma, eta, w, w1, w2);
+    if (n <= 0) return fail(c, "WGC ODE integration failed");
+    if (nk_out) *nk_out = n;
+    return ofd_set_wgc_table(c, n, eta.data(), w.data(), w1.data(), w2.data());
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
 }
 
 // ====================================================================== ion terms on the device
@@ -2402,10 +2407,17 @@ extern "C" long long ofd_request_bytes(int n1, int n2, int n3, int kinetic, int 
 
 extern "C" int ofd_pool_create(int n1, int n2, int n3, int capacity, int kinetic, int precision, ofd_pool** out) {
   if (!out || capacity < 1) return -1;
-  ofd_pool* p = new ofd_pool();
+  ofd_pool* p = new (std::nothrow) ofd_pool();
+  if (!p) return -1;
   p->n1 = n1; p->n2 = n2; p->n3 = n3; p->capacity = capacity; p->kinetic = kinetic; p->precision = precision;
-  p->slots.resize(capacity);
-  p->where.assign(capacity, -1);
+  try {
+    p->slots.resize(capacity);
+    p->where.assign(capacity, -1);
+  } catch (const std::exception&) {
+    p->err = "pool slot allocation failed";
+    *out = p;
+    return -1;
+  }
   const size_t rows = (size_t)ofd_pool::kGroups * capacity;
   if (hipHostMalloc((void**)&p->h_desc, sizeof(ReqDesc) * rows, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&p->h_sums, sizeof(double) * kNSum * rows, hipHostMallocDefault) != hipSuccess) {

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -70,7 +71,7 @@ void inverse3(const double a[3][3], double inv[3][3]) {
 
 }  // namespace
 
-extern "C" int ofd_read_geometry(const char* path, ofd_geometry* g, char* err, int errlen) {
+static int read_geometry(const char* path, ofd_geometry* g, char* err, int errlen) {
   if (!path || !g) {
     set_err(err, errlen, "null argument");
     return -1;
@@ -134,7 +135,12 @@ extern "C" int ofd_read_geometry(const char* path, ofd_geometry* g, char* err, i
       cell[1][1] = b * std::sin(ga);
       cell[0][2] = c * std::cos(be);
       cell[1][2] = c * (std::cos(al) - std::cos(be) * std::cos(ga)) / std::sin(ga);
-      cell[2][2] = std::sqrt(c * c - cell[0][2] * cell[0][2] - cell[1][2] * cell[1][2]);
+      const double c2 = c * c - cell[0][2] * cell[0][2] - cell[1][2] * cell[1][2];
+      if (!(c2 > 0.0) || !(std::sin(ga) != 0.0)) {
+        set_err(err, errlen, "LATTICE_ABC: lengths and angles do not form a cell");
+        return -2;
+      }
+      cell[2][2] = std::sqrt(c2);
       found_lattice = true;
     } else if (cmd == "POSITIONS_FRAC" || cmd == "POSITIONS_CART" || cmd == "POSITIONS_ABS" ||
                cmd == "POSITIONS_BOHR") {
@@ -184,6 +190,19 @@ extern "C" int ofd_read_geometry(const char* path, ofd_geometry* g, char* err, i
     set_err(err, errlen, "no POSITIONS block");
     return -2;
   }
+  // handedness (ReadIonFile.f90:640-648): (a1 x a2) . a3 must be positive
+  const double cx = cell[1][0] * cell[2][1] - cell[2][0] * cell[1][1];
+  const double cy = cell[2][0] * cell[0][1] - cell[0][0] * cell[2][1];
+  const double cz = cell[0][0] * cell[1][1] - cell[1][0] * cell[0][1];
+  const double handed = cx * cell[0][2] + cy * cell[1][2] + cz * cell[2][2];
+  if (handed < 0.0) {
+    set_err(err, errlen, "left-handed lattice vectors");
+    return -2;
+  }
+  if (!(handed > 0.0)) {
+    set_err(err, errlen, "degenerate lattice vectors (zero volume)");
+    return -2;
+  }
   const int nion = (int)elem.size();
   // Cartesian (Angstrom) or bohr positions to fractional (ReadIonFile.f90:515-530)
   if (cart || in_bohr) {
@@ -197,14 +216,6 @@ extern "C" int ofd_read_geometry(const char* path, ofd_geometry* g, char* err, i
     }
   }
   for (double& v : coord) v = v - std::floor(v);   // coord - FLOOR(coord)
-  // handedness (ReadIonFile.f90:640-648): (a1 x a2) . a3 must be positive
-  const double cx = cell[1][0] * cell[2][1] - cell[2][0] * cell[1][1];
-  const double cy = cell[2][0] * cell[0][1] - cell[0][0] * cell[2][1];
-  const double cz = cell[0][0] * cell[1][1] - cell[1][0] * cell[0][1];
-  if (cx * cell[0][2] + cy * cell[1][2] + cz * cell[2][2] < 0.0) {
-    set_err(err, errlen, "left-handed lattice vectors");
-    return -2;
-  }
   // ions grouped by species in order of first appearance, stable within a
   // species (the exchange sort of ReadIonFile.f90:556-577 gives this order)
   std::vector<int> order(nion);
@@ -241,6 +252,16 @@ extern "C" int ofd_read_geometry(const char* path, ofd_geometry* g, char* err, i
   return 0;
 }
 
+extern "C" int ofd_read_geometry(const char* path, ofd_geometry* g, char* err, int errlen) {
+  try {
+    return read_geometry(path, g, err, errlen);
+  } catch (const std::exception& e) {
+    if (g) ofd_free_geometry(g);
+    set_err(err, errlen, e.what());
+    return -1;
+  }
+}
+
 extern "C" void ofd_free_geometry(ofd_geometry* g) {
   if (!g) return;
   std::free(g->type_start);
@@ -255,8 +276,8 @@ extern "C" void ofd_free_geometry(ofd_geometry* g) {
   g->pseudo_file = nullptr;
 }
 
-extern "C" int ofd_read_recpot(const char* path, double* maxg_invA, double* values, int capacity, int* npts,
-                               char* err, int errlen) {
+static int read_recpot(const char* path, double* maxg_invA, double* values, int capacity, int* npts, char* err,
+                       int errlen) {
   if (!path || !maxg_invA || !npts) {
     set_err(err, errlen, "null argument");
     return -1;
@@ -319,6 +340,10 @@ extern "C" int ofd_read_recpot(const char* path, double* maxg_invA, double* valu
     return -2;
   }
   if (!v.empty() && v.back() == 1000.0) v.back() = 0.0;
+  if (v.size() > (size_t)0x7fffffff) {
+    set_err(err, errlen, "table too long");
+    return -2;
+  }
   *npts = (int)v.size();
   if (values) {
     if (capacity < (int)v.size()) {
@@ -330,11 +355,26 @@ extern "C" int ofd_read_recpot(const char* path, double* maxg_invA, double* valu
   return 0;
 }
 
+extern "C" int ofd_read_recpot(const char* path, double* maxg_invA, double* values, int capacity, int* npts,
+                               char* err, int errlen) {
+  try {
+    return read_recpot(path, maxg_invA, values, capacity, npts, err, errlen);
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return -1;
+  }
+}
+
 // One value as FORMAT(E26.20e2): 0.ddddddddddddddddddddE+xx (26 characters).
-static void e26_20(double v, char* out) {
+// The format has two exponent digits: magnitudes below 1e-99 are written as
+// zero (the Fortran edit would fill the field with asterisks, which no reader
+// accepts), and magnitudes of 1e99 and above are rejected.
+static bool e26_20(double v, char* out) {
+  if (!std::isfinite(v) || std::fabs(v) >= 1e99) return false;
+  if (std::fabs(v) < 1e-99) v = 0.0;
   if (v == 0.0) {
     std::snprintf(out, 27, "%26s", "0.00000000000000000000E+00");
-    return;
+    return true;
   }
   char buf[64];
   std::snprintf(buf, sizeof(buf), "%.19E", std::fabs(v));   // d.ddd...E+xx, 20 significant digits
@@ -349,6 +389,7 @@ static void e26_20(double v, char* out) {
     std::snprintf(out, 27, "-.%sE%c%02d", digits, ex < 0 ? '-' : '+', std::abs(ex));
   else
     std::snprintf(out, 27, "0.%sE%c%02d", digits, ex < 0 ? '-' : '+', std::abs(ex));
+  return true;
 }
 
 extern "C" int ofd_write_density(const char* path, int n1, int n2, int n3, int nspin, const double* rho, char* err,
@@ -371,7 +412,11 @@ extern "C" int ofd_write_density(const char* path, int n1, int n2, int n3, int n
   const long long n = (long long)n1 * n2 * n3 * nspin;
   char rec[32];
   for (long long k = 0; k < n; ++k) {
-    e26_20(rho[k], rec);
+    if (!e26_20(rho[k], rec)) {
+      std::fclose(f);
+      set_err(err, errlen, "value not representable in E26.20e2 at index " + std::to_string(k));
+      return -2;
+    }
     std::fprintf(f, "%s\n", rec);
   }
   const bool ok = std::fclose(f) == 0;
@@ -379,8 +424,8 @@ extern "C" int ofd_write_density(const char* path, int n1, int n2, int n3, int n
   return ok ? 0 : -1;
 }
 
-extern "C" int ofd_read_density(const char* path, int n1, int n2, int n3, const double* cell, double num_electrons,
-                                int repeat_x, int repeat_y, int repeat_z, double* rho, char* err, int errlen) {
+static int read_density(const char* path, int n1, int n2, int n3, const double* cell, double num_electrons,
+                        int repeat_x, int repeat_y, int repeat_z, double* rho, char* err, int errlen) {
   if (!path || !rho || n1 <= 0 || n2 <= 0 || n3 <= 0) {
     set_err(err, errlen, "bad argument");
     return -1;
@@ -409,6 +454,10 @@ extern "C" int ofd_read_density(const char* path, int n1, int n2, int n3, const 
     return -2;
   }
   const long long nf = (long long)xl * yl * zl;
+  if (nf > (1LL << 34)) {
+    set_err(err, errlen, "header dimensions too large");
+    return -2;
+  }
   std::vector<double> d((size_t)nf);
   for (long long k = 0; k < nf; ++k) {
     if (!std::getline(in, ln)) {
@@ -435,7 +484,9 @@ extern "C" int ofd_read_density(const char* path, int n1, int n2, int n3, const 
     auto frac_of = [](long long len, int rep, int n, int i) {
       double f = ((double)(len * rep) / (double)n * (1.0 + 2.0 * (double)i) - 1.0) * 0.5;
       f = f - (double)len * std::floor(f / (double)len);   // MODULO(f, len)
-      return f;
+      // a tiny negative f (a rounded 0) wraps to len - tiny, which can round to
+      // len itself: that is the point 0
+      return f < (double)len ? f : 0.0;
     };
     for (int iz = 0; iz < n3; ++iz) {
       const double fz = frac_of(zl, rz, n3, iz);
@@ -479,4 +530,14 @@ extern "C" int ofd_read_density(const char* path, int n1, int n2, int n3, const 
     for (long long k = 0; k < n; ++k) rho[k] *= sc;
   }
   return 0;
+}
+
+extern "C" int ofd_read_density(const char* path, int n1, int n2, int n3, const double* cell, double num_electrons,
+                                int repeat_x, int repeat_y, int repeat_z, double* rho, char* err, int errlen) {
+  try {
+    return read_density(path, n1, n2, n3, cell, num_electrons, repeat_x, repeat_y, repeat_z, rho, err, errlen);
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return -1;
+  }
 }

@@ -271,6 +271,27 @@ def test_fast_mixed_lengths_match_generic_path(pa, monkeypatch, grid):
         _check_terms(e32, e1, rtol=1e-5, floor=1e-8)
 
 
+@pytest.mark.parametrize("grid,kinetic", [((192, 192, 192), 4), ((256, 256, 256), 4), ((256, 256, 256), 5)])
+def test_calpot_full_size_matches_oracle(pa, grid, kinetic):
+    """The evaluation at the full workload grids (192^3, 256^3; WT and WGC)
+    against the oracle on the same density and ion potential, at the FP64
+    bar."""
+    _say("calpot full size %s kinetic %d" % (grid, kinetic))
+    cell, frac, vion, ne, f, rho = _case(grid, _tri_lattice(2.0), kinetic=kinetic, seed=21, nion=4)
+    sysm = O.System(cell, f, vion, e_ionion=-2.5)
+    s = pa.OFDSystem(cell.lattice, grid, ne, kinetic=kinetic)
+    s.set_ion_terms(vion, -2.5)
+    if kinetic == 5:
+        eta, w, w1, w2 = _wgc_table(f)
+        sysm.wgc_kernels = O.wgc_kernels_from_table(sysm.qtable, eta, w, w1, w2, f)
+        s.set_wgc_table(eta, w, w1, w2)
+    pr, er = O.calculate_potential_plus(rho, sysm, True, True)
+    pot, et = pa.CalculatePotentialPlus(s, rho)
+    _check_terms(et, er)
+    assert np.max(np.abs(pot - pr)) <= POT_RTOL * np.max(np.abs(pr))
+    s.close()
+
+
 def test_ntn_fused_dots_matches_reference_products(pa):
     """fused_dots takes the CG norms from the evaluation's sums and the
     recurrences: the converged energy must match the reference-product solve

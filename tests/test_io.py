@@ -117,3 +117,44 @@ def test_density_repeat_regrid_and_scaling(io, tmp_path):
     lat = np.diag([7.0, 8.0, 9.0])
     s = io.read_density(p, (5, 4, 3), lattice=lat, num_electrons=12.0)
     assert abs(s.sum() * np.prod(np.diag(lat)) / s.size - 12.0) < 1e-12
+
+
+def test_regrid_wrap_rounding_and_bad_inputs(io, tmp_path):
+    """A cell centre whose position rounds to a tiny negative number wraps to
+    the first file point (file x-length 2 onto 98 points: i = 24 sits at
+    exactly 0). Malformed inputs are errors, not crashes: degenerate or
+    inconsistent lattices, headers too large to hold, values the E26.20e2
+    records cannot carry (magnitudes below 1e-99 are written as zero)."""
+    from profess_amd._lib import OFDError
+    rng = np.random.default_rng(9)
+    rho = rng.random((2, 2, 2)) + 0.5
+    p = tmp_path / "w.den"
+    io.write_density(p, rho)
+    out = io.read_density(p, (98, 2, 2))
+    assert np.array_equal(out[:, :, 24], rho[:, :, 0])
+    assert np.all(out >= rho.min()) and np.all(out <= rho.max())
+    # lattices
+    bad = tmp_path / "bad.ion"
+    bad.write_text("%BLOCK LATTICE_CART\n1 0 0\n2 0 0\n0 0 1\n%END LATTICE_CART\n"
+                   "%BLOCK POSITIONS_FRAC\nAl 0 0 0\n%END POSITIONS_FRAC\n")
+    with pytest.raises(OFDError, match="degenerate"):
+        io.read_geometry(bad)
+    bad.write_text("%BLOCK LATTICE_ABC\n3 3 3\n10 10 120\n%END LATTICE_ABC\n"
+                   "%BLOCK POSITIONS_FRAC\nAl 0 0 0\n%END POSITIONS_FRAC\n")
+    with pytest.raises(OFDError, match="do not form a cell"):
+        io.read_geometry(bad)
+    # a header promising more than can be held
+    big = tmp_path / "big.den"
+    big.write_text("x-dimension:       100000\ny-dimension:       100000\n"
+                   "z-dimension:       100000\n# of spins:             1\n")
+    with pytest.raises(OFDError, match="too large"):
+        io.read_density(big, (4, 4, 4))
+    # values outside the record format
+    tiny = np.full((1, 1, 2), 0.5)
+    tiny[0, 0, 1] = 1e-120
+    io.write_density(p, tiny)
+    assert np.array_equal(io.read_density(p, (2, 1, 1))[0, 0], [0.5, 0.0])
+    for v in (1e120, np.inf, np.nan):
+        tiny[0, 0, 1] = v
+        with pytest.raises(OFDError, match="not representable"):
+            io.write_density(p, tiny)
