@@ -175,7 +175,7 @@ int ofd_ewald(ofd_ctx* ctx, double* energy, double* forces, double* info5);
 /* Ion relaxation by conjugate gradients, ConjugateGradient of IonOptCG2
  * (IonOptCG2.f90:60-520) with its LnSrch line search (MathLNsrch.f90:52-296),
  * called as OptimizeIons does with RhoOptimizer = ProjNormRhoMinimization
- * (dir_type 1) and CalculateForces = Ewald + ion-electron spline forces
+ * (dir_type from ion->rho_dir_type) and CalculateForces = Ewald + ion-electron spline forces
  * (CalForce.f90:134-194). Every geometry refreshes the ion terms (ionic
  * potential and Ewald energy, RefreshIons.f90:125-238), optimizes the density
  * from the previous one (trashPreviousRho = .FALSE.) and computes forces.
@@ -189,6 +189,8 @@ typedef struct {
   double force_cutoff;  /* stop when max |F| falls below it (Ha/bohr); <= 0: 5e-5 (:29) */
   int max_steps;        /* <= 0: 200 (:41) */
   double nn_dist;       /* nearest-ion distance guard (bohr); < 0: 2.0 (Neighbor.f90:33); 0: off */
+  int rho_dir_type;     /* density optimizer at every geometry, as the input's METH selects
+                           (Optimizer.f90:336-341): 1 NTN, 2 NCG, 3 NBF; <= 0: 1 */
 } ofd_ion_params;
 typedef struct {
   int steps;            /* CG iterations */

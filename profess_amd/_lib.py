@@ -54,7 +54,8 @@ class ofd_geometry(C.Structure):
 
 
 class ofd_ion_params(C.Structure):
-    _fields_ = [("cg_type", C.c_int), ("force_cutoff", C.c_double), ("max_steps", C.c_int), ("nn_dist", C.c_double)]
+    _fields_ = [("cg_type", C.c_int), ("force_cutoff", C.c_double), ("max_steps", C.c_int), ("nn_dist", C.c_double),
+                ("rho_dir_type", C.c_int)]
 
 
 class ofd_ion_stats(C.Structure):

@@ -191,17 +191,17 @@ def EwaldEnergy(system: OFDSystem, forces=False):
 
 
 def ConjugateGradient(system: OFDSystem, rhoR, frac, cg_type=2, force_cutoff=-1.0, max_steps=-1, nn_dist=-1.0,
-                      max_geometries=1000, **rho_kw):
+                      max_geometries=1000, rho_dir_type=1, **rho_kw):
     """Ion relaxation (IonOptCG2::ConjugateGradient, IonOptCG2.f90:60-520)
     from fractional positions `frac` (ions in the order and species grouping
-    given to build_ion_potential) and the starting density rhoR, with the NTN
-    density optimizer at every geometry (rho_kw as for
-    ProjNormRhoMinimization). Returns (rho, frac, energy9, forces, energies of
+    given to build_ion_potential) and the starting density rhoR, with the
+    density optimizer of rho_dir_type (1 NTN, 2 NCG, 3 NBF) at every geometry
+    (rho_kw as for ProjNormRhoMinimization). Returns (rho, frac, energy9, forces, energies of
     every geometry, stats)."""
     from ._lib import ofd_ion_params, ofd_ion_stats
     r = np.ascontiguousarray(rhoR, dtype=np.float64).copy()
     fr = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1, 3).copy()
-    ip = ofd_ion_params(int(cg_type), float(force_cutoff), int(max_steps), float(nn_dist))
+    ip = ofd_ion_params(int(cg_type), float(force_cutoff), int(max_steps), float(nn_dist), int(rho_dir_type))
     p = opt_params(**rho_kw)
     et = np.zeros(9)
     forces = np.zeros_like(fr)

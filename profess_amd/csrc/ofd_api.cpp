@@ -2059,12 +2059,14 @@ extern "C" int ofd_ion_relax(ofd_ctx* c, double* rhoR, double* frac_xyz, const o
   const double fcut = (ion && ion->force_cutoff > 0) ? ion->force_cutoff : (double)5e-5f;
   const int max_steps = (ion && ion->max_steps > 0) ? ion->max_steps : 200;
   const double nn_dist = (ion && ion->nn_dist >= 0) ? ion->nn_dist : 2.0;
+  const int rho_dir = (ion && ion->rho_dir_type > 0) ? ion->rho_dir_type : 1;
   const int max_line = 10;
   const char* tr = std::getenv("OFD_TRACE");
   const bool trace = tr && std::atoi(tr) != 0;
   // IonOptCG2.f90:134-139: ftol, gtol default-real literals, xtol 1e-12_DP
   const double ftol = (double)1e-4f, gtol = (double)1e-1f;
   if (cg_type < 1 || cg_type > 3) return fail(c, "cg_type must be 1, 2 or 3");
+  if (rho_dir > 3) return fail(c, "rho_dir_type must be 1 (NTN), 2 (NCG) or 3 (NBF)");
   const int ns = (int)c->type_start.size() - 1;
   const int nion = c->type_start[ns];
   const long long n = c->nr();
@@ -2092,7 +2094,7 @@ extern "C" int ofd_ion_relax(ofd_ctx* c, double* rhoR, double* frac_xyz, const o
     if (ofd_ewald(c, &eii, fii.data(), nullptr)) return false;
     int outcome = 0;
     ofd_opt_stats st;
-    if (ofd_rho_opt(c, rhoR, 1, rho_params, et, &outcome, &st)) return false;
+    if (ofd_rho_opt(c, rhoR, rho_dir, rho_params, et, &outcome, &st)) return false;
     if (ofd_ion_forces(c, rhoR, fie.data())) return false;
     for (size_t k = 0; k < F.size(); ++k) F[k] = fii[k] + fie[k];
     if (energies && ngeo < max_energies) energies[ngeo] = et[0];

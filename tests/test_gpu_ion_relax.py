@@ -26,7 +26,7 @@ def pa():
     return profess_amd
 
 
-def _oracle_relax():
+def _oracle_relax(dir_type=1, params=O.OptParams()):
     psp = O.make_psp(*O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read()))
     cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
     f = O.Functional(kinetic=4).resolve(12.0, cell.vol)
@@ -37,7 +37,7 @@ def _oracle_relax():
         vion = O.ion_potential(cell, [(psp, frac)])
         eii, _, _ = O.ewald_energy(cell, q, frac)
         full, pot = O.make_evaluators(O.System(cell, f, vion, eii))
-        res = O.rho_minimization(state["rho"], full, pot, cell.dV)
+        res = O.rho_minimization(state["rho"], full, pot, cell.dV, params, dir_type=dir_type)
         state["rho"] = res.rho
         return res.energy[0], O.ion_forces(cell, [(psp, frac)], res.rho) + O.ewald_forces(cell, q, frac)
     return cell, O.ion_relax_cg2(cell, S.OPTION_FRAC, evaluate)
@@ -69,3 +69,26 @@ def test_ion_relaxation_optIon(pa):
     assert np.max(np.abs(d @ lat.T)) < 1e-3
     assert st["max_force"] < float(np.float32(5e-5))
     assert et[0] == energies[-1]
+
+
+@pytest.mark.parametrize("dir_type", [2, 3])
+def test_ion_relaxation_optIon_ncg_nbf(pa, dir_type):
+    """The same relaxation with NCG (2) or NBF (3) density solves to a
+    potential norm of 1e-6: the oracle's energies at every geometry, and the
+    reference's recorded final energy (1e-6 eV/atom) and geometry (1e-3 bohr)."""
+    _say("ion relaxation optIon, density dir_type %d" % dir_type)
+    mg, v = O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read())
+    lat = S.OPTION_LATTICE_A.T / O.BOHR
+    s = pa.OFDSystem(lat, (16, 16, 16), 12.0, kinetic=4)
+    s.build_ion_potential([(mg, v, S.OPTION_FRAC)])
+    rho0 = np.full(s.shape, 12.0 / s.vol)
+    rho, frac, et, forces, energies, st = pa.ConjugateGradient(s, rho0, S.OPTION_FRAC, rho_dir_type=dir_type,
+                                                               conv_check="POT", pot_tol=1e-6)
+    _, ref = _oracle_relax(dir_type, O.OptParams(conv_check="POT", pot_tol=1e-6))
+    ev = energies * O.HARTREE_TO_EV
+    ev_or = np.array(ref.energies) * O.HARTREE_TO_EV
+    assert len(ev) == len(ev_or) == len(S.OPTION_RELAX_EV)
+    assert np.max(np.abs(ev - ev_or)) / 4 < 1e-6
+    assert abs(ev[-1] - S.OPTION_RELAX_EV[-1]) / 4 < 1e-6
+    d = np.mod(frac - S.OPTION_FINAL_FRAC + 0.5, 1.0) - 0.5
+    assert np.max(np.abs(d @ lat.T)) < 1e-3
