@@ -26,7 +26,7 @@ def pa():
     return profess_amd
 
 
-def _oracle_relax(dir_type=1, params=O.OptParams()):
+def _oracle_relax(dir_type=1, params=O.OptParams(), cg_type=2):
     psp = O.make_psp(*O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read()))
     cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
     f = O.Functional(kinetic=4).resolve(12.0, cell.vol)
@@ -40,7 +40,7 @@ def _oracle_relax(dir_type=1, params=O.OptParams()):
         res = O.rho_minimization(state["rho"], full, pot, cell.dV, params, dir_type=dir_type)
         state["rho"] = res.rho
         return res.energy[0], O.ion_forces(cell, [(psp, frac)], res.rho) + O.ewald_forces(cell, q, frac)
-    return cell, O.ion_relax_cg2(cell, S.OPTION_FRAC, evaluate)
+    return cell, O.ion_relax_cg2(cell, S.OPTION_FRAC, evaluate, cg_type=cg_type)
 
 
 def test_ion_relaxation_optIon(pa):
@@ -92,3 +92,27 @@ def test_ion_relaxation_optIon_ncg_nbf(pa, dir_type):
     assert abs(ev[-1] - S.OPTION_RELAX_EV[-1]) / 4 < 1e-6
     d = np.mod(frac - S.OPTION_FINAL_FRAC + 0.5, 1.0) - 0.5
     assert np.max(np.abs(d @ lat.T)) < 1e-3
+
+
+@pytest.mark.parametrize("cg_type", [1, 3])
+def test_ion_relaxation_optIon_pr_dy(pa, cg_type):
+    """Polak-Ribiere (1) and Dai-Yuan (3) ion directions on test/optIon: the
+    oracle's relaxation with the same update, geometry for geometry, and the
+    minimum the reference's Hager-Zhang run records (final energy within
+    1e-6 eV/atom)."""
+    _say("ion relaxation optIon, cg_type %d" % cg_type)
+    mg, v = O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read())
+    lat = S.OPTION_LATTICE_A.T / O.BOHR
+    s = pa.OFDSystem(lat, (16, 16, 16), 12.0, kinetic=4)
+    s.build_ion_potential([(mg, v, S.OPTION_FRAC)])
+    rho0 = np.full(s.shape, 12.0 / s.vol)
+    rho, frac, et, forces, energies, st = pa.ConjugateGradient(s, rho0, S.OPTION_FRAC, cg_type=cg_type)
+    _, ref = _oracle_relax(cg_type=cg_type)
+    ev = energies * O.HARTREE_TO_EV
+    ev_or = np.array(ref.energies) * O.HARTREE_TO_EV
+    assert len(ev) == len(ev_or)
+    assert np.max(np.abs(ev - ev_or)) / 4 < 1e-6
+    d = np.mod(frac - ref.frac + 0.5, 1.0) - 0.5
+    assert np.max(np.abs(d @ lat.T)) < 1e-5
+    assert abs(ev[-1] - S.OPTION_RELAX_EV[-1]) / 4 < 1e-6
+    assert st["max_force"] < float(np.float32(5e-5))

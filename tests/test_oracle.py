@@ -517,3 +517,24 @@ def test_reference_sample_optIon_relaxation():
     assert sum(a != b for a, b in zip(state["iters"], S.OPTION_RELAX_NTN_STEPS)) <= 1
     d = np.mod(r.frac - S.OPTION_FINAL_FRAC + 0.5, 1.0) - 0.5
     assert np.max(np.abs(d @ cell.lattice.T)) < 1e-3
+
+
+@pytest.mark.parametrize("cg_type", [1, 2, 3])
+def test_ion_cg_variants_terminate_on_quadratic(cg_type):
+    """The ion relaxation's Polak-Ribiere, Hager-Zhang and Dai-Yuan updates
+    (IonOptCG2.f90:286-308) on E = 1/2 sum_i r_i^T K r_i: LnSrch's secant
+    step is exact on a quadratic, and the Hessian blockdiag(K, K, K) has three
+    distinct eigenvalues, so conjugate directions reach the minimum in three
+    steps (steepest descent does not)."""
+    cell = O.Cell(np.diag([10.0, 11.0, 12.0]), 8, 8, 8)
+    k = np.array([[1.0, 0.2, 0.0], [0.2, 0.5, 0.1], [0.0, 0.1, 0.3]])
+    c = np.array([[0.2, 0.3, 0.4], [0.6, 0.7, 0.1], [0.45, 0.15, 0.8]])
+
+    def evaluate(frac):
+        r = (np.mod(frac - c + 0.5, 1.0) - 0.5) @ cell.lattice.T
+        return 0.5 * float(np.sum((r @ k) * r)), -(r @ k)
+    start = c + np.array([[0.03, -0.02, 0.01], [-0.01, 0.02, 0.03], [0.02, 0.01, -0.03]])
+    res = O.ion_relax_cg2(cell, start, evaluate, cg_type=cg_type, nn_dist=0.0)
+    assert res.steps <= 3 and res.max_force < 1e-12
+    d = np.mod(res.frac - c + 0.5, 1.0) - 0.5
+    assert np.max(np.abs(d @ cell.lattice.T)) < 1e-12
