@@ -40,3 +40,15 @@ OPTDEN_STEP_EV = [-2.226243159654e+02, -2.312025858301e+02, -2.317471544164e+02,
                   -2.317607263207e+02]
 OPTDEN_NEWTON_ITERS = [2, 8, 6, 8, 4, 7]
 OPTDEN_EII_EV = -2.994046165635e+02
+
+# test/optCell: the same lattice with the ions at the ideal positions
+# (test/optCell/displaced.ion), first geometry of its run
+# (test/optCell/sampleoutput/displaced.out:93-113), eV
+OPTCELL_FRAC = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5]])
+OPTCELL_STEP0_EV = -2.224205733115e+02
+OPTCELL_ITER_EV = [-2.311374927723e+02, -2.316989773208e+02, -2.317102291421e+02, -2.317117591695e+02,
+                   -2.317118278762e+02, -2.317118334728e+02, -2.317118335995e+02]
+OPTCELL_EII_EV = -3.011568607522e+02
+# TF, vW, WT, LDA, Hartree, ion-electron at the converged density (:104-110)
+OPTCELL_TERMS_EV = {"TF": 8.990205775509e+01, "vW": 7.344980972680e+00, "WT": -2.785239973122e+00,
+                    "XC": -8.927974807017e+01, "H": 4.503089276774e-01, "IE": 6.381266754054e+01}

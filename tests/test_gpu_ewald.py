@@ -94,3 +94,28 @@ def test_system_from_reference_files(pa):
     _, et = pa.CalculatePotentialPlus(s, rho)
     assert abs(et[0] * O.HARTREE_TO_EV - S.OPTION_STEP0_EV) <= 1e-10 * abs(S.OPTION_STEP0_EV)
     assert abs(et[5] * O.HARTREE_TO_EV - S.OPTION_EII_EV) <= 1e-10 * abs(S.OPTION_EII_EV)
+
+
+def test_optCell_first_geometry_from_files(pa):
+    """test/optCell/displaced.ion (ideal positions) through the readers, the
+    device ionic potential, the device Ewald sum and the NTN solve: its
+    recorded ion-ion energy, uniform-density energy and converged total energy
+    (test/optCell/sampleoutput/displaced.out:93,108,112), and the oracle's
+    iteration count on the same device-built ionic potential."""
+    _say("optCell first geometry")
+    from profess_amd import inputs
+    s, g = inputs.system_from_files(os.path.join(GOLDEN, "optCell_displaced.ion"), (16, 16, 16))
+    assert np.array_equal(g["frac"], S.OPTCELL_FRAC)
+    rho0 = np.full(s.shape, 12.0 / s.vol)
+    _, et = pa.CalculatePotentialPlus(s, rho0)
+    assert abs(et[5] * O.HARTREE_TO_EV - S.OPTCELL_EII_EV) <= 1e-10 * abs(S.OPTCELL_EII_EV)
+    assert abs(et[0] * O.HARTREE_TO_EV - S.OPTCELL_STEP0_EV) <= 1e-10 * abs(S.OPTCELL_STEP0_EV)
+    rho, et, outcome, st = pa.ProjNormRhoMinimization(s, rho0)
+    assert outcome == 0
+    assert abs(et[0] * O.HARTREE_TO_EV - S.OPTCELL_ITER_EV[-1]) / 4 < 1e-6
+    cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
+    f = O.Functional(kinetic=4).resolve(12.0, cell.vol)
+    full, pot = O.make_evaluators(O.System(cell, f, s.ion_potential(), et[5]))
+    res = O.rho_minimization(rho0, full, pot, cell.dV)
+    assert st["iterations"] == res.iters
+    assert abs(res.energy[0] - et[0]) * O.HARTREE_TO_EV / 4 < 1e-9

@@ -538,3 +538,35 @@ def test_ion_cg_variants_terminate_on_quadratic(cg_type):
     assert res.steps <= 3 and res.max_force < 1e-12
     d = np.mod(res.frac - c + 0.5, 1.0) - 0.5
     assert np.max(np.abs(d @ cell.lattice.T)) < 1e-12
+
+
+def test_reference_sample_optCell_first_geometry():
+    """test/optCell's first geometry (the optIon lattice with the ions at the
+    ideal positions, test/optCell/displaced.ion): its ion-ion energy
+    (displaced.out:108) to 1e-11 relative, the uniform-density step 0 (:93) to
+    1e-12, the same inner Newton counts on the first two NTN steps (2 and 8,
+    :94-95), the converged total energy (:112) within 1e-6 eV/atom. The
+    reference stops after 7 steps, the oracle after 6: from step 3 on the
+    inner CG exits one iteration apart and the ENE test (three energies within
+    1e-6 Ha) is met one step earlier; the energy terms at the two stopping
+    points differ at first order in the density residual (2e-4 eV/atom)."""
+    psp = _al_psp()
+    g = os.path.join(GOLDEN, "optCell_displaced.ion")
+    with open(g) as fh:
+        assert "0.000000000000000   0.500000000000000   0.500000000000000" in fh.read()
+    cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
+    eii, _, _ = O.ewald_energy(cell, np.full(4, 3.0), S.OPTCELL_FRAC)
+    assert eii * O.HARTREE_TO_EV == pytest.approx(S.OPTCELL_EII_EV, rel=1e-11)
+    vion = O.ion_potential(cell, [(psp, S.OPTCELL_FRAC)])
+    f = O.Functional(kinetic=4).resolve(12.0, cell.vol)
+    sysm = O.System(cell, f, vion, eii)
+    rho = np.full(cell.shape, 12.0 / cell.vol)
+    full, pot = O.make_evaluators(sysm)
+    assert full(rho)[1][0] * O.HARTREE_TO_EV == pytest.approx(S.OPTCELL_STEP0_EV, rel=1e-12)
+    res = O.rho_minimization(rho, full, pot, cell.dV)
+    assert res.outcome == 0
+    assert [h["newton_iter"] for h in res.history[1:3]] == [2, 8]
+    assert abs(res.energy[0] * O.HARTREE_TO_EV - S.OPTCELL_ITER_EV[-1]) / 4 < 1e-6
+    e = res.energy * O.HARTREE_TO_EV
+    for k, i in {"TF": 6, "vW": 7, "WT": 8, "XC": 4, "H": 3, "IE": 2}.items():
+        assert abs(e[i] - S.OPTCELL_TERMS_EV[k]) / 4 < 2e-4, k
