@@ -77,3 +77,37 @@ def test_input_generators():
     assert np.all(c["rho"] >= 0.7 * rho0 - 1e-15) and np.all(c["rho"] <= 1.3 * rho0 + 1e-15)
     mg, v = inputs.synthetic_recpot(3, 1.2)
     assert v.size == 15001 and abs(mg * inputs.BOHR - 30.0) < 1e-12
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every struct _lib.py passes through ctypes has the size and field
+    offsets the C compiler gives the same struct in include/ofd.h."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    from profess_amd import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = [v for k, v in vars(_lib).items()
+               if k.startswith("ofd_") and isinstance(v, type) and issubclass(v, C.Structure)]
+    assert len(structs) >= 10
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "ofd.h"', "int main(void) {"]
+    for s in structs:
+        n = s.__name__
+        lines.append('  printf("%s %%zu\\n", sizeof(%s));' % (n, n))
+        for f, _ in s._fields_:
+            # a C field named as a Python keyword carries a trailing _ in ctypes
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (n, f, n, f.rstrip("_")))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                  check=True).stdout.splitlines())
+    for s in structs:
+        n = s.__name__
+        assert int(got[n]) == C.sizeof(s), n
+        for f, _ in s._fields_:
+            assert int(got["%s.%s" % (n, f)]) == getattr(s, f).offset, (n, f)
