@@ -111,3 +111,47 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert int(got[n]) == C.sizeof(s), n
         for f, _ in s._fields_:
             assert int(got["%s.%s" % (n, f)]) == getattr(s, f).offset, (n, f)
+
+
+def test_fortran_bind_c_types_match_the_header(tmp_path):
+    """The Fortran binding's BIND(C) types (profess_amd/fortran/ofd_profess.f90)
+    have the size and the field offsets of the header's structs they stand
+    for (offsets matter: a missing trailing int can hide in the padding)."""
+    import shutil
+    import subprocess
+    from profess_amd import _lib
+    fc = "/opt/rocm/bin/amdflang"
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not os.path.exists(fc) or cc is None:
+        pytest.skip("no Fortran or C compiler")
+    pairs = {"ofd_grid_t": "ofd_grid", "ofd_functional_t": "ofd_functional",
+             "ofd_opt_params_t": "ofd_opt_params", "ofd_opt_stats_t": "ofd_opt_stats"}
+    fields = {c: [f.rstrip("_") for f, _ in getattr(_lib, c)._fields_] for c in pairs.values()}
+    f90 = ["PROGRAM layout", "  USE ISO_C_BINDING", "  USE ofd_c", "  INTEGER(C_INTPTR_T) :: b"]
+    f90 += ["  TYPE(%s), TARGET :: v%d" % (t, k) for k, t in enumerate(pairs)]
+    for k, (t, c) in enumerate(pairs.items()):
+        f90.append('  PRINT *, "%s", C_SIZEOF(v%d)' % (c, k))
+        f90.append("  b = TRANSFER(C_LOC(v%d), b)" % k)
+        for f in fields[c]:
+            f90.append('  PRINT *, "%s.%s", TRANSFER(C_LOC(v%d%%%s), b) - b' % (c, f, k, f))
+    f90 += ["END PROGRAM layout"]
+    (tmp_path / "layout.f90").write_text("\n".join(f90) + "\n")
+    src = os.path.join(ROOT, "profess_amd", "fortran", "ofd_profess.f90")
+    subprocess.run([fc, "-c", "-module-dir", str(tmp_path), src, "-o", str(tmp_path / "ofd_profess.o")], check=True)
+    subprocess.run([fc, "-I", str(tmp_path), str(tmp_path / "layout.f90"), "-o", str(tmp_path / "layout_f")],
+                   check=True, cwd=str(tmp_path))
+    out = subprocess.run([str(tmp_path / "layout_f")], capture_output=True, text=True, check=True).stdout.split()
+    fort = {out[i]: int(out[i + 1]) for i in range(0, len(out), 2)}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "ofd.h"', "int main(void) {"]
+    for c in pairs.values():
+        lines.append('  printf("%s %%zu\\n", sizeof(%s));' % (c, c))
+        lines += ['  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (c, f, c, f) for f in fields[c]]
+    lines += ["  return 0;", "}"]
+    (tmp_path / "layout.c").write_text("\n".join(lines) + "\n")
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(tmp_path / "layout.c"), "-o",
+                    str(tmp_path / "layout_c")], check=True)
+    cout = dict(l.split() for l in subprocess.run([str(tmp_path / "layout_c")], capture_output=True, text=True,
+                                                   check=True).stdout.splitlines())
+    assert len(fort) == len(cout) >= 30
+    for k, v in cout.items():
+        assert fort[k] == int(v), (k, fort[k], v)
