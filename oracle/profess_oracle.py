@@ -1442,9 +1442,12 @@ class LnSrch:
 
     @staticmethod
     def _new_alp(x1, y1, x2, y2, stpmin, stpmax):
-        slope = (y1 - y2) / (x1 - x2)
-        b = y1 - x1 * slope
-        alp_t = -b / slope
+        # IEEE division as in the Fortran: a zero slope gives +-Inf, which the
+        # bounds test below turns into -1
+        with np.errstate(divide="ignore", invalid="ignore"):
+            slope = np.float64(y1 - y2) / np.float64(x1 - x2)
+            b = y1 - x1 * slope
+            alp_t = float(-b / slope)
         if alp_t >= stpmax or alp_t <= stpmin:
             return -1.0
         return alp_t

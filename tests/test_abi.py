@@ -11,7 +11,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    txt = open(os.path.join(ROOT, "include", "ofd.h")).read()
+    with open(os.path.join(ROOT, "include", "ofd.h")) as fh:
+        txt = fh.read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(ofd_[a-z_0-9]+)\s*\(", txt)))
 

@@ -26,8 +26,13 @@ def pa():
     return profess_amd
 
 
+def _recpot_text():
+    with open(os.path.join(GOLDEN, "al_HC.lda.recpot")) as fh:
+        return fh.read()
+
+
 def _oracle_relax(dir_type=1, params=O.OptParams(), cg_type=2):
-    psp = O.make_psp(*O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read()))
+    psp = O.make_psp(*O.read_recpot_text(_recpot_text()))
     cell = O.Cell(S.OPTION_LATTICE_A.T / O.BOHR, 16, 16, 16)
     f = O.Functional(kinetic=4).resolve(12.0, cell.vol)
     q = np.full(4, 3.0)
@@ -45,7 +50,7 @@ def _oracle_relax(dir_type=1, params=O.OptParams(), cg_type=2):
 
 def test_ion_relaxation_optIon(pa):
     _say("ion relaxation optIon")
-    mg, v = O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read())
+    mg, v = O.read_recpot_text(_recpot_text())
     lat = S.OPTION_LATTICE_A.T / O.BOHR
     s = pa.OFDSystem(lat, (16, 16, 16), 12.0, kinetic=4)
     s.build_ion_potential([(mg, v, S.OPTION_FRAC)])
@@ -77,7 +82,7 @@ def test_ion_relaxation_optIon_ncg_nbf(pa, dir_type):
     potential norm of 1e-6: the oracle's energies at every geometry, and the
     reference's recorded final energy (1e-6 eV/atom) and geometry (1e-3 bohr)."""
     _say("ion relaxation optIon, density dir_type %d" % dir_type)
-    mg, v = O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read())
+    mg, v = O.read_recpot_text(_recpot_text())
     lat = S.OPTION_LATTICE_A.T / O.BOHR
     s = pa.OFDSystem(lat, (16, 16, 16), 12.0, kinetic=4)
     s.build_ion_potential([(mg, v, S.OPTION_FRAC)])
@@ -101,7 +106,7 @@ def test_ion_relaxation_optIon_pr_dy(pa, cg_type):
     minimum the reference's Hager-Zhang run records (final energy within
     1e-6 eV/atom)."""
     _say("ion relaxation optIon, cg_type %d" % cg_type)
-    mg, v = O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read())
+    mg, v = O.read_recpot_text(_recpot_text())
     lat = S.OPTION_LATTICE_A.T / O.BOHR
     s = pa.OFDSystem(lat, (16, 16, 16), 12.0, kinetic=4)
     s.build_ion_potential([(mg, v, S.OPTION_FRAC)])

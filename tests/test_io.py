@@ -64,7 +64,8 @@ def test_recpot_reader_matches_reading_rules(io, tmp_path):
     """al_HC.lda.recpot as ReadPseudo reads it; a table whose last data line is
     short reads the terminator as 0 (ReadIonFile.f90:885-888)."""
     mg, v = io.read_recpot_file(os.path.join(GOLDEN, "al_HC.lda.recpot"))
-    mg2, v2 = O.read_recpot_text(open(os.path.join(GOLDEN, "al_HC.lda.recpot")).read())
+    with open(os.path.join(GOLDEN, "al_HC.lda.recpot")) as fh:
+        mg2, v2 = O.read_recpot_text(fh.read())
     assert mg == mg2 and np.array_equal(v, v2)
     psp = O.make_psp(mg, v)
     assert psp.charge == 3
