@@ -118,9 +118,25 @@ int ofd_get_ion_potential(ofd_ctx* ctx, double* vion_real);
  * KEDF_WGCkernel.f90:109-277). */
 int ofd_set_wgc_table(ofd_ctx* ctx, int nknots, const double* eta, const double* w, const double* w1,
                       const double* w2);
-/* Same table integrated by this library (adaptive RK on the host, on
- * makeKernel2's knots); returns the knot count in *nknots_out if not NULL. */
+/* Same table integrated by this library (ofd_wgc_kernel_table below) and set
+ * as by ofd_set_wgc_table; returns the knot count in *nknots_out if not NULL. */
 int ofd_build_wgc_table(ofd_ctx* ctx, int* nknots_out);
+/* The 1D WGC table of makeKernel2 (KEDF_KernelODE.f90:254-478) computed on the
+ * host, no device and no context: the ODE of KEDF_KernelODE.f90:481-537
+ * integrated from eta = 100 toward 0 on makeKernel2's 12,696 knots by an
+ * adaptive Dormand-Prince 5(4) pair at rtol 1e-11. The reference integrates
+ * with RKSUITE at TOL 5e-5 (KEDF_KernelODE.f90:376), whose table carries that
+ * tolerance's error: against it this table differs by up to about 1e-4 of
+ * max|w| in w and w' and 5e-3 of max|w''| in w'' near eta = 1, and WGC
+ * energies built on it differ from the reference-table energies at the
+ * 1e-6 eV/atom level (4.7e-7 eV/atom in the total, 3e-6 relative in the
+ * nonlocal term on the test/optDen cell), not at the 1e-10 per-term level. A caller who needs the reference's own
+ * numbers passes the reference's table to ofd_set_wgc_table instead.
+ * Writes at most nmax knots; returns the knot count (when nmax is smaller or
+ * an output is NULL nothing is written), or a negative code on bad input or
+ * failure. */
+int ofd_wgc_kernel_table(double alpha, double beta, double gamma, int nmax, double* eta, double* w, double* w1,
+                         double* w2);
 
 /* CalculatePotentialPlus(rho, optSqrt, potential, eTable) (CalPotential.f90:41).
  * etable9 == NULL means the reference's call without eTable (no energies). */

@@ -131,9 +131,17 @@ def q_mask(n1, n2, n3, offset=None):
 # FFT conventions (Fourier_v1.f90:333-465). numpy's pocketfft stands in for
 # FFTW (not vendored by the reference): same transform, same normalization.
 # --------------------------------------------------------------------------
+# None: numpy.fft, one thread. An integer k: scipy.fft (the same pocketfft)
+# with k worker threads; used only to time the oracle on several cores.
+FFT_WORKERS = None
+
+
 def fft_forward(a):
     """R2C, e^{-iG.r}, scaled by 1/N (Fourier_v1.f90:361,388-393)."""
     n = a.size
+    if FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.rfftn(a, workers=FFT_WORKERS) / float(n)
     return np.fft.rfftn(a) / float(n)
 
 
@@ -141,6 +149,9 @@ def fft_backward(c, shape):
     """C2R, e^{+iG.r}, unnormalized (Fourier_v1.f90:436). The imaginary parts
     of the x = 0 and x = n1/2 columns are ignored, as in any Hermitian C2R."""
     n = shape[0] * shape[1] * shape[2]
+    if FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.irfftn(c, s=shape, axes=(0, 1, 2), workers=FFT_WORKERS) * float(n)
     return np.fft.irfftn(c, s=shape, axes=(0, 1, 2)) * float(n)
 
 

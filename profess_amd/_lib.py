@@ -72,6 +72,7 @@ _lib = None
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd_set_ion_terms",
            "ofd_build_ion_potential", "ofd_get_ion_potential", "ofd_set_wgc_table", "ofd_build_wgc_table",
+           "ofd_wgc_kernel_table",
            "ofd_calpot", "ofd_calpot_device", "ofd_rho_opt", "ofd_ion_forces", "ofd_pool_create",
            "ofd_pool_destroy", "ofd_pool_last_error", "ofd_pool_admit", "ofd_pool_step", "ofd_pool_collect",
            "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
@@ -103,6 +104,7 @@ def lib():
     L.ofd_get_ion_potential.argtypes = [vp, dp]
     L.ofd_set_wgc_table.argtypes = [vp, C.c_int, dp, dp, dp, dp]
     L.ofd_build_wgc_table.argtypes = [vp, ip]
+    L.ofd_wgc_kernel_table.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp]
     L.ofd_calpot.argtypes = [vp, dp, C.c_int, dp, dp]
     L.ofd_calpot_device.argtypes = [vp, C.c_void_p, C.c_int, C.c_void_p, dp]
     L.ofd_rho_opt.argtypes = [vp, dp, C.c_int, C.POINTER(ofd_opt_params), dp, ip, C.POINTER(ofd_opt_stats)]

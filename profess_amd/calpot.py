@@ -128,6 +128,20 @@ class OFDSystem:
         return n.value
 
 
+def wgc_kernel_table(alpha, beta, gamma):
+    """The library's 1D WGC kernel table (eta, w, w', w'') on makeKernel2's knots
+    (KEDF_KernelODE.f90:254-478), integrated on the host (ofd_wgc_kernel_table)."""
+    L = lib()
+    n = L.ofd_wgc_kernel_table(alpha, beta, gamma, 0, None, None, None, None)
+    if n <= 0:
+        raise OFDError("WGC table integration failed (%d)" % n)
+    out = [np.zeros(n) for _ in range(4)]
+    got = L.ofd_wgc_kernel_table(alpha, beta, gamma, n, *[dptr(x) for x in out])
+    if got != n:
+        raise OFDError("WGC table integration failed (%d)" % got)
+    return tuple(out)
+
+
 def CalculatePotentialPlus(system: OFDSystem, rho, optSqrt=True, eTable=True):
     """CalculatePotentialPlus(rho, optSqrt, potential[, eTable]) (CalPotential.f90:41).
     Returns (potential, eTable(9) in Hartree or None)."""

@@ -564,18 +564,21 @@ extern "C" int ofd_get_ion_potential(ofd_ctx* c, double* vion) {
 }
 
 // eTable from the finalized sums (CalculateEnergy, CalPotential.f90:610-644)
-static void etable_from_sums(const ofd_ctx* c, const double* s, double* et) {
+static void etable_calc(const double* s, double lam, double mu, double dV, double eii, double* et) {
   double loc[9] = {0};
   loc[2] = s[0];
   loc[3] = s[1] / 2.0;
   loc[4] = s[2];
-  loc[6] = c->lam * kCTF * s[3];
-  loc[7] = 0.5 * s[4] * c->mu;
+  loc[6] = lam * kCTF * s[3];
+  loc[7] = 0.5 * s[4] * mu;
   loc[8] = kCTF * s[5];
   loc[1] = loc[6] + loc[7] + loc[8];
-  for (int i = 0; i < 9; ++i) et[i] = loc[i] * c->dV;
-  et[5] = c->e_ionion;
+  for (int i = 0; i < 9; ++i) et[i] = loc[i] * dV;
+  et[5] = eii;
   et[0] = et[1] + et[2] + et[3] + et[4] + et[5];
+}
+static void etable_from_sums(const ofd_ctx* c, const double* s, double* et) {
+  etable_calc(s, c->lam, c->mu, c->dV, c->e_ionion, et);
 }
 
 // One evaluation on the context's own descriptor; sums (kNSum) copied back.
@@ -1685,6 +1688,24 @@ static int wgc_ode_table(double alpha, double beta, double gamma, std::vector<do
   return n;
 }
 
+extern "C" int ofd_wgc_kernel_table(double alpha, double beta, double gamma, int nmax, double* eta_out,
+                                    double* w_out, double* w1_out, double* w2_out) {
+  if (!(alpha > 0.0) || !(beta > 0.0) || !(gamma > 0.0)) return -1;
+  try {
+    std::vector<double> eta, w, w1, w2;
+    const int n = wgc_ode_table(alpha, beta, gamma, eta, w, w1, w2);
+    if (n <= 0) return -2;
+    if (nmax < n || !eta_out || !w_out || !w1_out || !w2_out) return n;
+    std::copy(eta.begin(), eta.end(), eta_out);
+    std::copy(w.begin(), w.end(), w_out);
+    std::copy(w1.begin(), w1.end(), w1_out);
+    std::copy(w2.begin(), w2.end(), w2_out);
+    return n;
+  } catch (const std::exception&) {
+    return -3;
+  }
+}
+
 extern "C" int ofd_build_wgc_table(ofd_ctx* c, int* nk_out) {
   if (!c) return -1;
   if (c->kind != kWGC8) return fail(c, "context is not a WGC functional");
@@ -1753,7 +1774,7 @@ extern "C" int ofd_build_ion_potential(ofd_ctx* c, int nspecies, const int* type
                           c->st));
     const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
     void* sc = dalloc<char>(scratch);
-    int rc = build_ion_potential(c->sp, c->rb, c->vol, nspecies, c->type_start.data(), c->d_frac, nion, c->d_psp_vqs,
+    int rc = build_ion_potential(c->sp, c->rb, c->vol, nspecies, c->type_start.data(), c->frac.data(), c->d_frac, nion, c->d_psp_vqs,
                                  c->d_psp_dd, c->psp_off.data(), c->psp_npts.data(), c->psp_maxg.data(),
                                  c->psp_v0.data(), c->psp_charge.data(), order, c->vion, sc, c->st);
     hipStreamSynchronize(c->st);
@@ -1850,7 +1871,7 @@ extern "C" int ofd_ewald(ofd_ctx* c, double* energy, double* forces, double* inf
     const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
     void* sc = dalloc<char>(scratch);
     double comps[4];
-    const int rcode = ewald_device(c->sp, c->rb, c->lat, c->vol, ns, c->type_start.data(), charge.data(), c->d_frac,
+    const int rcode = ewald_device(c->sp, c->rb, c->lat, c->vol, ns, c->type_start.data(), charge.data(), c->frac.data(), c->d_frac,
                                    nion, c->spline_order, eta, rc, kc, cinv, comps, forces, sc, c->st);
     hipStreamSynchronize(c->st);
     dev_free(sc);
@@ -1882,7 +1903,7 @@ static int refresh_ion_potential(ofd_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->d_frac, c->frac.data(), sizeof(double) * c->frac.size(), hipMemcpyHostToDevice, c->st));
   const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
   void* sc = dalloc<char>(scratch);
-  const int rc = build_ion_potential(c->sp, c->rb, c->vol, ns, c->type_start.data(), c->d_frac, nion, c->d_psp_vqs,
+  const int rc = build_ion_potential(c->sp, c->rb, c->vol, ns, c->type_start.data(), c->frac.data(), c->d_frac, nion, c->d_psp_vqs,
                                      c->d_psp_dd, c->psp_off.data(), c->psp_npts.data(), c->psp_maxg.data(),
                                      c->psp_v0.data(), c->psp_charge.data(), c->spline_order, c->vion, sc, c->st);
   hipStreamSynchronize(c->st);
@@ -1923,10 +1944,19 @@ static double nn_min_distance(const double* lat, const std::vector<double>& frac
 }
 
 // LnSrch (MathLNsrch.f90:52-296) with its module state points(2,3) and dsave:
-// two bracket points (alpha, E, dE/dalpha), secant steps on dE/dalpha.
+// two bracket points (alpha, E, dE/dalpha), secant steps on dE/dalpha. The
+// reference reports its state as a text task whose first word (FG, CONV,
+// WARNING, ERROR) drives the caller; here that word is an enum, and `why`
+// numbers the branch that gave a warning or an error (1: no usable second
+// point yet, 2-3: secant step outside the bracket with the second point
+// first, 4-5: the points out of order, 6-7: secant step outside the bracket
+// after a positive slope, 8-9: the same after a negative slope).
+enum class LsTask { Start, FG, Conv, Warn, Error };
+
 struct IonLnSrch {
   double P[2][3];
   double d0 = 0, d1 = 0;   // dsave(1:2)
+  int why = 0;
   static double new_alp(double x1, double y1, double x2, double y2, double stpmin, double stpmax) {
     const double slope = (y1 - y2) / (x1 - x2);
     const double b = y1 - x1 * slope;
@@ -1935,40 +1965,36 @@ struct IonLnSrch {
   }
   double na(double stpmin, double stpmax) const { return new_alp(P[0][0], P[0][2], P[1][0], P[1][2], stpmin, stpmax); }
   void set(int k, double a, double f, double g) { P[k][0] = a; P[k][1] = f; P[k][2] = g; }
-  // task in/out: the reference's CHARACTER task, here as a string
-  void step(double& alp, double f, double g, std::string& task, double c1, double c2, double stpmin, double stpmax) {
-    if (task.compare(0, 5, "START") == 0) {
-      if (g > 0.0) { task = "ERROR: Initial G>0"; return; }
+  LsTask warn(int w) { why = w; return LsTask::Warn; }
+  LsTask error(int w) { why = w; return LsTask::Error; }
+  LsTask step(double& alp, double f, double g, LsTask task, double c1, double c2, double stpmin, double stpmax) {
+    if (task == LsTask::Start) {
+      if (g > 0.0) return error(0);   // not a descent direction
       for (auto& r : P) r[0] = r[1] = r[2] = -1.0;
       set(0, 0.0, f, g);
       d0 = f;
       d1 = g;
       P[1][0] = -1.0;
-      task = "FG" + task.substr(2);
-      return;
+      return LsTask::FG;
     }
-    if (task.compare(0, 2, "FG") == 0 && P[1][0] > 0.0) {
-      if (f <= d0 + c1 * alp * d1 && std::fabs(g) <= c2 * std::fabs(d1)) { task = "CONV"; return; }
+    if (task == LsTask::FG && P[1][0] > 0.0) {
+      if (f <= d0 + c1 * alp * d1 && std::fabs(g) <= c2 * std::fabs(d1)) return LsTask::Conv;
     }
     if (P[1][0] < 0.0) {
       if (g < P[0][2]) {
         alp = f < P[0][1] ? alp * 5.0 : (P[0][0] + alp) / 2.0;
-        task = "WARNING: Need a better second point.";
-        return;
+        return warn(1);
       }
       set(1, alp, f, g);
       alp = na(stpmin, stpmax);
       if (alp < 0.0 || alp < P[0][0]) {
         if (P[1][1] < P[0][1]) {
-          task = "WARNING: Interpolation error, code:-1, energy decreases, step further";
           alp = 5.0 * P[1][0];
-        } else {
-          task = "ERROR: Interpolation error, code:-1";
+          return warn(2);
         }
-        return;
+        return error(3);
       }
-      task = "FG";
-      return;
+      return LsTask::FG;
     }
     if (g > 0) {
       if (P[1][2] > 0.0) {
@@ -1977,20 +2003,16 @@ struct IonLnSrch {
           alp = na(stpmin, stpmax);
           if (alp < P[0][0]) {
             alp = (P[0][0] + P[1][0]) / 2.0;
-            task = "WARNING: Order flips, code:0, energy decreases, step further";
-          } else {
-            task = "FG";
+            return warn(4);
           }
-          return;
+          return LsTask::FG;
         }
         if (f < P[1][1]) {
-          task = "WARNINIG: Order flipped, code:1";
           alp = alp * 5.0;
-        } else {
-          alp = (P[0][0] + P[1][0]) / 2.0;
-          task = "WARNING: Order flipped, code:2";
+          return warn(5);
         }
-        return;
+        alp = (P[0][0] + P[1][0]) / 2.0;
+        return warn(5);
       }
       if (P[1][2] < 0.0) {
         std::memcpy(P[0], P[1], sizeof(P[0]));
@@ -1999,15 +2021,12 @@ struct IonLnSrch {
         if (alp < P[0][0] || alp > P[1][0]) {
           if (P[0][1] > P[1][1]) {
             alp = P[1][0] * 5.0;
-            task = "WARNING: Interpolation error, code:1, energy decreases, step further";
-          } else {
-            alp = (P[0][0] + P[1][0]) / 2.0;
-            task = "WARNING: Interpolation error, code:1";
+            return warn(6);
           }
-        } else {
-          task = "FG";
+          alp = (P[0][0] + P[1][0]) / 2.0;
+          return warn(7);
         }
-        return;
+        return LsTask::FG;
       }
     }
     if (g < 0.0) {
@@ -2016,16 +2035,13 @@ struct IonLnSrch {
         alp = na(stpmin, stpmax);
         if (alp < P[0][0] || alp > P[1][0]) {
           if (P[0][1] > P[1][1]) {
-            task = "WARNING: Interpolation, code:2, f decreases, step further";
             alp = 5.0 * P[1][0];
-          } else {
-            alp = (P[0][0] + P[1][0]) / 2.0;
-            task = "WARNING: Interpolation, code:4, brackets the minimum by half.";
+            return warn(8);
           }
-        } else {
-          task = "FG";
+          alp = (P[0][0] + P[1][0]) / 2.0;
+          return warn(8);
         }
-        return;
+        return LsTask::FG;
       }
       if (P[1][2] < 0.0) {
         std::memcpy(P[0], P[1], sizeof(P[0]));
@@ -2034,16 +2050,14 @@ struct IonLnSrch {
         if (alp < P[1][0]) {
           if (P[0][1] > P[1][1]) {
             alp = P[1][0] * 5.0;
-            task = "WARNING: Interpolation error: code:3, f decrease, step further";
-          } else {
-            task = "ERROR: Interpolation error: code:3";
+            return warn(9);
           }
-        } else {
-          task = "FG";
+          return error(9);
         }
-        return;
+        return LsTask::FG;
       }
     }
+    return task;   // g == 0 with the second point set: the reference leaves task as it was
   }
 };
 
@@ -2170,11 +2184,11 @@ extern "C" int ofd_ion_relax(ofd_ctx* c, double* rhoR, double* frac_xyz, const o
       const double stpmax = nn_dist > 0 ? std::fabs((nn_min_distance(c->lat, frac) - nn_dist) / dmax)
                                         : std::fabs(1.0 / dmax);
       alpha = std::min(alpha, stpmax);
-      std::string task = "START";
+      LsTask task = LsTask::Start;
       int nline = 0;
       bool done = false;
       for (;;) {
-        ls.step(alpha, et[0], grad_alp, task, ftol, gtol, stpmin, stpmax);
+        task = ls.step(alpha, et[0], grad_alp, task, ftol, gtol, stpmin, stpmax);
         if (nline >= 4 && std::fabs(ls.P[0][1] - ls.P[1][1]) < (double)1.e-6f) break;
         if (ls.P[0][0] > 0.0) {
           double bmax = 0.0;
@@ -2188,13 +2202,13 @@ extern "C" int ofd_ion_relax(ofd_ctx* c, double* rhoR, double* frac_xyz, const o
           }
           if (bmax < (double)1e-5f) break;
         }
-        if (task.compare(0, 4, "CONV") == 0) break;
-        if (task.compare(0, 5, "ERROR") == 0) {
+        if (task == LsTask::Conv) break;
+        if (task == LsTask::Error) {
           restart = true;
           alpha = 1.0;
           break;
         }
-        if (task.compare(0, 2, "FG") == 0 || task.compare(0, 4, "WARN") == 0) {
+        if (task == LsTask::FG || task == LsTask::Warn) {
           // coord = MODULO(cart2frac (oldCart + alpha dir), 1)
           for (int a = 0; a < nion; ++a)
             for (int j = 0; j < 3; ++j) {

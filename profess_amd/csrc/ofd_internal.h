@@ -94,6 +94,8 @@ struct ShapePlan {
   int lds_target = 65536;  // soft budget used for the automatic tiles
   int force_generic = 0;   // 1: never use the register-resident fast path
   int no_plane = 0;        // 1: never use the fused x+y plane kernel
+  int no_plane_inv = 1;    // 1: never use the fused inverse y+x plane kernel
+  int plane_fwd2 = 0;      // 1: the small-LDS fused x+y forward plane kernel where it applies
 };
 
 struct Tiles {
@@ -179,7 +181,7 @@ int build_wgc_kernels(const ShapePlan& sp, const double* rb, const double* d_eta
                       const double* d_pp2, int nknots, double tkfs, double gamma, double rhoS, double* d_out,
                       hipStream_t st);
 int build_ion_potential(const ShapePlan& sp, const double* rb, double vol, int ntypes, const int* h_type_start,
-                        const double* d_frac, int nion, const double* d_psp_vqs, const double* d_psp_dd,
+                        const double* h_frac, const double* d_frac, int nion, const double* d_psp_vqs, const double* d_psp_dd,
                         const int* h_psp_off, const int* h_psp_npts, const double* h_psp_maxg,
                         const double* h_psp_v0, const int* h_psp_charge, int order, double* d_vion,
                         void* d_scratch, hipStream_t st);
@@ -194,7 +196,8 @@ int ion_forces_spline(const ShapePlan& sp, const double* rb, const double* cell_
 // comps4: real, self, reciprocal, average-potential terms (Hartree); h_forces
 // (host, 3 per ion) may be NULL. Scratch as for build_ion_potential.
 int ewald_device(const ShapePlan& sp, const double* rb, const double* lat, double vol, int ntypes,
-                 const int* h_type_start, const double* h_charge, const double* d_frac, int nion, int order,
+                 const int* h_type_start, const double* h_charge, const double* h_frac, const double* d_frac,
+                 int nion, int order,
                  double eta, double rc, double kc, const double* cell_inv_t, double* comps4, double* h_forces,
                  void* d_scratch, hipStream_t st);
 

@@ -6,6 +6,7 @@
 //    (IonElectronForcesSpline / CalculateSplineForces, :219-355, :642-753)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -139,29 +140,164 @@ __device__ __forceinline__ void card_bspline(double* s, int order, double x) {
 
 constexpr int kMaxOrder = 16;
 
-// FillQIonTable (IonElectronSpline.f90:537-638): one thread per (ion, i3, i2)
-// stencil line; weights added with FP64 atomics.
-__global__ void k_qion_scatter(GridDims d, const double* __restrict__ frac, int ion0, int nion, int order,
-                               double weight, double* __restrict__ q) {
-  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const long long total = (long long)nion * order * order;
-  if (idx >= total) return;
-  const int ion = ion0 + (int)(idx / (order * order));
-  const int rem = (int)(idx % (order * order));
-  const int i3 = rem / order, i2 = rem % order;
-  const double g1 = frac[3 * ion] * d.n1, g2 = frac[3 * ion + 1] * d.n2, g3 = frac[3 * ion + 2] * d.n3;
-  const double f1 = floor(g1), f2 = floor(g2), f3 = floor(g3);
-  double M1[kMaxOrder + 1], M2[kMaxOrder + 1], M3[kMaxOrder + 1];
-  card_bspline<kMaxOrder>(M1, order, g1 - f1);
-  card_bspline<kMaxOrder>(M2, order, g2 - f2);
-  card_bspline<kMaxOrder>(M3, order, g3 - f3);
-  auto md = [](long long a, int n) { long long r = a % n; return (int)(r < 0 ? r + n : r); };
-  const int l3 = md(-(long long)f3 + i3, d.n3);
-  const int l2 = md(-(long long)f2 + i2, d.n2);
-  for (int ix = 0; ix < order; ++ix) {
-    const int l1 = md(-(long long)f1 + ix, d.n1);
-    atomicAdd(&q[((size_t)l3 * d.n2 + l2) * d.n1 + l1], M1[ix + 1] * M2[i2 + 1] * M3[i3 + 1] * weight);
+// FillQIonTable (IonElectronSpline.f90:537-638), as a gather with a fixed
+// summation order (no atomics, so the table is bitwise reproducible).
+// Ion i at fractional x has, along each axis of n points, the base
+// b = (-floor(n x)) mod n and covers the points (b + j) mod n, j < order,
+// with weight M[j + 1] (M the cardinal B-spline of n x - floor(n x)). Its
+// contribution to point l is w_i * S_1 S_2 S_3 with S = sum of M[j + 1] over
+// j < order, j = (l - b) mod n (+ n, + 2n, ... when n < order).
+// Prep: one thread per ion writes its three M tables (order + 1 each).
+__global__ void k_qion_prep(GridDims d, const double* __restrict__ frac, const int* __restrict__ ions, int nion,
+                            int order, double* __restrict__ mtab) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nion) return;
+  const int ion = ions[k];
+  const int n[3] = {d.n1, d.n2, d.n3};
+  for (int a = 0; a < 3; ++a) {
+    const double g = frac[3 * ion + a] * n[a];
+    double m[kMaxOrder + 1];
+    card_bspline<kMaxOrder>(m, order, g - floor(g));
+    for (int j = 0; j <= order; ++j) mtab[((size_t)k * 3 + a) * (kMaxOrder + 1) + j] = m[j];
   }
+}
+
+struct QBins {
+  int nb[3], bs[3];   // bins and points per bin along each axis
+};
+
+// Candidate bins along one axis for point l: the bins holding the bases
+// (l - j) mod n, j < order, in the order met (at most three distinct).
+__device__ __forceinline__ int qion_axis_bins(int l, int n, int order, int nb, int bs, int* out) {
+  if (nb == 1) {
+    out[0] = 0;
+    return 1;
+  }
+  int c = 0;
+  for (int j = 0; j < order; ++j) {
+    int b = l - j;
+    if (b < 0) b += n;
+    const int bin = b / bs;
+    if (c == 0 || out[c - 1] != bin) out[c++] = bin;
+  }
+  return c;
+}
+
+__device__ __forceinline__ double qion_axis_weight(int l, int b, int n, int order, const double* __restrict__ m) {
+  int j = l - b;
+  if (j < 0) j += n;
+  double s = 0.0;
+  for (; j < order; j += n) s += m[j + 1];
+  return s;
+}
+
+// One thread per grid point (x fastest): bins in (z, y, x) order, ions of a
+// bin in index order.
+__global__ void __launch_bounds__(256) k_qion_gather(GridDims d, QBins qb, const int* __restrict__ bin_start,
+                                                     const int* __restrict__ bin_ion, const int* __restrict__ base,
+                                                     const double* __restrict__ weight,
+                                                     const double* __restrict__ mtab, int order,
+                                                     double* __restrict__ q) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= d.nr) return;
+  const int l1 = (int)(idx % d.n1);
+  const long long r = idx / d.n1;
+  const int l2 = (int)(r % d.n2), l3 = (int)(r / d.n2);
+  int c1[3], c2[3], c3[3];
+  const int n1b = qion_axis_bins(l1, d.n1, order, qb.nb[0], qb.bs[0], c1);
+  const int n2b = qion_axis_bins(l2, d.n2, order, qb.nb[1], qb.bs[1], c2);
+  const int n3b = qion_axis_bins(l3, d.n3, order, qb.nb[2], qb.bs[2], c3);
+  constexpr int MS = kMaxOrder + 1;
+  double acc = 0.0;
+  for (int a3 = 0; a3 < n3b; ++a3)
+    for (int a2 = 0; a2 < n2b; ++a2)
+      for (int a1 = 0; a1 < n1b; ++a1) {
+        const int bin = (c3[a3] * qb.nb[1] + c2[a2]) * qb.nb[0] + c1[a1];
+        for (int k = bin_start[bin]; k < bin_start[bin + 1]; ++k) {
+          const int i = bin_ion[k];
+          const double* m = mtab + (size_t)i * 3 * MS;
+          const double s1 = qion_axis_weight(l1, base[3 * i], d.n1, order, m);
+          if (s1 == 0.0) continue;
+          const double s2 = qion_axis_weight(l2, base[3 * i + 1], d.n2, order, m + MS);
+          if (s2 == 0.0) continue;
+          const double s3 = qion_axis_weight(l3, base[3 * i + 2], d.n3, order, m + 2 * MS);
+          acc += weight[i] * (s1 * s2 * s3);
+        }
+      }
+  q[idx] = acc;
+}
+
+// The B-spline table of ions ion0 .. ion0 + nion - 1 (host fractional
+// coordinates h_frac, device copy d_frac) with per-ion weights h_w, written
+// to every point of q. The bins are built on the host: the base of an ion is
+// -floor(n x) mod n, the same IEEE product and floor the device evaluates.
+int qion_table(const GridDims& d, const double* h_frac, const double* d_frac, int ion0, int nion,
+               const double* h_w, int order, double* q, hipStream_t st) {
+  const int n[3] = {d.n1, d.n2, d.n3};
+  QBins qb;
+  for (int a = 0; a < 3; ++a) {
+    if (n[a] < 2 * order) {
+      qb.nb[a] = 1;
+      qb.bs[a] = n[a];
+    } else {
+      qb.bs[a] = order;
+      qb.nb[a] = (n[a] + order - 1) / order;
+    }
+  }
+  const int nbins = qb.nb[0] * qb.nb[1] * qb.nb[2];
+  std::vector<int> base(3 * (size_t)std::max(nion, 1)), binof(std::max(nion, 1)), start(nbins + 1, 0),
+      order_ions(std::max(nion, 1)), ions(std::max(nion, 1));
+  for (int k = 0; k < nion; ++k) {
+    const int ion = ion0 + k;
+    ions[k] = ion;
+    int bi[3];
+    for (int a = 0; a < 3; ++a) {
+      const double g = h_frac[3 * (size_t)ion + a] * n[a];
+      long long b = -(long long)std::floor(g);
+      b %= n[a];
+      if (b < 0) b += n[a];
+      base[3 * k + a] = (int)b;
+      bi[a] = (int)b / qb.bs[a];
+    }
+    binof[k] = (bi[2] * qb.nb[1] + bi[1]) * qb.nb[0] + bi[0];
+    ++start[binof[k] + 1];
+  }
+  for (int b = 0; b < nbins; ++b) start[b + 1] += start[b];
+  {
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int k = 0; k < nion; ++k) order_ions[fill[binof[k]]++] = k;   // stable: index order within a bin
+  }
+  std::vector<double> w(std::max(nion, 1));
+  for (int k = 0; k < nion; ++k) w[k] = h_w[k];
+  const size_t MS = kMaxOrder + 1;
+  int* di = (int*)dev_alloc(sizeof(int) * ((size_t)nbins + 1 + 5 * (size_t)std::max(nion, 1)));
+  double* dm = (double*)dev_alloc(sizeof(double) * (3 * MS + 1) * (size_t)std::max(nion, 1));
+  if (!di || !dm) {
+    dev_free(di);
+    dev_free(dm);
+    return -1;
+  }
+  int* d_start = di;
+  int* d_binion = d_start + nbins + 1;
+  int* d_base = d_binion + std::max(nion, 1);
+  int* d_ions = d_base + 3 * std::max(nion, 1);
+  double* d_w = dm;
+  double* d_mtab = dm + std::max(nion, 1);
+  hipMemcpyAsync(d_start, start.data(), sizeof(int) * (nbins + 1), hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(d_binion, order_ions.data(), sizeof(int) * std::max(nion, 1), hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(d_base, base.data(), sizeof(int) * 3 * std::max(nion, 1), hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(d_ions, ions.data(), sizeof(int) * std::max(nion, 1), hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(d_w, w.data(), sizeof(double) * std::max(nion, 1), hipMemcpyHostToDevice, st);
+  if (nion > 0)
+    hipLaunchKernelGGL(k_qion_prep, dim3((unsigned)((nion + 127) / 128)), dim3(128), 0, st, d, d_frac, d_ions, nion,
+                       order, d_mtab);
+  hipLaunchKernelGGL(k_qion_gather, dim3((unsigned)((d.nr + 255) / 256)), dim3(256), 0, st, d, qb, d_start,
+                     d_binion, d_base, d_w, d_mtab, order, q);
+  // the host vectors above must outlive the copies
+  const hipError_t e = hipStreamSynchronize(st);
+  dev_free(di);
+  dev_free(dm);
+  return e == hipSuccess && hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 struct PspDev {
@@ -356,7 +492,7 @@ static PspDev psp_dev(int s, const double* d_vqs, const double* d_dd, const int*
 
 // scratch layout: [real grid nr doubles][complex sf nc][complex acc nc]
 int build_ion_potential(const ShapePlan& sp, const double* rb, double vol, int ntypes, const int* h_type_start,
-                        const double* d_frac, int nion, const double* d_psp_vqs, const double* d_psp_dd,
+                        const double* h_frac, const double* d_frac, int nion, const double* d_psp_vqs, const double* d_psp_dd,
                         const int* h_psp_off, const int* h_psp_npts, const double* h_psp_maxg,
                         const double* h_psp_v0, const int* h_psp_charge, int order, double* d_vion,
                         void* d_scratch, hipStream_t st) {
@@ -375,11 +511,13 @@ int build_ion_potential(const ShapePlan& sp, const double* rb, double vol, int n
   const unsigned gb = (unsigned)((d.nc + 255) / 256);
   for (int s = 0; s < ntypes; ++s) {
     const int i0 = h_type_start[s], n_s = h_type_start[s + 1] - i0;
-    hipMemsetAsync(q, 0, sizeof(double) * d.nr, st);
-    const long long tot = (long long)n_s * order * order;
-    if (tot > 0)
-      hipLaunchKernelGGL(k_qion_scatter, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, d, d_frac, i0, n_s,
-                         order, 1.0, q);
+    {
+      const std::vector<double> ones(std::max(n_s, 1), 1.0);
+      if (qion_table(d, h_frac, d_frac, i0, n_s, ones.data(), order, q, st)) {
+        dev_free(dd);
+        return -1;
+      }
+    }
     D.u = q;
     D.work = sf;
     hipMemcpyAsync(dd, &D, sizeof(D), hipMemcpyHostToDevice, st);
@@ -580,7 +718,8 @@ __global__ void k_ewald_recip_f(GridDims d, RB rb, const double2* __restrict__ s
 
 // scratch layout: [real grid nr doubles][complex sf nc][complex tmp nc]
 int ewald_device(const ShapePlan& sp, const double* rb, const double* lat, double vol, int ntypes,
-                 const int* h_type_start, const double* h_charge, const double* d_frac, int nion, int order,
+                 const int* h_type_start, const double* h_charge, const double* h_frac, const double* d_frac,
+                 int nion, int order,
                  double eta, double rc, double kc, const double* cell_inv_t, double* comps4, double* h_forces,
                  void* d_scratch, hipStream_t st) {
   const GridDims& d = sp.d;
@@ -614,13 +753,9 @@ int ewald_device(const ShapePlan& sp, const double* rb, const double* lat, doubl
   double* tab = (double*)d_scratch;
   double2* sf = (double2*)(tab + d.nr);
   double2* tmp = sf + d.nc;
-  hipMemsetAsync(tab, 0, sizeof(double) * d.nr, st);
-  for (int s = 0; s < ntypes; ++s) {
-    const int i0 = h_type_start[s], n_s = h_type_start[s + 1] - i0;
-    const long long tot = (long long)n_s * order * order;
-    if (tot > 0)
-      hipLaunchKernelGGL(k_qion_scatter, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, d, d_frac, i0, n_s,
-                         order, h_charge[s], tab);
+  if (qion_table(d, h_frac, d_frac, 0, nion, q.data(), order, tab, st)) {
+    dev_free(qd); dev_free(eo); dev_free(fo); dev_free(fr); dev_free(part); dev_free(dd);
+    return -1;
   }
   ReqDesc D;
   std::memset(&D, 0, sizeof(D));

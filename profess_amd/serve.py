@@ -56,6 +56,12 @@ def _reps_for(species, n, max_atoms):
     return (r, r, rc)
 
 
+def atom_count(req: Request):
+    """Atoms in the request's supercell (4 per fcc cell, 2 per hcp cell)."""
+    r = req.reps
+    return (4 if req.species == "Al" else 2) * r[0] * r[1] * r[2]
+
+
 def make_requests(n, seed=0, grids=(32, 48, 64, 80, 96, 120, 128), max_atoms=500, rate=50.0):
     """Poisson arrivals at `rate` per second; species, grid and displacement
     amplitude (0.05-0.15 bohr) drawn from a seeded generator."""

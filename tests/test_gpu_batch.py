@@ -73,6 +73,34 @@ def test_mixed_precision_solve_within_1e5_ev_per_atom(pa):
     assert out[1][1]["fp32_evals"] > 0
 
 
+def test_serving_mix_within_1e5_ev_per_atom_of_fp64(pa):
+    """Sixteen requests over all seven serving grids (32 ... 128^3, fcc Al and
+    hcp Mg): each converged energy of the FP32-then-FP64 pool lies within
+    1e-5 eV/atom of the same request's plain FP64 solve (ofd_rho_opt alone)."""
+    from profess_amd import serve
+    grids = (32, 48, 64, 80, 96, 120, 128)
+    def mk():
+        reqs = serve.make_requests(16, seed=12, grids=grids, max_atoms=64, rate=1e6)
+        for i, r in enumerate(reqs):   # every grid at least twice, both species
+            g = grids[i % len(grids)]
+            r.grid = (g, g, g)
+            r.species = "Al" if i % 2 == 0 else "Mg"
+            r.reps = serve._reps_for(r.species, g, 64)
+        return reqs
+    pool_reqs = mk()
+    out = serve.run_workload(pool_reqs, mixed=True)
+    assert out["converged"] == len(pool_reqs)
+    plain = mk()
+    ref = serve.run_sequential(plain)
+    assert ref["converged"] == len(plain)
+    worst = 0.0
+    for a, b in zip(sorted(pool_reqs, key=lambda r: r.rid), sorted(plain, key=lambda r: r.rid)):
+        de = abs(a.energy[0] - b.energy[0]) * O.HARTREE_TO_EV / serve.atom_count(a)
+        worst = max(worst, de)
+        assert de < 1e-5, (a.rid, a.grid, de)
+    print("serving mix: max |E_pool - E_fp64| = %.2e eV/atom over %d requests" % (worst, len(plain)), flush=True)
+
+
 def test_graph_replays_match_eager(pa, monkeypatch):
     """A density optimization replays the captured evaluation graphs many
     times with eager vector work and synchronizations between the replays;

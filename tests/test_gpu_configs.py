@@ -6,7 +6,8 @@ must land within 1e-5 eV/atom of the FP64 solve; the per-term differences
 are printed.
 
 config 5: hcp Mg 2000 atoms (sigma 0.1 bohr), 256^3, WGC second order
-(gamma 2.7) on the reference's own RKSUITE kernel table. The NTN solve must
+(gamma 2.7) on the reference's own RKSUITE kernel table (the committed
+fixture tests/golden/wgc_table_default.npz). The NTN solve must
 converge; at the converged density every energy term must match the oracle to
 1e-10 relative and the ion-electron forces to 1e-6 Ha/bohr.
 """
@@ -16,7 +17,7 @@ import numpy as np
 import pytest
 
 from oracle import profess_oracle as O
-from oracle import refbuild as R
+from tests import wgc_tables as W
 
 pytestmark = pytest.mark.gpu
 
@@ -61,8 +62,6 @@ def test_config3_mixed_precision_192(pa):
 
 
 def test_config5_wgc_mg2000_forces(pa):
-    if not R.available():
-        pytest.skip("reference RKSUITE table builder not built")
     from profess_amd import inputs
     _say("config 5: hcp Mg 2000 atoms, 256^3, WGC")
     c = inputs.mg_hcp_case((10, 10, 10), (256, 256, 256), sigma=0.1, seed_ion=53)
@@ -71,7 +70,7 @@ def test_config5_wgc_mg2000_forces(pa):
     s = pa.OFDSystem(c["lattice"], c["grid"], ne, kinetic=5, gamma=2.7)
     s.build_ion_potential([(mg, vals, c["frac"])])
     p = s.params
-    table = R.wgc_eta_table(p["alpha"], p["beta"], p["gamma"])
+    table = W.for_params(p["alpha"], p["beta"], p["gamma"])
     s.set_wgc_table(*table)
     rho0 = np.full(s.shape, ne / s.vol)
     rho, et, oc, st = pa.ProjNormRhoMinimization(s, rho0, conv_check="POT", pot_tol=1e-5)

@@ -9,6 +9,7 @@ import pytest
 
 from oracle import profess_oracle as O
 from oracle import refbuild as R
+from tests import wgc_tables as W
 from tests import sample_cases as S
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -102,6 +103,50 @@ def test_uniform_density_closed_forms():
     rs = 0.62035049089940009 / rho0 ** (1 / 3)
     exc = -0.73855876638202234 * rho0 ** (1 / 3) + (-0.1423) / (1 + 1.0529 * math.sqrt(rs) + 0.3334 * rs)
     assert et[4] == pytest.approx(exc * rho0 * cell.vol, rel=1e-13)
+
+
+def _pz81_eps_c(rs):
+    """Perdew-Zunger correlation per electron (Perdew & Zunger, PRB 23, 5048
+    (1981), Appendix C): A ln rs + B + C rs ln rs + D rs for rs < 1,
+    gamma / (1 + beta1 sqrt(rs) + beta2 rs) for rs >= 1 (unpolarized)."""
+    A, B, C, D = 0.0311, -0.048, 0.0020, -0.0116
+    g, b1, b2 = -0.1423, 1.0529, 0.3334
+    hi = A * np.log(rs) + B + C * rs * np.log(rs) + D * rs
+    lo = g / (1 + b1 * np.sqrt(rs) + b2 * rs)
+    dhi = A / rs + C * (np.log(rs) + 1) + D
+    dlo = -g * (b1 / (2 * np.sqrt(rs)) + b2) / (1 + b1 * np.sqrt(rs) + b2 * rs) ** 2
+    return np.where(rs < 1, hi, lo), np.where(rs < 1, dhi, dlo)
+
+
+def test_lda_both_branches_from_definitions():
+    """LDA (XC_LDA.f90:51-310) at uniform densities on both sides of rs = 1,
+    against the definitions: Dirac exchange eps_x = -(3/4)(3/pi)^(1/3) rho^(1/3),
+    rs = (3/(4 pi rho))^(1/3), PZ81 correlation, and the potential
+    v = d(rho eps)/d rho = 4/3 eps_x + eps_c - (rs/3) d eps_c/d rs. This pins
+    the high-density branch (rs < 1, XC_LDA.f90:141-142 and :283-286) that
+    no sample run reaches, including its derived potential constants."""
+    cell = _tri_cell((4, 4, 4))
+    for rho0 in (0.5, 0.3, 0.027, 2.0):
+        rho = np.full(cell.shape, rho0)
+        rs = (3.0 / (4.0 * math.pi * rho0)) ** (1.0 / 3.0)
+        ex = -0.75 * (3.0 / math.pi) ** (1.0 / 3.0) * rho0 ** (1.0 / 3.0)
+        ec, dec = _pz81_eps_c(np.array(rs))
+        assert O.lda_energy(rho) == pytest.approx((ex + ec) * rho0 * rho.size, rel=1e-13)
+        v = 4.0 / 3.0 * ex + ec - rs / 3.0 * dec
+        np.testing.assert_allclose(O.lda_potential(rho), v, rtol=1e-13)
+    assert (3.0 / (4.0 * math.pi * 0.5)) ** (1.0 / 3.0) < 1.0
+
+
+def test_lda_potential_is_energy_derivative_across_rs_1():
+    """Finite-difference dE_xc/d rho against the LDA potential on densities
+    in [0.2, 0.6], on both sides of rs = 1 (rho = 0.2387)."""
+    rng = np.random.default_rng(9)
+    rho = 0.2 + 0.4 * rng.random(400)
+    rho = rho[np.abs(rho - 3.0 / (4.0 * math.pi)) > 1e-3]
+    assert np.any(rho < 0.2387) and np.any(rho > 0.2387)
+    h = 1e-6
+    fd = np.array([(O.lda_energy(np.array([r + h])) - O.lda_energy(np.array([r - h]))) / (2 * h) for r in rho])
+    np.testing.assert_allclose(O.lda_potential(rho), fd, rtol=1e-8)
 
 
 def test_potential_is_energy_derivative():
@@ -242,7 +287,6 @@ def test_reference_sample_optIon_wt_triclinic():
         assert h["energy"] == pytest.approx(ref, rel=2e-6)
 
 
-@needs_ref
 def test_reference_sample_optDen_wgc():
     """test/optDen sample (WGC, second order): same Newton inner-iteration counts
     per step and the same energies within the accuracy of the 1D kernel table
@@ -252,7 +296,7 @@ def test_reference_sample_optDen_wgc():
     vion = O.ion_potential(cell, [(psp, S.OPTDEN_FRAC)])
     f = O.Functional(kinetic=5, alpha=S.OPTDEN_ALPHA, beta=S.OPTDEN_BETA).resolve(12.0, cell.vol)
     sysm = O.System(cell, f, vion, S.OPTDEN_EII_EV / O.HARTREE_TO_EV)
-    eta, w, w1, w2 = R.wgc_eta_table(f.alpha, f.beta, f.gamma)
+    eta, w, w1, w2 = W.for_params(f.alpha, f.beta, f.gamma)
     sysm.wgc_kernels = O.wgc_kernels_from_table(sysm.qtable, eta, w, w1, w2, f)
     rho = np.full(cell.shape, 12.0 / cell.vol)
     full, pot = O.make_evaluators(sysm)
