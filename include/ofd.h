@@ -152,6 +152,22 @@ int ofd_calpot_device(ofd_ctx* ctx, const double* rho_device, int opt_sqrt, doub
 int ofd_calpot_enqueue(ofd_ctx* ctx, const double* rho_device, int opt_sqrt, double* potential_device,
                        double* sums9_device);
 int ofd_sums_to_etable(const ofd_ctx* ctx, const double* sums9, double* etable9);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory during a call (code under HIP stream capture, serving loops that
+ * must not allocate). The evaluation of ofd_calpot_enqueue, enqueued on the
+ * context's stream and not waited for, with all of its per-call device
+ * scratch (descriptor, energy partial sums, the transformed fields' spectra)
+ * in one static device array of ofd_noalloc_limit_bytes() = 64 MiB that each
+ * device holds until the process ends: no allocation, free, copy, memset or
+ * graph. rho_device, potential_device: n1*n2*n3 device arrays; etable9_device:
+ * 9 device doubles receiving eTable (Hartree), or NULL. A grid and functional
+ * whose scratch exceeds the limit (Wang-Teter with alpha == beta up to
+ * 128^3) is refused with a nonzero return and ofd_last_error. All calls on
+ * one device share the array: calls that overlap in time on one device race
+ * (order them on one stream). */
+int ofd_calpot_noalloc(ofd_ctx* ctx, const double* rho_device, int opt_sqrt, double* potential_device,
+                       double* etable9_device);
+long long ofd_noalloc_limit_bytes(void);
 /* the context's HIP stream (hipStream_t) and a blocking wait on it */
 void* ofd_get_stream(ofd_ctx* ctx);
 int ofd_sync(ofd_ctx* ctx);

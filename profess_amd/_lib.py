@@ -75,7 +75,8 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_wgc_kernel_table",
            "ofd_calpot", "ofd_calpot_device", "ofd_rho_opt", "ofd_ion_forces", "ofd_pool_create",
            "ofd_pool_destroy", "ofd_pool_last_error", "ofd_pool_admit", "ofd_pool_step", "ofd_pool_collect",
-           "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable", "ofd_get_stream", "ofd_sync",
+           "ofd_request_bytes", "ofd_calpot_enqueue", "ofd_sums_to_etable",
+           "ofd_calpot_noalloc", "ofd_noalloc_limit_bytes", "ofd_get_stream", "ofd_sync",
            "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished", "ofd_server_create",
            "ofd_server_add_pool", "ofd_server_wait", "ofd_server_last_error", "ofd_server_destroy",
            "ofd_get_volume", "ofd_ewald", "ofd_ion_relax", "ofd_read_geometry", "ofd_free_geometry",
@@ -131,6 +132,9 @@ def lib():
     L.ofd_request_bytes.restype = C.c_longlong
     L.ofd_calpot_enqueue.argtypes = [vp, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     L.ofd_sums_to_etable.argtypes = [vp, dp, dp]
+    L.ofd_calpot_noalloc.argtypes = [vp, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.ofd_noalloc_limit_bytes.argtypes = []
+    L.ofd_noalloc_limit_bytes.restype = C.c_longlong
     L.ofd_get_stream.argtypes = [vp]
     L.ofd_get_stream.restype = C.c_void_p
     L.ofd_sync.argtypes = [vp]

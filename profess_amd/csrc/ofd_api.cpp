@@ -564,7 +564,8 @@ extern "C" int ofd_get_ion_potential(ofd_ctx* c, double* vion) {
 }
 
 // eTable from the finalized sums (CalculateEnergy, CalPotential.f90:610-644)
-static void etable_calc(const double* s, double lam, double mu, double dV, double eii, double* et) {
+static __host__ __device__ void etable_calc(const double* s, double lam, double mu, double dV, double eii,
+                                            double* et) {
   double loc[9] = {0};
   loc[2] = s[0];
   loc[3] = s[1] / 2.0;
@@ -631,6 +632,84 @@ extern "C" int ofd_calpot_enqueue(ofd_ctx* c, const double* rho, int opt_sqrt, d
   if (launch_eval(c->graphs, c->sp, c->kind, kInRho, kOutPot, fp32, c->desc, 1, sums_dev ? sums_dev : c->fin,
                   c->st))
     return fail(c, "kernel launch failed");
+  return 0;
+}
+
+// ====================================================================== evaluation without allocation
+// ofd_calpot_noalloc: the evaluation of ofd_calpot_enqueue with its device
+// scratch (descriptor, finalized sums, per-workgroup partial sums, the
+// spectra of the transformed fields) in one file-scope device array instead
+// of the context's buffers, launched eagerly: no allocation, no graph, no
+// copy and no memset, so that it can be called inside a stream capture or a
+// loop that must not allocate. Every buffer in the array is written before it
+// is read within one call, so nothing needs clearing between calls; calls
+// that overlap on one device share the array and race.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_put_desc(ReqDesc d, ReqDesc* dst) { *dst = d; }
+__global__ void k_etable_dev(const double* s, double lam, double mu, double dV, double eii, double* et) {
+  etable_calc(s, lam, mu, dV, eii, et);
+}
+
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+}  // namespace
+
+// Bytes of scratch an evaluation of context c takes: descriptor, finalized
+// sums, partial sums, spectra (FP64 size, as the context's own buffers).
+static size_t noalloc_bytes(const ofd_ctx* c, size_t* off_fin, size_t* off_sums, size_t* off_work) {
+  const size_t nrows = (size_t)c->sp.d.n2 * c->sp.d.n3;
+  *off_fin = up256(sizeof(ReqDesc));
+  *off_sums = *off_fin + up256(kNSum * sizeof(double));
+  *off_work = *off_sums + up256(nrows * kNSum * sizeof(double));
+  return *off_work + (size_t)func_nf(c->kind) * (size_t)c->sp.d.nc * sizeof(double2);
+}
+
+extern "C" long long ofd_noalloc_limit_bytes(void) { return (long long)kNoAllocBytes; }
+
+extern "C" int ofd_calpot_noalloc(ofd_ctx* c, const double* rho, int opt_sqrt, double* pot, double* etable9) {
+  if (!c || !rho || !pot) return fail(c, "null argument");
+  if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set (ofd_set_wgc_table)");
+  size_t off_fin, off_sums, off_work;
+  const size_t need = noalloc_bytes(c, &off_fin, &off_sums, &off_work);
+  if (need > kNoAllocBytes)
+    return fail(c, "ofd_calpot_noalloc: this grid and functional need " + std::to_string(need) +
+                       " bytes of scratch, above its limit of " + std::to_string(kNoAllocBytes) +
+                       " bytes (use ofd_calpot_enqueue)");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return fail(c, "no current HIP device");
+  unsigned char* base;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+        return fail(c, "ofd_calpot_noalloc: device scratch address unavailable");
+      g_scratch_ptr[dev] = p;
+    }
+    base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  }
+  ReqDesc D = c->base_desc();
+  D.u = rho;
+  D.out = pot;
+  D.opt_sqrt = opt_sqrt ? 1 : 0;
+  D.fin = reinterpret_cast<double*>(base + off_fin);
+  D.sums = reinterpret_cast<double*>(base + off_sums);
+  D.work = base + off_work;
+  ReqDesc* dd = reinterpret_cast<ReqDesc*>(base);
+  hipLaunchKernelGGL(k_put_desc, dim3(1), dim3(1), 0, c->st, D, dd);
+  const int fp32 = c->precision == 1;
+  if (run_eval(c->sp, c->kind, kInRho, kOutPot, fp32, dd, 1, c->st) ||
+      run_finalize(c->sp, c->kind, fp32, dd, 1, D.fin, c->st))
+    return fail(c, "kernel launch failed");
+  if (etable9)
+    hipLaunchKernelGGL(k_etable_dev, dim3(1), dim3(1), 0, c->st, D.fin, c->lam, c->mu, c->dV, c->e_ionion, etable9);
+  if (hipGetLastError() != hipSuccess) return fail(c, "kernel launch failed");
   return 0;
 }
 
