@@ -77,9 +77,7 @@ bool fast_plane_available(const ShapePlan& sp, int kind, int fp32) {
 int fast_finalize(const ShapePlan& sp, int kind, int, const ReqDesc* dd, int nb, double*, hipStream_t st) {
   const int rpb = fast::inv_rpb(func_nf(kind), len_T(sp.d.M));
   const long long nrows = (long long)sp.d.n2 * sp.d.n3;
-  // epilogue workgroups: two per z-plane in the fused inverse plane pass,
-  // one per rpb rows in the x-inverse pass
-  const int nwg = plane_inv_ok_impl(sp, kind) ? 2 * sp.d.n3 : (int)((nrows + rpb - 1) / rpb);
+  const int nwg = (int)((nrows + rpb - 1) / rpb);
   hipLaunchKernelGGL(fast::k_fin, dim3(kNSum, nb), dim3(256), 0, st, dd, nwg);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

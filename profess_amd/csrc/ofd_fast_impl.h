@@ -406,161 +406,6 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
   }
 }
 
-// ------------------------------------------------------------ fused x + y forward, small LDS
-// One workgroup per (z-plane i3, field f, request), the fast form of
-// k_plane_fwd for n2 = n1 in {32, 64, 128}: 70-74 KB of LDS and at most 128
-// VGPRs, so two workgroups share a CU and one's transforms overlap the
-// other's loads and stores.
-//  x step (XR rounds of the plane's rows): the field from rho, the
-//    half-length complex FFT, the R2C split; the Nyquist value X[M] of a
-//    row (real, as X[0]) is carried as the imaginary part of column 0, so a
-//    row has M complex values; each thread keeps its values in registers.
-//  transpose to columns through LDS one real component at a time (a plane
-//    of M x n2 reals, 64 KB at 128^2).
-//  y step: column c (fastest) by TY lanes, the forward y FFT; column 0's
-//    transform P = F[X0] + i F[XM] of two real columns is split with the
-//    Hermitian symmetry, F[X0](k) = (P(k) + conj P(-k))/2 and
-//    F[XM](k) = (P(k) - conj P(-k))/(2i), and stored as columns 0 and M.
-template <int M, int N2> struct PlaneFwdGeo {
-  static constexpr int R1X = P2<M>::R1, TX = P2<M>::R2;
-  static constexpr int R1Y = P2<N2>::R1, TY = P2<N2>::R2;
-  static constexpr int THREADS = M * TY;           // y step: M columns
-  static constexpr int XROWS = THREADS / TX;       // rows per x round
-  static constexpr int XR = N2 / XROWS;            // x rounds
-  static constexpr int LLX = line_lds<M>();        // x exchange line (complex)
-  static constexpr int PS = M | 1;                 // transpose row stride (reals), odd
-  static constexpr int CS = col_stride<N2>();      // y exchange column stride (reals)
-  static_assert(XROWS * XR == N2 && XR >= 1, "the x rounds cover the plane");
-  static_assert(THREADS % 64 == 0, "whole waves");
-};
-template <int M, int N2, typename R> constexpr size_t plane_fwd2_lds() {
-  using G = PlaneFwdGeo<M, N2>;
-  const size_t a = (size_t)G::XROWS * G::LLX * sizeof(typename V2<R>::T);
-  const size_t b = (size_t)N2 * G::PS * sizeof(R);
-  const size_t c = (size_t)M * G::CS * sizeof(R);
-  const size_t m = a > b ? a : b;
-  return m > c ? m : c;
-}
-
-template <int M, int N2, int KIND, int INMODE, typename R>
-__global__ void __launch_bounds__((PlaneFwdGeo<M, N2>::THREADS),
-                                  (PlaneFwdGeo<M, N2>::THREADS >= 256 ? PlaneFwdGeo<M, N2>::THREADS / 128 : 1))
-k_plane_fwd2(const ReqDesc* __restrict__ descs, GridDims d, const typename V2<R>::T* __restrict__ twx,
-             const typename V2<R>::T* __restrict__ tws, const typename V2<R>::T* __restrict__ twy) {
-  using C = typename V2<R>::T;
-  using G = PlaneFwdGeo<M, N2>;
-  constexpr int R1X = G::R1X, TX = G::TX, R1Y = G::R1Y, TY = G::TY, XR = G::XR, XROWS = G::XROWS;
-  constexpr int NFK = NF_<KIND>::v;
-  constexpr int N1 = 2 * M;
-  extern __shared__ __align__(16) unsigned char smem[];
-  C* xl = reinterpret_cast<C*>(smem);
-  R* pl = reinterpret_cast<R*>(smem);
-  const ReqDesc& D = descs[blockIdx.y];
-  const int pf = xcd_block(blockIdx.x, gridDim.x);
-  const int i3 = pf / NFK, f = pf - i3 * NFK;
-  const int tid = threadIdx.x;
-  const int r = tid / TX, tx = tid % TX;      // x roles
-  const int c = tid % M, t = tid / M;         // y roles
-  // ---- x step: every round leaves the thread's R1X values of its row in xo
-  C xo[XR][R1X];
-#pragma unroll
-  for (int q = 0; q < XR; ++q) {
-    const int row = q * XROWS + r;
-    const size_t base = ((size_t)i3 * N2 + row) * N1;
-    C z[R1X];
-#pragma unroll
-    for (int m = 0; m < R1X; ++m) {
-      const int k = tx + TX * m;
-      const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + base + 2 * k)));
-      double a, b;
-      if (INMODE == kInRho) {
-        a = uu.x;
-        b = uu.y;
-      } else {
-        double ta = D.ca * uu.x, tb = D.ca * uu.y;
-        if (D.cb != 0.0) {
-          const double2 vv = ldc(glb(reinterpret_cast<const double2*>(D.v + base + 2 * k)));
-          ta = fma(D.cb, vv.x, ta);
-          tb = fma(D.cb, vv.y, tb);
-        }
-        a = ta * ta;
-        b = tb * tb;
-      }
-      z[m].x = (R)field_of<KIND>(D, f, a);
-      z[m].y = (R)field_of<KIND>(D, f, b);
-    }
-    C* line = xl + (size_t)r * G::LLX;
-    fft2<M, -1>(z, line, tx, twx, true);
-#pragma unroll
-    for (int m = 0; m < R1X; ++m) line[tx + TX * m] = z[m];
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < R1X; ++m) {
-      const int k = tx + TX * m;
-      const C zk = line[k];
-      const C zm = line[k == 0 ? 0 : M - k];
-      const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
-      const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
-      const C w = tws[k];
-      C x;
-      x.x = er + (w.x * orr - w.y * oi);
-      x.y = ei + (w.x * oi + w.y * orr);
-      if (k == 0) {
-        // X[0] = Z0.x + Z0.y and X[M] = Z0.x - Z0.y, both real: packed
-        x.x = zk.x + zk.y;
-        x.y = zk.x - zk.y;
-      }
-      xo[q][m] = x;
-    }
-    __syncthreads();
-  }
-  // ---- transpose rows -> columns, one real component at a time
-  C y[R1Y];
-#pragma unroll
-  for (int part = 0; part < 2; ++part) {
-#pragma unroll
-    for (int q = 0; q < XR; ++q)
-#pragma unroll
-      for (int m = 0; m < R1X; ++m)
-        pl[(size_t)(q * XROWS + r) * G::PS + tx + TX * m] = part ? xo[q][m].y : xo[q][m].x;
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < R1Y; ++m) {
-      const R v = pl[(size_t)(t + TY * m) * G::PS + c];
-      if (part) y[m].y = v; else y[m].x = v;
-    }
-    __syncthreads();
-  }
-  // ---- y step
-  fft2s<N2, -1>(y, pl + (size_t)c * G::CS, t, twy, true);
-  gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)i3 * (N2 * (M + 1));
-  if (c != 0) {
-#pragma unroll
-    for (int m = 0; m < R1Y; ++m) stc(out + (size_t)(t + TY * m) * (M + 1) + c, y[m]);
-  }
-  // column 0: separate F[X0] and F[XM] (P(-k) from the column's other lanes)
-  C* u = reinterpret_cast<C*>(smem);
-  if (c == 0) {
-#pragma unroll
-    for (int m = 0; m < R1Y; ++m) u[t + TY * m] = y[m];
-  }
-  __syncthreads();
-  if (c == 0) {
-#pragma unroll
-    for (int m = 0; m < R1Y; ++m) {
-      const int k = t + TY * m;
-      const C p = y[m], pn = u[(N2 - k) & (N2 - 1)];
-      C a, b;
-      a.x = (R)0.5 * (p.x + pn.x);
-      a.y = (R)0.5 * (p.y - pn.y);
-      b.x = (R)0.5 * (p.y + pn.y);
-      b.y = (R)-0.5 * (p.x - pn.x);
-      stc(out + (size_t)k * (M + 1), a);
-      stc(out + (size_t)k * (M + 1) + M, b);
-    }
-  }
-}
-
 // ------------------------------------------------------------ y columns
 // columns flattened over (i3, k1); blockDim = KB * T, thread = (t, c), c fastest
 template <int N, int S, typename R>
@@ -817,270 +662,6 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
     for (int m = 0; m < R1; ++m) mine[t + T * m] = z[m];
   }
   x_epilogue<M, KIND, INMODE, OUTMODE, RPB, RPB * NF * T, LL>(D, reinterpret_cast<const R*>(lds), row0, nrows, blk);
-}
-
-// ------------------------------------------------------------ fused y + x inverse + epilogue (half plane)
-// One workgroup per (z-plane i3, row half h, request), Wang-Teter with
-// alpha == beta (three fields). For each field in turn it
-//  * loads the plane's K1 spectrum columns,
-//  * runs the inverse y FFT of every column, with the Nyquist column k1 = M
-//    carried in the imaginary part of column 0: after the inverse z and y
-//    transforms both columns are real, so one complex transform of
-//    X[.,0] + i X[.,M] gives both,
-//  * keeps the N2/2 output rows of its half in LDS and runs their C2R merge
-//    and inverse x FFT,
-//  * adds the field's share to the potential of each of its points.
-// The potential and the energy sums are linear in the three
-// inverse-transformed fields, with coefficients that depend on rho only:
-//   opt_sqrt: V = 2 phi (vion + vxc + vtf) + 2 phi o0 + 2 phi kTF (a rho^(a-1) + b rho^(b-1)) o1 + mu o2
-//   else:     V =        (vion + vxc + vtf) +       o0 +       kTF (a rho^(a-1) + b rho^(b-1)) o1 + mu o2 / (2 phi)
-// so a thread keeps, for the 2*R1X points its x transform produces (the same
-// points for every field), rho^(-1/6) and the running potential in registers
-// and rho in LDS. The
-// y transform of all columns is done by both halves of a plane (they share
-// the loads through the XCD's L2); in exchange no spectrum goes back to HBM
-// between the y and x inverse transforms.
-template <int M, int N2> struct PlaneInvGeo {
-  static constexpr int R1Y = P2<N2>::R1, TY = P2<N2>::R2;
-  static constexpr int R1X = P2<M>::R1, TX = P2<M>::R2;
-  static constexpr int HR = N2 / 2;          // rows per half
-  static constexpr int NTY = M * TY;         // y: M columns (column 0 carries M too)
-  static constexpr int NTX = HR * TX;        // x: HR rows
-  static constexpr int THREADS = ((NTY > NTX ? NTY : NTX) + 63) / 64 * 64;
-  static constexpr int LLX = line_lds<M>();  // row stride (complex): holds M values and the x exchange
-  static constexpr int CS = col_stride<N2>();   // y column scratch stride (reals)
-  static constexpr int NP = 2 * R1X;         // points per x thread
-  static_assert(R1Y % 2 == 0, "half of a column's outputs per thread");
-  static_assert(LLX >= M, "a row holds its M complex values");
-};
-// LDS: the row / column-scratch region, then rho of every point of the
-// half plane (point-major: [j][thread], conflict-free).
-template <int M, int N2, typename R> constexpr size_t plane_inv_lds_rows() {
-  using G = PlaneInvGeo<M, N2>;
-  const size_t a = (size_t)G::HR * G::LLX * sizeof(typename V2<R>::T);
-  const size_t b = (size_t)M * G::CS * sizeof(R);
-  return ((a > b ? a : b) + 15) / 16 * 16;
-}
-template <int M, int N2, typename R> constexpr size_t plane_inv_lds() {
-  using G = PlaneInvGeo<M, N2>;
-  return plane_inv_lds_rows<M, N2, R>() + (size_t)G::NTX * G::NP * sizeof(double);
-}
-
-// Column c of a plane (rows of K1 = M + 1 complex): rows t + TY m; column 0
-// also takes column M into its imaginary part. Offsets are compile-time
-// multiples of the row length from one per-thread pointer.
-template <int M, int N2, typename C>
-__device__ __forceinline__ void pinv_load(gptr<const C> plane, int c, int t, bool ay, C (&x)[P2<N2>::R1]) {
-  constexpr int R1Y = P2<N2>::R1, TY = P2<N2>::R2, K1 = M + 1;
-  if (ay) {
-    gptr<const C> p = plane + (t * K1 + c);
-#pragma unroll
-    for (int m = 0; m < R1Y; ++m) x[m] = ldc(p + TY * K1 * m);
-    if (c == 0) {
-#pragma unroll
-      for (int m = 0; m < R1Y; ++m) {
-        const C q = ldc(p + (TY * K1 * m + M));
-        x[m].x -= q.y;
-        x[m].y += q.x;
-      }
-    }
-  }
-}
-
-template <int M, int N2, int INMODE, int OUTMODE, typename R>
-__global__ void __launch_bounds__((PlaneInvGeo<M, N2>::THREADS))
-k_plane_inv(const ReqDesc* __restrict__ descs, GridDims d, const typename V2<R>::T* __restrict__ twx,
-            const typename V2<R>::T* __restrict__ tws, const typename V2<R>::T* __restrict__ twy) {
-  using C = typename V2<R>::T;
-  using G = PlaneInvGeo<M, N2>;
-  constexpr int R1Y = G::R1Y, TY = G::TY, R1X = G::R1X, TX = G::TX, HR = G::HR, LLX = G::LLX, NP = G::NP;
-  constexpr int N1 = 2 * M;
-  extern __shared__ __align__(16) unsigned char smem[];
-  C* rows = reinterpret_cast<C*>(smem);
-  R* ysc = reinterpret_cast<R*>(smem);
-  double* lrho = reinterpret_cast<double*>(smem + plane_inv_lds_rows<M, N2, R>());
-  const ReqDesc& D = descs[blockIdx.y];
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int i3 = blk >> 1, h = blk & 1;
-  const int tid = threadIdx.x;
-  // y roles: column c (fastest), lane t
-  const int c = tid % M, t = tid / M;
-  const bool ay = tid < G::NTY;
-  // x roles: row r, lane tx; points (h*HR + r, 2(tx + TX m) + e)
-  const int r = tid / TX, tx = tid % TX;
-  const bool ax = tid < G::NTX;
-  const size_t prow = ((size_t)i3 * N2 + (size_t)(h * HR + r)) * N1;   // first point of the thread's row
-  const bool five6 = D.alpha == 5.0 / 6.0 && D.beta == 5.0 / 6.0;
-  double acc[kNSum];
-#pragma unroll
-  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
-  gptr<const C> ws = glb(reinterpret_cast<const C*>(D.work)) + (size_t)i3 * (N2 * (M + 1));
-  C cur[R1Y];
-  pinv_load<M, N2>(ws, c, t, ay, cur);
-  // ---- per-point setup: rho (to LDS) and rho^(-1/6); the rho-only part of
-  // the potential is added at the end, when the transform registers are free
-  double py[NP], pot[NP];
-  if (ax) {
-#pragma unroll
-    for (int m = 0; m < R1X; ++m) {
-      const size_t g = prow + 2 * (size_t)(tx + TX * m);
-      const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + g)));
-      double2 vv = make_double2(0.0, 0.0);
-      if (INMODE == kInPhi && D.cb != 0.0) vv = ldc(glb(reinterpret_cast<const double2*>(D.v + g)));
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const double u = e ? uu.y : uu.x;
-        double rho;
-        if (INMODE == kInRho) {
-          rho = u;
-        } else {
-          const double v = e ? vv.y : vv.x;
-          const double tphi = D.cb != 0.0 ? fma(D.cb, v, D.ca * u) : D.ca * u;
-          rho = tphi * tphi;
-        }
-        lrho[(size_t)(2 * m + e) * G::NTX + tid] = rho;
-        py[2 * m + e] = rho_m16(rho);
-        pot[2 * m + e] = 0.0;
-      }
-    }
-  }
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    if (f > 0) pinv_load<M, N2>(ws + (size_t)f * d.nc, c, t, ay, cur);
-    // inverse y FFT of the plane's columns (column scratch: ysc + c * CS)
-    fft2s<N2, +1>(cur, ysc + (size_t)c * G::CS, t, twy, ay);
-    if (ay) {
-#pragma unroll
-      for (int m = 0; m < R1Y; ++m) {
-        const int lr = t + TY * m - h * HR;   // local output row
-        if (lr >= 0 && lr < HR) rows[(size_t)lr * LLX + c] = cur[m];
-      }
-    }
-    __syncthreads();
-    // C2R merge of the half's rows (X[0] = Re p, X[M] = Im p of the packed column)
-    C z[R1X];
-    if (ax) {
-      const C* row = rows + (size_t)r * LLX;
-#pragma unroll
-      for (int m = 0; m < R1X; ++m) {
-        const int k = tx + TX * m;
-        C xk, xm;
-        if (k == 0) {
-          const C p0 = row[0];
-          xk.x = p0.x; xk.y = (R)0;
-          xm.x = p0.y; xm.y = (R)0;
-        } else {
-          xk = row[k];
-          xm = row[M - k];
-        }
-        const R er = xk.x + xm.x, ei = xk.y - xm.y;
-        const R dr = xk.x - xm.x, di = xk.y + xm.y;
-        const C w = tws[k];
-        const R orr = w.x * dr + w.y * di, oi = w.x * di - w.y * dr;
-        z[m].x = er - oi;
-        z[m].y = ei + orr;
-      }
-    }
-    __syncthreads();
-    fft2<M, +1>(z, rows + (size_t)r * LLX, tx, twx, ax);
-    // the field's share of the potential and of the energy sums
-    if (ax) {
-#pragma unroll
-      for (int m = 0; m < R1X; ++m)
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const double o = (double)(e ? z[m].y : z[m].x);
-          const double rho = lrho[(size_t)(2 * m + e) * G::NTX + tid], y = py[2 * m + e];
-          const double phi = rho * (y * y) * y;
-          if (f == 0) {
-            acc[1] += o * rho;
-            pot[2 * m + e] += (D.opt_sqrt ? 2.0 * phi : 1.0) * o;
-          } else if (f == 1) {
-            double ra, cw;
-            if (five6) {
-              ra = rho * y;
-              cw = kCTF * (D.alpha + D.beta) * y;
-            } else {
-              ra = pow(rho, D.alpha);
-              cw = kCTF * (D.alpha * pow(rho, D.alpha - 1.0) + D.beta * pow(rho, D.beta - 1.0));
-            }
-            acc[5] += ra * o;
-            pot[2 * m + e] += (D.opt_sqrt ? 2.0 * phi * cw : cw) * o;
-          } else {
-            acc[4] += phi * o;
-            pot[2 * m + e] += D.opt_sqrt ? D.mu * o : D.mu * o / (2.0 * phi);
-          }
-        }
-    }
-  }
-  // ---- the rho-only part of the potential and energies, then the outputs
-  if (ax) {
-#pragma unroll
-    for (int m = 0; m < R1X; ++m) {
-      const size_t g = prow + 2 * (size_t)(tx + TX * m);
-      const double2 vi = ldc(glb(reinterpret_cast<const double2*>(D.vion + g)));
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const double rho = lrho[(size_t)(2 * m + e) * G::NTX + tid], vion = e ? vi.y : vi.x;
-        PtVals pv;
-        point_vals(rho, D.alpha, D.beta, pv);
-        const double r23 = pv.r13 * pv.r13;
-        const double vtf = (5.0 / 3.0 * kCTF * D.lam) * r23;
-        acc[0] += rho * vion;
-        acc[2] += pv.exc;
-        acc[3] += rho * r23;
-        const double base = vion + pv.vxc + vtf;
-        pot[2 * m + e] += D.opt_sqrt ? 2.0 * pv.phi * base : base;
-      }
-      double2 o2;
-      if (OUTMODE == kOutPot) {
-        o2 = make_double2(pot[2 * m], pot[2 * m + 1]);
-      } else {
-        const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + g)));
-        double2 vv = make_double2(0.0, 0.0);
-        if (D.v != nullptr) vv = ldc(glb(reinterpret_cast<const double2*>(D.v + g)));
-        double2 bb = make_double2(0.0, 0.0), ww = make_double2(0.0, 0.0);
-        if (OUTMODE == kOutNewton) {
-          bb = ldc(glb(reinterpret_cast<const double2*>(D.bvec + g)));
-          if (D.w != nullptr) ww = ldc(glb(reinterpret_cast<const double2*>(D.w + g)));
-        }
-        double res[2];
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const double u = e ? uu.y : uu.x, v = e ? vv.y : vv.x;
-          const double tphi = D.cb != 0.0 ? fma(D.cb, v, D.ca * u) : D.ca * u;
-          const double de = pot[2 * m + e] * fsign(tphi);
-          if (OUTMODE == kOutSigned) {
-            res[e] = de;
-            const double dir = D.s1 * u + (D.v != nullptr ? D.s2 * v : 0.0);
-            acc[6] += de * dir;
-            const double dl = de - 2.0 * tphi * D.mu0;
-            acc[7] += dl * dl;
-            acc[8] += de * tphi;
-            acc[9] += de * de;
-            acc[10] += tphi * tphi;
-          } else {
-            const double b = e ? bb.y : bb.x, w = e ? ww.y : ww.x;
-            const double ap = ((de - D.mu0 * 2.0 * tphi) - b) / D.eps;
-            res[e] = ap;
-            acc[6] += v * ap;
-            if (D.w != nullptr) {
-              acc[7] += w * ap;
-              acc[8] += ap * ap;
-            }
-          }
-        }
-        o2 = make_double2(res[0], res[1]);
-      }
-      gptr<double2> out = glb(reinterpret_cast<double2*>(D.out + g));
-      out->x = o2.x;
-      out->y = o2.y;
-      // one pair of points at a time (interleaving all of them keeps every
-      // point's temporaries live together)
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-  block_sum_strided<kNSum>(acc, glb(D.sums) + blk, gridDim.x);
 }
 
 // ------------------------------------------------------------ general two-stage lengths
@@ -1622,26 +1203,10 @@ static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStre
   return true;
 }
 
-template <typename R, int M, int N2, int KIND, int INMODE>
-static void launch_plane2(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st,
-                          const typename V2<R>::T* twx, const typename V2<R>::T* tws,
-                          const typename V2<R>::T* twy) {
-  using namespace fast;
-  constexpr size_t lds = plane_fwd2_lds<M, N2, R>();
-  allow_lds<k_plane_fwd2<M, N2, KIND, INMODE, R>>(lds);
-  hipLaunchKernelGGL((k_plane_fwd2<M, N2, KIND, INMODE, R>), dim3(sp.d.n3 * NF_<KIND>::v, nb),
-                     dim3(PlaneFwdGeo<M, N2>::THREADS), lds, st, dd, sp.d, twx, tws, twy);
-}
-
 template <typename R, int KIND, int INMODE>
 static bool try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st, const typename V2<R>::T* twx,
                       const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
   if (sp.no_plane) return false;
-  if (sp.plane_fwd2) {
-    if (sp.d.M == 64 && sp.d.n2 == 128) return launch_plane2<R, 64, 128, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy), true;
-    if (sp.d.M == 32 && sp.d.n2 == 64) return launch_plane2<R, 32, 64, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy), true;
-    if (sp.d.M == 16 && sp.d.n2 == 32) return launch_plane2<R, 16, 32, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy), true;
-  }
   if (sp.d.M == 64 && sp.d.n2 == 128) return launch_plane<R, 64, 128, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   if (sp.d.M == 32 && sp.d.n2 == 64) return launch_plane<R, 32, 64, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   if (sp.d.M == 16 && sp.d.n2 == 32) return launch_plane<R, 16, 32, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
@@ -1649,38 +1214,6 @@ static bool try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_
   if (sp.d.M == 40 && sp.d.n2 == 80) return launch_plane<R, 40, 80, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   if (sp.d.M == 24 && sp.d.n2 == 48) return launch_plane<R, 24, 48, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
   return false;
-}
-
-// Whether the fused inverse plane pass (k_plane_inv) serves this shape and
-// functional: Wang-Teter with alpha == beta, n2 = n1, n1 in {32, 64, 128}.
-inline bool plane_inv_ok_impl(const ShapePlan& sp, int kind) {
-  if (sp.no_plane_inv || kind != kWT3) return false;
-  const int M = sp.d.M, n2 = sp.d.n2;
-  if (!((M == 64 && n2 == 128) || (M == 32 && n2 == 64) || (M == 16 && n2 == 32))) return false;
-  return is_fast_len(sp.d.n3) || is_gp_len(sp.d.n3);
-}
-
-template <typename R, int M, int N2, int INMODE, int OUTMODE>
-static void launch_plane_inv(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st,
-                             const typename V2<R>::T* twx, const typename V2<R>::T* tws,
-                             const typename V2<R>::T* twy) {
-  using namespace fast;
-  using G = PlaneInvGeo<M, N2>;
-  constexpr size_t lds = plane_inv_lds<M, N2, R>();
-  auto kern = k_plane_inv<M, N2, INMODE, OUTMODE, R>;
-  allow_lds<k_plane_inv<M, N2, INMODE, OUTMODE, R>>(lds);
-  hipLaunchKernelGGL(kern, dim3(2 * sp.d.n3, nb), dim3(G::THREADS), lds, st, dd, sp.d, twx, tws, twy);
-}
-
-template <typename R, int INMODE, int OUTMODE>
-static bool try_plane_inv(const ShapePlan& sp, int kind, const ReqDesc* dd, int nb, hipStream_t st,
-                          const typename V2<R>::T* twx, const typename V2<R>::T* tws,
-                          const typename V2<R>::T* twy) {
-  if (!plane_inv_ok_impl(sp, kind)) return false;
-  if (sp.d.M == 64) launch_plane_inv<R, 64, 128, INMODE, OUTMODE>(sp, dd, nb, st, twx, tws, twy);
-  else if (sp.d.M == 32) launch_plane_inv<R, 32, 64, INMODE, OUTMODE>(sp, dd, nb, st, twx, tws, twy);
-  else launch_plane_inv<R, 16, 32, INMODE, OUTMODE>(sp, dd, nb, st, twx, tws, twy);
-  return true;
 }
 
 template <typename R, int KIND, int INMODE, int OUTMODE, int M>
@@ -1723,7 +1256,6 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
     case 192: launch_z<R, 192, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
     default: launch_z<R, 256, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
   }
-  if (KIND == kWT3 && try_plane_inv<R, INMODE, OUTMODE>(sp, KIND, dd, nb, st, twx, tws, twy)) return;
   ylaunch(std::integral_constant<int, +1>());
   launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, false, twx, tws);
 }

@@ -94,8 +94,6 @@ struct ShapePlan {
   int lds_target = 65536;  // soft budget used for the automatic tiles
   int force_generic = 0;   // 1: never use the register-resident fast path
   int no_plane = 0;        // 1: never use the fused x+y plane kernel
-  int no_plane_inv = 1;    // 1: never use the fused inverse y+x plane kernel
-  int plane_fwd2 = 0;      // 1: the small-LDS fused x+y forward plane kernel where it applies
 };
 
 struct Tiles {
