@@ -185,6 +185,37 @@ int ofd_calpot_many(ofd_ctx* const* ctxs, int nb, const double* const* rho_devic
 int ofd_rho_opt(ofd_ctx* ctx, double* rhoR, int dir_type, const ofd_opt_params* params, double* energy9,
                 int* rho_outcome, ofd_opt_stats* stats);
 
+/* ---- spin polarization (numSpin = 2). After ofd_set_spin with nspin = 2
+ * every density and potential array of ofd_calpot, ofd_calpot_device,
+ * ofd_calpot_enqueue, ofd_calpot_noalloc, ofd_calpot_many, ofd_rho_opt and
+ * ofd_ion_forces holds both channels, 2*n1*n2*n3 values: spin up, then spin
+ * down (the reference's rho(n1,n2,n3,2)). The evaluation is CalculatePotentialPlus
+ * for numSpin = 2 (CalPotential.f90:120-160): ion and Hartree terms on the
+ * total density, the LSDA exchange-correlation (:262-282), Thomas-Fermi,
+ * Wang-Teter and vW on 2 rho_s, halved (KEDF_TF.f90:66-74, CalPotential.f90:
+ * 400-407, KEDF_VW.f90:100-140). ofd_rho_opt runs the numSpin = 2 NTN solve
+ * (dir_type 1) with SpinThetaSearch (RhoLineSearch.f90:447-657) inside the
+ * magnetization loop (RhoOptN.f90:336-480), in FP64; ofd_ion_forces sums the
+ * channels (CalForce.f90:180). Wang-Teter (kinetic 4) only; pools and
+ * ofd_ion_relax take numSpin = 1 contexts. */
+typedef struct {
+  int nspin;        /* 1 or 2 (input keyword SPIN, ReadInputFile.f90:729-745) */
+  int lsda;         /* 1 Perdew-Zunger (LSDAPotPZ, the reference's default), 2 Perdew-Wang 92
+                       (LSDAPotPW92); <= 0: 1 (XC_LDA.f90:44) */
+  int fixed_mag;    /* 1: one solve at the initial magnetization (FIXMAG); 0: the
+                       magnetization loop (RhoOptimizers.f90:85) */
+  int nmag;         /* magnetization loop steps at most; <= 0: 100 (RhoOptimizers.f90:97) */
+  double tolm;      /* stop when |dE/dm| < tolm (Hartree); <= 0: 5e-6 as a
+                       single-precision literal (RhoOptimizers.f90:100) */
+} ofd_spin;
+int ofd_set_spin(ofd_ctx* ctx, const ofd_spin* spin);
+/* after a spin ofd_rho_opt: out3 = final magnetization, N_up, N_down; the
+ * number of solves of the magnetization loop */
+int ofd_get_magnetization(const ofd_ctx* ctx, double* out3, int* mag_iters);
+/* initial channels from a total density and the input MAGMOM (System.f90:
+ * 326-329): rho2[s] = (rho - (-1)^s magmom / volume) / 2, s = 1, 2 */
+int ofd_spin_split(const ofd_ctx* ctx, const double* rho, double magmom, double* rho2);
+
 /* Ion-electron forces from the B-spline structure factor
  * (IonElectronForcesSpline, IonElectronSpline.f90:219-355, called with
  * FFT(rho) as in CalForce.f90:180). forces: 3 per ion (Ha/bohr), ion order as

@@ -417,6 +417,178 @@ def calculate_potential_plus(rho, sysm: System, opt_sqrt=True, want_energy=True)
 
 
 # --------------------------------------------------------------------------
+# Spin-polarized evaluation, numSpin = 2. Arrays of both channels have numpy
+# shape (2, n3, n2, n1) == Fortran rho(n1,n2,n3,2): spin up, then spin down.
+# --------------------------------------------------------------------------
+LSDA_AX = -0.7385588              # ax of LSDAPotPZ/PW92 (XC_LDA.f90:330, :434)
+LSDA_FZZ = 1.709921               # f''(0)
+LSDA_GAMMA = 0.5198421            # 2^(4/3) - 2
+# Perdew-Zunger 1981 parameters in Rydberg (XC_LDA.f90:453-467): unpolarized
+# (c1..c4 for rs < 1; g, b1, b2 for rs >= 1), fully polarized (as..ds; gs, b1s, b2s)
+_PZR = dict(c1=0.0622, c2=0.0960, c3=0.0040, c4=0.0232, as_=0.0311, bs=-0.0538, cs=0.0014, ds=-0.0096,
+            g=-0.2846, b1=1.0529, b2=0.3334, gs=-0.1686, b1s=1.3981, b2s=0.2611)
+# Perdew-Wang 1992 G(rs) parameter rows (a, a1, b1, b2, b3, b4) as the reference
+# passes them to spn_gcor (XC_LDA.f90:363-368): -ec unpolarized, -ec polarized,
+# alpha_c (spin stiffness, as -alfm)
+PW92_U = (0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294)
+PW92_P = (0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517)
+PW92_A = (0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671)
+
+
+def spn_gcor(a, a1, b1, b2, b3, b4, p, rs):
+    """spn_gcor (XC_LDA.f90:550-592): G(rs) = -2a(1 + a1 rs) ln(1 + 1/(2a(b1 rs^1/2
+    + b2 rs + b3 rs^3/2 + b4 rs^(p+1)))) and dG/drs."""
+    p1 = p + 1.0
+    q0 = -2.0 * a * (1.0 + a1 * rs)
+    rs12 = np.sqrt(rs)
+    rs32 = rs12 ** 3
+    rsp = rs ** p
+    q1 = 2.0 * a * (b1 * rs12 + b2 * rs + b3 * rs32 + b4 * rs * rsp)
+    q2 = np.log(1.0 + 1.0 / q1)
+    gg = q0 * q2
+    q3 = a * (b1 / rs12 + 2.0 * b2 + 3.0 * b3 * rs12 + 2.0 * b4 * p1 * rsp)
+    ggrs = -2.0 * a * a1 * q2 - q0 * q3 / (q1 ** 2 + q1)
+    return gg, ggrs
+
+
+def _spin_common(rho2):
+    """Exchange per channel (on 2 rho_s), rs, zeta and the spin interpolation f,
+    f' shared by LSDAPotPZ and LSDAPotPW92 (XC_LDA.f90:361-376, :493-503)."""
+    third, ac2 = 1.0 / 3.0, 4.0 / 3.0
+    ex = LSDA_AX * (2.0 * rho2) ** third
+    vx = ex * ac2
+    d = rho2[0] + rho2[1]
+    rs = (0.75 / PI / d) ** third
+    zet = (rho2[0] - rho2[1]) / d
+    f = ((1.0 + zet) ** ac2 + (1.0 - zet) ** ac2 - 2.0) / LSDA_GAMMA
+    fz = ac2 * ((1.0 + zet) ** third - (1.0 - zet) ** third) / LSDA_GAMMA
+    return ex, vx, d, rs, zet, f, fz
+
+
+def lsda_pz(rho2):
+    """LSDAPotPZ (XC_LDA.f90:417-547), the reference's default LSDA (lsda = 1):
+    Perdew-Zunger correlation in Rydberg, halved. Returns (potential (2, ...),
+    energy without dV)."""
+    ex, vx, d, rs, zet, f, fz = _spin_common(rho2)
+    P = _PZR
+    eu = np.empty_like(rs)
+    vc1 = np.empty_like(rs)
+    es = np.empty_like(rs)
+    vs = np.empty_like(rs)
+    lo = rs < 1.0
+    r = rs[lo]
+    lrs = np.log(r)
+    eu[lo] = P["c1"] * lrs - P["c2"] + (P["c3"] * lrs - P["c4"]) * r
+    vc1[lo] = eu[lo] - (P["c1"] + (P["c3"] * lrs + P["c3"] - P["c4"]) * r) / 3.0
+    es[lo] = P["as_"] * lrs + P["bs"] + (P["cs"] * lrs + P["ds"]) * r
+    vs[lo] = es[lo] - (P["as_"] + (P["cs"] * lrs + P["cs"] + P["ds"]) * r) / 3.0
+    r = rs[~lo]
+    srs = np.sqrt(r)
+    eu[~lo] = P["g"] / (1.0 + P["b1"] * srs + P["b2"] * r)
+    vc1[~lo] = eu[~lo] * eu[~lo] * (1.0 + 7.0 * P["b1"] * srs / 6.0 + 4.0 / 3.0 * P["b2"] * r) / P["g"]
+    es[~lo] = P["gs"] / (1.0 + P["b1s"] * srs + P["b2s"] * r)
+    vs[~lo] = es[~lo] * es[~lo] * (1.0 + 7.0 * P["b1s"] * srs / 6.0 + 4.0 / 3.0 * P["b2s"] * r) / P["gs"]
+    vc2 = vc1 + f * (vs - vc1) - (es - eu) * (1.0 + zet) * fz
+    vc1 = vc1 + f * (vs - vc1) + (es - eu) * (1.0 - zet) * fz
+    ec = eu + f * (es - eu)
+    pot = np.stack([0.5 * (2.0 * vx[0] + vc1), 0.5 * (2.0 * vx[1] + vc2)])
+    e = 0.5 * np.sum(2.0 * (rho2[0] * ex[0] + rho2[1] * ex[1]) + ec * d)
+    return pot, e
+
+
+def lsda_pw92(rho2):
+    """LSDAPotPW92 (XC_LDA.f90:313-414), lsda = 2: Perdew-Wang 1992 correlation.
+    Returns (potential (2, ...), energy without dV)."""
+    ex, vx, d, rs, zet, f, fz = _spin_common(rho2)
+    eu, eurs = spn_gcor(*PW92_U, 1.0, rs)
+    ep, eprs = spn_gcor(*PW92_P, 1.0, rs)
+    alfm, alfrsm = spn_gcor(*PW92_A, 1.0, rs)
+    z4 = zet ** 4
+    ec = eu * (1.0 - f * z4) + ep * f * z4 - alfm * f * (1.0 - z4) / LSDA_FZZ
+    ecrs = eurs * (1.0 - f * z4) + eprs * f * z4 - alfrsm * f * (1.0 - z4) / LSDA_FZZ
+    eczet = 4.0 * (zet ** 3) * f * (ep - eu + alfm / LSDA_FZZ) + fz * (z4 * ep - z4 * eu - (1.0 - z4) * alfm / LSDA_FZZ)
+    comm = ec - rs * ecrs / 3.0 - zet * eczet
+    pot = np.stack([vx[0] + comm + eczet, vx[1] + comm - eczet])
+    e = np.sum(rho2[0] * ex[0] + rho2[1] * ex[1] + ec * d)
+    return pot, e
+
+
+def lda_energy_spin(rho2):
+    """LDAEnergy for numSpin = 2 (XC_LDA.f90:152-185): the PW92 energy with the
+    exchange constant cX of the unpolarized branch. Without dV."""
+    cx = -0.73855876638202234
+    third = 1.0 / 3.0
+    exa = cx * (2.0 * rho2[0]) ** third
+    exb = cx * (2.0 * rho2[1]) ** third
+    d = rho2[0] + rho2[1]
+    rs = (0.75 / (PI * d)) ** third
+    eu, _ = spn_gcor(*PW92_U, 1.0, rs)
+    ep, _ = spn_gcor(*PW92_P, 1.0, rs)
+    alfm, _ = spn_gcor(*PW92_A, 1.0, rs)
+    zet = (rho2[0] - rho2[1]) / d
+    f = ((1.0 + zet) ** (4.0 / 3.0) + (1.0 - zet) ** (4.0 / 3.0) - 2.0) / LSDA_GAMMA
+    z4 = zet ** 4
+    ec = eu * (1.0 - f * z4) + ep * f * z4 - alfm * f * (1.0 - z4) / LSDA_FZZ
+    return np.sum(rho2[0] * exa + rho2[1] * exb + ec * d)
+
+
+def calculate_potential_plus_spin(rho2, sysm: System, opt_sqrt=True, want_energy=True, lsda=1):
+    """CalculatePotentialPlus for numSpin = 2 (CalPotential.f90:120-160):
+    ionic and Hartree terms on rho_up + rho_down, copied to both channels; LSDA
+    exchange-correlation (:262-282, lsda 1 = PZ, 2 = PW92); the kinetic terms
+    evaluated on 2 rho_s and halved (CalTF, KEDF_TF.f90:60-78; WT/WGC,
+    CalPotential.f90:400-407, :426-431); the chain rule to sqrt(rho_s); spin vW
+    on sqrt(2 rho_s) divided by sqrt(2) (CalVW, KEDF_VW.f90:100-140).
+    Returns (potential (2, ...), eTable(9) in Hartree or None)."""
+    f = sysm.func
+    et = np.zeros(9)
+    rho = rho2[0] + rho2[1]
+    common = sysm.vion.copy()
+    et[2] = np.sum(rho * common)
+    vh, et[3] = hartree(rho, sysm.qtable)
+    common = common + vh
+    pot = np.stack([common, common])
+    vxc, et[4] = (lsda_pz if lsda == 1 else lsda_pw92)(rho2)
+    pot = pot + vxc
+    for s in range(2):
+        x = 2.0 * rho2[s]
+        pot[s] = pot[s] + tf_potential(x, f.lam)
+        et[6] += 0.5 * tf_energy(x, f.lam)
+    for s in range(2):
+        x = 2.0 * rho2[s]
+        if f.kinetic == 4:
+            pnl, e = wt_potplus(x, sysm.kernel, f)
+        elif f.kinetic == 5:
+            pnl, e = wgc_potplus(x, sysm.wgc_kernels, f)
+        else:
+            raise ValueError("kinetic must be 4 or 5")
+        pot[s] = pot[s] + pnl
+        et[8] += 0.5 * e
+    if opt_sqrt:
+        pot = 2.0 * np.sqrt(rho2) * pot
+    for s in range(2):
+        tvw, e = vw_sqrt(np.sqrt(2.0 * rho2[s]), sysm.qtable, f.mu)
+        tvw = tvw / math.sqrt(2.0)
+        if not opt_sqrt:
+            tvw = tvw / (2.0 * np.sqrt(rho2[s]))
+        pot[s] = pot[s] + tvw
+        et[7] += e * 0.5
+    if not want_energy:
+        return pot, None
+    et[1] = et[6] + et[7] + et[8]
+    et = et * sysm.cell.dV
+    et[5] = sysm.e_ionion
+    et[0] = np.sum(et[1:6])
+    return pot, et
+
+
+def spin_split(rho, magmom, vol):
+    """Initial spin densities from a total density and the input MAGMOM
+    (System.f90:326-329): rho_s = (rho - (-1)^s magmom / vol) / 2, s = 1, 2."""
+    return np.stack([0.5 * (rho - (-1) ** s * magmom / vol) for s in (1, 2)])
+
+
+# --------------------------------------------------------------------------
 # Cubic splines (MathSplines.f90:178-560)
 # --------------------------------------------------------------------------
 def spline_cubic_set(t, y, ibcbeg, ybcbeg, ibcend, ybcend):
@@ -1421,6 +1593,258 @@ def rho_minimization(rho_init, evaluate_full, evaluate_pot, dV, params: OptParam
         it += 1
     return OptResult(rho=phi ** 2, energy=energy, outcome=outcome, iters=it,
                      history=history, num_evals=nevals)
+
+
+def spin_theta_search(theta, energy, gradth, phi, direction, dV, evaluate_full, ftol, gtol, xtol):
+    """SpinThetaSearch (RhoLineSearch.f90:447-657): at most two Polak-Ribiere
+    (thetaCGDirection, :660-700) line searches in (theta_up, theta_down), each a
+    Dcsrch over the step alpha along thetadir in [0, pi / max|thetadir|] with at
+    most 10 trial points. theta and gradth are updated in place. Returns
+    (temp_phi, dE/dphi, energy, success, evaluations)."""
+    alphatol = _f32(1e-4)
+    thetadir_old = np.zeros(2)
+    gradth_old = np.zeros(2)
+    success = 1
+    alpha = 0.0
+    temp_phi = np.stack([phi[k] * math.cos(theta[k]) + direction[k] * math.sin(theta[k]) for k in range(2)])
+    dedphi, energy = evaluate_full(temp_phi ** 2)
+    nevals = 1
+    dedphi = dedphi * _sign(temp_phi)
+
+    def grad_of_theta(th, k):         # gradOftheta (RhoLineSearch.f90:142-198)
+        return dV * np.sum(dedphi[k] * (-phi[k] * math.sin(th) + direction[k] * math.cos(th)))
+
+    j = 1
+    while True:
+        gam = 0.0 if j == 1 else max(0.0, float(np.sum(gradth * (gradth - gradth_old))) /
+                                      float(np.sum(gradth_old * gradth_old)))
+        gradth_old = gradth.copy()
+        thetadir = -gradth + gam * thetadir_old
+        for k in range(2):
+            gradth[k] = grad_of_theta(theta[k], k)
+        while True:
+            dedalpha = gradth[0] * thetadir[0] + gradth[1] * thetadir[1]
+            if dedalpha >= 0.0:
+                thetadir = -gradth.copy()
+            else:
+                break
+        thetadir_old = thetadir.copy()
+        maxdir = max(abs(thetadir[0]), abs(thetadir[1]))
+        # MIN(0.1, 0.1*pi/maxthetadir) with the single-precision literal 0.1
+        alpha = min(_f32(0.1), _f32(0.1) * PI / maxdir)
+        ls = Dcsrch(ftol, gtol, xtol, 0.0, PI / maxdir)
+        nline = 0
+        while True:
+            alpha, task = ls.step(alpha, energy[0], dedalpha)
+            if task.startswith("CONV"):
+                success = 2
+                break
+            if task.startswith("ERROR"):
+                success = -1
+                break
+            if task.startswith("WARN"):
+                success = -2
+                break
+            tem = theta + alpha * thetadir
+            temp_phi = np.stack([phi[k] * math.cos(tem[k]) + direction[k] * math.sin(tem[k]) for k in range(2)])
+            dedphi, energy = evaluate_full(temp_phi ** 2)
+            nevals += 1
+            dedphi = dedphi * _sign(temp_phi)
+            for k in range(2):
+                gradth[k] = grad_of_theta(tem[k], k)
+            dedalpha = gradth[0] * thetadir[0] + gradth[1] * thetadir[1]
+            nline += 1
+            if nline > 10:
+                success = -1
+                break
+        theta += alpha * thetadir
+        if success == -1:
+            break
+        if math.sqrt(gradth[0] * gradth[0] + gradth[1] * gradth[1]) < alphatol:
+            break
+        j += 1
+        if j > 2:
+            break
+    return temp_phi, dedphi, energy, success, nevals
+
+
+def rho_minimization_spin(rho2_init, evaluate_full, evaluate_pot, dV, params: OptParams = OptParams(),
+                          tot_ele=None):
+    """ProjNormRhoMinimization for numSpin = 2 with dir_type = 1 (NTN), one
+    pass of the magnetization loop (RhoOptN.f90:336-420, DensityOptimization
+    :550-641): per-spin electron numbers, chemical potentials and Newton
+    directions (NewtonDirection perturbs one channel, RhoDirNew.f90:25-210),
+    getNextGradient / checkGradient per channel (RhoLineSearch.f90:40-139,
+    :201-309), the joint theta search, the potential norm over both channels
+    (:745-760). evaluate_full(rho2) -> (dE/dphi (2, ...), eTable);
+    evaluate_pot(rho2) -> dE/dphi. tot_ele: the electron numbers per spin
+    (default: sum(rho_s) dV, :298-301)."""
+    num_points = float(rho2_init[0].size)
+    ftol = float(np.float32(1e-4))
+    gtol = float(np.float32(2e-1))
+    xtol = 1e-12
+    tot = np.array([np.sum(rho2_init[s]) * dV for s in range(2)]) if tot_ele is None else np.asarray(tot_ele, float)
+    theta = np.array([0.2, 0.2])
+    gradth = np.zeros(2)
+    phi = np.sqrt(np.abs(rho2_init))
+    dedphi, energy = evaluate_full(phi ** 2)
+    nevals = 1
+    dedphi = dedphi * _sign(phi)
+    mu0 = np.array([dV * np.sum(dedphi[s] * phi[s]) / (2.0 * tot[s]) for s in range(2)])
+    dldphi = np.stack([dedphi[s] - mu0[s] * 2.0 * phi[s] for s in range(2)])
+    pnorm0 = math.sqrt((np.sum(dldphi[0] ** 2) + np.sum(dldphi[1] ** 2)) / num_points / 2.0)
+    history = [dict(iter=0, energy=float(energy[0]), pot_norm=pnorm0)]
+    e = [energy[0], 0.0, 0.0]
+    outcome = -1
+    it = 1
+    temp_phi = phi
+    while True:
+        direction = np.zeros_like(phi)
+        newton_iters = []
+        for s in range(2):
+            other = phi[1 - s] ** 2
+
+            def ev(rho_s, s=s, other=other):
+                rr = np.empty((2,) + rho_s.shape)
+                rr[s] = rho_s
+                rr[1 - s] = other
+                return evaluate_pot(rr)[s]
+            direction[s], nit, _, ne = newton_direction(phi[s], dldphi[s], mu0[s], ev, dV)
+            nevals += ne
+            newton_iters.append(nit)
+        sw1 = sw2 = False
+        while True:
+            for s in range(2):                 # getNextGradient, numSpin = 2
+                pd = phi[s] + direction[s]
+                _norm_phi_dir = math.sqrt(np.sum(pd * pd) * dV)   # computed, unused for numSpin = 2
+                direction[s] = direction[s] - phi[s] * (np.sum(phi[s] * direction[s]) * dV) / tot[s]
+                norm_perp = math.sqrt(np.sum(direction[s] * direction[s]) * dV)
+                theta[s] = 0.0
+                direction[s] = direction[s] * math.sqrt(tot[s]) / norm_perp
+                gradth[s] = dV * np.sum(dedphi[s] * (-phi[s] * math.sin(0.0) + direction[s] * math.cos(0.0)))
+            done = False                         # checkGradient, numSpin = 2
+            for s in range(2):
+                if gradth[s] > 1.0e5:
+                    raise RuntimeError("The gradient %d is too large" % (s + 1))
+                if gradth[s] >= 0.0:
+                    if sw1 and sw2:
+                        outcome = 1
+                        done = True
+                        break
+                    if s == 0:
+                        sw1 = True
+                    else:
+                        sw2 = True
+                    direction[s] = -dldphi[s]
+            if done or (gradth[0] < 0.0 and gradth[1] < 0.0):
+                break
+        if outcome == 1:
+            break
+        temp_phi, dedphi, energy, success, ne = spin_theta_search(theta, energy, gradth, phi, direction, dV,
+                                                                  evaluate_full, ftol, gtol, xtol)
+        nevals += ne
+        if success == -1 and it > params.max_iter:
+            raise RuntimeError("Fail rho optimizations")
+        pot_norm = math.sqrt((np.sum((dedphi[0] - 2.0 * temp_phi[0] * mu0[0]) ** 2) +
+                              np.sum((dedphi[1] - 2.0 * temp_phi[1] * mu0[1]) ** 2)) / num_points / 2.0)
+        history.append(dict(iter=it, energy=float(energy[0]), pot_norm=pot_norm, theta=theta.copy(),
+                            newton_iter=newton_iters, success=success))
+        exit_flag = False
+        pconv = pot_norm < params.pot_tol
+        e = [energy[0], e[0], e[1]]
+        econv = it >= 3 and abs(e[0] - e[1]) < params.tole and abs(e[0] - e[2]) < params.tole
+        cc = params.conv_check
+        if cc.startswith("ENE") and econv:
+            outcome, exit_flag = 0, True
+        if cc.startswith("POT") and pconv:
+            outcome, exit_flag = 0, True
+        if cc.startswith("BOTH") and (pconv or econv):
+            outcome, exit_flag = 0, True
+        if outcome == 1:
+            exit_flag = True
+        if it >= params.max_iter:
+            outcome, exit_flag = 1, True
+        phi = np.stack([phi[s] * math.cos(theta[s]) + direction[s] * math.sin(theta[s]) for s in range(2)])
+        mu0 = np.array([dV * np.sum(dedphi[s] * phi[s]) / (2.0 * tot[s]) for s in range(2)])
+        dldphi = np.stack([dedphi[s] - 2.0 * mu0[s] * phi[s] for s in range(2)])
+        if exit_flag:
+            break
+        it += 1
+    return OptResult(rho=phi ** 2, energy=energy, outcome=outcome, iters=it, history=history, num_evals=nevals)
+
+
+@dataclass
+class SpinOptResult:
+    rho: np.ndarray
+    energy: np.ndarray
+    outcome: int
+    mag_iters: int
+    magnetization: float
+    tot_ele: np.ndarray
+    solves: list
+
+
+def rho_minimization_spin_mag(rho2_init, evaluate_full, evaluate_pot, dV, params: OptParams = OptParams(),
+                              fixed_mag=False, nmag=100, tolm=float(np.float32(5.0e-6))):
+    """The magnetization loop around the numSpin = 2 solve (RhoOptN.f90:336-
+    480): with fixed_mag (the input's FIXMAG) one solve at the initial
+    magnetization; otherwise up to nmag solves, each restarted from the initial
+    density rescaled to the spin populations (Ntot +- m)/2, m moved by the
+    secant/parabola steps of :461-472 until |dE/dm| < tolm (or 60 steps). The
+    reference reads m1 before its first assignment when it forms dE/dm after
+    the first solve; here it starts at 0, which only decides whether the
+    second solve's test passes (it cannot for any finite energy at m = 0)."""
+    rho_r = np.array(rho2_init, dtype=float)
+    tot = np.array([np.sum(rho_r[s]) * dV for s in range(2)])
+    ntot = float(np.sum(tot))
+    magnetization = tot[0] - tot[1]
+    de = 1.0
+    mag_iter = 0
+    ee1 = ee2 = ee3 = 0.0
+    m1 = m2 = m3 = 0.0
+    solves = []
+    res = None
+    for _ in range(nmag):
+        res = rho_minimization_spin(rho_r, evaluate_full, evaluate_pot, dV, params, tot_ele=tot.copy())
+        solves.append(res)
+        mag_iter += 1
+        if fixed_mag:
+            break
+        if abs(de) < tolm:
+            break
+        ee3, ee2, ee1 = ee2, ee1, float(res.energy[0])
+        m3, m2, m1 = m2, m1, magnetization
+        with np.errstate(divide="ignore", invalid="ignore"):
+            de = float(np.float64(ee1 - ee2) / np.float64(m1 - m2))   # IEEE, as the Fortran
+        if mag_iter == 1:
+            magnetization = m1 + _f32(0.1)
+        if mag_iter == 2:
+            magnetization = m1 - _f32(0.2)
+        if mag_iter == 3:
+            magnetization = m1 - de / ee1
+        if mag_iter > 3:
+            b = (m1 - m2) / (m3 - m2)
+            de1 = 2.0 * (ee1 - (1 - b) * ee2 - b * ee3) / ((1.0 - 1.0 / b) * (m1 - m2) ** 2)
+            if mag_iter > 60:
+                break
+            magnetization = m1 - de / de1
+        rho_r[0] = rho_r[0] / tot[0]
+        rho_r[1] = rho_r[1] / tot[1]
+        tot = np.array([0.5 * (ntot + magnetization), 0.5 * (ntot - magnetization)])
+        rho_r[0] = rho_r[0] * tot[0]
+        rho_r[1] = rho_r[1] * tot[1]
+    return SpinOptResult(rho=res.rho, energy=res.energy, outcome=res.outcome, mag_iters=mag_iter,
+                         magnetization=magnetization, tot_ele=tot, solves=solves)
+
+
+def make_spin_evaluators(sysm: System, lsda=1):
+    """calculate_potential_plus_spin in the optimizer's two call forms."""
+    def full(rho2):
+        return calculate_potential_plus_spin(rho2, sysm, True, True, lsda)
+
+    def pot_only(rho2):
+        return calculate_potential_plus_spin(rho2, sysm, True, False, lsda)[0]
+    return full, pot_only
 
 
 def make_evaluators(sysm: System):

@@ -62,6 +62,11 @@ class ofd_ion_stats(C.Structure):
     _fields_ = [("steps", C.c_int), ("geometries", C.c_int), ("max_force", C.c_double), ("seconds", C.c_double)]
 
 
+class ofd_spin(C.Structure):
+    _fields_ = [("nspin", C.c_int), ("lsda", C.c_int), ("fixed_mag", C.c_int), ("nmag", C.c_int),
+                ("tolm", C.c_double)]
+
+
 class ofd_opt_stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("num_evals", C.c_int), ("newton_iters_total", C.c_int),
                 ("fp32_evals", C.c_int), ("final_pot_norm", C.c_double), ("seconds", C.c_double)]
@@ -80,7 +85,8 @@ EXPORTS = ["ofd_create", "ofd_destroy", "ofd_last_error", "ofd_get_params", "ofd
            "ofd_calpot_many", "ofd_autotune", "ofd_pools_step", "ofd_pool_finished", "ofd_server_create",
            "ofd_server_add_pool", "ofd_server_wait", "ofd_server_last_error", "ofd_server_destroy",
            "ofd_get_volume", "ofd_ewald", "ofd_ion_relax", "ofd_read_geometry", "ofd_free_geometry",
-           "ofd_read_recpot", "ofd_write_density", "ofd_read_density"]
+           "ofd_read_recpot", "ofd_write_density", "ofd_read_density", "ofd_set_spin", "ofd_get_magnetization",
+           "ofd_spin_split"]
 
 
 def lib():
@@ -149,6 +155,9 @@ def lib():
     L.ofd_server_last_error.restype = C.c_char_p
     L.ofd_server_destroy.argtypes = [vp]
     L.ofd_server_destroy.restype = None
+    L.ofd_set_spin.argtypes = [vp, C.POINTER(ofd_spin)]
+    L.ofd_get_magnetization.argtypes = [vp, dp, ip]
+    L.ofd_spin_split.argtypes = [vp, dp, C.c_double, dp]
     _lib = L
     return L
 

@@ -46,6 +46,7 @@ class OFDSystem:
         L.ofd_get_volume(h, C.byref(v))
         self.vol = v.value   # the library's cell%vol: uniform densities agree bit for bit
         self.dV = self.vol / (self.n1 * self.n2 * self.n3)
+        self.nspin = 1
 
     # -- lifetime
     def close(self):
@@ -72,6 +73,34 @@ class OFDSystem:
     @property
     def shape(self):
         return (self.n3, self.n2, self.n1)
+
+    @property
+    def rho_shape(self):
+        """Shape of density and potential arrays: (2, n3, n2, n1) with spin."""
+        return self.shape if self.nspin == 1 else (2,) + self.shape
+
+    def set_spin(self, nspin=2, lsda=1, fixed_mag=False, nmag=-1, tolm=-1.0):
+        """numSpin (input SPIN), the LSDA form (1 Perdew-Zunger, 2 Perdew-Wang
+        92), FIXMAG, and the magnetization loop's nmag and tolm (ofd_set_spin)."""
+        from ._lib import ofd_spin
+        sp = ofd_spin(int(nspin), int(lsda), int(bool(fixed_mag)), int(nmag), float(tolm))
+        self._check(lib().ofd_set_spin(self._h, C.byref(sp)))
+        self.nspin = int(nspin)
+
+    def spin_split(self, rho, magmom):
+        """Initial channels from a total density and MAGMOM (System.f90:326-329)."""
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        assert r.shape == self.shape
+        out = np.empty((2,) + self.shape)
+        self._check(lib().ofd_spin_split(self._h, dptr(r), float(magmom), dptr(out)))
+        return out
+
+    def magnetization(self):
+        """After a spin solve: (magnetization, N_up, N_down, magnetization-loop solves)."""
+        out = np.zeros(3)
+        it = C.c_int(0)
+        self._check(lib().ofd_get_magnetization(self._h, dptr(out), C.byref(it)))
+        return out[0], out[1], out[2], it.value
 
     @property
     def params(self):
@@ -146,8 +175,8 @@ def CalculatePotentialPlus(system: OFDSystem, rho, optSqrt=True, eTable=True):
     """CalculatePotentialPlus(rho, optSqrt, potential[, eTable]) (CalPotential.f90:41).
     Returns (potential, eTable(9) in Hartree or None)."""
     r = np.ascontiguousarray(rho, dtype=np.float64)
-    if r.shape != system.shape:
-        raise ValueError("rho has shape %r, grid is %r" % (r.shape, system.shape))
+    if r.shape != system.rho_shape:
+        raise ValueError("rho has shape %r, expected %r" % (r.shape, system.rho_shape))
     pot = np.empty_like(r)
     et = np.zeros(9) if eTable else None
     system._check(lib().ofd_calpot(system._h, dptr(r), 1 if optSqrt else 0, dptr(pot), dptr(et)))
@@ -173,6 +202,8 @@ def ProjNormRhoMinimization(system: OFDSystem, rhoR, dir_type=1, **kw):
     dir_type 1 (NTN), 2 (NCG; cg_alg="HZ" or "PR" as DENC 2 or 1) or 3 (NBF).
     Returns (rho, energy(9), rhoOutcome, stats dict)."""
     r = np.array(rhoR, dtype=np.float64, order="C", copy=True)
+    if r.shape != system.rho_shape:
+        raise ValueError("rhoR has shape %r, expected %r" % (r.shape, system.rho_shape))
     et = np.zeros(9)
     outc = C.c_int(-1)
     st = ofd_opt_stats()
@@ -186,6 +217,8 @@ def ProjNormRhoMinimization(system: OFDSystem, rhoR, dir_type=1, **kw):
 def IonElectronForcesSpline(system: OFDSystem, rho):
     """Ion-electron forces (Ha/bohr), ions in the order given to build_ion_potential."""
     r = np.ascontiguousarray(rho, dtype=np.float64)
+    if r.shape != system.rho_shape:
+        raise ValueError("rho has shape %r, expected %r" % (r.shape, system.rho_shape))
     out = np.zeros((system.nion, 3))
     system._check(lib().ofd_ion_forces(system._h, dptr(r), dptr(out)))
     return out

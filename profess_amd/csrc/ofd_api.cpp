@@ -328,6 +328,13 @@ struct ofd_ctx {
   double lam = 1, mu = 1, alpha = -100, beta = -100, gamma = -1, rho0 = -1, rhoS = -1, coef = 0, tkf = 0;
   double e_ionion = 0;
   bool have_vion = false, have_wgc = false;
+  // spin (CellInfo numSpin, XC_LDA lsda, RhoOptimizers fixedmag/nmag/tolm):
+  // with nspin = 2 every density/potential array holds both channels
+  // (Fortran rho(n1,n2,n3,2): spin up, then spin down)
+  int nspin = 1, lsda = 1, fixed_mag = 0, nmag = 100;
+  double tolm = (double)5.0e-6f;
+  double mag_state[3] = {0, 0, 0};   // last spin solve: magnetization, N_up, N_down
+  int mag_iters = 0;
   hipStream_t st = nullptr;
   // device buffers
   double* vion = nullptr;
@@ -377,9 +384,22 @@ struct ofd_ctx {
     for (int i = 0; i < 9; ++i) D.rb[i] = rb[i];
     D.lam = lam; D.mu = mu; D.alpha = alpha; D.beta = beta; D.rhoS = rhoS; D.coef = coef; D.tkf = tkf;
     D.opt_sqrt = 1;
+    D.lsda = lsda;
     return D;
   }
   long long nr() const { return sp.d.nr; }
+  // points of one density array: n1*n2*n3 per spin channel
+  long long nr_all() const { return sp.d.nr * nspin; }
+  // the density/potential arrays of an evaluation (kInRho, kOutPot): both
+  // channels from one array of nr_all() values when nspin = 2
+  void set_rho_io(ReqDesc& D, const double* rho, double* pot) const {
+    D.u = rho;
+    D.out = pot;
+    if (nspin == 2) {
+      D.u2 = rho + nr();
+      D.out2 = pot + nr();
+    }
+  }
 };
 
 static int fail(ofd_ctx* c, const std::string& m) {
@@ -603,8 +623,7 @@ static int eval_one(ofd_ctx* c, const ReqDesc& D, int inmode, int outmode, int f
 extern "C" int ofd_calpot_device(ofd_ctx* c, const double* rho, int opt_sqrt, double* pot, double* et) {
   if (!c || !rho || !pot) return fail(c, "null argument");
   ReqDesc D = c->base_desc();
-  D.u = rho;
-  D.out = pot;
+  c->set_rho_io(D, rho, pot);
   D.opt_sqrt = opt_sqrt ? 1 : 0;
   double s[kNSum];
   if (eval_one(c, D, kInRho, kOutPot, c->precision == 1, s)) return -1;
@@ -616,8 +635,7 @@ extern "C" int ofd_calpot_enqueue(ofd_ctx* c, const double* rho, int opt_sqrt, d
   if (!c || !rho || !pot) return fail(c, "null argument");
   if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set (ofd_set_wgc_table)");
   ReqDesc D = c->base_desc();
-  D.u = rho;
-  D.out = pot;
+  c->set_rho_io(D, rho, pot);
   D.opt_sqrt = opt_sqrt ? 1 : 0;
   if (sums_dev) D.fin = sums_dev;
   // the descriptor is uploaded only when it changes, from the context's own
@@ -695,8 +713,7 @@ extern "C" int ofd_calpot_noalloc(ofd_ctx* c, const double* rho, int opt_sqrt, d
     base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
   }
   ReqDesc D = c->base_desc();
-  D.u = rho;
-  D.out = pot;
+  c->set_rho_io(D, rho, pot);
   D.opt_sqrt = opt_sqrt ? 1 : 0;
   D.fin = reinterpret_cast<double*>(base + off_fin);
   D.sums = reinterpret_cast<double*>(base + off_sums);
@@ -748,8 +765,7 @@ extern "C" int ofd_calpot_many(ofd_ctx* const* cs, int nb, const double* const* 
     std::vector<ReqDesc> h(nb);
     for (int i = 0; i < nb; ++i) {
       h[i] = cs[i]->base_desc();
-      h[i].u = rho[i];
-      h[i].out = pot[i];
+      cs[i]->set_rho_io(h[i], rho[i], pot[i]);
       h[i].opt_sqrt = opt_sqrt ? 1 : 0;
       h[i].fin = c0->many_sums + (size_t)i * kNSum;
     }
@@ -777,7 +793,7 @@ extern "C" int ofd_calpot_many(ofd_ctx* const* cs, int nb, const double* const* 
 
 extern "C" int ofd_calpot(ofd_ctx* c, const double* rho, int opt_sqrt, double* pot, double* et) {
   if (!c || !rho || !pot) return fail(c, "null argument");
-  const size_t b = c->nr() * sizeof(double);
+  const size_t b = c->nr_all() * sizeof(double);
   if (hipMemcpyAsync(c->rho, rho, b, hipMemcpyHostToDevice, c->st) != hipSuccess) return fail(c, "copy failed");
   if (ofd_calpot_device(c, c->rho, opt_sqrt, c->pot, et)) return -1;
   if (copy_sync(pot, c->pot, b, hipMemcpyDeviceToHost, c->st) != hipSuccess) return fail(c, "copy failed");
@@ -1407,12 +1423,14 @@ void fill_params(const ofd_opt_params* in, ofd_opt_params* p) {
   p->fused_dots = in->fused_dots != 0 ? 1 : 0;
 }
 
+// phi, dir, dE/dphi, dL/dphi and tmp hold every spin channel; the Newton
+// vectors r, p, Ap one channel (NewtonDirection works on one at a time)
 void alloc_opt(ofd_ctx* c) {
   if (c->phi) return;
-  const long long n = c->nr();
-  c->phi = dalloc<double>(n); c->dir = dalloc<double>(n); c->dedphi = dalloc<double>(n);
-  c->dldphi = dalloc<double>(n); c->r = dalloc<double>(n); c->p = dalloc<double>(n);
-  c->ap = dalloc<double>(n); c->tmp = dalloc<double>(n);
+  const long long n = c->nr(), na = c->nr_all();
+  c->phi = dalloc<double>(na); c->dir = dalloc<double>(na); c->dedphi = dalloc<double>(na);
+  c->dldphi = dalloc<double>(na); c->r = dalloc<double>(n); c->p = dalloc<double>(n);
+  c->ap = dalloc<double>(n); c->tmp = dalloc<double>(na);
 }
 
 ReqDesc desc_for(ofd_ctx* c, const EvalReq& q) {
@@ -1578,10 +1596,14 @@ void solver_prepare(NtnSolver& S, ofd_ctx* c, const ofd_opt_params* prm, BatchSc
 
 }  // namespace
 
+static int rho_opt_spin(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt_params* prm, double* energy9,
+                        int* rho_outcome, ofd_opt_stats* stats);
+
 extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt_params* prm, double* energy9,
                            int* rho_outcome, ofd_opt_stats* stats) {
   if (!c || !rhoR) return fail(c, "null argument");
   if (dir_type < 1 || dir_type > 3) return fail(c, "dir_type 1 (NTN), 2 (NCG) or 3 (NBF)");
+  if (c->nspin == 2) return rho_opt_spin(c, rhoR, dir_type, prm, energy9, rho_outcome, stats);
   if (c->kind == kWGC8 && !c->have_wgc) return fail(c, "WGC kernel table not set");
   auto t0 = std::chrono::steady_clock::now();
   BatchScratch bs;
@@ -1623,6 +1645,412 @@ extern "C" int ofd_rho_opt(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt
   } catch (const std::exception& e) {
     bs.free_all();
     return fail(c, e.what());
+  }
+  return 0;
+}
+
+// ====================================================================== spin-polarized solver
+// ProjNormRhoMinimization for numSpin = 2, dir_type = 1 (RhoOptN.f90:60-850):
+// per-channel electron numbers and chemical potentials, one NewtonDirection
+// per channel (RhoDirNew.f90:25-210, the finite-difference Hessian product
+// perturbing that channel only), getNextGradient / checkGradient per channel
+// (RhoLineSearch.f90:40-139, :201-309), SpinThetaSearch's joint search of
+// (theta_up, theta_down) (:447-657), and around it the magnetization loop
+// (RhoOptN.f90:336-480). Each step runs on the device (evaluations, vector
+// updates, deterministic dot products); the host holds the scalars and waits
+// for each result, as the reference's loop does.
+namespace {
+
+struct SpinSolver {
+  ofd_ctx* c = nullptr;
+  hipStream_t st = nullptr;
+  long long n = 0;            // points per channel
+  ofd_opt_params prm{};
+  BatchScratch bs;
+  double* phi[2] = {};
+  double* dir[2] = {};
+  double* de[2] = {};
+  double* dl[2] = {};
+  double tot[2] = {0, 0}, mu0[2] = {0, 0}, theta[2] = {0.2, 0.2}, gradth[2] = {0, 0};
+  double energy[9] = {0};
+  double s[kNSum] = {0};
+  double pot_norm = 0;
+  int outcome = -1, it = 1, success = 1;
+  ofd_opt_stats stats{};
+  const double mach = 2.220446049250313e-16;
+
+  void bind(ofd_ctx* cc) {
+    c = cc;
+    st = c->st;
+    n = c->nr();
+    for (int k = 0; k < 2; ++k) {
+      phi[k] = c->phi + k * n;
+      dir[k] = c->dir + k * n;
+      de[k] = c->dedphi + k * n;
+      dl[k] = c->dldphi + k * n;
+    }
+  }
+  // up to four dot products in one launch, deterministic order
+  void dots(int np, const double* const* a, const double* const* b, double* out) {
+    vec_dots(a, b, np, n, c->dot_part, c->dot_out, st);
+    HIPCHK(copy_sync(out, c->dot_out, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  double dot(const double* a, const double* b) {
+    double o[4];
+    const double* A[1] = {a};
+    const double* B[1] = {b};
+    dots(1, A, B, o);
+    return o[0];
+  }
+  // one evaluation at phi_t,s = cos(t_s) phi_s + sin(t_s) dir_s (t = 0: phi),
+  // dE/dphi (signed) into de; sums: gradth per channel at t, the potential
+  // norm with the current mu, sum dE/dphi_s phi_t,s
+  void eval_at(const double* t) {
+    ReqDesc D = c->base_desc();
+    const bool zero = t[0] == 0.0 && t[1] == 0.0;
+    D.u = phi[0];
+    D.v = zero ? nullptr : dir[0];
+    D.ca = std::cos(t[0]);
+    D.cb = zero ? 0.0 : std::sin(t[0]);
+    D.s1 = -std::sin(t[0]);
+    D.s2 = std::cos(t[0]);
+    D.u2 = phi[1];
+    D.v2 = zero ? nullptr : dir[1];
+    D.ca2 = std::cos(t[1]);
+    D.cb2 = zero ? 0.0 : std::sin(t[1]);
+    D.s1b = -std::sin(t[1]);
+    D.s2b = std::cos(t[1]);
+    D.out = de[0];
+    D.out2 = de[1];
+    D.mu0 = mu0[0];
+    D.mu0b = mu0[1];
+    D.opt_sqrt = 1;
+    if (eval_one(c, D, kInPhi, kOutSigned, 0, s)) throw std::runtime_error(c->err);
+    etable_from_sums(c, s, energy);
+    ++stats.num_evals;
+  }
+  // dL/dphi_s = dE/dphi_s - 2 mu_s phi_s with mu_s = dV sum(dE/dphi_s phi_s) / (2 N_s)
+  // (ChemicalPotential, RhoDirBFGS.f90:294-343) from the last evaluation's sums
+  void chem_pot_and_dl() {
+    mu0[0] = c->dV * s[8] / (2.0 * tot[0]);
+    mu0[1] = c->dV * s[10] / (2.0 * tot[1]);
+    for (int k = 0; k < 2; ++k) vec_lincomb(dl[k], de[k], phi[k], 1.0, -2.0 * mu0[k], n, st);
+  }
+
+  // NewtonDirection for channel ch (RhoDirNew.f90:25-210) into dir[ch]
+  int newton(int ch) {
+    double* r = c->r;
+    double* p = c->p;
+    double* ap = c->ap;
+    vec_fill(dir[ch], 0.0, n, st);
+    vec_lincomb(r, dl[ch], dl[ch], -1.0, 0.0, n, st);   // r = -b
+    double rr = dot(r, r);
+    const double criteria = rr;
+    const double s1 = dot(phi[ch], phi[ch]);
+    double ry = 0.0, ry_last = 0.0, rr_last;
+    int iter = 1;
+    while (true) {
+      ry_last = ry;
+      ry = rr;   // y = r: r.y is |r|^2
+      if (iter == 1) vec_lincomb(p, r, r, 1.0, 0.0, n, st);
+      else vec_lincomb(p, r, p, 1.0, ry / ry_last, n, st);
+      const double s2 = dot(p, p);
+      const double eps = 2.0 * std::sqrt(mach) * (1.0 + std::sqrt(s1)) / std::sqrt(s2);
+      ReqDesc D = c->base_desc();
+      D.u = phi[0];
+      D.u2 = phi[1];
+      D.ca = 1.0;
+      D.ca2 = 1.0;
+      if (ch == 0) { D.v = p; D.cb = eps; } else { D.v2 = p; D.cb2 = eps; }
+      D.bvec = dl[ch];
+      D.out = ap;
+      D.mu0 = mu0[ch];
+      D.eps = eps;
+      D.newton_ch = ch;
+      D.opt_sqrt = 1;
+      if (eval_one(c, D, kInPhi, kOutNewton, 0, s)) throw std::runtime_error(c->err);
+      ++stats.num_evals;
+      const double pap = s[6];
+      if (pap < 0.0) {
+        if (iter == 1) vec_lincomb(dir[ch], dir[ch], p, -1.0, ry / pap, n, st);
+        break;
+      }
+      const double alpha = ry / pap;
+      vec_axpby(dir[ch], p, alpha, 1.0, n, st);
+      vec_axpby(r, ap, -alpha, 1.0, n, st);
+      rr_last = rr;
+      rr = dot(r, r);
+      if (rr < 0.1 * criteria) break;
+      if (iter > 50) break;
+      if (std::fabs(rr - rr_last) / rr < 0.01 && iter > 9) break;
+      ++iter;
+    }
+    stats.newton_iters_total += iter;
+    return iter;
+  }
+
+  // getNextGradient, numSpin = 2 (RhoLineSearch.f90:108-131): project out
+  // phi_s, theta_s = 0, normalize to sqrt(N_s), gradth_s at theta = 0
+  void next_gradient() {
+    for (int k = 0; k < 2; ++k) {
+      const double pd = dot(phi[k], dir[k]);
+      vec_axpby(dir[k], phi[k], -(pd * c->dV) / tot[k], 1.0, n, st);
+      const double nperp = std::sqrt(dot(dir[k], dir[k]) * c->dV);
+      theta[k] = 0.0;
+      vec_axpby(dir[k], dir[k], 0.0, std::sqrt(tot[k]) / nperp, n, st);
+      gradth[k] = c->dV * dot(de[k], dir[k]);
+    }
+  }
+
+  // SpinThetaSearch (RhoLineSearch.f90:447-657). Its first evaluation, at
+  // theta = 0, is the current point, whose dE/dphi, energy and gradients are
+  // already in hand, and is not repeated. Returns success as the reference
+  // sets it (2 converged, -1 error or more than 10 trial points, -2 warning).
+  int theta_search(double ftol, double gtol, double xtol) {
+    const double alphatol = (double)1e-4f;
+    double td_old[2] = {0, 0}, g_old[2] = {0, 0};
+    int ok = success;
+    for (int j = 1;; ++j) {
+      double gam = 0.0;
+      if (j > 1) {
+        const double num = gradth[0] * (gradth[0] - g_old[0]) + gradth[1] * (gradth[1] - g_old[1]);
+        gam = std::max(0.0, num / (g_old[0] * g_old[0] + g_old[1] * g_old[1]));
+      }
+      g_old[0] = gradth[0];
+      g_old[1] = gradth[1];
+      double td[2] = {-gradth[0] + gam * td_old[0], -gradth[1] + gam * td_old[1]};
+      double dedalpha;
+      while (true) {
+        dedalpha = gradth[0] * td[0] + gradth[1] * td[1];
+        if (dedalpha >= 0.0) {
+          td[0] = -gradth[0];
+          td[1] = -gradth[1];
+        } else {
+          break;
+        }
+      }
+      td_old[0] = td[0];
+      td_old[1] = td[1];
+      const double maxdir = std::max(std::fabs(td[0]), std::fabs(td[1]));
+      // MIN(0.1, 0.1*pi/maxthetadir) with the single-precision literal 0.1
+      double alpha = std::min((double)0.1f, (double)0.1f * kPi / maxdir);
+      LineSearch ls{ftol, gtol, xtol, 0.0, kPi / maxdir};
+      int nline = 0;
+      while (true) {
+        const int task = ls.next(alpha, energy[0], dedalpha);
+        if (task == 2) { ok = 2; break; }
+        if (task == 4) { ok = -1; break; }
+        if (task == 3) { ok = -2; break; }
+        const double tem[2] = {theta[0] + alpha * td[0], theta[1] + alpha * td[1]};
+        eval_at(tem);
+        gradth[0] = c->dV * s[6];
+        gradth[1] = c->dV * s[9];
+        dedalpha = gradth[0] * td[0] + gradth[1] * td[1];
+        if (++nline > 10) {
+          ok = -1;
+          break;
+        }
+      }
+      theta[0] += alpha * td[0];
+      theta[1] += alpha * td[1];
+      if (ok == -1) break;
+      if (std::sqrt(gradth[0] * gradth[0] + gradth[1] * gradth[1]) < alphatol) break;
+      if (j + 1 > 2) break;
+    }
+    return ok;
+  }
+
+  // One fixed-magnetization solve from the density in c->rho (both channels)
+  // with electron numbers tot (RhoOptN.f90:336-420, DensityOptimization).
+  void solve() {
+    outcome = -1;
+    success = 1;
+    it = 1;
+    theta[0] = theta[1] = 0.2;
+    const double ftol = (double)1e-4f, gtol = (double)2e-1f, xtol = 1e-12;
+    const double npts = (double)n;
+    vec_sqrt_abs(c->phi, c->rho, 2 * n, st);
+    const double zero[2] = {0.0, 0.0};
+    mu0[0] = mu0[1] = 0.0;
+    eval_at(zero);
+    chem_pot_and_dl();
+    double e_hist[3] = {energy[0], 0.0, 0.0};
+    while (true) {
+      for (int k = 0; k < 2; ++k) newton(k);
+      bool sw1 = false, sw2 = false;
+      while (true) {
+        next_gradient();
+        bool done = false;
+        for (int k = 0; k < 2 && !done; ++k) {
+          if (gradth[k] > 1.0e5) throw std::runtime_error(k == 0 ? "The gradient 1 is too large" : "The gradient 2 is too large");
+          if (gradth[k] >= 0.0) {
+            if (sw1 && sw2) {
+              outcome = 1;
+              done = true;
+              break;
+            }
+            (k == 0 ? sw1 : sw2) = true;
+            vec_lincomb(dir[k], dl[k], dl[k], -1.0, 0.0, n, st);
+          }
+        }
+        if (done || (gradth[0] < 0.0 && gradth[1] < 0.0)) break;
+      }
+      if (outcome == 1) break;
+      success = theta_search(ftol, gtol, xtol);
+      if (success == -1 && it > prm.max_iter) throw std::runtime_error("Fail rho optimizations");
+      // StepInformation: both channels, the previous mu (RhoOptN.f90:745-760)
+      pot_norm = std::sqrt(s[7] / npts / 2.0);
+      bool exit_flag = false;
+      const bool pconv = pot_norm < prm.pot_tol;
+      e_hist[2] = e_hist[1];
+      e_hist[1] = e_hist[0];
+      e_hist[0] = energy[0];
+      const bool econv = it >= 3 && std::fabs(e_hist[0] - e_hist[1]) < prm.tole &&
+                         std::fabs(e_hist[0] - e_hist[2]) < prm.tole;
+      if ((prm.conv_check == 1 && econv) || (prm.conv_check == 2 && pconv) ||
+          (prm.conv_check == 3 && (pconv || econv))) {
+        outcome = 0;
+        exit_flag = true;
+      }
+      if (outcome == 1) exit_flag = true;
+      if (it >= prm.max_iter) {
+        outcome = 1;
+        exit_flag = true;
+      }
+      // phi_s = phi_s cos(theta_s) + dir_s sin(theta_s): the point of the last
+      // evaluation, so its sums give the new mu
+      for (int k = 0; k < 2; ++k) vec_lincomb(phi[k], phi[k], dir[k], std::cos(theta[k]), std::sin(theta[k]), n, st);
+      chem_pot_and_dl();
+      if (exit_flag) break;
+      ++it;
+    }
+    stats.iterations = it;
+  }
+};
+
+}  // namespace
+
+static int rho_opt_spin(ofd_ctx* c, double* rhoR, int dir_type, const ofd_opt_params* prm, double* energy9,
+                        int* rho_outcome, ofd_opt_stats* stats) {
+  if (dir_type != 1) return fail(c, "spin-polarized density optimization is built for dir_type 1 (NTN)");
+  auto t0 = std::chrono::steady_clock::now();
+  SpinSolver S;
+  try {
+    alloc_opt(c);
+    S.bind(c);
+    fill_params(prm, &S.prm);
+    const long long n = c->nr();
+    HIPCHK(hipMemcpyAsync(c->rho, rhoR, 2 * n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    for (int k = 0; k < 2; ++k) S.tot[k] = dsum(c->rho + k * n, n, S.bs, c->st) * c->dV;
+    // the magnetization loop (RhoOptN.f90:336-480): FIXMAG solves once
+    const double ntot = S.tot[0] + S.tot[1];
+    double mag = S.tot[0] - S.tot[1], de = 1.0;
+    double ee1 = 0, ee2 = 0, ee3 = 0, m1 = 0, m2 = 0, m3 = 0;
+    int mag_iter = 0;
+    for (int imag = 0; imag < c->nmag; ++imag) {
+      S.solve();
+      ++mag_iter;
+      if (c->fixed_mag) break;
+      if (std::fabs(de) < c->tolm) break;
+      ee3 = ee2; ee2 = ee1; ee1 = S.energy[0];
+      m3 = m2; m2 = m1; m1 = mag;
+      de = (ee1 - ee2) / (m1 - m2);
+      if (mag_iter == 1) mag = m1 + (double)0.1f;
+      if (mag_iter == 2) mag = m1 - (double)0.2f;
+      if (mag_iter == 3) mag = m1 - de / ee1;
+      if (mag_iter > 3) {
+        const double b = (m1 - m2) / (m3 - m2);
+        const double de1 = 2.0 * (ee1 - (1 - b) * ee2 - b * ee3) / ((1.0 - 1.0 / b) * (m1 - m2) * (m1 - m2));
+        if (mag_iter > 60) break;
+        mag = m1 - de / de1;
+      }
+      // each solve restarts from the initial density, rescaled to the new
+      // spin populations (Ntot +- m) / 2
+      const double t_new[2] = {0.5 * (ntot + mag), 0.5 * (ntot - mag)};
+      for (int k = 0; k < 2; ++k) {
+        vec_axpby(c->rho + k * n, c->rho + k * n, 0.0, 1.0 / S.tot[k], n, c->st);
+        vec_axpby(c->rho + k * n, c->rho + k * n, 0.0, t_new[k], n, c->st);
+        S.tot[k] = t_new[k];
+      }
+    }
+    c->mag_state[0] = mag;
+    c->mag_state[1] = S.tot[0];
+    c->mag_state[2] = S.tot[1];
+    c->mag_iters = mag_iter;
+    vec_square(c->rho, c->phi, 2 * n, c->st);   // rhoR = phi**2 (:495)
+    HIPCHK(hipMemcpyAsync(rhoR, c->rho, 2 * n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    S.bs.free_all();
+  } catch (const std::exception& e) {
+    S.bs.free_all();
+    return fail(c, e.what());
+  }
+  if (energy9) std::memcpy(energy9, S.energy, sizeof(S.energy));
+  if (rho_outcome) *rho_outcome = S.outcome;
+  if (stats) {
+    *stats = S.stats;
+    stats->iterations = S.it;
+    stats->final_pot_norm = S.pot_norm;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return 0;
+}
+
+extern "C" int ofd_set_spin(ofd_ctx* c, const ofd_spin* sp) {
+  if (!c || !sp) return fail(c, "null argument");
+  if (sp->nspin != 1 && sp->nspin != 2) return fail(c, "nspin must be 1 or 2");
+  const int lsda = sp->lsda <= 0 ? 1 : sp->lsda;
+  if (lsda != 1 && lsda != 2) return fail(c, "lsda must be 1 (Perdew-Zunger) or 2 (Perdew-Wang 92)");
+  if (sp->nspin == 2 && c->kinetic != 4)
+    return fail(c, "spin-polarized evaluation is built for the Wang-Teter functional (kinetic 4)");
+  const int base = c->kinetic == 5 ? (int)kWGC8 : (c->alpha == c->beta ? (int)kWT3 : (int)kWT4);
+  const int kind = sp->nspin == 2 ? (base == kWT3 ? (int)kWT3S : (int)kWT4S) : base;
+  try {
+    HIPCHK(hipStreamSynchronize(c->st));
+    const long long nr = c->nr(), nc = c->sp.d.nc;
+    if (func_nf(kind) > func_nf(c->kind)) {
+      dev_free(c->work);
+      c->work = nullptr;
+      c->work = dalloc<double2>((size_t)func_nf(kind) * nc);
+    }
+    if (sp->nspin != c->nspin) {
+      free_opt(c);
+      if (sp->nspin == 2) {
+        dev_free(c->rho);
+        dev_free(c->pot);
+        c->rho = c->pot = nullptr;
+        c->rho = dalloc<double>(2 * nr);
+        c->pot = dalloc<double>(2 * nr);
+      }
+    }
+    graphs_clear(c->graphs);
+    std::memset(&c->last_desc, 0, sizeof(c->last_desc));
+    c->many_last.clear();
+    c->kind = kind;
+    c->nspin = sp->nspin;
+    c->lsda = lsda;
+    c->fixed_mag = sp->fixed_mag ? 1 : 0;
+    c->nmag = sp->nmag > 0 ? sp->nmag : 100;
+    c->tolm = sp->tolm > 0.0 ? sp->tolm : (double)5.0e-6f;
+  } catch (const std::exception& e) {
+    return fail(c, e.what());
+  }
+  return 0;
+}
+
+extern "C" int ofd_get_magnetization(const ofd_ctx* c, double* out3, int* mag_iters) {
+  if (!c || !out3) return -1;
+  for (int k = 0; k < 3; ++k) out3[k] = c->mag_state[k];
+  if (mag_iters) *mag_iters = c->mag_iters;
+  return 0;
+}
+
+extern "C" int ofd_spin_split(const ofd_ctx* c, const double* rho, double magmom, double* rho2) {
+  if (!c || !rho || !rho2) return -1;
+  const long long n = c->nr();
+  for (int s = 1; s <= 2; ++s) {
+    const double sgn = (s % 2 == 0) ? 1.0 : -1.0;   // (-1)^s
+    for (long long i = 0; i < n; ++i) rho2[(s - 1) * n + i] = 0.5 * (rho[i] - sgn * magmom / c->vol);
   }
   return 0;
 }
@@ -1875,7 +2303,9 @@ extern "C" int ofd_ion_forces(ofd_ctx* c, const double* rho, double* forces) {
     const int nion = c->type_start[ns];
     double cinv[9], rb[9], vol;
     cell_setup(c->lat, rb, &vol, cinv);
-    HIPCHK(hipMemcpyAsync(c->rho, rho, c->nr() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->rho, rho, c->nr_all() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    // forces act on the total density, SUM(rhoR, 4) (CalForce.f90:180)
+    if (c->nspin == 2) vec_axpby(c->rho, c->rho + c->nr(), 1.0, 1.0, c->nr(), c->st);
     double* df = dalloc<double>(3 * (size_t)nion);
     const size_t scratch = sizeof(double) * c->sp.d.nr + 2 * sizeof(double2) * c->sp.d.nc;
     void* sc = dalloc<char>(scratch);
@@ -2145,6 +2575,7 @@ extern "C" int ofd_ion_relax(ofd_ctx* c, double* rhoR, double* frac_xyz, const o
                              int max_energies, ofd_ion_stats* stats) {
   if (!c || !rhoR || !frac_xyz || !rho_params) return fail(c, "null argument");
   if (c->type_start.empty()) return fail(c, "ions not set (ofd_build_ion_potential)");
+  if (c->nspin != 1) return fail(c, "ion relaxation is built for numSpin = 1");
   auto t0 = std::chrono::steady_clock::now();
   // IonOptimizers.f90:29,36,41 and Neighbor.f90:33 defaults; forceCutoff is
   // a default-real literal (5e-5 in single precision)
@@ -2586,6 +3017,10 @@ extern "C" int ofd_pool_admit(ofd_pool* p, ofd_ctx* c, const double* rho, const 
   }
   if (c->kind == kWGC8 && !c->have_wgc) {
     p->err = "WGC table not set";
+    return -1;
+  }
+  if (c->nspin != 1) {
+    p->err = "pools serve numSpin = 1 requests";
     return -1;
   }
   int slot = -1;

@@ -17,8 +17,20 @@ enum FuncKind : int {
   kWT3 = 0,   // Wang-Teter, alpha == beta: F[rho], F[rho^b], F[sqrt rho]
   kWT4 = 1,   // Wang-Teter, alpha != beta: F[rho], F[rho^b], F[rho^a], F[sqrt rho]
   kWGC8 = 2,  // WGC 2nd order: F[rho], F[phi], 3 x rho^a terms, 3 x rho^b terms
+  // two spin channels (numSpin = 2); x_s = 2 rho_s, rho = rho_up + rho_dn
+  kWT3S = 3,  // F[rho], F[x_up^b], F[x_dn^b], F[sqrt x_up], F[sqrt x_dn]
+  kWT4S = 4,  // F[rho], F[x_up^b], F[x_dn^b], F[x_up^a], F[x_dn^a], F[sqrt x_up], F[sqrt x_dn]
 };
-inline int func_nf(int k) { return k == kWT3 ? 3 : (k == kWT4 ? 4 : 8); }
+constexpr bool is_spin_kind(int k) { return k == kWT3S || k == kWT4S; }
+inline int func_nf(int k) {
+  switch (k) {
+    case kWT3: return 3;
+    case kWT4: return 4;
+    case kWT3S: return 5;
+    case kWT4S: return 7;
+    default: return 8;
+  }
+}
 
 // Input forms of the density: rho directly, or rho = (ca*u + cb*v)^2 with
 // u, v two phi-like vectors (optimizer trial points).
@@ -30,6 +42,10 @@ enum InMode : int { kInRho = 0, kInPhi = 1 };
 //                s2 = sum dEdphi*phi_t
 //  kOutNewton  aux = (potential*sign(phi_t) - 2 mu0 phi_t - b)/eps written to
 //              `out`; extra sum s0 = sum v*aux   (v = the CG direction p)
+// Spin kinds write both channels' potentials (out, out2); kOutSigned then sums
+//   6 dE/dphi_up*dir_up, 7 both channels' (dE/dphi_s - 2 mu_s phi_s)^2,
+//   8 dE/dphi_up*phi_up, 9 dE/dphi_dn*dir_dn, 10 dE/dphi_dn*phi_dn
+// and kOutNewton acts on channel newton_ch only (its p is v or v2).
 enum OutMode : int { kOutPot = 0, kOutSigned = 1, kOutNewton = 2 };
 
 // Per-pass partial sums written by every epilogue workgroup.
@@ -61,6 +77,14 @@ struct ReqDesc {
   double lam, mu, alpha, beta, rhoS, coef, tkf;
   int opt_sqrt;
   int pad_;
+  // second spin channel (spin kinds): rho_dn (kInRho) or phi_dn = ca2*u2 + cb2*v2,
+  // its potential, its kOutSigned direction weights and chemical potential
+  const double* u2;
+  const double* v2;
+  double* out2;
+  double ca2, cb2, s1b, s2b, mu0b;
+  int lsda;        // exchange-correlation of the spin kinds: 1 Perdew-Zunger, 2 Perdew-Wang 92
+  int newton_ch;   // kOutNewton with spin kinds: the perturbed channel (0 up, 1 down)
 };
 
 struct GridDims {

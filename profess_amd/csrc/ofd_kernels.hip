@@ -32,6 +32,8 @@ template <int KIND> struct NFields;
 template <> struct NFields<kWT3> { static constexpr int v = 3; };
 template <> struct NFields<kWT4> { static constexpr int v = 4; };
 template <> struct NFields<kWGC8> { static constexpr int v = 8; };
+template <> struct NFields<kWT3S> { static constexpr int v = 5; };
+template <> struct NFields<kWT4S> { static constexpr int v = 7; };
 constexpr int kPlain = 9;   // one real field, no functional
 template <> struct NFields<kPlain> { static constexpr int v = 1; };
 
@@ -51,6 +53,30 @@ __device__ __forceinline__ void gen_fields(const ReqDesc& D, double rho, double*
     f[2] = ra; f[3] = th * ra; f[4] = th * th * ra / 2.0;
     f[5] = rb; f[6] = th * rb; f[7] = th * th * rb / 2.0;
   }
+}
+
+// Spin kinds: the fields of both channels at one point (CalPotential.f90:
+// 120-160, 400-407: Hartree on rho_up + rho_dn, the kinetic terms on 2 rho_s).
+template <int KIND>
+__device__ __forceinline__ void gen_fields_spin(const ReqDesc& D, double r1, double r2, double* f) {
+  const double x1 = 2.0 * r1, x2 = 2.0 * r2;
+  double ra1, rb1, ra2, rb2;
+  fwd_pows(x1, D.alpha, D.beta, ra1, rb1);
+  fwd_pows(x2, D.alpha, D.beta, ra2, rb2);
+  f[0] = r1 + r2;
+  f[1] = rb1;
+  f[2] = rb2;
+  if (KIND == kWT3S) {
+    f[3] = sqrt(x1); f[4] = sqrt(x2);
+  } else {
+    f[3] = ra1; f[4] = ra2;
+    f[5] = sqrt(x1); f[6] = sqrt(x2);
+  }
+}
+
+// phi_t of one channel: ca*u + cb*v (cb == 0: v may be null)
+__device__ __forceinline__ double phi_t(double ca, double cb, const double* u, const double* v, size_t g) {
+  return cb != 0.0 ? fma(cb, v[g], ca * u[g]) : ca * u[g];
 }
 
 // ------------------------------------------------------------ block reduce
@@ -130,15 +156,28 @@ __global__ void __launch_bounds__(256) k_xfwd(const ReqDesc* __restrict__ descs,
   for (int e = threadIdx.x; e < nrow * n1; e += blockDim.x) {
     const int r = e / n1, i1 = e - r * n1;
     const long long g = (row0 + r) * n1 + i1;
-    double rho;
-    if (INMODE == kInRho) {
-      rho = D.u[g];
-    } else {
-      const double t = D.cb != 0.0 ? fma(D.cb, D.v[g], D.ca * D.u[g]) : D.ca * D.u[g];
-      rho = t * t;
-    }
     double f[NF];
-    gen_fields<KIND>(D, rho, f);
+    if constexpr (is_spin_kind(KIND)) {
+      double r1, r2;
+      if (INMODE == kInRho) {
+        r1 = D.u[g];
+        r2 = D.u2[g];
+      } else {
+        const double t1 = phi_t(D.ca, D.cb, D.u, D.v, g), t2 = phi_t(D.ca2, D.cb2, D.u2, D.v2, g);
+        r1 = t1 * t1;
+        r2 = t2 * t2;
+      }
+      gen_fields_spin<KIND>(D, r1, r2, f);
+    } else {
+      double rho;
+      if (INMODE == kInRho) {
+        rho = D.u[g];
+      } else {
+        const double t = D.cb != 0.0 ? fma(D.cb, D.v[g], D.ca * D.u[g]) : D.ca * D.u[g];
+        rho = t * t;
+      }
+      gen_fields<KIND>(D, rho, f);
+    }
 #pragma unroll
     for (int j = 0; j < NF; ++j) Ar[2 * ((r * NF + j) * ld + (i1 >> 1)) + (i1 & 1)] = (R)f[j];
   }
@@ -211,6 +250,17 @@ __device__ __forceinline__ void gop(const ReqDesc& D, const double* __restrict__
       x[2][0] *= K; x[2][1] *= K;
       x[3][0] *= vw; x[3][1] *= vw;
     }
+  } else if (KIND == kWT3S || KIND == kWT4S) {
+    // Hartree on the total; the Wang-Teter kernel on each channel's x^b (and
+    // x^a); vW on each channel's sqrt(x)
+    constexpr int nk = KIND == kWT3S ? 2 : 4;
+    const double K = D.wtk[gidx];
+    const double vw = q2 * invn;
+    x[0][0] *= hart; x[0][1] *= hart;
+#pragma unroll
+    for (int f = 1; f <= nk; ++f) { x[f][0] *= K; x[f][1] *= K; }
+#pragma unroll
+    for (int f = nk + 1; f <= nk + 2; ++f) { x[f][0] *= vw; x[f][1] *= vw; }
   } else if (KIND == kWGC8) {
     const double k1 = wk[gidx] * invn, k2 = wk[nc + gidx] * invn, k3 = wk[2 * nc + gidx] * invn,
                  k4 = wk[3 * nc + gidx] * invn;
@@ -330,6 +380,56 @@ __device__ __forceinline__ double epi_point(const ReqDesc& D, double rho, double
   return loc + D.mu * t / (2.0 * p.phi);
 }
 
+// Spin kinds: both channels' potentials at one point and the energy sums.
+// Per channel s (x_s = 2 rho_s): ion + Hartree of the total, the LSDA
+// potential, TF and Wang-Teter on x_s (WTPotentialPlus(2 rho_s)), the chain
+// rule 2 sqrt(rho_s) under optSqrt, and vW on sqrt(x_s) divided by sqrt(2)
+// (CalVW, KEDF_VW.f90:100-140); the kinetic energies are halved sums over the
+// channels (KEDF_TF.f90:66-74, CalPotential.f90:400-407).
+template <int KIND>
+__device__ __forceinline__ void epi_point_spin(const ReqDesc& D, double r1, double r2, double vion, const double* o,
+                                               double (&acc)[kNSum], double& p1, double& p2) {
+  constexpr double kInvSqrt2 = 0.70710678118654752440;
+  double vx[2], exc;
+  lsda_point(r1, r2, D.lsda, vx[0], vx[1], exc);
+  const double common = vion + o[0];
+  const double rs[2] = {r1, r2};
+  double pot[2], etf = 0.0, evw = 0.0, enl = 0.0;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double x = 2.0 * rs[s];
+    PtVals p;
+    point_vals(x, D.alpha, D.beta, p);
+    const double r23 = p.r13 * p.r13;
+    const double vtf = (5.0 / 3.0 * kCTF * D.lam) * r23;
+    double vnl, t;
+    if (KIND == kWT3S) {
+      vnl = kCTF * (D.alpha * p.ram1 * o[1 + s] + D.beta * p.rbm1 * o[1 + s]);
+      enl += p.ra * o[1 + s];
+      t = o[3 + s];
+    } else {
+      vnl = kCTF * (D.alpha * p.ram1 * o[1 + s] + D.beta * p.rbm1 * o[3 + s]);
+      enl += p.ra * o[1 + s];
+      t = o[5 + s];
+    }
+    const double loc = common + vx[s] + vtf + vnl;
+    const double tw = D.mu * t * kInvSqrt2;
+    const double sq = p.phi * kInvSqrt2;   // sqrt(rho_s)
+    pot[s] = D.opt_sqrt ? 2.0 * sq * loc + tw : loc + tw / (2.0 * sq);
+    etf += x * r23;
+    evw += p.phi * t;
+  }
+  const double rho = r1 + r2;
+  acc[0] += rho * vion;
+  acc[1] += o[0] * rho;
+  acc[2] += exc;
+  acc[3] += 0.5 * etf;
+  acc[4] += 0.5 * evw;
+  acc[5] += 0.5 * enl;
+  p1 = pot[0];
+  p2 = pot[1];
+}
+
 template <typename R, int KIND, int INMODE, int OUTMODE>
 __global__ void __launch_bounds__(256) k_xinv(const ReqDesc* __restrict__ descs, GridDims d, Plan1D p,
                                               const typename V2<R>::T* __restrict__ tw,
@@ -382,6 +482,44 @@ __global__ void __launch_bounds__(256) k_xinv(const ReqDesc* __restrict__ descs,
     for (int j = 0; j < NF; ++j) o[j] = (double)rr[2 * ((r * NF + j) * ld + (i1 >> 1)) + (i1 & 1)];
     if (KIND == kPlain) {
       D.out[g] = o[0];
+      continue;
+    }
+    if constexpr (is_spin_kind(KIND)) {
+      double t1 = 0.0, t2 = 0.0, r1, r2;
+      if (INMODE == kInRho) {
+        r1 = D.u[g];
+        r2 = D.u2[g];
+      } else {
+        t1 = phi_t(D.ca, D.cb, D.u, D.v, g);
+        t2 = phi_t(D.ca2, D.cb2, D.u2, D.v2, g);
+        r1 = t1 * t1;
+        r2 = t2 * t2;
+      }
+      double p1, p2;
+      epi_point_spin<KIND>(D, r1, r2, D.vion[g], o, acc, p1, p2);
+      if (OUTMODE == kOutPot) {
+        D.out[g] = p1;
+        D.out2[g] = p2;
+      } else if (OUTMODE == kOutSigned) {
+        const double de1 = p1 * fsign(t1), de2 = p2 * fsign(t2);
+        D.out[g] = de1;
+        D.out2[g] = de2;
+        const double dir1 = D.s1 * D.u[g] + (D.v != nullptr ? D.s2 * D.v[g] : 0.0);
+        const double dir2 = D.s1b * D.u2[g] + (D.v2 != nullptr ? D.s2b * D.v2[g] : 0.0);
+        const double dl1 = de1 - 2.0 * t1 * D.mu0, dl2 = de2 - 2.0 * t2 * D.mu0b;
+        acc[6] += de1 * dir1;
+        acc[7] += dl1 * dl1 + dl2 * dl2;
+        acc[8] += de1 * t1;
+        acc[9] += de2 * dir2;
+        acc[10] += de2 * t2;
+      } else {
+        const bool up = D.newton_ch == 0;
+        const double tt = up ? t1 : t2;
+        const double de = (up ? p1 : p2) * fsign(tt);
+        const double ap = ((de - D.mu0 * 2.0 * tt) - D.bvec[g]) / D.eps;
+        D.out[g] = ap;
+        acc[6] += (up ? D.v[g] : D.v2[g]) * ap;
+      }
       continue;
     }
     double tphi = 0.0, rho;
@@ -500,6 +638,8 @@ static int eval_r(const ShapePlan& sp, int kind, int inmode, int outmode, const 
     case kWT3: return eval_k<R, kWT3>(sp, inmode, outmode, dd, nb, st);
     case kWT4: return eval_k<R, kWT4>(sp, inmode, outmode, dd, nb, st);
     case kWGC8: return eval_k<R, kWGC8>(sp, inmode, outmode, dd, nb, st);
+    case kWT3S: return eval_k<R, kWT3S>(sp, inmode, outmode, dd, nb, st);
+    case kWT4S: return eval_k<R, kWT4S>(sp, inmode, outmode, dd, nb, st);
     default: return -3;
   }
 }

@@ -100,6 +100,72 @@ __device__ __forceinline__ void point_vals(double rho, double alpha, double beta
   p.exc = cXe * rho * p.r13 + rho * ec;
 }
 
+// Spin-polarized LDA at one point (numSpin = 2): both channels' potentials and
+// the energy density. lsda 1: LSDAPotPZ (XC_LDA.f90:417-547, Perdew-Zunger
+// correlation in Rydberg, halved with the doubled exchange); lsda 2:
+// LSDAPotPW92 (XC_LDA.f90:313-414, spn_gcor :550-592). The constants are the
+// reference's parametrization.
+__device__ __forceinline__ void pw92_g(double a, double a1, double b1, double b2, double b3, double b4, double rs,
+                                       double rs12, double& g, double& grs) {
+  // p = 1: rs^p = rs, rs^(3/2) = rs12^3
+  const double q0 = -2.0 * a * (1.0 + a1 * rs);
+  const double q1 = 2.0 * a * (b1 * rs12 + b2 * rs + b3 * (rs12 * rs12 * rs12) + b4 * rs * rs);
+  const double q2 = log(1.0 + 1.0 / q1);
+  g = q0 * q2;
+  const double q3 = a * (b1 / rs12 + 2.0 * b2 + 3.0 * b3 * rs12 + 4.0 * b4 * rs);
+  grs = -2.0 * a * a1 * q2 - q0 * q3 / (q1 * q1 + q1);
+}
+
+__device__ __forceinline__ void lsda_point(double r1, double r2, int lsda, double& v1, double& v2, double& e) {
+  constexpr double ax = -0.7385588, fzz = 1.709921, gam = 0.5198421;
+  constexpr double p75vpi = 0.75 / 3.141592653589793238462643;
+  const double ex1 = ax * cbrt(2.0 * r1), ex2 = ax * cbrt(2.0 * r2);
+  const double vx1 = ex1 * (4.0 / 3.0), vx2 = ex2 * (4.0 / 3.0);
+  const double d = r1 + r2;
+  const double rs = cbrt(p75vpi / d);
+  const double zet = (r1 - r2) / d;
+  const double cp = cbrt(1.0 + zet), cm = cbrt(1.0 - zet);
+  const double f = ((1.0 + zet) * cp + (1.0 - zet) * cm - 2.0) / gam;
+  const double fz = (4.0 / 3.0) * (cp - cm) / gam;
+  if (lsda == 2) {
+    const double rs12 = sqrt(rs);
+    double eu, eurs, ep, eprs, alfm, alfrsm;
+    pw92_g(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, rs, rs12, eu, eurs);
+    pw92_g(0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, rs, rs12, ep, eprs);
+    pw92_g(0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671, rs, rs12, alfm, alfrsm);
+    const double z3 = zet * zet * zet, z4 = z3 * zet;
+    const double ec = eu * (1.0 - f * z4) + ep * f * z4 - alfm * f * (1.0 - z4) / fzz;
+    const double ecrs = eurs * (1.0 - f * z4) + eprs * f * z4 - alfrsm * f * (1.0 - z4) / fzz;
+    const double eczet = 4.0 * z3 * f * (ep - eu + alfm / fzz) + fz * (z4 * ep - z4 * eu - (1.0 - z4) * alfm / fzz);
+    const double comm = ec - rs * ecrs / 3.0 - zet * eczet;
+    v1 = vx1 + comm + eczet;
+    v2 = vx2 + comm - eczet;
+    e = r1 * ex1 + r2 * ex2 + ec * d;
+    return;
+  }
+  double eu, vc, es, vs;
+  if (rs < 1.0) {
+    const double lrs = log(rs);
+    eu = 0.0622 * lrs - 0.0960 + (0.0040 * lrs - 0.0232) * rs;
+    vc = eu - (0.0622 + (0.0040 * lrs + 0.0040 - 0.0232) * rs) / 3.0;
+    es = 0.0311 * lrs - 0.0538 + (0.0014 * lrs - 0.0096) * rs;
+    vs = es - (0.0311 + (0.0014 * lrs + 0.0014 - 0.0096) * rs) / 3.0;
+  } else {
+    const double srs = sqrt(rs);
+    eu = -0.2846 / (1.0 + 1.0529 * srs + 0.3334 * rs);
+    vc = eu * eu * (1.0 + 7.0 * 1.0529 * srs / 6.0 + 4.0 / 3.0 * 0.3334 * rs) / -0.2846;
+    es = -0.1686 / (1.0 + 1.3981 * srs + 0.2611 * rs);
+    vs = es * es * (1.0 + 7.0 * 1.3981 * srs / 6.0 + 4.0 / 3.0 * 0.2611 * rs) / -0.1686;
+  }
+  const double base = vc + f * (vs - vc);
+  const double vc2 = base - (es - eu) * (1.0 + zet) * fz;
+  const double vc1 = base + (es - eu) * (1.0 - zet) * fz;
+  const double ec = eu + f * (es - eu);
+  v1 = 0.5 * (2.0 * vx1 + vc1);
+  v2 = 0.5 * (2.0 * vx2 + vc2);
+  e = 0.5 * (2.0 * (r1 * ex1 + r2 * ex2) + ec * d);
+}
+
 // rho^alpha and rho^beta for the forward fields
 __device__ __forceinline__ void fwd_pows(double rho, double alpha, double beta, double& ra, double& rb) {
   if (alpha == 5.0 / 6.0 && beta == 5.0 / 6.0) {
