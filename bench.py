@@ -42,6 +42,16 @@ def b_wt(n1, n2, n3):
     return 4 * 8 * nr + 6 * 16 * nc
 
 
+def b_wgc(n1, n2, n3):
+    """Algorithmic HBM bytes of one second-order WGC evaluation: the four
+    real grids as for WT and eight half-complex spectra (rho, sqrt rho, and
+    theta^k rho^alpha, theta^k rho^beta for k = 0, 1, 2) each written once and
+    read once (16 complex)."""
+    nr = n1 * n2 * n3
+    nc = (n1 // 2 + 1) * n2 * n3
+    return 4 * 8 * nr + 16 * 16 * nc
+
+
 class Hip:
     def __init__(self):
         self.L = C.CDLL("/opt/rocm/lib/libamdhip64.so")
@@ -123,6 +133,59 @@ def time_single(pa, hip, case, steps, warmup, precision=0):
         raise RuntimeError("non-finite energies")
     vion = s.ion_potential()
     return s, ms / steps, et, vion, (d_rho, d_pot, d_sum)
+
+
+def time_wgc(pa, hip, n=256, steps=10, warmup=2):
+    """One second-order WGC evaluation (gamma = 2.7) of hcp Mg with 2000
+    atoms on n^3, device-resident rho: microseconds per evaluation by HIP
+    events on the context's stream, and the eTable."""
+    from profess_amd import inputs
+    L = pa.lib()
+    reps = 10 if n == 256 else max(1, round(n / 25.6))
+    c = inputs.mg_hcp_case((reps, reps, reps), (n, n, n), sigma=0.1, seed_ion=53)
+    s = pa.OFDSystem(c["lattice"], c["grid"], c["num_electrons"], kinetic=5, gamma=2.7)
+    s.build_ion_potential([(c["recpot"][0], c["recpot"][1], c["frac"])])
+    p = s.params
+    try:
+        from tests import wgc_tables
+        s.set_wgc_table(*wgc_tables.for_params(p["alpha"], p["beta"], p["gamma"]))
+        table = "the reference's makeKernel2 table (tests/golden fixture)"
+    except (ImportError, KeyError, OSError):
+        s.build_wgc_table()
+        table = "ofd_build_wgc_table"
+    rho = np.ascontiguousarray(c["rho"])
+    d_rho, d_pot, d_sum = hip.malloc(rho.nbytes), hip.malloc(rho.nbytes), hip.malloc(9 * 8)
+    hip.h2d(d_rho, rho)
+    stream = L.ofd_get_stream(s._h)
+
+    def enq():
+        s._check(L.ofd_calpot_enqueue(s._h, C.c_void_p(d_rho), 1, C.c_void_p(d_pot), C.c_void_p(d_sum)))
+
+    for _ in range(warmup):
+        enq()
+    s._check(L.ofd_sync(s._h))
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    for _ in range(steps):
+        enq()
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1) / steps
+    sums = np.zeros(9)
+    hip.d2h(sums, d_sum)
+    et = np.zeros(9)
+    L.ofd_sums_to_etable(s._h, sums.ctypes.data_as(C.POINTER(C.c_double)), et.ctypes.data_as(C.POINTER(C.c_double)))
+    for b in (d_rho, d_pot, d_sum):
+        hip.L.hipFree(C.c_void_p(b))
+    s.close()
+    if not np.all(np.isfinite(et)):
+        raise RuntimeError("non-finite WGC energies")
+    b = b_wgc(n, n, n)
+    return {"workload": "WGC second order (gamma 2.7) + LDA + Hartree + ion-electron energy and potential, hcp Mg "
+                        "%d atoms (sigma 0.1 bohr), %d^3 grid, FP64, device-resident rho; kernel table: %s"
+                        % (len(c["frac"]), n, table),
+            "grid": [n, n, n], "us_per_eval": ms * 1e3, "evals_per_s": 1000.0 / ms, "steps": steps,
+            "algorithmic_bytes": b, "achieved_GBs": b / (ms * 1e-3) / 1e9,
+            "roofline_frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "etable": list(et)}
 
 
 def time_batched(pa, hip, s0, case, vion, nb, steps, warmup):
@@ -334,6 +397,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-wgc", action="store_true", help="skip the 256^3 WGC evaluation leg")
+    ap.add_argument("--wgc-only", action="store_true", help="only the 256^3 WGC evaluation leg")
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--serve", type=int, default=0, metavar="NREQ",
                     help="only the mixed-grid serving workload with NREQ requests (pool alone, no plain run)")
@@ -354,6 +419,9 @@ def main():
     n = args.grid
     if args.probe:
         probe(pa, hip, n, args.steps)
+        return
+    if args.wgc_only:
+        print(json.dumps({"wgc256": time_wgc(pa, hip, steps=args.steps, warmup=args.warmup)}), flush=True)
         return
     case = make_case(n)
     if args.serve:
@@ -400,6 +468,11 @@ def main():
         msb, _ = time_batched(pa, hip, s, case, vion, args.batch, max(10, args.steps // args.batch), args.warmup)
         out["batched"] = {"batch": args.batch, "evals_per_s": args.batch * 1000.0 / msb, "ms_per_batch": msb,
                           "roofline_frac": bw * args.batch * 1000.0 / msb / 1e9 / HBM_PEAK_GBS}
+    if not args.no_wgc:
+        try:
+            out["wgc256"] = time_wgc(pa, hip)
+        except Exception as e:   # reported, never hidden
+            out["wgc256"] = {"error": str(e)}
     if not args.no_serving:
         try:
             out["serving"] = serving_sample(pa)

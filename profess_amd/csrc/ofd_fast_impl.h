@@ -143,6 +143,7 @@ __device__ __forceinline__ void fft2s(C (&x)[P2<N>::R1], R* __restrict__ lds, in
 template <int KIND> struct NF_;
 template <> struct NF_<kWT3> { static constexpr int v = 3; };
 template <> struct NF_<kWT4> { static constexpr int v = 4; };
+template <> struct NF_<kWGC8> { static constexpr int v = 8; };
 
 template <int KIND>
 __device__ __forceinline__ void fields(const ReqDesc& D, double rho, double* f) {
@@ -187,6 +188,17 @@ __device__ __forceinline__ void block_sum_strided(double (&acc)[N], gptr<double>
 
 template <int KIND>
 __device__ __forceinline__ double field_of(const ReqDesc& D, int f, double rho) {
+  if (KIND == kWGC8) {
+    // rho, sqrt(rho), then theta^k rho^alpha / k! and theta^k rho^beta / k!
+    // (k = 0, 1, 2; theta = rho - rho*) as the second-order WGC expansion
+    // takes them (KEDF_WGC.f90:208-286)
+    if (f == 0) return rho;
+    if (f == 1) return sqrt(rho);
+    const double pw = exp2((f < 5 ? D.alpha : D.beta) * log2(rho));
+    const double th = rho - D.rhoS;
+    const int k = (f - 2) % 3;
+    return k == 0 ? pw : (k == 1 ? th * pw : th * th * pw / 2.0);
+  }
   if (f == 0) return rho;
   if (f == NF_<KIND>::v - 1) return sqrt(rho);
   double ra, rb;
@@ -223,7 +235,54 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
   const long long row = (long long)xcd_block(blockIdx.x, gridDim.x) * RPB + r;
   const bool active = row < nrows;
   C z[R1];
-  if (active) {
+  auto rho_at = [&](size_t g) {
+    if (INMODE == kInRho) return (double)glb(D.u)[g];
+    double tp = D.ca * glb(D.u)[g];
+    if (D.cb != 0.0) tp = fma(D.cb, (double)glb(D.v)[g], tp);
+    return tp * tp;
+  };
+  if constexpr (KIND == kWGC8) {
+    // six of the eight fields are rho^alpha or rho^beta times a factor: every
+    // point's two powers are computed once (one log2, two exp2) by one of the
+    // row's NF*T threads and shared through LDS, which the lines' transforms
+    // reuse afterwards
+    double* pa = reinterpret_cast<double*>(smem) + (size_t)r * 4 * M;
+    double* pb = pa + 2 * M;
+    if (active) {
+      const size_t base = (size_t)row * (2 * M);
+      for (int q = f * T + t; q < 2 * M; q += NF * T) {
+        const double l2 = log2(rho_at(base + q));
+        pa[q] = exp2(D.alpha * l2);
+        pb[q] = exp2(D.beta * l2);
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const size_t base = (size_t)row * (2 * M);
+      const int kk = (f - 2) % 3;
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        const int k = t + T * m;
+        double v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const double rho = rho_at(base + 2 * k + h);
+          if (f == 0) {
+            v[h] = rho;
+          } else if (f == 1) {
+            v[h] = sqrt(rho);
+          } else {
+            const double pw = f < 5 ? pa[2 * k + h] : pb[2 * k + h];
+            const double th = rho - D.rhoS;
+            v[h] = kk == 0 ? pw : (kk == 1 ? th * pw : th * th * pw / 2.0);
+          }
+        }
+        z[m].x = (R)v[0];
+        z[m].y = (R)v[1];
+      }
+    }
+    __syncthreads();
+  } else if (active) {
     const size_t base = (size_t)row * (2 * M);
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
@@ -489,6 +548,125 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   }
 }
 
+// z columns of the WGC fields with the coupled G-space operator of the
+// second-order expansion (KEDF_WGC.f90:222-231, :262-280): on each block of
+// three spectra (A, B, C) = (F[rho^p], F[theta rho^p], F[theta^2 rho^p / 2])
+// it makes a' = K1 A + K2 B + K3 C, b' = K2 A + K4 B, c' = K3 A.
+// blockIdx.y = 0 takes F[rho] (Hartree) and F[sqrt rho] (vW), 1 the
+// rho^alpha block, 2 the rho^beta block. A block is done in three phases with
+// two columns per lane in registers: c' from A; then b' and the partial
+// K1 A + K2 B once B is in; then a' once C is in. Each output is written to
+// the slot of an input already read, so the block leaves (c', b', a') in the
+// slots of (A, B, C); the x-inverse epilogue reads them in that order.
+template <int N, typename R>
+__global__ void __launch_bounds__(256) k_fz_wgc(const ReqDesc* __restrict__ descs, GridDims d,
+                                                const typename V2<R>::T* __restrict__ tw, int kb) {
+  using C = typename V2<R>::T;
+  constexpr int R1 = P2<N>::R1, T = P2<N>::R2;
+  constexpr int LL = col_stride<N>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* lds = reinterpret_cast<C*>(smem);
+  const ReqDesc& D = descs[blockIdx.z];
+  const int g = blockIdx.y;
+  const int f0 = g == 0 ? 0 : 3 * g - 1;   // fields 0-1, 2-4, 5-7
+  const int c = threadIdx.x % kb, t = threadIdx.x / kb;
+  const long long col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + c;
+  const long long ncol = (long long)d.K1 * d.n2;
+  const bool active = col < ncol;
+  const int i2 = active ? (int)(col / d.K1) : 0;
+  const int m1 = active ? (int)(col - (long long)i2 * d.K1) : 0;
+  const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
+  const size_t es = (size_t)d.n2 * d.K1;
+  const double invn = 1.0 / (double)d.nr;
+  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)f0 * d.nc + (size_t)col;
+  C* line = lds + (size_t)c * LL;
+  C xa[R1], xb[R1];
+  auto load = [&](C (&x)[R1], int j) {
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) x[m] = ldc(base + (size_t)j * d.nc + (size_t)(t + T * m) * es);
+    }
+  };
+  auto store = [&](const C (&x)[R1], int j) {
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) stc(base + (size_t)j * d.nc + (size_t)(t + T * m) * es, x[m]);
+    }
+  };
+  if (g == 0) {
+    load(xa, 0);
+    load(xb, 1);
+    fft2<N, -1>(xa, line, t, tw, active);
+    fft2<N, -1>(xb, line, t, tw, active);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        const int i3 = t + T * m;
+        const int m3 = i3 > N / 2 ? i3 - N : i3;
+        const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
+        const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
+        const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
+        const double q2 = gx * gx + gy * gy + gz * gz;
+        const double hart = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0, vw = q2 * invn;
+        xa[m].x = (R)((double)xa[m].x * hart);
+        xa[m].y = (R)((double)xa[m].y * hart);
+        xb[m].x = (R)((double)xb[m].x * vw);
+        xb[m].y = (R)((double)xb[m].y * vw);
+      }
+    }
+    fft2<N, +1>(xa, line, t, tw, active);
+    fft2<N, +1>(xb, line, t, tw, active);
+    store(xa, 0);
+    store(xb, 1);
+    return;
+  }
+  const gptr<const double> wk = glb(D.wgck);
+  auto kidx = [&](int m) { return (size_t)(t + T * m) * es + (size_t)col; };
+  // phase 1: c' = K3 A into the slot of A
+  load(xa, 0);
+  fft2<N, -1>(xa, line, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const double k3 = wk[2 * d.nc + kidx(m)] * invn;
+      xb[m].x = (R)(k3 * (double)xa[m].x);
+      xb[m].y = (R)(k3 * (double)xa[m].y);
+    }
+  }
+  fft2<N, +1>(xb, line, t, tw, active);
+  store(xb, 0);
+  // phase 2: b' = K2 A + K4 B into the slot of B; xa <- K1 A + K2 B
+  load(xb, 1);
+  fft2<N, -1>(xb, line, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const size_t gi = kidx(m);
+      const double k1 = wk[gi] * invn, k2 = wk[d.nc + gi] * invn, k4 = wk[3 * d.nc + gi] * invn;
+      const double ar = xa[m].x, ai = xa[m].y, br = xb[m].x, bi = xb[m].y;
+      xa[m].x = (R)(k1 * ar + k2 * br);
+      xa[m].y = (R)(k1 * ai + k2 * bi);
+      xb[m].x = (R)(k2 * ar + k4 * br);
+      xb[m].y = (R)(k2 * ai + k4 * bi);
+    }
+  }
+  fft2<N, +1>(xb, line, t, tw, active);
+  store(xb, 1);
+  // phase 3: a' = (K1 A + K2 B) + K3 C into the slot of C
+  load(xb, 2);
+  fft2<N, -1>(xb, line, t, tw, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const double k3 = wk[2 * d.nc + kidx(m)] * invn;
+      xa[m].x = (R)((double)xa[m].x + k3 * (double)xb[m].x);
+      xa[m].y = (R)((double)xa[m].y + k3 * (double)xb[m].y);
+    }
+  }
+  fft2<N, +1>(xa, line, t, tw, active);
+  store(xa, 2);
+}
+
 // ------------------------------------------------------------ x inverse (C2R) + epilogue
 template <int KIND>
 __device__ __forceinline__ double epi(const ReqDesc& D, double rho, double vion, const double* o, double (&acc)[kNSum]) {
@@ -501,10 +679,20 @@ __device__ __forceinline__ double epi(const ReqDesc& D, double rho, double vion,
     vnl = kCTF * (D.alpha * p.ram1 * o[1] + D.beta * p.rbm1 * o[1]);
     enl = p.ra * o[1];
     t = o[2];
-  } else {
+  } else if (KIND == kWT4) {
     vnl = kCTF * (D.alpha * p.ram1 * o[1] + D.beta * p.rbm1 * o[2]);
     enl = p.ra * o[1];
     t = o[3];
+  } else {
+    // WGCPotentialPlus, second-order branch (KEDF_WGC.f90:208-286). The z
+    // pass leaves each block as (c', b', a') in the slots of (A, B, C):
+    // o[4], o[3], o[2] are a', b', c' of the rho^alpha block, o[7], o[6],
+    // o[5] those of the rho^beta block
+    const double th = rho - D.rhoS, a = D.alpha, b = D.beta;
+    t = o[1];
+    enl = p.rb * (o[4] + th * o[3] + th * th / 2.0 * o[2]);
+    vnl = kCTF * (p.rbm1 * (b * o[4] + (rho + b * th) * o[3] + th * (rho + b * th / 2.0) * o[2]) +
+                  p.ram1 * (a * o[7] + (rho + a * th) * o[6] + th * (rho + a * th / 2.0) * o[5]));
   }
   const double loc = vion + o[0] + p.vxc + vtf + vnl;
   acc[0] += rho * vion;
@@ -1032,8 +1220,9 @@ static inline bool is_fast_half(int m) {
 static inline bool is_gp_len(int n) { return n == 60 || n == 120; }
 
 inline bool fast_path_ok_impl(const ShapePlan& sp, int kind) {
-  return (kind == kWT3 || kind == kWT4) && (is_fast_half(sp.d.M) || is_gp_len(sp.d.M)) &&
-         (is_fast_len(sp.d.n2) || is_gp_len(sp.d.n2)) && (is_fast_len(sp.d.n3) || is_gp_len(sp.d.n3));
+  const bool xy = (is_fast_half(sp.d.M) || is_gp_len(sp.d.M)) && (is_fast_len(sp.d.n2) || is_gp_len(sp.d.n2));
+  if (kind == kWGC8) return xy && is_fast_len(sp.d.n3);   // the coupled z pass has the two-stage form only
+  return (kind == kWT3 || kind == kWT4) && xy && (is_fast_len(sp.d.n3) || is_gp_len(sp.d.n3));
 }
 
 struct FastTiles {
@@ -1172,7 +1361,13 @@ static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hip
   using C = typename V2<R>::T;
   const long long ncol = (long long)sp.d.K1 * sp.d.n2;
   const int nblk = (int)((ncol + kb - 1) / kb);
-  if constexpr (IsGP<N>::v) {
+  if constexpr (KIND == kWGC8) {
+    if constexpr (!IsGP<N>::v) {
+      const size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
+      allow_lds<k_fz_wgc<N, R>>(lds);
+      hipLaunchKernelGGL((k_fz_wgc<N, R>), dim3(nblk, 3, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb);
+    }
+  } else if constexpr (IsGP<N>::v) {
     const size_t lds = (size_t)kb * GPD<N>::LINE * sizeof(C);
     allow_lds<g_fz_gop<N, KIND, R>>(lds);
     hipLaunchKernelGGL((g_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * GPD<N>::T), lds, st, dd,

@@ -82,10 +82,12 @@ __device__ __forceinline__ void point_vals(double rho, double alpha, double beta
     p.rb = p.ra;
     p.rbm1 = i16;
   } else {
-    p.ra = pow(rho, alpha);
-    p.ram1 = pow(rho, alpha - 1.0);
-    p.rb = pow(rho, beta);
-    p.rbm1 = pow(rho, beta - 1.0);
+    // one log2 for both exponents; rho^(a-1) = rho^a / rho
+    const double l2 = log2(rho);
+    p.ra = exp2(alpha * l2);
+    p.rb = beta == alpha ? p.ra : exp2(beta * l2);
+    p.ram1 = p.ra / rho;
+    p.rbm1 = p.rb / rho;
   }
   double ec;
   if (rs < 1.0) {
@@ -166,14 +168,19 @@ __device__ __forceinline__ void lsda_point(double r1, double r2, int lsda, doubl
   e = 0.5 * (2.0 * (r1 * ex1 + r2 * ex2) + ec * d);
 }
 
+// Powers of a density with a general exponent are taken as exp2(a log2 rho)
+// (positive rho): the relative error is about |a log2 rho| ulp, below 1e-14
+// for the densities and exponents here, at a fraction of the cost of pow.
+
 // rho^alpha and rho^beta for the forward fields
 __device__ __forceinline__ void fwd_pows(double rho, double alpha, double beta, double& ra, double& rb) {
   if (alpha == 5.0 / 6.0 && beta == 5.0 / 6.0) {
     ra = rho * rho_m16(rho);
     rb = ra;
   } else {
-    ra = pow(rho, alpha);
-    rb = pow(rho, beta);
+    const double l2 = log2(rho);
+    ra = exp2(alpha * l2);
+    rb = beta == alpha ? ra : exp2(beta * l2);
   }
 }
 

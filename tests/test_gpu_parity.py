@@ -106,7 +106,11 @@ def _wgc_table(f):
     return W.for_params(f.alpha, f.beta, f.gamma)
 
 
-@pytest.mark.parametrize("grid", [(16, 16, 16), (20, 24, 18)])
+# (20, 24, 18) runs on the generic passes, the others on the register FFT
+# passes (the coupled WGC z pass; the fused x+y plane kernel at 32^3; a
+# general two-stage x length at 120)
+@pytest.mark.parametrize("grid", [(16, 16, 16), (20, 24, 18), (32, 32, 32), (64, 48, 40), (120, 96, 64),
+                                  (128, 64, 128)])
 def test_calpot_wgc_matches_oracle(pa, grid):
     _say("calpot WGC grid %s" % (grid,))
     cell, frac, vion, ne, f, rho = _case(grid, _tri_lattice(), kinetic=5)
@@ -283,6 +287,31 @@ def test_fast_mixed_lengths_match_generic_path(pa, monkeypatch, grid):
             monkeypatch.setenv("OFD_GENERIC", flag)
             s = pa.OFDSystem(cell.lattice, grid, ne, kinetic=4, precision=prec)
             s.set_ion_terms(vion, 0.0)
+            res[prec, flag] = pa.CalculatePotentialPlus(s, rho)
+            s.close()
+    (p0, e0), (p1, e1) = res[0, "0"], res[0, "1"]
+    assert np.max(np.abs(p0 - p1)) <= 1e-13 * np.max(np.abs(p1))
+    _check_terms(e0, e1, rtol=1e-13)
+    for flag in ("0", "1"):
+        p32, e32 = res[1, flag]
+        assert np.max(np.abs(p32 - p1)) <= 1e-5 * np.max(np.abs(p1))
+        _check_terms(e32, e1, rtol=1e-5, floor=1e-8)
+
+
+@pytest.mark.parametrize("grid", [(64, 64, 64), (256, 48, 32)])
+def test_fast_wgc_matches_generic_path(pa, monkeypatch, grid):
+    """WGC on the register FFT passes (coupled z pass) and on the generic LDS
+    passes agree to rounding in FP64, and in the FP32 passes within FP32
+    rounding, for the plain evaluation and the optimizer's signed form."""
+    cell, frac, vion, ne, f, rho = _case(grid, _tri_lattice(), kinetic=5, seed=9)
+    table = _wgc_table(f)
+    res = {}
+    for prec in (0, 1):
+        for flag in ("0", "1"):
+            monkeypatch.setenv("OFD_GENERIC", flag)
+            s = pa.OFDSystem(cell.lattice, grid, ne, kinetic=5, precision=prec)
+            s.set_ion_terms(vion, 0.0)
+            s.set_wgc_table(*table)
             res[prec, flag] = pa.CalculatePotentialPlus(s, rho)
             s.close()
     (p0, e0), (p1, e1) = res[0, "0"], res[0, "1"]
