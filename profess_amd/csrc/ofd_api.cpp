@@ -345,6 +345,7 @@ struct ofd_ctx {
   double* fin = nullptr;     // kNSum finalized sums
   double* wgck = nullptr;    // 4 x nc
   double* wtk = nullptr;     // nc: WT kernel coef * LindG / N
+  float* wtk32 = nullptr;    // its FP32 copy (precision 1)
   ReqDesc* desc = nullptr;   // one descriptor
   ReqDesc last_desc{};       // what `desc` holds after an enqueue
   ReqDesc* many_desc = nullptr;
@@ -379,6 +380,7 @@ struct ofd_ctx {
     D.sums = sums;
     D.wgck = wgck;
     D.wtk = wtk;
+    D.wtk32 = wtk32;
     D.work = work;
     D.fin = fin;
     for (int i = 0; i < 9; ++i) D.rb[i] = rb[i];
@@ -500,6 +502,10 @@ extern "C" int ofd_create(const ofd_grid* g, const ofd_functional* f, double num
       c->wtk = dalloc<double>(nc);
       if (build_wt_kernel(c->sp, c->rb, c->coef, c->tkf, c->lam, c->mu, c->wtk, c->st))
         throw std::runtime_error("WT kernel build failed");
+      if (c->precision == 1) {   // the FP32 passes read the table in FP32
+        c->wtk32 = dalloc<float>(nc);
+        vec_to_f32(c->wtk32, c->wtk, nc, c->st);
+      }
       HIPCHK(hipStreamSynchronize(c->st));
     }
     c->desc = dalloc<ReqDesc>(1);
@@ -543,7 +549,7 @@ extern "C" void ofd_destroy(ofd_ctx* c) {
   if (c->st) hipStreamSynchronize(c->st);
   free_opt(c);
   for (void* b : {(void*)c->vion, (void*)c->rho, (void*)c->pot, c->work, (void*)c->sums, (void*)c->fin,
-                  (void*)c->wgck, (void*)c->wtk, (void*)c->desc, (void*)c->dot_part, (void*)c->dot_out,
+                  (void*)c->wgck, (void*)c->wtk, (void*)c->wtk32, (void*)c->desc, (void*)c->dot_part, (void*)c->dot_out,
                   (void*)c->many_desc, (void*)c->many_sums, (void*)c->d_frac, (void*)c->d_psp_vqs,
                   (void*)c->d_psp_dd})
     dev_free(b);

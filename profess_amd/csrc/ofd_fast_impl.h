@@ -206,6 +206,22 @@ __device__ __forceinline__ double field_of(const ReqDesc& D, int f, double rho) 
   return (KIND == kWT4 && f == 2) ? ra : rb;
 }
 
+// A field value in the pass's precision. The FP32 passes of the Wang-Teter
+// kinds form their powers with the hardware log2 / exp2 (relative error about
+// 1e-6, as the FP32 transforms); otherwise the FP64 value, rounded.
+template <int KIND, typename R>
+__device__ __forceinline__ R field_r(const ReqDesc& D, int f, double rho) {
+  if constexpr (sizeof(R) == 4 && (KIND == kWT3 || KIND == kWT4)) {
+    const float r = (float)rho;
+    if (f == 0) return r;
+    if (f == NF_<KIND>::v - 1) return __builtin_amdgcn_sqrtf(r);
+    const float p = (KIND == kWT4 && f == 2) ? (float)D.alpha : (float)D.beta;
+    return __builtin_amdgcn_exp2f(p * __builtin_amdgcn_logf(r));
+  } else {
+    return (R)field_of<KIND>(D, f, rho);
+  }
+}
+
 // Rows per workgroup of the x passes: RPB * NF lines of T threads,
 // whole wavefronts, at most 256 threads.
 __host__ __device__ constexpr int inv_rpb(int nf, int t) {
@@ -302,8 +318,8 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
         a = ta * ta;
         b = tb * tb;
       }
-      z[m].x = (R)field_of<KIND>(D, f, a);
-      z[m].y = (R)field_of<KIND>(D, f, b);
+      z[m].x = field_r<KIND, R>(D, f, a);
+      z[m].y = field_r<KIND, R>(D, f, b);
     }
   }
   C* mine = lds + (size_t)line * LL;
@@ -386,8 +402,8 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
         a = ta * ta;
         b = tb * tb;
       }
-      z[m].x = (R)field_of<KIND>(D, f, a);
-      z[m].y = (R)field_of<KIND>(D, f, b);
+      z[m].x = field_r<KIND, R>(D, f, a);
+      z[m].y = field_r<KIND, R>(D, f, b);
     }
   }
   C* myrow = P + (size_t)row * S;
@@ -498,6 +514,33 @@ __global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, G
 // One field per workgroup (blockIdx.y): for the Wang-Teter layouts the
 // G-space operator is diagonal in the fields (Hartree 4 pi/G^2 on F[rho],
 // K(|G|) on F[rho^b] (and F[rho^a]), G^2 on F[sqrt rho]).
+// The diagonal G-space multiplier of field f at (m1, m2, m3): Hartree
+// 4 pi / G^2 on F[rho], G^2 on F[sqrt rho], the Wang-Teter kernel on the
+// others, with the forward 1/N. FP32 passes form it in FP32 and read the
+// kernel table's FP32 copy when there is one.
+template <int KIND, typename R>
+__device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, int m3, size_t gi, double invn) {
+  constexpr int NF = NF_<KIND>::v;
+  if constexpr (sizeof(R) == 4) {
+    if (f != 0 && f != NF - 1) return D.wtk32 != nullptr ? glb(D.wtk32)[gi] : (float)glb(D.wtk)[gi];
+    const float x1 = (float)m1, x2 = (float)m2, x3 = (float)m3;
+    const float gx = (float)D.rb[0] * x1 + (float)D.rb[1] * x2 + (float)D.rb[2] * x3;
+    const float gy = (float)D.rb[3] * x1 + (float)D.rb[4] * x2 + (float)D.rb[5] * x3;
+    const float gz = (float)D.rb[6] * x1 + (float)D.rb[7] * x2 + (float)D.rb[8] * x3;
+    const float q2 = gx * gx + gy * gy + gz * gz;
+    if (f == 0) return q2 > 0.0f ? (float)(4.0 * kPi * invn) / q2 : 0.0f;
+    return q2 * (float)invn;
+  } else {
+    if (f != 0 && f != NF - 1) return glb(D.wtk)[gi];
+    const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
+    const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
+    const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
+    const double q2 = gx * gx + gy * gy + gz * gz;
+    if (f == 0) return q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
+    return q2 * invn;
+  }
+}
+
 template <int N, int KIND, typename R>
 __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
                                                 const typename V2<R>::T* __restrict__ tw, int kb) {
@@ -529,16 +572,9 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
     for (int m = 0; m < R1; ++m) {
       const int i3 = t + T * m;
       const int m3 = i3 > N / 2 ? i3 - N : i3;
-      const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
-      const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
-      const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
-      const double q2 = gx * gx + gy * gy + gz * gz;
-      double sc;
-      if (f == 0) sc = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
-      else if (f == NF - 1) sc = q2 * invn;
-      else sc = glb(D.wtk)[(size_t)i3 * es + col];
-      x[m].x = (R)((double)x[m].x * sc);
-      x[m].y = (R)((double)x[m].y * sc);
+      const R scr = gop_scale<KIND, R>(D, f, m1, m2, m3, (size_t)i3 * es + col, invn);
+      x[m].x *= scr;
+      x[m].y *= scr;
     }
   }
   fft2<N, +1>(x, lds + (size_t)c * LL, t, tw, active);
@@ -705,6 +741,52 @@ __device__ __forceinline__ double epi(const ReqDesc& D, double rho, double vion,
   return loc + D.mu * t / (2.0 * p.phi);
 }
 
+// The epilogue of the FP32 passes (Wang-Teter kinds): the pointwise terms in
+// FP32 with the hardware log2 / exp2 / sqrt (relative error about 1e-6, the
+// order of the FP32 transforms' own), the energy sums in FP64.
+template <int KIND>
+__device__ __forceinline__ double epi32(const ReqDesc& D, double rho_d, double vion, const double* o,
+                                        double (&acc)[kNSum]) {
+  constexpr float cC = 0.62035049089940009f, cXp = -0.98474502184269641f, cXe = -0.73855876638202234f;
+  constexpr float kLn2 = 0.69314718055994531f;
+  const float rho = (float)rho_d, al = (float)D.alpha, be = (float)D.beta;
+  const float l2 = __builtin_amdgcn_logf(rho);                     // log2 rho
+  const float r13 = __builtin_amdgcn_exp2f(l2 * (1.0f / 3.0f));
+  const float phi = __builtin_amdgcn_sqrtf(rho);
+  const float ra = __builtin_amdgcn_exp2f(al * l2), ram1 = __builtin_amdgcn_exp2f((al - 1.0f) * l2);
+  const float rb = KIND == kWT3 ? ra : __builtin_amdgcn_exp2f(be * l2);
+  const float rbm1 = KIND == kWT3 ? ram1 : __builtin_amdgcn_exp2f((be - 1.0f) * l2);
+  const float l2rs = __builtin_amdgcn_logf(cC) - l2 * (1.0f / 3.0f);  // log2 rs
+  const float rs = __builtin_amdgcn_exp2f(l2rs);
+  float vxc, ec;
+  if (rs < 1.0f) {
+    const float lg = l2rs * kLn2;
+    vxc = cXp * r13 + lg * (3.11e-2f + 1.33333333333333333e-3f * rs) - 5.83666666666666666e-2f - 8.4e-3f * rs;
+    ec = lg * (3.11e-2f + 2e-3f * rs) - 4.8e-2f - 1.16e-2f * rs;
+  } else {
+    const float sq = __builtin_amdgcn_sqrtf(rs);
+    const float iden = __builtin_amdgcn_rcpf(1.0f + 1.0529f * sq + 3.334e-1f * rs);
+    vxc = cXp * r13 + (-1.423e-1f - 0.174798948333333333f * sq - 6.325709333333333333e-2f * rs) * iden * iden;
+    ec = -1.423e-1f * iden;
+  }
+  const float r23 = r13 * r13;
+  const float vtf = (float)(5.0 / 3.0 * kCTF * D.lam) * r23;
+  const float o1 = (float)o[1];
+  const float o2 = KIND == kWT3 ? o1 : (float)o[2];
+  const float vnl = (float)kCTF * (al * ram1 * o1 + be * rbm1 * o2);
+  const float t = (float)o[KIND == kWT3 ? 2 : 3];
+  const float loc = (float)vion + (float)o[0] + vxc + vtf + vnl;
+  acc[0] += rho_d * vion;
+  acc[1] += o[0] * rho_d;
+  acc[2] += (double)(cXe * rho * r13 + rho * ec);
+  acc[3] += rho_d * (double)r23;
+  acc[4] += (double)phi * (double)t;
+  acc[5] += (double)ra * o[1];
+  const float mu = (float)D.mu;
+  if (D.opt_sqrt) return (double)(2.0f * phi * loc + mu * t);
+  return (double)(loc + mu * t / (2.0f * phi));
+}
+
 // Epilogue of the x-inverse passes over the workgroup's RPB rows, whose NF
 // inverse-transformed fields sit in LDS (line rr*NF + j of stride LL complex;
 // real point i of a row is real component i of its line). Consecutive
@@ -774,7 +856,9 @@ __device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict
         tphi = D.cb != 0.0 ? fma(D.cb, pv[q], D.ca * pu[q]) : D.ca * pu[q];
         rho = tphi * tphi;
       }
-      const double pot = epi<KIND>(D, rho, pion[q], o, acc);
+      double pot;
+      if constexpr (sizeof(R) == 4 && (KIND == kWT3 || KIND == kWT4)) pot = epi32<KIND>(D, rho, pion[q], o, acc);
+      else pot = epi<KIND>(D, rho, pion[q], o, acc);
       if (OUTMODE == kOutPot) {
         glb(D.out)[g] = pot;
       } else if (OUTMODE == kOutSigned) {
@@ -1047,17 +1131,10 @@ __global__ void __launch_bounds__(256) g_fz_gop(const ReqDesc* __restrict__ desc
       for (int k2 = 0; k2 < G::R2; ++k2) {
         const int i3 = t + G::T * q + G::R1 * k2;
         const int m3 = i3 > N / 2 ? i3 - N : i3;
-        const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
-        const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
-        const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
-        const double q2 = gx * gx + gy * gy + gz * gz;
-        double sc;
-        if (f == 0) sc = q2 > 0.0 ? 4.0 * kPi / q2 * invn : 0.0;
-        else if (f == NF - 1) sc = q2 * invn;
-        else sc = glb(D.wtk)[(size_t)i3 * es + col];
+        const R scr = gop_scale<KIND, R>(D, f, m1, m2, m3, (size_t)i3 * es + col, invn);
         C& v = xs[q * G::R2 + k2];
-        v.x = (R)((double)v.x * sc);
-        v.y = (R)((double)v.y * sc);
+        v.x *= scr;
+        v.y *= scr;
       }
     }
   }
@@ -1113,8 +1190,8 @@ __global__ void __launch_bounds__(256) g_fx_fwd(const ReqDesc* __restrict__ desc
           a = ta * ta;
           b = tb * tb;
         }
-        xc[j * G::R1 + m].x = (R)field_of<KIND>(D, f, a);
-        xc[j * G::R1 + m].y = (R)field_of<KIND>(D, f, b);
+        xc[j * G::R1 + m].x = field_r<KIND, R>(D, f, a);
+        xc[j * G::R1 + m].y = field_r<KIND, R>(D, f, b);
       }
     }
   }

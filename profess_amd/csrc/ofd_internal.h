@@ -85,6 +85,7 @@ struct ReqDesc {
   double ca2, cb2, s1b, s2b, mu0b;
   int lsda;        // exchange-correlation of the spin kinds: 1 Perdew-Zunger, 2 Perdew-Wang 92
   int newton_ch;   // kOutNewton with spin kinds: the perturbed channel (0 up, 1 down)
+  const float* wtk32;   // FP32 copy of wtk for the FP32 passes (may be null)
 };
 
 struct GridDims {
@@ -198,6 +199,8 @@ void vec_dots_batched(const DotDesc* d_dd, int ndots, long long n, double* d_par
 // coef * LindG(|G|/tkf; lam, mu) / N on the half grid (WT kinds)
 int build_wt_kernel(const ShapePlan& sp, const double* rb, double coef, double tkf, double lam, double mu,
                     double* d_out, hipStream_t st);
+// dst = (float)src, n values (device)
+void vec_to_f32(float* dst, const double* src, long long n, hipStream_t st);
 int build_wgc_kernels(const ShapePlan& sp, const double* rb, const double* d_eta, const double* d_w,
                       const double* d_w1, const double* d_w2, const double* d_pp0, const double* d_pp1,
                       const double* d_pp2, int nknots, double tkfs, double gamma, double rhoS, double* d_out,

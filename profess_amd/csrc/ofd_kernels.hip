@@ -710,6 +710,10 @@ __global__ void k_square(double* y, const double* x, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = x[i] * x[i];
 }
+__global__ void k_to_f32(float* y, const double* x, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
 __global__ void k_scale_sign(double* y, const double* x, const double* s, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = x[i] * fsign(s[i]);
@@ -819,6 +823,9 @@ void vec_fill(double* y, double v, long long n, hipStream_t st) {
 }
 void vec_square(double* y, const double* x, long long n, hipStream_t st) {
   hipLaunchKernelGGL(k_square, dim3(nblocks(n)), dim3(256), 0, st, y, x, n);
+}
+void vec_to_f32(float* y, const double* x, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n)), dim3(256), 0, st, y, x, n);
 }
 void vec_scale_sign(double* y, const double* x, const double* s, long long n, hipStream_t st) {
   hipLaunchKernelGGL(k_scale_sign, dim3(nblocks(n)), dim3(256), 0, st, y, x, s, n);

@@ -5,7 +5,10 @@
 ! and a full ProjNormRhoMinimization. Prints the energies and returns a
 ! nonzero status when they miss the reference's recorded values
 ! (displaced.out:92-99: -219.9483598873 eV at step 0, -231.1425327622 eV
-! converged; tolerance 1e-6 eV/atom on the converged value).
+! converged; tolerance 1e-6 eV/atom on the converged value). Then the same
+! cell with two spin channels (ofd_set_spin, fixed magnetization 0): at
+! rho_up = rho_down = rho/2 every term but exchange-correlation equals the
+! unpolarized one, and the spin solve reaches the unpolarized minimum.
 ! Usage: ofd_fortran_test <path to al_HC.lda.recpot>
 !------------------------------------------------------------------------------
 PROGRAM ofd_fortran_test
@@ -23,7 +26,10 @@ PROGRAM ofd_fortran_test
   INTEGER(C_INT), TARGET :: tstart(2), npts(1), poff(1)
   REAL(KIND=8), ALLOCATABLE, TARGET :: vals(:)
   REAL(KIND=8), ALLOCATABLE :: pot(:,:,:,:)
-  REAL(KIND=8) :: energy(9), e0, ef
+  REAL(KIND=8) :: energy(9), e0, ef, e_unpol(9), e_spin(9)
+  REAL(KIND=8), ALLOCATABLE :: rho_conv(:,:,:)
+  TYPE(ofd_spin_t) :: sp
+  INTEGER :: k
   TYPE(ofd_grid_t) :: g
   TYPE(ofd_functional_t) :: f
   INTEGER(C_INT) :: st
@@ -79,6 +85,30 @@ PROGRAM ofd_fortran_test
   IF (ABS(ef - (-2.311425327622d2)) / 4.d0 > 1.d-6) nerr = nerr + 1
   IF (rhoOutcome /= 0) nerr = nerr + 1
   IF (ABS(eii * h2ev - (-2.986846473280d2)) > 1.d-10 * 298.7d0) nerr = nerr + 1   ! displaced.out:119
+
+  ! numSpin = 2 at the converged density, split evenly
+  ALLOCATE(rho_conv(16,16,16))
+  rho_conv = rhoR(:,:,:,1)
+  CALL CalculatePotentialPlus(rhoR, .TRUE., pot, e_unpol)
+  sp%nspin = 2; sp%lsda = 1; sp%fixed_mag = 1; sp%nmag = -1; sp%tolm = -1.d0
+  st = ofd_set_spin(ofd_context, sp)
+  CALL ofd_check(ofd_context, st, 'ofd_set_spin')
+  DEALLOCATE(rhoR, pot)
+  ALLOCATE(rhoR(16,16,16,2), pot(16,16,16,2))
+  rhoR(:,:,:,1) = 0.5d0 * rho_conv
+  rhoR(:,:,:,2) = 0.5d0 * rho_conv
+  CALL CalculatePotentialPlus(rhoR, .TRUE., pot, e_spin)
+  WRITE(*, '(A,9ES20.12)') ' spin eTable (Ha)', e_spin
+  DO k = 2, 9        ! kinetic, ion-electron, Hartree, ion-ion, TF, vW, WT (Fortran slots)
+    IF (k == 5) CYCLE
+    IF (ABS(e_spin(k) - e_unpol(k)) > 1.d-12 * ABS(e_unpol(k)) + 1.d-14) nerr = nerr + 1
+  END DO
+  ! LSDAPotPZ's rounded exchange constant (-0.7385588) against LDA's
+  IF (ABS(e_spin(5) - e_unpol(5)) > 1.d-7 * ABS(e_unpol(5))) nerr = nerr + 1
+  CALL ProjNormRhoMinimization(energy, dir_type)
+  WRITE(*, '(A,F20.10,A,I3)') ' spin converged     ', energy(1) * h2ev, ' eV  outcome', rhoOutcome
+  IF (rhoOutcome /= 0) nerr = nerr + 1
+  IF (ABS(energy(1) * h2ev - ef) / 4.d0 > 1.d-5) nerr = nerr + 1
   CALL ofd_destroy(ofd_context)
   IF (nerr /= 0) THEN
     WRITE(*, '(A,I2,A)') ' FAILED (', nerr, ' checks)'

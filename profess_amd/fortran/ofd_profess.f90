@@ -35,6 +35,13 @@ MODULE ofd_c
     INTEGER(C_INT) :: cg_alg
   END TYPE ofd_opt_params_t
 
+  ! numSpin = 2 controls (CellInfo numSpin, XC_LDA lsda, RhoOptimizers
+  ! fixedmag/nmag/tolm); see ofd_set_spin in include/ofd.h
+  TYPE, BIND(C) :: ofd_spin_t
+    INTEGER(C_INT) :: nspin, lsda, fixed_mag, nmag
+    REAL(C_DOUBLE) :: tolm
+  END TYPE ofd_spin_t
+
   TYPE, BIND(C) :: ofd_opt_stats_t
     INTEGER(C_INT) :: iterations, num_evals, newton_iters_total, fp32_evals
     REAL(C_DOUBLE) :: final_pot_norm, seconds
@@ -110,6 +117,19 @@ MODULE ofd_c
       INTEGER(C_INT), VALUE :: max_energies
       INTEGER(C_INT) :: ofd_ion_relax
     END FUNCTION
+    FUNCTION ofd_set_spin(ctx, spin) BIND(C, name="ofd_set_spin")
+      IMPORT :: C_PTR, C_INT, ofd_spin_t
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ofd_spin_t), INTENT(IN) :: spin
+      INTEGER(C_INT) :: ofd_set_spin
+    END FUNCTION
+    FUNCTION ofd_get_magnetization(ctx, out3, mag_iters) BIND(C, name="ofd_get_magnetization")
+      IMPORT :: C_PTR, C_INT, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: out3(3)
+      INTEGER(C_INT), INTENT(OUT) :: mag_iters
+      INTEGER(C_INT) :: ofd_get_magnetization
+    END FUNCTION
     FUNCTION c_strlen(s) BIND(C, name="strlen")
       IMPORT :: C_PTR, C_SIZE_T
       TYPE(C_PTR), VALUE :: s
@@ -143,7 +163,7 @@ MODULE OFD_Sys
   IMPLICIT NONE
   TYPE(C_PTR), SAVE :: ofd_context = C_NULL_PTR
   INTEGER, SAVE :: n1G = 0, n2G = 0, n3G = 0
-  ! SYS::rhoR (System.f90:79) with numSpin = 1
+  ! SYS::rhoR (System.f90:79), rhoR(n1,n2,n3,numSpin)
   REAL(KIND=8), ALLOCATABLE, TARGET, SAVE :: rhoR(:,:,:,:)
   ! RhoOptimizers module variables (RhoOptimizers.f90:44-101)
   INTEGER, SAVE :: maxIter = 500
@@ -170,11 +190,13 @@ CONTAINS
     LOGICAL, INTENT(IN) :: optSqrt
     REAL(KIND=8), DIMENSION(:,:,:,:), INTENT(OUT) :: potential
     REAL(KIND=8), DIMENSION(:), OPTIONAL, INTENT(OUT) :: eTable
-    REAL(KIND=8), ALLOCATABLE, TARGET :: r(:,:,:), p(:,:,:)
+    REAL(KIND=8), ALLOCATABLE, TARGET :: r(:,:,:,:), p(:,:,:,:)
     REAL(KIND=8), TARGET :: et(9)
     INTEGER(C_INT) :: st, os
-    ALLOCATE(r(SIZE(rho,1), SIZE(rho,2), SIZE(rho,3)), p(SIZE(rho,1), SIZE(rho,2), SIZE(rho,3)))
-    r = rho(:,:,:,1)     ! contiguous copy: callers pass expressions such as phi**2
+    ! every spin channel (numSpin = SIZE(rho, 4), as set by ofd_set_spin)
+    ALLOCATE(r(SIZE(rho,1), SIZE(rho,2), SIZE(rho,3), SIZE(rho,4)), &
+             p(SIZE(rho,1), SIZE(rho,2), SIZE(rho,3), SIZE(rho,4)))
+    r = rho              ! contiguous copy: callers pass expressions such as phi**2
     os = 0
     IF (optSqrt) os = 1
     IF (PRESENT(eTable)) THEN
@@ -185,7 +207,7 @@ CONTAINS
       st = ofd_calpot(ofd_context, C_LOC(r), os, C_LOC(p), C_NULL_PTR)
       CALL ofd_check(ofd_context, st, 'CalculatePotentialPlus')
     END IF
-    potential(:,:,:,1) = p
+    potential = p
     DEALLOCATE(r, p)
   END SUBROUTINE CalculatePotentialPlus
 END MODULE CalPotPlus
@@ -203,13 +225,13 @@ CONTAINS
   SUBROUTINE ProjNormRhoMinimization(energy, dir_type)
     REAL(KIND=8), INTENT(OUT) :: energy(:)
     INTEGER, INTENT(INOUT) :: dir_type
-    REAL(KIND=8), ALLOCATABLE, TARGET :: r(:,:,:)
+    REAL(KIND=8), ALLOCATABLE, TARGET :: r(:,:,:,:)
     REAL(KIND=8), TARGET :: et(9)
     TYPE(ofd_opt_params_t) :: prm
     TYPE(ofd_opt_stats_t) :: stt
     INTEGER(C_INT) :: st, oc
-    ALLOCATE(r(SIZE(rhoR,1), SIZE(rhoR,2), SIZE(rhoR,3)))
-    r = rhoR(:,:,:,1)
+    ALLOCATE(r(SIZE(rhoR,1), SIZE(rhoR,2), SIZE(rhoR,3), SIZE(rhoR,4)))
+    r = rhoR
     prm%max_iter = maxIter
     prm%pot_tol = pot_tol
     prm%tole = tole
@@ -223,7 +245,7 @@ CONTAINS
     IF (cg_alg(1:2) == 'PR') prm%cg_alg = 1
     st = ofd_rho_opt(ofd_context, C_LOC(r), INT(dir_type, C_INT), prm, C_LOC(et), oc, stt)
     CALL ofd_check(ofd_context, st, 'ProjNormRhoMinimization')
-    rhoR(:,:,:,1) = r
+    rhoR = r
     energy(1:9) = et
     rhoOutcome = oc
     numEnergyTotal = stt%num_evals
