@@ -160,12 +160,13 @@ __device__ __forceinline__ void fields(const ReqDesc& D, double rho, double* f) 
 }
 
 // Fixed-order workgroup sum: wave butterflies, then the waves in order.
-// out[k * stride] = sum over the workgroup of acc[k].
-template <int N>
+// out[k * stride] = sum over the workgroup of acc[k] for the first NU slots,
+// 0 for the rest (slots the epilogue form never adds to).
+template <int N, int NU = N>
 __device__ __forceinline__ void block_sum_strided(double (&acc)[N], gptr<double> out, int stride) {
-  __shared__ double red[16][N];
+  __shared__ double red[16][NU];
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
+  for (int k = 0; k < NU; ++k) {
     double v = acc[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -175,13 +176,15 @@ __device__ __forceinline__ void block_sum_strided(double (&acc)[N], gptr<double>
   __syncthreads();
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) red[wv][k] = acc[k];
+    for (int k = 0; k < NU; ++k) red[wv][k] = acc[k];
   }
   __syncthreads();
   if (threadIdx.x < N) {
-    const int nw = (blockDim.x + 63) >> 6;
     double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += red[w][threadIdx.x];
+    if (threadIdx.x < NU) {
+      const int nw = (blockDim.x + 63) >> 6;
+      for (int w = 0; w < nw; ++w) s += red[w][threadIdx.x];
+    }
     out[(size_t)threadIdx.x * stride] = s;
   }
 }
@@ -886,7 +889,10 @@ __device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict
   // partials laid out [slot][workgroup]; the finalize kernel sums each slot
   // in a fixed order (no device-scope fence or atomic here: on a multi-XCD
   // part a release fence per workgroup writes back the whole L2)
-  block_sum_strided<kNSum>(acc, glb(D.sums) + blk, gridDim.x);
+  // the energy sums (slots 0-5), plus the optimizer sums of the signed form
+  // (6-10) or of the Newton form (6-8)
+  constexpr int NU = OUTMODE == kOutPot ? 6 : (OUTMODE == kOutNewton ? 9 : kNSum);
+  block_sum_strided<kNSum, NU>(acc, glb(D.sums) + blk, gridDim.x);
 }
 
 // One workgroup = RPB rows x NF fields; thread (line = row*NF + field, lane t).
