@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "libprofess_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["ofd_kernels.hip", "ofd_fast.hip", "ofd_fast_d3.hip", "ofd_fast_d4.hip", "ofd_fast_f3.hip",
-           "ofd_fast_f4.hip", "ofd_fast_d8.hip", "ofd_fast_f8.hip", "ofd_tables.hip", "ofd_plan.cpp",
+           "ofd_fast_f4.hip", "ofd_fast_d8.hip", "ofd_fast_f8.hip",
+           "ofd_fast_d5.hip", "ofd_fast_f5.hip", "ofd_fast_d7.hip", "ofd_fast_f7.hip", "ofd_tables.hip", "ofd_plan.cpp",
            "ofd_api.cpp", "ofd_io.cpp"]
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-pthread", "-Wno-unused-result", "-Wno-unused-value"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

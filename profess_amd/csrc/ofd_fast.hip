@@ -1,7 +1,7 @@
 // Dispatch of the register-resident fast path (two-stage line lengths,
 // Wang-Teter and WGC functionals). The kernels live in ofd_fast_impl.h; each
 // (precision, functional) pair is instantiated in its own translation unit
-// (ofd_fast_{d,f}{3,4,8}.hip).
+// (ofd_fast_{d,f}{3,4,5,7,8}.hip: the number is the kind's field count).
 #include "ofd_fast_impl.h"
 
 namespace ofd {
@@ -13,6 +13,10 @@ extern template int fast_eval_k<float, kWT3>(const ShapePlan&, int, int, const R
 extern template int fast_eval_k<float, kWT4>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
 extern template int fast_eval_k<double, kWGC8>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
 extern template int fast_eval_k<float, kWGC8>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+extern template int fast_eval_k<double, kWT3S>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+extern template int fast_eval_k<float, kWT3S>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+extern template int fast_eval_k<double, kWT4S>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+extern template int fast_eval_k<float, kWT4S>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
 
 namespace fast {
 // One workgroup per (slot, request): fin[slot] = sum over the workgroups'
@@ -47,6 +51,12 @@ int fast_eval(const ShapePlan& sp, int kind, int inmode, int outmode, int fp32, 
   if (kind == kWGC8)
     return fp32 ? fast_eval_k<float, kWGC8>(sp, inmode, outmode, dd, nb, st)
                 : fast_eval_k<double, kWGC8>(sp, inmode, outmode, dd, nb, st);
+  if (kind == kWT3S)
+    return fp32 ? fast_eval_k<float, kWT3S>(sp, inmode, outmode, dd, nb, st)
+                : fast_eval_k<double, kWT3S>(sp, inmode, outmode, dd, nb, st);
+  if (kind == kWT4S)
+    return fp32 ? fast_eval_k<float, kWT4S>(sp, inmode, outmode, dd, nb, st)
+                : fast_eval_k<double, kWT4S>(sp, inmode, outmode, dd, nb, st);
   return fp32 ? fast_eval_k<float, kWT4>(sp, inmode, outmode, dd, nb, st)
               : fast_eval_k<double, kWT4>(sp, inmode, outmode, dd, nb, st);
 }

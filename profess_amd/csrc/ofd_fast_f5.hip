@@ -1,0 +1,6 @@
+// Instantiation of the fast evaluation path for (float, kWT3S).
+#include "ofd_fast_impl.h"
+
+namespace ofd {
+template int fast_eval_k<float, kWT3S>(const ShapePlan&, int, int, const ReqDesc*, int, hipStream_t);
+}  // namespace ofd

@@ -144,6 +144,8 @@ template <int KIND> struct NF_;
 template <> struct NF_<kWT3> { static constexpr int v = 3; };
 template <> struct NF_<kWT4> { static constexpr int v = 4; };
 template <> struct NF_<kWGC8> { static constexpr int v = 8; };
+template <> struct NF_<kWT3S> { static constexpr int v = 5; };
+template <> struct NF_<kWT4S> { static constexpr int v = 7; };
 
 template <int KIND>
 __device__ __forceinline__ void fields(const ReqDesc& D, double rho, double* f) {
@@ -225,6 +227,61 @@ __device__ __forceinline__ R field_r(const ReqDesc& D, int f, double rho) {
   }
 }
 
+// rho of one spin channel at reals g and g + 1: the density itself (kInRho)
+// or phi_t^2 with phi_t = ca*u + cb*v
+template <int INMODE>
+__device__ __forceinline__ void rho_pair(const double* u, const double* v, double ca, double cb, size_t g, double& a,
+                                         double& b) {
+  const double2 uu = ldc(glb(reinterpret_cast<const double2*>(u + g)));
+  if (INMODE == kInRho) {
+    a = uu.x;
+    b = uu.y;
+    return;
+  }
+  double ta = ca * uu.x, tb = ca * uu.y;
+  if (cb != 0.0) {
+    const double2 vv = ldc(glb(reinterpret_cast<const double2*>(v + g)));
+    ta = fma(cb, vv.x, ta);
+    tb = fma(cb, vv.y, tb);
+  }
+  a = ta * ta;
+  b = tb * tb;
+}
+
+// Field f of the spin kinds from both channels' densities (x = 2 rho_s):
+// 0 rho_up + rho_dn; then, odd fields spin up and even fields spin down,
+// x^beta (kWT3S, kWT4S), x^alpha (kWT4S), sqrt(x) (CalPotential.f90:120-160,
+// 400-407; KEDF_VW.f90:100-140)
+template <int KIND>
+__device__ __forceinline__ double spin_field(const ReqDesc& D, int f, double r1, double r2) {
+  constexpr int NF = NF_<KIND>::v;
+  if (f == 0) return r1 + r2;
+  const double x = 2.0 * ((f & 1) ? r1 : r2);
+  if (f >= NF - 2) return sqrt(x);
+  double ra, rb;
+  fwd_pows(x, D.alpha, D.beta, ra, rb);
+  return (KIND == kWT4S && f >= 3) ? ra : rb;
+}
+
+// The x-forward passes' complex input element at reals g, g + 1 of field f
+template <int KIND, int INMODE, typename R>
+__device__ __forceinline__ typename V2<R>::T field_pair(const ReqDesc& D, int f, size_t g) {
+  typename V2<R>::T z;
+  double a, b;
+  if constexpr (is_spin_kind(KIND)) {
+    double a1 = 0.0, b1 = 0.0, a2 = 0.0, b2 = 0.0;
+    if (f == 0 || (f & 1)) rho_pair<INMODE>(D.u, D.v, D.ca, D.cb, g, a1, b1);
+    if (f == 0 || !(f & 1)) rho_pair<INMODE>(D.u2, D.v2, D.ca2, D.cb2, g, a2, b2);
+    z.x = (R)spin_field<KIND>(D, f, a1, a2);
+    z.y = (R)spin_field<KIND>(D, f, b1, b2);
+  } else {
+    rho_pair<INMODE>(D.u, D.v, D.ca, D.cb, g, a, b);
+    z.x = field_r<KIND, R>(D, f, a);
+    z.y = field_r<KIND, R>(D, f, b);
+  }
+  return z;
+}
+
 // Rows per workgroup of the x passes: RPB * NF lines of T threads,
 // whole wavefronts, at most 256 threads.
 __host__ __device__ constexpr int inv_rpb(int nf, int t) {
@@ -304,26 +361,7 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
   } else if (active) {
     const size_t base = (size_t)row * (2 * M);
 #pragma unroll
-    for (int m = 0; m < R1; ++m) {
-      const int k = t + T * m;
-      const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + base + 2 * k)));
-      double a, b;
-      if (INMODE == kInRho) {
-        a = uu.x;
-        b = uu.y;
-      } else {
-        double ta = D.ca * uu.x, tb = D.ca * uu.y;
-        if (D.cb != 0.0) {
-          const double2 vv = ldc(glb(reinterpret_cast<const double2*>(D.v + base + 2 * k)));
-          ta = fma(D.cb, vv.x, ta);
-          tb = fma(D.cb, vv.y, tb);
-        }
-        a = ta * ta;
-        b = tb * tb;
-      }
-      z[m].x = field_r<KIND, R>(D, f, a);
-      z[m].y = field_r<KIND, R>(D, f, b);
-    }
+    for (int m = 0; m < R1; ++m) z[m] = field_pair<KIND, INMODE, R>(D, f, base + 2 * (t + T * m));
   }
   C* mine = lds + (size_t)line * LL;
   fft2<M, -1>(z, mine, t, tw, active);
@@ -388,26 +426,7 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
   if (ax) {
     const size_t base = ((size_t)i3 * N2 + row) * (2 * M);
 #pragma unroll
-    for (int m = 0; m < R1X; ++m) {
-      const int k = t_plus(tx, TX, m);
-      const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + base + 2 * k)));
-      double a, b;
-      if (INMODE == kInRho) {
-        a = uu.x;
-        b = uu.y;
-      } else {
-        double ta = D.ca * uu.x, tb = D.ca * uu.y;
-        if (D.cb != 0.0) {
-          const double2 vv = ldc(glb(reinterpret_cast<const double2*>(D.v + base + 2 * k)));
-          ta = fma(D.cb, vv.x, ta);
-          tb = fma(D.cb, vv.y, tb);
-        }
-        a = ta * ta;
-        b = tb * tb;
-      }
-      z[m].x = field_r<KIND, R>(D, f, a);
-      z[m].y = field_r<KIND, R>(D, f, b);
-    }
+    for (int m = 0; m < R1X; ++m) z[m] = field_pair<KIND, INMODE, R>(D, f, base + 2 * t_plus(tx, TX, m));
   }
   C* myrow = P + (size_t)row * S;
   fft2<M, -1>(z, myrow, tx, twx, ax);
@@ -524,8 +543,9 @@ __global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, G
 template <int KIND, typename R>
 __device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, int m3, size_t gi, double invn) {
   constexpr int NF = NF_<KIND>::v;
+  constexpr int FVW = is_spin_kind(KIND) ? NF - 2 : NF - 1;   // first vW field (one per spin channel)
   if constexpr (sizeof(R) == 4) {
-    if (f != 0 && f != NF - 1) return D.wtk32 != nullptr ? glb(D.wtk32)[gi] : (float)glb(D.wtk)[gi];
+    if (f != 0 && f < FVW) return D.wtk32 != nullptr ? glb(D.wtk32)[gi] : (float)glb(D.wtk)[gi];
     const float x1 = (float)m1, x2 = (float)m2, x3 = (float)m3;
     const float gx = (float)D.rb[0] * x1 + (float)D.rb[1] * x2 + (float)D.rb[2] * x3;
     const float gy = (float)D.rb[3] * x1 + (float)D.rb[4] * x2 + (float)D.rb[5] * x3;
@@ -534,7 +554,7 @@ __device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, 
     if (f == 0) return q2 > 0.0f ? (float)(4.0 * kPi * invn) / q2 : 0.0f;
     return q2 * (float)invn;
   } else {
-    if (f != 0 && f != NF - 1) return glb(D.wtk)[gi];
+    if (f != 0 && f < FVW) return glb(D.wtk)[gi];
     const double gx = D.rb[0] * m1 + D.rb[1] * m2 + D.rb[2] * m3;
     const double gy = D.rb[3] * m1 + D.rb[4] * m2 + D.rb[5] * m3;
     const double gz = D.rb[6] * m1 + D.rb[7] * m2 + D.rb[8] * m3;
@@ -851,6 +871,46 @@ __device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict
       double o[NF];
 #pragma unroll
       for (int j = 0; j < NF; ++j) o[j] = (double)lr[2 * (size_t)(rr * NF + j) * LL + i];
+      if constexpr (is_spin_kind(KIND)) {
+        // both channels: the second one's inputs are loaded here
+        double t1 = 0.0, t2 = 0.0, r1, r2;
+        if (INMODE == kInRho) {
+          r1 = pu[q];
+          r2 = glb(D.u2)[g];
+        } else {
+          t1 = D.cb != 0.0 ? fma(D.cb, pv[q], D.ca * pu[q]) : D.ca * pu[q];
+          const double u2 = glb(D.u2)[g];
+          t2 = D.cb2 != 0.0 ? fma(D.cb2, (double)glb(D.v2)[g], D.ca2 * u2) : D.ca2 * u2;
+          r1 = t1 * t1;
+          r2 = t2 * t2;
+        }
+        double p1, p2;
+        epi_point_spin<KIND>(D, r1, r2, pion[q], o, acc, p1, p2);
+        if (OUTMODE == kOutPot) {
+          glb(D.out)[g] = p1;
+          glb(D.out2)[g] = p2;
+        } else if (OUTMODE == kOutSigned) {
+          const double de1 = p1 * fsign(t1), de2 = p2 * fsign(t2);
+          glb(D.out)[g] = de1;
+          glb(D.out2)[g] = de2;
+          const double dir1 = D.s1 * pu[q] + (D.v != nullptr ? D.s2 * pv[q] : 0.0);
+          const double dir2 = D.s1b * glb(D.u2)[g] + (D.v2 != nullptr ? D.s2b * glb(D.v2)[g] : 0.0);
+          const double dl1 = de1 - 2.0 * t1 * D.mu0, dl2 = de2 - 2.0 * t2 * D.mu0b;
+          acc[6] += de1 * dir1;
+          acc[7] += dl1 * dl1 + dl2 * dl2;
+          acc[8] += de1 * t1;
+          acc[9] += de2 * dir2;
+          acc[10] += de2 * t2;
+        } else {
+          const bool up = D.newton_ch == 0;
+          const double tt = up ? t1 : t2;
+          const double de = (up ? p1 : p2) * fsign(tt);
+          const double ap = ((de - D.mu0 * 2.0 * tt) - pb[q]) / D.eps;
+          glb(D.out)[g] = ap;
+          acc[6] += (up ? pv[q] : (double)glb(D.v2)[g]) * ap;
+        }
+        continue;
+      }
       double tphi, rho;
       if (INMODE == kInRho) {
         tphi = 0.0;
@@ -1179,26 +1239,8 @@ __global__ void __launch_bounds__(256) g_fx_fwd(const ReqDesc* __restrict__ desc
     for (int j = 0; j < G::J1; ++j) {
       if (!G::col_ok(t, j)) continue;
 #pragma unroll
-      for (int m = 0; m < G::R1; ++m) {
-        const int k = t + T * j + G::R2 * m;
-        const double2 uu = ldc(glb(reinterpret_cast<const double2*>(D.u + base + 2 * k)));
-        double a, b;
-        if (INMODE == kInRho) {
-          a = uu.x;
-          b = uu.y;
-        } else {
-          double ta = D.ca * uu.x, tb = D.ca * uu.y;
-          if (D.cb != 0.0) {
-            const double2 vv = ldc(glb(reinterpret_cast<const double2*>(D.v + base + 2 * k)));
-            ta = fma(D.cb, vv.x, ta);
-            tb = fma(D.cb, vv.y, tb);
-          }
-          a = ta * ta;
-          b = tb * tb;
-        }
-        xc[j * G::R1 + m].x = field_r<KIND, R>(D, f, a);
-        xc[j * G::R1 + m].y = field_r<KIND, R>(D, f, b);
-      }
+      for (int m = 0; m < G::R1; ++m)
+        xc[j * G::R1 + m] = field_pair<KIND, INMODE, R>(D, f, base + 2 * (t + T * j + G::R2 * m));
     }
   }
   C* mine = lds + (size_t)line * G::LINE;
@@ -1305,7 +1347,8 @@ static inline bool is_gp_len(int n) { return n == 60 || n == 120; }
 inline bool fast_path_ok_impl(const ShapePlan& sp, int kind) {
   const bool xy = (is_fast_half(sp.d.M) || is_gp_len(sp.d.M)) && (is_fast_len(sp.d.n2) || is_gp_len(sp.d.n2));
   if (kind == kWGC8) return xy && is_fast_len(sp.d.n3);   // the coupled z pass has the two-stage form only
-  return (kind == kWT3 || kind == kWT4) && xy && (is_fast_len(sp.d.n3) || is_gp_len(sp.d.n3));
+  return (kind == kWT3 || kind == kWT4 || kind == kWT3S || kind == kWT4S) && xy &&
+         (is_fast_len(sp.d.n3) || is_gp_len(sp.d.n3));
 }
 
 struct FastTiles {

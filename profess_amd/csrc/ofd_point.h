@@ -2,6 +2,8 @@
 // fast kernels:
 //   LDA Perdew-Zunger, numSpin = 1: potential XC_LDA.f90:199-310, energy
 //   density XC_LDA.f90:51-196 (the constants below are the formula's);
+//   LSDA for numSpin = 2 (Perdew-Zunger and Perdew-Wang 92) and the
+//   two-channel epilogue of the spin kinds;
 //   LindG, MathFunctions.f90:440-528 (including its 15 Taylor terms for
 //   eta > 3.65); the rho powers of the Wang-Teter terms.
 // rho^(1/3), rho^(-1/3), rs and sqrt(rs) are computed once per point, all
@@ -9,6 +11,8 @@
 // exchange-correlation terms.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "ofd_internal.h"
 
 namespace ofd {
 
@@ -166,6 +170,57 @@ __device__ __forceinline__ void lsda_point(double r1, double r2, int lsda, doubl
   v1 = 0.5 * (2.0 * vx1 + vc1);
   v2 = 0.5 * (2.0 * vx2 + vc2);
   e = 0.5 * (2.0 * (r1 * ex1 + r2 * ex2) + ec * d);
+}
+
+// Spin kinds: both channels' potentials at one point and the energy sums.
+// Per channel s (x_s = 2 rho_s): ion + Hartree of the total, the LSDA
+// potential, TF and Wang-Teter on x_s (WTPotentialPlus(2 rho_s)), the chain
+// rule 2 sqrt(rho_s) under optSqrt, and vW on sqrt(x_s) divided by sqrt(2)
+// (CalVW, KEDF_VW.f90:100-140); the kinetic energies are halved sums over the
+// channels (KEDF_TF.f90:66-74, CalPotential.f90:400-407).
+template <int KIND, int NS>
+__device__ __forceinline__ void epi_point_spin(const ReqDesc& D, double r1, double r2, double vion, const double* o,
+                                               double (&acc)[NS], double& p1, double& p2) {
+  constexpr double kCTF = 2.87123400018819;   // KEDF_TF.f90:42
+  constexpr double kInvSqrt2 = 0.70710678118654752440;
+  double vx[2], exc;
+  lsda_point(r1, r2, D.lsda, vx[0], vx[1], exc);
+  const double common = vion + o[0];
+  const double rs[2] = {r1, r2};
+  double pot[2], etf = 0.0, evw = 0.0, enl = 0.0;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double x = 2.0 * rs[s];
+    PtVals p;
+    point_vals(x, D.alpha, D.beta, p);
+    const double r23 = p.r13 * p.r13;
+    const double vtf = (5.0 / 3.0 * kCTF * D.lam) * r23;
+    double vnl, t;
+    if (KIND == kWT3S) {
+      vnl = kCTF * (D.alpha * p.ram1 * o[1 + s] + D.beta * p.rbm1 * o[1 + s]);
+      enl += p.ra * o[1 + s];
+      t = o[3 + s];
+    } else {
+      vnl = kCTF * (D.alpha * p.ram1 * o[1 + s] + D.beta * p.rbm1 * o[3 + s]);
+      enl += p.ra * o[1 + s];
+      t = o[5 + s];
+    }
+    const double loc = common + vx[s] + vtf + vnl;
+    const double tw = D.mu * t * kInvSqrt2;
+    const double sq = p.phi * kInvSqrt2;   // sqrt(rho_s)
+    pot[s] = D.opt_sqrt ? 2.0 * sq * loc + tw : loc + tw / (2.0 * sq);
+    etf += x * r23;
+    evw += p.phi * t;
+  }
+  const double rho = r1 + r2;
+  acc[0] += rho * vion;
+  acc[1] += o[0] * rho;
+  acc[2] += exc;
+  acc[3] += 0.5 * etf;
+  acc[4] += 0.5 * evw;
+  acc[5] += 0.5 * enl;
+  p1 = pot[0];
+  p2 = pot[1];
 }
 
 // Powers of a density with a general exponent are taken as exp2(a log2 rho)

@@ -156,3 +156,60 @@ def test_spin_forces_use_the_total_density(pa):
     np.testing.assert_allclose(f2, f1, rtol=1e-12, atol=1e-14)
     s1.close()
     s2.close()
+
+
+@pytest.mark.parametrize("grid,alpha,beta,lsda", [((64, 64, 64), -100.0, -100.0, 1), ((48, 40, 60), 0.9, 0.7, 2),
+                                                  ((120, 64, 48), 0.9, 0.7, 1), ((32, 96, 32), -100.0, -100.0, 2)])
+def test_fast_spin_matches_generic_path(pa, monkeypatch, grid, alpha, beta, lsda):
+    """The spin kinds on the register FFT passes and on the generic LDS passes
+    agree to rounding in FP64 (and the FP32 passes within FP32 rounding); the
+    FP64 result also meets the oracle at the FP64 bar."""
+    _say("spin fast/generic grid %s alpha %g beta %g lsda %d" % (grid, alpha, beta, lsda))
+    cell, vion, ne, f, rho2 = _spin_case(grid, alpha, beta, seed=4)
+    res = {}
+    for prec in (0, 1):
+        for flag in ("0", "1"):
+            monkeypatch.setenv("OFD_GENERIC", flag)
+            s = pa.OFDSystem(cell.lattice, grid, ne, kinetic=4, alpha=alpha, beta=beta, precision=prec)
+            s.set_ion_terms(vion, -1.25)
+            s.set_spin(2, lsda=lsda)
+            res[prec, flag] = pa.CalculatePotentialPlus(s, rho2, eTable=True)
+            s.close()
+    (p0, e0), (p1, e1) = res[0, "0"], res[0, "1"]
+    for k in range(2):
+        assert np.max(np.abs(p0[k] - p1[k])) <= 1e-13 * np.max(np.abs(p1[k]))
+    _check_terms(e0, e1, rtol=1e-13)
+    for flag in ("0", "1"):
+        p32, e32 = res[1, flag]
+        for k in range(2):
+            assert np.max(np.abs(p32[k] - p1[k])) <= 1e-5 * np.max(np.abs(p1[k]))
+        _check_terms(e32, e1, rtol=1e-5, floor=1e-8)
+    sysm = O.System(cell, f, vion, e_ionion=-1.25)
+    pr, er = O.calculate_potential_plus_spin(rho2, sysm, True, True, lsda=lsda)
+    _check_terms(e0, er)
+    for k in range(2):
+        assert np.max(np.abs(p0[k] - pr[k])) <= POT_RTOL * np.max(np.abs(pr[k]))
+
+
+def test_fast_spin_solve_matches_generic_path(pa, monkeypatch):
+    """fcc Al4 on 32^3 at a fixed magnetization of 0.4, solved with the spin
+    evaluations (trial points, signed and Newton forms) on the register FFT
+    passes and on the generic passes. The two differ in the last bits of each
+    evaluation, and the stop at a potential norm of 1e-6 can fall one
+    iteration apart: the iteration counts agree within one, the energies
+    within 1e-8 eV/atom and the densities within 1e-4 of max rho (a potential
+    norm of 1e-6 fixes the density to about 1e-5 of its maximum)."""
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("OFD_GENERIC", flag)
+        s, sysm, ne = _al4(pa, 32, 2, True)
+        _say("spin NTN fixed m = 0.4 on 32^3, OFD_GENERIC=%s" % flag)
+        rho0 = np.full(s.shape, ne / s.vol)
+        out[flag] = pa.ProjNormRhoMinimization(s, s.spin_split(rho0, 0.4), conv_check="POT", pot_tol=1e-6)
+        s.close()
+    (rf, ef, of, sf), (rg, eg, og, sg) = out["0"], out["1"]
+    _say("  iterations fast %d generic %d, E %.14f %.14f" % (sf["iterations"], sg["iterations"], ef[0], eg[0]))
+    assert of == 0 and og == 0
+    assert abs(sf["iterations"] - sg["iterations"]) <= 1
+    assert abs(ef[0] - eg[0]) * O.HARTREE_TO_EV / 4 < 1e-8
+    assert np.max(np.abs(rf - rg)) <= 1e-4 * np.max(rg)
