@@ -89,6 +89,47 @@ __device__ __forceinline__ void fft2(C (&x)[P2<N>::R1], C* __restrict__ lds, int
   __syncthreads();
 }
 
+// fft2 of one column of a pair that a radix-2 step joins first: on exit x
+// holds the FFT of E + sw O, where E and O are the pair's two lines (this
+// line's own input is one of them, written to `lds` by the first stage) and
+// sw = +w or -w. The join is linear and the first stage acts on each line
+// alone, so it is applied as the stage-two inputs are read from the two LDS
+// lines (no extra exchange or barrier).
+template <int N, int S, typename C>
+__device__ __forceinline__ void fft2_pair(C (&x)[P2<N>::R1], C* __restrict__ lds, const C* le, const C* lo, int t,
+                                          const C* __restrict__ tw, bool active, C sw) {
+  constexpr int R1 = P2<N>::R1, R2 = P2<N>::R2, T = R2, Q = R1 / R2;
+  if (active) {
+    Bfly<R1, S, C>::run(x);
+#pragma unroll
+    for (int m = 0; m < R1; ++m) lds[padded<N>(t * R1 + m)] = x[m];
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b2 = t + T * q;
+      C y[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        const C e = le[padded<N>(b2 + m * R1)], o = cmul(lo[padded<N>(b2 + m * R1)], sw);
+        y[m].x = e.x + o.x;
+        y[m].y = e.y + o.y;
+      }
+#pragma unroll
+      for (int m = 1; m < R2; ++m) {
+        C tw1 = tw[m * b2];
+        if (S > 0) tw1.y = -tw1.y;
+        y[m] = cmul(y[m], tw1);
+      }
+      Bfly<R2, S, C>::run(y);
+#pragma unroll
+      for (int m = 0; m < R2; ++m) x[q + Q * m] = y[m];
+    }
+  }
+  __syncthreads();
+}
+
 // ------------------------------------------------------------ pointwise
 // fft2 with the lane exchange done one real component at a time through a
 // scalar LDS line (half the LDS of fft2, two more barriers). Used by the
@@ -396,35 +437,44 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
 // spectrum (K1 x n2) of the field stays in LDS between the x R2C and the y
 // FFT: rows of stride S >= max(M + M/R1x, M + 1) complex, n2 + n2/R1y rows so
 // that the y transform can use the padded column layout as its scratch.
-template <int M, int N2> struct PlaneGeo {
-  static constexpr int LLX = M + M / P2<M>::R1;
+//
+// Split form (SPLIT = 1, N2 = n2/2): one workgroup per (z-plane, row parity,
+// field) takes the even or the odd rows y = 2j + p only and leaves their
+// N2-point y transform, E or O, at y-frequency rows p*N2 + k. The z pass
+// finishes the y transform with its last radix-2 step (Y[k] = E[k] + w^k O[k],
+// Y[k + N2] = E[k] - w^k O[k], w = exp(-2 pi i / n2)) as it loads its columns.
+// Half the rows take half the LDS (the tightest padded lengths: 80.7 KB for
+// a 128 x 128 plane in FP64), so two workgroups share a CU and one's loads
+// and stores overlap the other's transforms.
+template <int M, int N2, int SPLIT = 0> struct PlaneGeo {
+  static constexpr int LLX = SPLIT ? (M - 1) + (M - 1) / P2<M>::R1 + 1 : M + M / P2<M>::R1;
   static constexpr int S = LLX > M + 1 ? LLX : M + 1;
-  static constexpr int ROWS = N2 + N2 / P2<N2>::R1;
+  static constexpr int ROWS = SPLIT ? (N2 - 1) + (N2 - 1) / P2<N2>::R1 + 1 : N2 + N2 / P2<N2>::R1;
   static constexpr int TX = P2<M>::R2, TY = P2<N2>::R2;
   static constexpr int NTX = N2 * TX, NTY = (M + 1) * TY;
   static constexpr int THREADS = ((NTX > NTY ? NTX : NTY) + 63) / 64 * 64;
 };
 
-template <int M, int N2, int KIND, int INMODE, typename R>
-__global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ descs, GridDims d,
-                                                    const typename V2<R>::T* __restrict__ twx,
-                                                    const typename V2<R>::T* __restrict__ tws,
-                                                    const typename V2<R>::T* __restrict__ twy) {
+template <int M, int N2, int KIND, int INMODE, typename R, int SPLIT>
+__global__ void __launch_bounds__((SPLIT ? PlaneGeo<M, N2, 1>::THREADS : 1024))
+    k_plane_fwd(const ReqDesc* __restrict__ descs, GridDims d, const typename V2<R>::T* __restrict__ twx,
+                const typename V2<R>::T* __restrict__ tws, const typename V2<R>::T* __restrict__ twy) {
   using C = typename V2<R>::T;
-  using G = PlaneGeo<M, N2>;
+  using G = PlaneGeo<M, N2, SPLIT>;
   constexpr int R1X = P2<M>::R1, TX = G::TX, R1Y = P2<N2>::R1, TY = G::TY, S = G::S;
   extern __shared__ __align__(16) unsigned char smem[];
   C* P = reinterpret_cast<C*>(smem);
   const ReqDesc& D = descs[blockIdx.y];
   constexpr int NFK = NF_<KIND>::v;
   const int pf = xcd_block(blockIdx.x, gridDim.x);
-  const int i3 = pf / NFK, f = pf - i3 * NFK;
+  const int hp = pf / NFK, f = pf - hp * NFK;
+  const int i3 = hp >> SPLIT, par = SPLIT ? (hp & 1) : 0;
   // ---- x step: thread (row i2, lane t)
   const int row = threadIdx.x / TX, tx = threadIdx.x % TX;
   const bool ax = row < N2;
   C z[R1X];
   if (ax) {
-    const size_t base = ((size_t)i3 * N2 + row) * (2 * M);
+    const size_t base = ((size_t)i3 * (N2 << SPLIT) + (SPLIT ? 2 * row + par : row)) * (2 * M);
 #pragma unroll
     for (int m = 0; m < R1X; ++m) z[m] = field_pair<KIND, INMODE, R>(D, f, base + 2 * t_plus(tx, TX, m));
   }
@@ -489,7 +539,7 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
           w2[m] = P[(size_t)(j + j / R1) * S + c];
         }
 #pragma unroll
-        for (int m = 1; m < R2; ++m) w2[m] = cmul(w2[m], twy[m * b2]);
+        for (int m = 1; m < R2; ++m) w2[m] = cmul(w2[m], twy[(m * b2) << SPLIT]);
         Bfly<R2, -1, C>::run(w2);
 #pragma unroll
         for (int m = 0; m < R2; ++m) y[q + Q * m] = w2[m];
@@ -497,7 +547,8 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(const ReqDesc* __restrict__ 
     }
   }
   if (ay) {
-    gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)i3 * N2 * (M + 1) + c;
+    gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)i3 * (N2 << SPLIT) * (M + 1) +
+                  (size_t)par * N2 * (M + 1) + c;
 #pragma unroll
     for (int m = 0; m < R1Y; ++m) stc(out + (size_t)(ty + TY * m) * (M + 1), y[m]);
   }
@@ -564,9 +615,17 @@ __device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, 
   }
 }
 
-template <int N, int KIND, typename R>
+//
+// SPLITY = 1 (after the split plane pass): the first kb/2 columns of a
+// workgroup are y-frequencies k < n2/2, the last kb/2 the partners k + n2/2 at
+// the same k1; each pair finishes the y transform (the radix-2 step that
+// joins the even-row and odd-row transforms) as the first z FFT stage reads
+// its inputs (fft2_pair), then runs as two ordinary columns. The inverse z
+// transform leaves the spectra in the ordinary layout.
+template <int N, int KIND, typename R, int SPLITY>
 __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
-                                                const typename V2<R>::T* __restrict__ tw, int kb) {
+                                                const typename V2<R>::T* __restrict__ tw, int kb,
+                                                const typename V2<R>::T* __restrict__ twy) {
   using C = typename V2<R>::T;
   constexpr int R1 = P2<N>::R1, T = P2<N>::R2, NF = NF_<KIND>::v;
   constexpr int LL = col_stride<N>();
@@ -575,11 +634,23 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   const ReqDesc& D = descs[blockIdx.z];
   const int f = blockIdx.y;
   const int c = threadIdx.x % kb, t = threadIdx.x / kb;
-  const long long col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + c;
-  const long long ncol = (long long)d.K1 * d.n2;
-  const bool active = col < ncol;
-  const int i2 = active ? (int)(col / d.K1) : 0;
-  const int m1 = active ? (int)(col - (long long)i2 * d.K1) : 0;
+  const int kh = kb / 2, hi = SPLITY && c >= kh ? 1 : 0;
+  long long col;
+  bool active;
+  int i2, m1, i2lo = 0;
+  if constexpr (SPLITY) {
+    const long long colh = (long long)xcd_block(blockIdx.x, gridDim.x) * kh + (c - hi * kh);
+    active = colh < (long long)d.K1 * (d.n2 / 2);
+    i2lo = active ? (int)(colh / d.K1) : 0;
+    m1 = active ? (int)(colh - (long long)i2lo * d.K1) : 0;
+    i2 = i2lo + hi * (d.n2 / 2);
+    col = (long long)i2 * d.K1 + m1;
+  } else {
+    col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + c;
+    active = col < (long long)d.K1 * d.n2;
+    i2 = active ? (int)(col / d.K1) : 0;
+    m1 = active ? (int)(col - (long long)i2 * d.K1) : 0;
+  }
   const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
   const size_t es = (size_t)d.n2 * d.K1;
   const double invn = 1.0 / (double)d.nr;
@@ -589,7 +660,19 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
 #pragma unroll
     for (int m = 0; m < R1; ++m) x[m] = ldc(base + (size_t)(t + T * m) * es);
   }
-  fft2<N, -1>(x, lds + (size_t)c * LL, t, tw, active);
+  if constexpr (SPLITY) {
+    // x holds E (k < n2/2) or O (the partner); Y[k] = E + w O, Y[k + n2/2] = E - w O
+    C* own = lds + (size_t)c * LL;
+    C* other = lds + (size_t)(hi ? c - kh : c + kh) * LL;
+    C sw = active ? twy[i2lo] : C{};
+    if (hi) {
+      sw.x = -sw.x;
+      sw.y = -sw.y;
+    }
+    fft2_pair<N, -1>(x, own, hi ? other : own, hi ? own : other, t, tw, active, sw);
+  } else {
+    fft2<N, -1>(x, lds + (size_t)c * LL, t, tw, active);
+  }
   if (active) {
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
@@ -1394,13 +1477,13 @@ static inline long long col_line_bytes(int n, int cbytes, int which) {
 // Column tiles of the y and z passes: whole wavefronts, at most 256 threads
 // and at most lds_target bytes of LDS per workgroup (a tuned kb_y/kb_z is
 // bound by the LDS capacity instead).
-static inline FastTiles fast_tiles(const ShapePlan& sp, int nf, int cbytes) {
-  (void)nf;
+static inline FastTiles fast_tiles(const ShapePlan& sp, int cbytes) {
   FastTiles ft;
   const long long tgt = sp.lds_target;
   const int Ty = len_T(sp.d.n2), Tz = len_T(sp.d.n3);
   const int wy = 64 / Ty, wz = 64 / Tz;
-  const long long ly = col_line_bytes(sp.d.n2, cbytes, 1), lz = col_line_bytes(sp.d.n3, cbytes, 2);
+  const long long ly = col_line_bytes(sp.d.n2, cbytes, 1),
+                  lz = col_line_bytes(sp.d.n3, cbytes, 2);
   const long long ty = sp.kb_y > 0 ? sp.lds_cap : tgt, tz = sp.kb_z > 0 ? sp.lds_cap : tgt;
   ft.kby = sp.kb_y > 0 ? std::max(wy, sp.kb_y / wy * wy) : 256 / Ty;
   while (ft.kby > wy && (ft.kby * ly > ty || ft.kby * Ty > 256)) ft.kby -= wy;
@@ -1480,9 +1563,16 @@ static void launch_y(const ShapePlan& sp, int kb, int nf, const ReqDesc* dd, int
   }
 }
 
+template <auto K, typename C>
+static void launch_zk(dim3 grid, dim3 block, size_t lds, hipStream_t st, const ReqDesc* dd, const GridDims& d,
+                      const C* tw, int kb, const C* twy) {
+  allow_lds<K>(lds);
+  hipLaunchKernelGGL(K, grid, block, lds, st, dd, d, tw, kb, twy);
+}
+
 template <typename R, int N, int KIND>
 static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hipStream_t st,
-                     const typename V2<R>::T* tw) {
+                     const typename V2<R>::T* tw, bool splity, const typename V2<R>::T* twy) {
   using namespace fast;
   using C = typename V2<R>::T;
   const long long ncol = (long long)sp.d.K1 * sp.d.n2;
@@ -1499,42 +1589,70 @@ static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hip
     hipLaunchKernelGGL((g_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * GPD<N>::T), lds, st, dd,
                        sp.d, tw, kb);
   } else {
+    // split: kb/2 column pairs per workgroup (kb is a whole number of
+    // wavefronts' columns, so even)
+    const long long npair = (long long)sp.d.K1 * (sp.d.n2 / 2);
+    const int nblk2 = (int)((npair + kb / 2 - 1) / (kb / 2));
+    const dim3 grid(splity ? nblk2 : nblk, NF_<KIND>::v, nb), block(kb * P2<N>::R2);
     const size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
-    allow_lds<k_fz_gop<N, KIND, R>>(lds);
-    hipLaunchKernelGGL((k_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * P2<N>::R2), lds, st, dd,
-                       sp.d, tw, kb);
+    if (splity) launch_zk<k_fz_gop<N, KIND, R, 1>>(grid, block, lds, st, dd, sp.d, tw, kb, twy);
+    else launch_zk<k_fz_gop<N, KIND, R, 0>>(grid, block, lds, st, dd, sp.d, tw, kb, twy);
   }
 }
 
-template <typename R, int M, int N2, int KIND, int INMODE>
+// SPLIT = 1: N2 is half the grid's n2 (the rows of one parity)
+template <typename R, int M, int N2, int KIND, int INMODE, int SPLIT = 0>
 static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st,
                          const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
   using namespace fast;
   using C = typename V2<R>::T;
-  using G = PlaneGeo<M, N2>;
+  using G = PlaneGeo<M, N2, SPLIT>;
   const size_t lds = (size_t)G::ROWS * G::S * sizeof(C);
   if (lds > (size_t)sp.lds_cap || G::THREADS > 1024) return false;
-  auto kern = k_plane_fwd<M, N2, KIND, INMODE, R>;
+  auto kern = k_plane_fwd<M, N2, KIND, INMODE, R, SPLIT>;
   static bool attr_set = false;
   if (!attr_set) {
     hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
-  hipLaunchKernelGGL(kern, dim3(sp.d.n3 * NF_<KIND>::v, nb), dim3(G::THREADS), lds, st, dd, sp.d, twx, tws, twy);
+  hipLaunchKernelGGL(kern, dim3(sp.d.n3 * NF_<KIND>::v * (1 + SPLIT), nb), dim3(G::THREADS), lds, st, dd, sp.d,
+                     twx, tws, twy);
   return true;
 }
 
+// The split plane pass needs the z pass that finishes its y transform: the
+// ordinary z + G-space kernel (not WGC's coupled one, not the general
+// two-stage lengths).
+template <int KIND> static inline bool plane_split_ok(const ShapePlan& sp) {
+  return sp.plane_split && KIND != kWGC8 && !is_gp_len(sp.d.n3);
+}
+
+// 0: no plane pass (separate x and y forward passes), 1: the plane pass,
+// 2: its split form
 template <typename R, int KIND, int INMODE>
-static bool try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st, const typename V2<R>::T* twx,
-                      const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
-  if (sp.no_plane) return false;
-  if (sp.d.M == 64 && sp.d.n2 == 128) return launch_plane<R, 64, 128, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
-  if (sp.d.M == 32 && sp.d.n2 == 64) return launch_plane<R, 32, 64, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
-  if (sp.d.M == 16 && sp.d.n2 == 32) return launch_plane<R, 16, 32, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
-  if (sp.d.M == 48 && sp.d.n2 == 96) return launch_plane<R, 48, 96, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
-  if (sp.d.M == 40 && sp.d.n2 == 80) return launch_plane<R, 40, 80, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
-  if (sp.d.M == 24 && sp.d.n2 == 48) return launch_plane<R, 24, 48, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
-  return false;
+static int try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st, const typename V2<R>::T* twx,
+                     const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
+  if (sp.no_plane) return 0;
+  const bool split = plane_split_ok<KIND>(sp);
+  // the split form where the whole plane takes more than 64 KiB of LDS (at
+  // most two workgroups per CU); smaller planes already share CUs, and
+  // splitting them measured no faster
+  auto go = [&](auto m, auto n2) -> int {
+    constexpr int M = decltype(m)::value, N2 = decltype(n2)::value;
+    using G0 = fast::PlaneGeo<M, N2>;
+    constexpr size_t whole = (size_t)G0::ROWS * G0::S * sizeof(typename V2<R>::T);
+    if (split && whole > 65536 && launch_plane<R, M, N2 / 2, KIND, INMODE, 1>(sp, dd, nb, st, twx, tws, twy))
+      return 2;
+    return launch_plane<R, M, N2, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy) ? 1 : 0;
+  };
+  using std::integral_constant;
+  if (sp.d.M == 64 && sp.d.n2 == 128) return go(integral_constant<int, 64>(), integral_constant<int, 128>());
+  if (sp.d.M == 32 && sp.d.n2 == 64) return go(integral_constant<int, 32>(), integral_constant<int, 64>());
+  if (sp.d.M == 16 && sp.d.n2 == 32) return go(integral_constant<int, 16>(), integral_constant<int, 32>());
+  if (sp.d.M == 48 && sp.d.n2 == 96) return go(integral_constant<int, 48>(), integral_constant<int, 96>());
+  if (sp.d.M == 40 && sp.d.n2 == 80) return go(integral_constant<int, 40>(), integral_constant<int, 80>());
+  if (sp.d.M == 24 && sp.d.n2 == 48) return go(integral_constant<int, 24>(), integral_constant<int, 48>());
+  return 0;
 }
 
 template <typename R, int KIND, int INMODE, int OUTMODE, int M>
@@ -1559,23 +1677,25 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
       default: launch_y<R, 256, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
     }
   };
-  if (!try_plane<R, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy)) {
+  const int plane = try_plane<R, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
+  if (plane == 0) {
     launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, true, twx, tws);
     ylaunch(std::integral_constant<int, -1>());
   }
+  const bool sy = plane == 2;
   switch (sp.d.n3) {
-    case 24: launch_z<R, 24, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 40: launch_z<R, 40, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 48: launch_z<R, 48, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 60: launch_z<R, 60, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 64: launch_z<R, 64, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 80: launch_z<R, 80, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 96: launch_z<R, 96, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 120: launch_z<R, 120, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 128: launch_z<R, 128, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    case 192: launch_z<R, 192, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
-    default: launch_z<R, 256, KIND>(sp, ft.kbz, dd, nb, st, twz); break;
+    case 24: launch_z<R, 24, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 40: launch_z<R, 40, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 48: launch_z<R, 48, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 60: launch_z<R, 60, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 64: launch_z<R, 64, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 80: launch_z<R, 80, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 96: launch_z<R, 96, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 120: launch_z<R, 120, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 128: launch_z<R, 128, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    case 192: launch_z<R, 192, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
+    default: launch_z<R, 256, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
   }
   ylaunch(std::integral_constant<int, +1>());
   launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, false, twx, tws);
@@ -1583,7 +1703,7 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
 
 template <typename R, int KIND, int INMODE, int OUTMODE>
 static int fast_eval_t(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st) {
-  const FastTiles ft = fast_tiles(sp, fast::NF_<KIND>::v, (int)sizeof(typename V2<R>::T));
+  const FastTiles ft = fast_tiles(sp, (int)sizeof(typename V2<R>::T));
   const bool f32 = sizeof(R) == 4;
   const typename V2<R>::T *twx, *tws, *twy, *twz;
   if constexpr (sizeof(R) == 4) {

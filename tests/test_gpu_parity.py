@@ -69,6 +69,7 @@ GRIDS = [(16, 16, 16), (24, 20, 18), (14, 21, 10), (32, 30, 28), (6, 5, 4),
          (48, 48, 48), (96, 80, 40), (80, 192, 24), (192, 48, 96),
          (120, 120, 120), (120, 48, 120), (240, 24, 60),   # 60 = 4 x 15, 120 = 8 x 15 (uneven lanes)
          (128, 128, 32), (64, 64, 32), (32, 32, 32)]   # the last three take the fused x+y plane kernel
+                                                        # (128 x 128 planes in its split form)
 
 
 @pytest.mark.parametrize("grid", GRIDS)
@@ -296,6 +297,36 @@ def test_fast_mixed_lengths_match_generic_path(pa, monkeypatch, grid):
         p32, e32 = res[1, flag]
         assert np.max(np.abs(p32 - p1)) <= 1e-5 * np.max(np.abs(p1))
         _check_terms(e32, e1, rtol=1e-5, floor=1e-8)
+
+
+@pytest.mark.parametrize("grid,alpha,beta", [((128, 128, 64), -100.0, -100.0), ((96, 96, 48), 0.9, 0.7),
+                                             ((128, 128, 128), 0.9, 0.7)])
+def test_split_plane_matches_whole_plane(pa, monkeypatch, grid, alpha, beta):
+    """The plane pass in its split form (even and odd rows apart, the y
+    transform finished by the z pass) and whole agree to rounding: 1e-13 in
+    FP64, 1e-5 against FP64 in the FP32 passes; and the split FP64 result
+    meets the oracle at the FP64 bar."""
+    _say("split plane grid %s alpha %g beta %g" % (grid, alpha, beta))
+    cell, frac, vion, ne, f, rho = _case(grid, _tri_lattice(), alpha=alpha, beta=beta, seed=3)
+    res = {}
+    for prec in (0, 1):
+        for flag in ("0", "1"):
+            monkeypatch.setenv("OFD_PLANE_SPLIT", flag)
+            s = pa.OFDSystem(cell.lattice, grid, ne, kinetic=4, alpha=alpha, beta=beta, precision=prec)
+            s.set_ion_terms(vion, 0.0)
+            res[prec, flag] = pa.CalculatePotentialPlus(s, rho)
+            s.close()
+    (p1, e1), (p0, e0) = res[0, "1"], res[0, "0"]
+    assert np.max(np.abs(p1 - p0)) <= 1e-13 * np.max(np.abs(p0))
+    _check_terms(e1, e0, rtol=1e-13)
+    for flag in ("0", "1"):
+        p32, e32 = res[1, flag]
+        assert np.max(np.abs(p32 - p0)) <= 1e-5 * np.max(np.abs(p0))
+        _check_terms(e32, e0, rtol=1e-5, floor=1e-8)
+    sysm = O.System(cell, f, vion)
+    pr, er = O.calculate_potential_plus(rho, sysm, True, True)
+    _check_terms(e1, er)
+    assert np.max(np.abs(p1 - pr)) <= POT_RTOL * np.max(np.abs(pr))
 
 
 @pytest.mark.parametrize("grid", [(64, 64, 64), (256, 48, 32)])
