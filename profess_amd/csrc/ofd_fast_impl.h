@@ -89,15 +89,15 @@ __device__ __forceinline__ void fft2(C (&x)[P2<N>::R1], C* __restrict__ lds, int
   __syncthreads();
 }
 
-// fft2 of one column of a pair that a radix-2 step joins first: on exit x
-// holds the FFT of E + sw O, where E and O are the pair's two lines (this
-// line's own input is one of them, written to `lds` by the first stage) and
-// sw = +w or -w. The join is linear and the first stage acts on each line
-// alone, so it is applied as the stage-two inputs are read from the two LDS
-// lines (no extra exchange or barrier).
-template <int N, int S, typename C>
-__device__ __forceinline__ void fft2_pair(C (&x)[P2<N>::R1], C* __restrict__ lds, const C* le, const C* lo, int t,
-                                          const C* __restrict__ tw, bool active, C sw) {
+// fft2 of one column of a group of P lines X_0 .. X_{P-1} (this line's own
+// input is one of them, written to `lds` by the first stage; line p sits at
+// grp + p * gstride) that a radix-P step joins first: on exit x holds the FFT
+// of X_0 + sum_p w[p] X_p. The join is linear and the first stage acts on
+// each line alone, so it is applied as the stage-two inputs are read from the
+// group's LDS lines (no extra exchange or barrier).
+template <int N, int S, int P, typename C>
+__device__ __forceinline__ void fft2_join(C (&x)[P2<N>::R1], C* __restrict__ lds, const C* grp, int gstride, int t,
+                                          const C* __restrict__ tw, bool active, const C (&w)[P]) {
   constexpr int R1 = P2<N>::R1, R2 = P2<N>::R2, T = R2, Q = R1 / R2;
   if (active) {
     Bfly<R1, S, C>::run(x);
@@ -112,9 +112,14 @@ __device__ __forceinline__ void fft2_pair(C (&x)[P2<N>::R1], C* __restrict__ lds
       C y[R2];
 #pragma unroll
       for (int m = 0; m < R2; ++m) {
-        const C e = le[padded<N>(b2 + m * R1)], o = cmul(lo[padded<N>(b2 + m * R1)], sw);
-        y[m].x = e.x + o.x;
-        y[m].y = e.y + o.y;
+        const int j = padded<N>(b2 + m * R1);
+        y[m] = grp[j];
+#pragma unroll
+        for (int p = 1; p < P; ++p) {
+          const C o = cmul(grp[(size_t)p * gstride + j], w[p]);
+          y[m].x += o.x;
+          y[m].y += o.y;
+        }
       }
 #pragma unroll
       for (int m = 1; m < R2; ++m) {
@@ -438,15 +443,15 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
 // FFT: rows of stride S >= max(M + M/R1x, M + 1) complex, n2 + n2/R1y rows so
 // that the y transform can use the padded column layout as its scratch.
 //
-// Split form (SPLIT = 1, N2 = n2/2): one workgroup per (z-plane, row parity,
-// field) takes the even or the odd rows y = 2j + p only and leaves their
-// N2-point y transform, E or O, at y-frequency rows p*N2 + k. The z pass
-// finishes the y transform with its last radix-2 step (Y[k] = E[k] + w^k O[k],
-// Y[k + N2] = E[k] - w^k O[k], w = exp(-2 pi i / n2)) as it loads its columns.
-// Half the rows take half the LDS (the tightest padded lengths: 80.7 KB for
-// a 128 x 128 plane in FP64), so two workgroups share a CU and one's loads
-// and stores overlap the other's transforms.
-template <int M, int N2, int SPLIT = 0> struct PlaneGeo {
+// Split form (NP row classes, N2 = n2/NP; the plans use NP = 2): one workgroup per
+// (z-plane, row class p, field) takes the rows y = NP j + p only and leaves
+// their N2-point y transform X_p at y-frequency rows p*N2 + k. The z pass finishes
+// the y transform with its last radix-NP step (Y[k] = sum_p w^(p k) X_p[k mod
+// N2], w = exp(-2 pi i / n2)) as it loads its columns. A class of rows takes
+// 1/NP of the LDS (at the tightest padded lengths: 80.7 KB for half a 128 x
+// 128 plane in FP64, 39.8 KB for a quarter), so several workgroups share a CU
+// and the loads and stores of some overlap the transforms of others.
+template <int M, int N2, int SPLIT = 0> struct PlaneGeo {   // SPLIT: the tight lengths of the split form
   static constexpr int LLX = SPLIT ? (M - 1) + (M - 1) / P2<M>::R1 + 1 : M + M / P2<M>::R1;
   static constexpr int S = LLX > M + 1 ? LLX : M + 1;
   static constexpr int ROWS = SPLIT ? (N2 - 1) + (N2 - 1) / P2<N2>::R1 + 1 : N2 + N2 / P2<N2>::R1;
@@ -455,12 +460,12 @@ template <int M, int N2, int SPLIT = 0> struct PlaneGeo {
   static constexpr int THREADS = ((NTX > NTY ? NTX : NTY) + 63) / 64 * 64;
 };
 
-template <int M, int N2, int KIND, int INMODE, typename R, int SPLIT>
-__global__ void __launch_bounds__((SPLIT ? PlaneGeo<M, N2, 1>::THREADS : 1024))
+template <int M, int N2, int KIND, int INMODE, typename R, int NP>
+__global__ void __launch_bounds__((NP > 1 ? PlaneGeo<M, N2, 1>::THREADS : 1024))
     k_plane_fwd(const ReqDesc* __restrict__ descs, GridDims d, const typename V2<R>::T* __restrict__ twx,
                 const typename V2<R>::T* __restrict__ tws, const typename V2<R>::T* __restrict__ twy) {
   using C = typename V2<R>::T;
-  using G = PlaneGeo<M, N2, SPLIT>;
+  using G = PlaneGeo<M, N2, (NP > 1)>;
   constexpr int R1X = P2<M>::R1, TX = G::TX, R1Y = P2<N2>::R1, TY = G::TY, S = G::S;
   extern __shared__ __align__(16) unsigned char smem[];
   C* P = reinterpret_cast<C*>(smem);
@@ -468,13 +473,13 @@ __global__ void __launch_bounds__((SPLIT ? PlaneGeo<M, N2, 1>::THREADS : 1024))
   constexpr int NFK = NF_<KIND>::v;
   const int pf = xcd_block(blockIdx.x, gridDim.x);
   const int hp = pf / NFK, f = pf - hp * NFK;
-  const int i3 = hp >> SPLIT, par = SPLIT ? (hp & 1) : 0;
+  const int i3 = hp / NP, par = hp - i3 * NP;
   // ---- x step: thread (row i2, lane t)
   const int row = threadIdx.x / TX, tx = threadIdx.x % TX;
   const bool ax = row < N2;
   C z[R1X];
   if (ax) {
-    const size_t base = ((size_t)i3 * (N2 << SPLIT) + (SPLIT ? 2 * row + par : row)) * (2 * M);
+    const size_t base = ((size_t)i3 * (N2 * NP) + (NP * row + par)) * (2 * M);
 #pragma unroll
     for (int m = 0; m < R1X; ++m) z[m] = field_pair<KIND, INMODE, R>(D, f, base + 2 * t_plus(tx, TX, m));
   }
@@ -539,7 +544,7 @@ __global__ void __launch_bounds__((SPLIT ? PlaneGeo<M, N2, 1>::THREADS : 1024))
           w2[m] = P[(size_t)(j + j / R1) * S + c];
         }
 #pragma unroll
-        for (int m = 1; m < R2; ++m) w2[m] = cmul(w2[m], twy[(m * b2) << SPLIT]);
+        for (int m = 1; m < R2; ++m) w2[m] = cmul(w2[m], twy[m * b2 * NP]);
         Bfly<R2, -1, C>::run(w2);
 #pragma unroll
         for (int m = 0; m < R2; ++m) y[q + Q * m] = w2[m];
@@ -547,7 +552,7 @@ __global__ void __launch_bounds__((SPLIT ? PlaneGeo<M, N2, 1>::THREADS : 1024))
     }
   }
   if (ay) {
-    gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)i3 * (N2 << SPLIT) * (M + 1) +
+    gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)i3 * (N2 * NP) * (M + 1) +
                   (size_t)par * N2 * (M + 1) + c;
 #pragma unroll
     for (int m = 0; m < R1Y; ++m) stc(out + (size_t)(ty + TY * m) * (M + 1), y[m]);
@@ -616,13 +621,14 @@ __device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, 
 }
 
 //
-// SPLITY = 1 (after the split plane pass): the first kb/2 columns of a
-// workgroup are y-frequencies k < n2/2, the last kb/2 the partners k + n2/2 at
-// the same k1; each pair finishes the y transform (the radix-2 step that
-// joins the even-row and odd-row transforms) as the first z FFT stage reads
-// its inputs (fft2_pair), then runs as two ordinary columns. The inverse z
+// P > 1 (after the split plane pass with P row classes): the kb columns of a
+// workgroup are P members of kb/P groups; member q of a group holds
+// y-frequency k + q n2/P (k < n2/P) at one k1, and its input is the class-q
+// transform X_q[k]. Each group finishes the y transform (the radix-P step that
+// joins the classes' transforms) as the first z FFT stage reads its inputs
+// (fft2_join), then its members run as ordinary columns. The inverse z
 // transform leaves the spectra in the ordinary layout.
-template <int N, int KIND, typename R, int SPLITY>
+template <int N, int KIND, typename R, int P>
 __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
                                                 const typename V2<R>::T* __restrict__ tw, int kb,
                                                 const typename V2<R>::T* __restrict__ twy) {
@@ -634,16 +640,16 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   const ReqDesc& D = descs[blockIdx.z];
   const int f = blockIdx.y;
   const int c = threadIdx.x % kb, t = threadIdx.x / kb;
-  const int kh = kb / 2, hi = SPLITY && c >= kh ? 1 : 0;
+  const int kg = kb / P, q = c / kg, cg = c - q * kg;   // group size, member, group
   long long col;
   bool active;
-  int i2, m1, i2lo = 0;
-  if constexpr (SPLITY) {
-    const long long colh = (long long)xcd_block(blockIdx.x, gridDim.x) * kh + (c - hi * kh);
-    active = colh < (long long)d.K1 * (d.n2 / 2);
-    i2lo = active ? (int)(colh / d.K1) : 0;
+  int i2, m1;
+  if constexpr (P > 1) {
+    const long long colh = (long long)xcd_block(blockIdx.x, gridDim.x) * kg + cg;
+    active = colh < (long long)d.K1 * (d.n2 / P);
+    const int i2lo = active ? (int)(colh / d.K1) : 0;
     m1 = active ? (int)(colh - (long long)i2lo * d.K1) : 0;
-    i2 = i2lo + hi * (d.n2 / 2);
+    i2 = i2lo + q * (d.n2 / P);
     col = (long long)i2 * d.K1 + m1;
   } else {
     col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + c;
@@ -660,16 +666,12 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
 #pragma unroll
     for (int m = 0; m < R1; ++m) x[m] = ldc(base + (size_t)(t + T * m) * es);
   }
-  if constexpr (SPLITY) {
-    // x holds E (k < n2/2) or O (the partner); Y[k] = E + w O, Y[k + n2/2] = E - w O
-    C* own = lds + (size_t)c * LL;
-    C* other = lds + (size_t)(hi ? c - kh : c + kh) * LL;
-    C sw = active ? twy[i2lo] : C{};
-    if (hi) {
-      sw.x = -sw.x;
-      sw.y = -sw.y;
-    }
-    fft2_pair<N, -1>(x, own, hi ? other : own, hi ? own : other, t, tw, active, sw);
+  if constexpr (P > 1) {
+    // x holds X_q; Y[i2] = sum_p w^(p i2) X_p with w^j = twy[j mod n2]
+    C w[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) w[p] = active ? twy[(size_t)p * i2 % d.n2] : C{};
+    fft2_join<N, -1, P>(x, lds + (size_t)c * LL, lds + (size_t)cg * LL, kg * LL, t, tw, active, w);
   } else {
     fft2<N, -1>(x, lds + (size_t)c * LL, t, tw, active);
   }
@@ -1572,7 +1574,7 @@ static void launch_zk(dim3 grid, dim3 block, size_t lds, hipStream_t st, const R
 
 template <typename R, int N, int KIND>
 static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hipStream_t st,
-                     const typename V2<R>::T* tw, bool splity, const typename V2<R>::T* twy) {
+                     const typename V2<R>::T* tw, int P, const typename V2<R>::T* twy) {
   using namespace fast;
   using C = typename V2<R>::T;
   const long long ncol = (long long)sp.d.K1 * sp.d.n2;
@@ -1589,33 +1591,33 @@ static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hip
     hipLaunchKernelGGL((g_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * GPD<N>::T), lds, st, dd,
                        sp.d, tw, kb);
   } else {
-    // split: kb/2 column pairs per workgroup (kb is a whole number of
-    // wavefronts' columns, so even)
-    const long long npair = (long long)sp.d.K1 * (sp.d.n2 / 2);
-    const int nblk2 = (int)((npair + kb / 2 - 1) / (kb / 2));
-    const dim3 grid(splity ? nblk2 : nblk, NF_<KIND>::v, nb), block(kb * P2<N>::R2);
-    const size_t lds = (size_t)kb * col_stride<N>() * sizeof(C);
-    if (splity) launch_zk<k_fz_gop<N, KIND, R, 1>>(grid, block, lds, st, dd, sp.d, tw, kb, twy);
-    else launch_zk<k_fz_gop<N, KIND, R, 0>>(grid, block, lds, st, dd, sp.d, tw, kb, twy);
+    // after the split plane pass: kb/P groups of P columns per workgroup (kb
+    // rounded down to a multiple of P)
+    const int kbj = P > 1 ? std::max(P, kb / P * P) : kb;
+    const long long ngrp = (long long)sp.d.K1 * (sp.d.n2 / P);
+    const dim3 grid((unsigned)((ngrp + kbj / P - 1) / (kbj / P)), NF_<KIND>::v, nb), block(kbj * P2<N>::R2);
+    const size_t lds = (size_t)kbj * col_stride<N>() * sizeof(C);
+    if (P == 2) launch_zk<k_fz_gop<N, KIND, R, 2>>(grid, block, lds, st, dd, sp.d, tw, kbj, twy);
+    else launch_zk<k_fz_gop<N, KIND, R, 1>>(grid, block, lds, st, dd, sp.d, tw, kbj, twy);
   }
 }
 
-// SPLIT = 1: N2 is half the grid's n2 (the rows of one parity)
-template <typename R, int M, int N2, int KIND, int INMODE, int SPLIT = 0>
+// P > 1: the split form, N2 = n2 / P (the rows of one class)
+template <typename R, int M, int N2, int KIND, int INMODE, int P = 1>
 static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st,
                          const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
   using namespace fast;
   using C = typename V2<R>::T;
-  using G = PlaneGeo<M, N2, SPLIT>;
+  using G = PlaneGeo<M, N2, (P > 1)>;
   const size_t lds = (size_t)G::ROWS * G::S * sizeof(C);
   if (lds > (size_t)sp.lds_cap || G::THREADS > 1024) return false;
-  auto kern = k_plane_fwd<M, N2, KIND, INMODE, R, SPLIT>;
+  auto kern = k_plane_fwd<M, N2, KIND, INMODE, R, P>;
   static bool attr_set = false;
   if (!attr_set) {
     hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
-  hipLaunchKernelGGL(kern, dim3(sp.d.n3 * NF_<KIND>::v * (1 + SPLIT), nb), dim3(G::THREADS), lds, st, dd, sp.d,
+  hipLaunchKernelGGL(kern, dim3(sp.d.n3 * NF_<KIND>::v * P, nb), dim3(G::THREADS), lds, st, dd, sp.d,
                      twx, tws, twy);
   return true;
 }
@@ -1624,11 +1626,13 @@ static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStre
 // ordinary z + G-space kernel (not WGC's coupled one, not the general
 // two-stage lengths).
 template <int KIND> static inline bool plane_split_ok(const ShapePlan& sp) {
-  return sp.plane_split && KIND != kWGC8 && !is_gp_len(sp.d.n3);
+  return sp.plane_split > 1 && KIND != kWGC8 && !is_gp_len(sp.d.n3);
 }
 
-// 0: no plane pass (separate x and y forward passes), 1: the plane pass,
-// 2: its split form
+// The plane pass run, as its number of row classes: 0 none (separate x and y
+// forward passes), 1 the whole plane, 2 the split form. (Four classes, a
+// quarter of the rows per workgroup, measured slower at 128^3: the plane pass
+// no faster, 26.5 us, and the z pass's radix-4 join 7.7 us dearer.)
 template <typename R, int KIND, int INMODE>
 static int try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st, const typename V2<R>::T* twx,
                      const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
@@ -1641,7 +1645,7 @@ static int try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t
     constexpr int M = decltype(m)::value, N2 = decltype(n2)::value;
     using G0 = fast::PlaneGeo<M, N2>;
     constexpr size_t whole = (size_t)G0::ROWS * G0::S * sizeof(typename V2<R>::T);
-    if (split && whole > 65536 && launch_plane<R, M, N2 / 2, KIND, INMODE, 1>(sp, dd, nb, st, twx, tws, twy))
+    if (split && whole > 65536 && launch_plane<R, M, N2 / 2, KIND, INMODE, 2>(sp, dd, nb, st, twx, tws, twy))
       return 2;
     return launch_plane<R, M, N2, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy) ? 1 : 0;
   };
@@ -1682,7 +1686,7 @@ static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* 
     launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, true, twx, tws);
     ylaunch(std::integral_constant<int, -1>());
   }
-  const bool sy = plane == 2;
+  const int sy = plane > 1 ? plane : 1;   // row classes the z pass joins
   switch (sp.d.n3) {
     case 24: launch_z<R, 24, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
     case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;

@@ -119,7 +119,7 @@ struct ShapePlan {
   int lds_target = 65536;  // soft budget used for the automatic tiles
   int force_generic = 0;   // 1: never use the register-resident fast path
   int no_plane = 0;        // 1: never use the fused x+y plane kernel
-  int plane_split = 1;     // 1: plane passes above 64 KiB of LDS in their even/odd-row split form
+  int plane_split = 2;     // 2: plane passes above 64 KiB of LDS in their even/odd-row split form (1: whole)
 };
 
 struct Tiles {

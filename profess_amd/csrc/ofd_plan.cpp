@@ -186,7 +186,7 @@ int shape_plan_init(ShapePlan* sp, int n1, int n2, int n3, std::string* err) {
   }
   if (const char* e = std::getenv("OFD_GENERIC")) sp->force_generic = std::atoi(e) != 0;
   if (const char* e = std::getenv("OFD_NO_PLANE")) sp->no_plane = std::atoi(e) != 0;
-  if (const char* e = std::getenv("OFD_PLANE_SPLIT")) sp->plane_split = std::atoi(e) != 0;
+  if (const char* e = std::getenv("OFD_PLANE_SPLIT")) sp->plane_split = std::atoi(e);
   if (const char* e = std::getenv("OFD_KB_Y")) sp->kb_y = std::atoi(e);
   if (const char* e = std::getenv("OFD_KB_Z")) sp->kb_z = std::atoi(e);
   const size_t cb = sizeof(double2);
