@@ -448,11 +448,15 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
 // their N2-point y transform X_p at y-frequency rows p*N2 + k. The z pass finishes
 // the y transform with its last radix-NP step (Y[k] = sum_p w^(p k) X_p[k mod
 // N2], w = exp(-2 pi i / n2)) as it loads its columns. A class of rows takes
-// 1/NP of the LDS (at the tightest padded lengths: 80.7 KB for half a 128 x
-// 128 plane in FP64, 39.8 KB for a quarter), so several workgroups share a CU
-// and the loads and stores of some overlap the transforms of others.
-template <int M, int N2, int SPLIT = 0> struct PlaneGeo {   // SPLIT: the tight lengths of the split form
-  static constexpr int LLX = SPLIT ? (M - 1) + (M - 1) / P2<M>::R1 + 1 : M + M / P2<M>::R1;
+// 1/NP of the LDS (with the tightest padded column length: 81.8 KB for half a
+// 128 x 128 plane in FP64), so several workgroups share a CU and the loads and
+// stores of some overlap the transforms of others.
+template <int M, int N2, int SPLIT = 0> struct PlaneGeo {   // SPLIT: the tight row count of the split form
+  // the row stride keeps M + M/R1 (not the tight M + (M-1)/R1): at 72 complex
+  // two consecutive rows fall on opposite halves of the 64 LDS banks, so the
+  // 16 lanes of two rows served together do not conflict (71 measured 4x the
+  // bank conflicts)
+  static constexpr int LLX = M + M / P2<M>::R1;
   static constexpr int S = LLX > M + 1 ? LLX : M + 1;
   static constexpr int ROWS = SPLIT ? (N2 - 1) + (N2 - 1) / P2<N2>::R1 + 1 : N2 + N2 / P2<N2>::R1;
   static constexpr int TX = P2<M>::R2, TY = P2<N2>::R2;
