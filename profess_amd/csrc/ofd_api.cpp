@@ -263,9 +263,16 @@ static void graphs_clear(GraphCache& g) {
   g.exec.clear();
   g.seen.clear();
 }
+// Launch graphs are opt-in (OFD_GRAPH=1; OFD_NO_GRAPH=1 always turns them
+// off): on MI355X under ROCm 7 a replayed five-kernel evaluation took 4-5 us
+// longer on the GPU than the same kernels launched one by one, at every grid
+// from 32^3 to 256^3, and the launches of one evaluation cost the host less
+// than its GPU time even at 32^3.
 static bool graphs_enabled() {
-  const char* e = std::getenv("OFD_NO_GRAPH");
-  return !(e && std::atoi(e) != 0);
+  const char* off = std::getenv("OFD_NO_GRAPH");
+  if (off && std::atoi(off) != 0) return false;
+  const char* on = std::getenv("OFD_GRAPH");
+  return on && std::atoi(on) != 0;
 }
 // Runs run_eval + run_finalize for `nb` descriptors at `dd` on `st`,
 // through a cached graph when possible. Returns 0 on success.

@@ -43,6 +43,7 @@ __global__ void __launch_bounds__(256) k_fin(const ReqDesc* __restrict__ descs, 
 
 bool fast_path_ok(const ShapePlan& sp, int kind) { return fast_path_ok_impl(sp, kind); }
 
+
 int fast_eval(const ShapePlan& sp, int kind, int inmode, int outmode, int fp32, const ReqDesc* dd, int nb,
               hipStream_t st) {
   if (kind == kWT3)

@@ -467,16 +467,16 @@ template <int M, int N2, int SPLIT = 0> struct PlaneGeo {   // SPLIT: the tight 
 template <int M, int N2, int KIND, int INMODE, typename R, int NP>
 __global__ void __launch_bounds__((NP > 1 ? PlaneGeo<M, N2, 1>::THREADS : 1024))
     k_plane_fwd(const ReqDesc* __restrict__ descs, GridDims d, const typename V2<R>::T* __restrict__ twx,
-                const typename V2<R>::T* __restrict__ tws, const typename V2<R>::T* __restrict__ twy) {
+                const typename V2<R>::T* __restrict__ tws, const typename V2<R>::T* __restrict__ twy, int fo, int fn) {
   using C = typename V2<R>::T;
   using G = PlaneGeo<M, N2, (NP > 1)>;
   constexpr int R1X = P2<M>::R1, TX = G::TX, R1Y = P2<N2>::R1, TY = G::TY, S = G::S;
   extern __shared__ __align__(16) unsigned char smem[];
   C* P = reinterpret_cast<C*>(smem);
   const ReqDesc& D = descs[blockIdx.y];
-  constexpr int NFK = NF_<KIND>::v;
+  // fields fo .. fo + fn - 1 of the kind
   const int pf = xcd_block(blockIdx.x, gridDim.x);
-  const int hp = pf / NFK, f = pf - hp * NFK;
+  const int hp = pf / fn, f = fo + (pf - hp * fn);
   const int i3 = hp / NP, par = hp - i3 * NP;
   // ---- x step: thread (row i2, lane t)
   const int row = threadIdx.x / TX, tx = threadIdx.x % TX;
@@ -567,7 +567,7 @@ __global__ void __launch_bounds__((NP > 1 ? PlaneGeo<M, N2, 1>::THREADS : 1024))
 // columns flattened over (i3, k1); blockDim = KB * T, thread = (t, c), c fastest
 template <int N, int S, typename R>
 __global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, GridDims d,
-                                            const typename V2<R>::T* __restrict__ tw, int kb) {
+                                            const typename V2<R>::T* __restrict__ tw, int kb, int fo) {
   using C = typename V2<R>::T;
   constexpr int R1 = P2<N>::R1, T = P2<N>::R2;
   constexpr int LL = col_stride<N>();
@@ -579,7 +579,7 @@ __global__ void __launch_bounds__(256) k_fy(const ReqDesc* __restrict__ descs, G
   const long long ncol = (long long)d.K1 * d.n3;
   const bool active = col < ncol;
   const long long i3 = active ? col / d.K1 : 0, k1 = active ? col - i3 * d.K1 : 0;
-  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)blockIdx.y * d.nc + (size_t)i3 * d.n2 * d.K1 + k1;
+  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)(fo + blockIdx.y) * d.nc + (size_t)i3 * d.n2 * d.K1 + k1;
   C x[R1];
   if (active) {
 #pragma unroll
@@ -635,14 +635,14 @@ __device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, 
 template <int N, int KIND, typename R, int P>
 __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
                                                 const typename V2<R>::T* __restrict__ tw, int kb,
-                                                const typename V2<R>::T* __restrict__ twy) {
+                                                const typename V2<R>::T* __restrict__ twy, int fo) {
   using C = typename V2<R>::T;
   constexpr int R1 = P2<N>::R1, T = P2<N>::R2, NF = NF_<KIND>::v;
   constexpr int LL = col_stride<N>();
   extern __shared__ __align__(16) unsigned char smem[];
   C* lds = reinterpret_cast<C*>(smem);
   const ReqDesc& D = descs[blockIdx.z];
-  const int f = blockIdx.y;
+  const int f = fo + blockIdx.y;
   const int c = threadIdx.x % kb, t = threadIdx.x / kb;
   const int kg = kb / P, q = c / kg, cg = c - q * kg;   // group size, member, group
   long long col;
@@ -1194,7 +1194,7 @@ __device__ __forceinline__ void ginv(C (&xs)[GPD<N>::NS], C (&xc)[GPD<N>::NC], C
 // loads columns and stores rows, S = +1 the reverse.
 template <int N, int S, typename R>
 __global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, GridDims d,
-                                            const typename V2<R>::T* __restrict__ tw, int kb) {
+                                            const typename V2<R>::T* __restrict__ tw, int kb, int fo) {
   using C = typename V2<R>::T;
   using G = GPD<N>;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1205,7 +1205,7 @@ __global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, G
   const long long ncol = (long long)d.K1 * d.n3;
   const bool active = col < ncol;
   const long long i3 = active ? col / d.K1 : 0, k1c = active ? col - i3 * d.K1 : 0;
-  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)blockIdx.y * d.nc + (size_t)i3 * d.n2 * d.K1 + k1c;
+  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)(fo + blockIdx.y) * d.nc + (size_t)i3 * d.n2 * d.K1 + k1c;
   C xc[G::NC], xs[G::NS];
   if (S < 0) {
     if (active) {
@@ -1249,14 +1249,14 @@ __global__ void __launch_bounds__(256) g_fy(const ReqDesc* __restrict__ descs, G
 // z columns + the diagonal G-space operator, one field per blockIdx.y
 template <int N, int KIND, typename R>
 __global__ void __launch_bounds__(256) g_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
-                                                const typename V2<R>::T* __restrict__ tw, int kb) {
+                                                const typename V2<R>::T* __restrict__ tw, int kb, int fo) {
   using C = typename V2<R>::T;
   using G = GPD<N>;
   constexpr int NF = NF_<KIND>::v;
   extern __shared__ __align__(16) unsigned char smem[];
   C* lds = reinterpret_cast<C*>(smem);
   const ReqDesc& D = descs[blockIdx.z];
-  const int f = blockIdx.y;
+  const int f = fo + blockIdx.y;
   const int cl = threadIdx.x % kb, t = threadIdx.x / kb;
   const long long col = (long long)xcd_block(blockIdx.x, gridDim.x) * kb + cl;
   const long long ncol = (long long)d.K1 * d.n2;
@@ -1552,7 +1552,7 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
 }
 
 template <typename R, int N, int S>
-static void launch_y(const ShapePlan& sp, int kb, int nf, const ReqDesc* dd, int nb, hipStream_t st,
+static void launch_y(const ShapePlan& sp, int kb, int fo, int nf, const ReqDesc* dd, int nb, hipStream_t st,
                      const typename V2<R>::T* tw) {
   using namespace fast;
   using C = typename V2<R>::T;
@@ -1561,24 +1561,24 @@ static void launch_y(const ShapePlan& sp, int kb, int nf, const ReqDesc* dd, int
   if constexpr (IsGP<N>::v) {
     const size_t lds = (size_t)kb * GPD<N>::LINE * sizeof(C);
     allow_lds<g_fy<N, S, R>>(lds);
-    hipLaunchKernelGGL((g_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * GPD<N>::T), lds, st, dd, sp.d, tw, kb);
+    hipLaunchKernelGGL((g_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * GPD<N>::T), lds, st, dd, sp.d, tw, kb, fo);
   } else {
     const size_t lds = (size_t)kb * col_stride<N>() * sizeof(R);
     allow_lds<k_fy<N, S, R>>(lds);
-    hipLaunchKernelGGL((k_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb);
+    hipLaunchKernelGGL((k_fy<N, S, R>), dim3(nblk, nf, nb), dim3(kb * P2<N>::R2), lds, st, dd, sp.d, tw, kb, fo);
   }
 }
 
 template <auto K, typename C>
 static void launch_zk(dim3 grid, dim3 block, size_t lds, hipStream_t st, const ReqDesc* dd, const GridDims& d,
-                      const C* tw, int kb, const C* twy) {
+                      const C* tw, int kb, const C* twy, int fo) {
   allow_lds<K>(lds);
-  hipLaunchKernelGGL(K, grid, block, lds, st, dd, d, tw, kb, twy);
+  hipLaunchKernelGGL(K, grid, block, lds, st, dd, d, tw, kb, twy, fo);
 }
 
 template <typename R, int N, int KIND>
 static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hipStream_t st,
-                     const typename V2<R>::T* tw, int P, const typename V2<R>::T* twy) {
+                     const typename V2<R>::T* tw, int P, const typename V2<R>::T* twy, int fo, int fn) {
   using namespace fast;
   using C = typename V2<R>::T;
   const long long ncol = (long long)sp.d.K1 * sp.d.n2;
@@ -1592,24 +1592,25 @@ static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hip
   } else if constexpr (IsGP<N>::v) {
     const size_t lds = (size_t)kb * GPD<N>::LINE * sizeof(C);
     allow_lds<g_fz_gop<N, KIND, R>>(lds);
-    hipLaunchKernelGGL((g_fz_gop<N, KIND, R>), dim3(nblk, NF_<KIND>::v, nb), dim3(kb * GPD<N>::T), lds, st, dd,
-                       sp.d, tw, kb);
+    hipLaunchKernelGGL((g_fz_gop<N, KIND, R>), dim3(nblk, fn, nb), dim3(kb * GPD<N>::T), lds, st, dd, sp.d, tw, kb,
+                       fo);
   } else {
     // after the split plane pass: kb/P groups of P columns per workgroup (kb
     // rounded down to a multiple of P)
     const int kbj = P > 1 ? std::max(P, kb / P * P) : kb;
     const long long ngrp = (long long)sp.d.K1 * (sp.d.n2 / P);
-    const dim3 grid((unsigned)((ngrp + kbj / P - 1) / (kbj / P)), NF_<KIND>::v, nb), block(kbj * P2<N>::R2);
+    const dim3 grid((unsigned)((ngrp + kbj / P - 1) / (kbj / P)), fn, nb), block(kbj * P2<N>::R2);
     const size_t lds = (size_t)kbj * col_stride<N>() * sizeof(C);
-    if (P == 2) launch_zk<k_fz_gop<N, KIND, R, 2>>(grid, block, lds, st, dd, sp.d, tw, kbj, twy);
-    else launch_zk<k_fz_gop<N, KIND, R, 1>>(grid, block, lds, st, dd, sp.d, tw, kbj, twy);
+    if (P == 2) launch_zk<k_fz_gop<N, KIND, R, 2>>(grid, block, lds, st, dd, sp.d, tw, kbj, twy, fo);
+    else launch_zk<k_fz_gop<N, KIND, R, 1>>(grid, block, lds, st, dd, sp.d, tw, kbj, twy, fo);
   }
 }
 
 // P > 1: the split form, N2 = n2 / P (the rows of one class)
 template <typename R, int M, int N2, int KIND, int INMODE, int P = 1>
 static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st,
-                         const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
+                         const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy,
+                         int fo, int fn) {
   using namespace fast;
   using C = typename V2<R>::T;
   using G = PlaneGeo<M, N2, (P > 1)>;
@@ -1621,8 +1622,7 @@ static bool launch_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStre
     hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
-  hipLaunchKernelGGL(kern, dim3(sp.d.n3 * NF_<KIND>::v * P, nb), dim3(G::THREADS), lds, st, dd, sp.d,
-                     twx, tws, twy);
+  hipLaunchKernelGGL(kern, dim3(sp.d.n3 * fn * P, nb), dim3(G::THREADS), lds, st, dd, sp.d, twx, tws, twy, fo, fn);
   return true;
 }
 
@@ -1639,7 +1639,7 @@ template <int KIND> static inline bool plane_split_ok(const ShapePlan& sp) {
 // no faster, 26.5 us, and the z pass's radix-4 join 7.7 us dearer.)
 template <typename R, int KIND, int INMODE>
 static int try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t st, const typename V2<R>::T* twx,
-                     const typename V2<R>::T* tws, const typename V2<R>::T* twy) {
+                     const typename V2<R>::T* tws, const typename V2<R>::T* twy, int fo, int fn) {
   if (sp.no_plane) return 0;
   const bool split = plane_split_ok<KIND>(sp);
   // the split form where the whole plane takes more than 64 KiB of LDS (at
@@ -1649,9 +1649,9 @@ static int try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t
     constexpr int M = decltype(m)::value, N2 = decltype(n2)::value;
     using G0 = fast::PlaneGeo<M, N2>;
     constexpr size_t whole = (size_t)G0::ROWS * G0::S * sizeof(typename V2<R>::T);
-    if (split && whole > 65536 && launch_plane<R, M, N2 / 2, KIND, INMODE, 2>(sp, dd, nb, st, twx, tws, twy))
+    if (split && whole > 65536 && launch_plane<R, M, N2 / 2, KIND, INMODE, 2>(sp, dd, nb, st, twx, tws, twy, fo, fn))
       return 2;
-    return launch_plane<R, M, N2, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy) ? 1 : 0;
+    return launch_plane<R, M, N2, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy, fo, fn) ? 1 : 0;
   };
   using std::integral_constant;
   if (sp.d.M == 64 && sp.d.n2 == 128) return go(integral_constant<int, 64>(), integral_constant<int, 128>());
@@ -1663,49 +1663,85 @@ static int try_plane(const ShapePlan& sp, const ReqDesc* dd, int nb, hipStream_t
   return 0;
 }
 
+// Whether a plane pass (whole or split) serves this shape: the shapes
+// try_plane knows, when the whole plane fits in LDS (the split form takes
+// less), so that the branch launches it for certain.
+template <typename R> static inline bool plane_for(const ShapePlan& sp) {
+  using namespace fast;
+  if (sp.no_plane) return false;
+  auto fits = [&](auto m, auto n2) {
+    using G = PlaneGeo<decltype(m)::value, decltype(n2)::value>;
+    return (size_t)G::ROWS * G::S * sizeof(typename V2<R>::T) <= (size_t)sp.lds_cap && G::THREADS <= 1024;
+  };
+  using std::integral_constant;
+  const int M = sp.d.M, n2 = sp.d.n2;
+  if (M == 64 && n2 == 128) return fits(integral_constant<int, 64>(), integral_constant<int, 128>());
+  if (M == 32 && n2 == 64) return fits(integral_constant<int, 32>(), integral_constant<int, 64>());
+  if (M == 16 && n2 == 32) return fits(integral_constant<int, 16>(), integral_constant<int, 32>());
+  if (M == 48 && n2 == 96) return fits(integral_constant<int, 48>(), integral_constant<int, 96>());
+  if (M == 40 && n2 == 80) return fits(integral_constant<int, 40>(), integral_constant<int, 80>());
+  if (M == 24 && n2 == 48) return fits(integral_constant<int, 24>(), integral_constant<int, 48>());
+  return false;
+}
+
+// The pass chain of one launch set. (Forking the fields' independent
+// passes, plane or forward y through inverse y, onto a second stream for a
+// single request measured slower at 96^3 and 128^3 and 3 % faster at 256^3:
+// the x-inverse pass still waits for both branches, and the smaller launches
+// of each branch lose what the overlap gains.)
 template <typename R, int KIND, int INMODE, int OUTMODE, int M>
 static void fast_chain(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd, int nb, hipStream_t st,
                        const typename V2<R>::T* twx, const typename V2<R>::T* tws, const typename V2<R>::T* twy,
                        const typename V2<R>::T* twz) {
   constexpr int NF = fast::NF_<KIND>::v;
-  auto ylaunch = [&](auto sgn) {
+  auto ylaunch = [&](auto sgn, hipStream_t s, int fo, int fn) {
     constexpr int S = decltype(sgn)::value;
     switch (sp.d.n2) {
-      case 24: launch_y<R, 24, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 32: launch_y<R, 32, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 40: launch_y<R, 40, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 48: launch_y<R, 48, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 60: launch_y<R, 60, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 64: launch_y<R, 64, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 80: launch_y<R, 80, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 96: launch_y<R, 96, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 120: launch_y<R, 120, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 128: launch_y<R, 128, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      case 192: launch_y<R, 192, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
-      default: launch_y<R, 256, S>(sp, ft.kby, NF, dd, nb, st, twy); break;
+      case 24: launch_y<R, 24, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 32: launch_y<R, 32, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 40: launch_y<R, 40, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 48: launch_y<R, 48, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 60: launch_y<R, 60, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 64: launch_y<R, 64, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 80: launch_y<R, 80, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 96: launch_y<R, 96, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 120: launch_y<R, 120, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 128: launch_y<R, 128, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      case 192: launch_y<R, 192, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
+      default: launch_y<R, 256, S>(sp, ft.kby, fo, fn, dd, nb, s, twy); break;
     }
   };
-  const int plane = try_plane<R, KIND, INMODE>(sp, dd, nb, st, twx, tws, twy);
-  if (plane == 0) {
-    launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, true, twx, tws);
-    ylaunch(std::integral_constant<int, -1>());
-  }
-  const int sy = plane > 1 ? plane : 1;   // row classes the z pass joins
-  switch (sp.d.n3) {
-    case 24: launch_z<R, 24, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 40: launch_z<R, 40, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 48: launch_z<R, 48, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 60: launch_z<R, 60, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 64: launch_z<R, 64, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 80: launch_z<R, 80, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 96: launch_z<R, 96, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 120: launch_z<R, 120, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 128: launch_z<R, 128, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    case 192: launch_z<R, 192, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-    default: launch_z<R, 256, KIND>(sp, ft.kbz, dd, nb, st, twz, sy, twy); break;
-  }
-  ylaunch(std::integral_constant<int, +1>());
+  auto zlaunch = [&](hipStream_t s, int sy, int fo, int fn) {
+    switch (sp.d.n3) {
+      case 24: launch_z<R, 24, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 32: launch_z<R, 32, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 40: launch_z<R, 40, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 48: launch_z<R, 48, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 60: launch_z<R, 60, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 64: launch_z<R, 64, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 80: launch_z<R, 80, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 96: launch_z<R, 96, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 120: launch_z<R, 120, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 128: launch_z<R, 128, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      case 192: launch_z<R, 192, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+      default: launch_z<R, 256, KIND>(sp, ft.kbz, dd, nb, s, twz, sy, twy, fo, fn); break;
+    }
+  };
+  const bool plane = plane_for<R>(sp);
+  if (!plane) launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, true, twx, tws);
+  // one branch: the plane pass (or the forward y pass), z + G-space, inverse y
+  auto branch = [&](hipStream_t s, int fo, int fn) {
+    int sy = 1;
+    if (plane) {
+      const int pl = try_plane<R, KIND, INMODE>(sp, dd, nb, s, twx, tws, twy, fo, fn);
+      sy = pl > 1 ? pl : 1;   // row classes the z pass joins
+    } else {
+      ylaunch(std::integral_constant<int, -1>(), s, fo, fn);
+    }
+    zlaunch(s, sy, fo, fn);
+    ylaunch(std::integral_constant<int, +1>(), s, fo, fn);
+  };
+  branch(st, 0, NF);
   launch_x<R, M, KIND, INMODE, OUTMODE>(sp, ft, dd, nb, st, false, twx, tws);
 }
 

@@ -106,8 +106,9 @@ def test_graph_replays_match_eager(pa, monkeypatch):
     times with eager vector work and synchronizations between the replays;
     every result must equal the eager (uncaptured) run bit for bit."""
     res = {}
-    for mode in ("0", "1"):
-        monkeypatch.setenv("OFD_NO_GRAPH", mode)
+    monkeypatch.delenv("OFD_NO_GRAPH", raising=False)
+    for mode in ("1", "0"):   # OFD_GRAPH=1: captured and replayed; 0: eager
+        monkeypatch.setenv("OFD_GRAPH", mode)
         s, rho = _system(pa, (32, 32, 32), 11)
         r, et, oc, st = pa.ProjNormRhoMinimization(s, rho, conv_check="POT", pot_tol=1e-6)
         e2 = [pa.CalculatePotentialPlus(s, r)[1] for _ in range(3)]
@@ -134,11 +135,13 @@ def test_autotune_keeps_results(pa):
     assert np.all(np.isfinite(et2))
 
 
-def test_pool_replays_after_requests_leave(pa):
+def test_pool_replays_after_requests_leave(pa, monkeypatch):
     """Requests admitted while others finish (the first admitted context is
     destroyed at collection while launch graphs of the same batch width are
     replayed later): every result equals the request's solve run alone."""
     from profess_amd import serve
+    monkeypatch.delenv("OFD_NO_GRAPH", raising=False)
+    monkeypatch.setenv("OFD_GRAPH", "1")
     reqs = serve.make_requests(6, seed=9, grids=(32,), max_atoms=8, rate=1e6)
     params = pa.calpot.opt_params(pot_tol=1e-5, conv_check="POT", fused_dots=True)   # run_workload's settings
     pool = serve.Pool((32, 32, 32), 4, 0)
