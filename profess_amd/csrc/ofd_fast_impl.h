@@ -770,13 +770,18 @@ __global__ void __launch_bounds__(256) k_fz_wgc(const ReqDesc* __restrict__ desc
   }
   const gptr<const double> wk = glb(D.wgck);
   auto kidx = [&](int m) { return (size_t)(t + T * m) * es + (size_t)col; };
-  // phase 1: c' = K3 A into the slot of A
+  // phase 1: c' = K3 A into the slot of A. With at most 16 points per lane
+  // K3 stays in registers for phase 3 (one table read fewer, within the
+  // VGPRs of two waves per SIMD); longer lanes read it again.
+  constexpr bool keep_k3 = R1 <= 16;
+  double k3s[keep_k3 ? R1 : 1];
   load(xa, 0);
   fft2<N, -1>(xa, line, t, tw, active);
   if (active) {
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
       const double k3 = wk[2 * d.nc + kidx(m)] * invn;
+      if constexpr (keep_k3) k3s[m] = k3;
       xb[m].x = (R)(k3 * (double)xa[m].x);
       xb[m].y = (R)(k3 * (double)xa[m].y);
     }
@@ -806,7 +811,9 @@ __global__ void __launch_bounds__(256) k_fz_wgc(const ReqDesc* __restrict__ desc
   if (active) {
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
-      const double k3 = wk[2 * d.nc + kidx(m)] * invn;
+      double k3;
+      if constexpr (keep_k3) k3 = k3s[m];
+      else k3 = wk[2 * d.nc + kidx(m)] * invn;
       xa[m].x = (R)((double)xa[m].x + k3 * (double)xb[m].x);
       xa[m].y = (R)((double)xa[m].y + k3 * (double)xb[m].y);
     }
