@@ -662,13 +662,18 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
     m1 = active ? (int)(col - (long long)i2 * d.K1) : 0;
   }
   const int m2 = i2 > d.n2 / 2 ? i2 - d.n2 : i2;
-  const size_t es = (size_t)d.n2 * d.K1;
+  // element offsets within a field in 32 bits (a field's K1 n2 n3 elements
+  // stay below 2^31 on the grids this pass serves) from a base that is
+  // uniform across the workgroup: scalar base + 32-bit lane offset, no 64-bit
+  // index arithmetic per element
+  const unsigned es = (unsigned)(d.n2 * d.K1);
   const double invn = 1.0 / (double)d.nr;
-  gptr<C> base = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)col;
+  gptr<C> fbase = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc;
+  const unsigned o0 = (unsigned)col + (unsigned)t * es;
   C x[R1];
   if (active) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m) x[m] = ldc(base + (size_t)(t + T * m) * es);
+    for (int m = 0; m < R1; ++m) x[m] = ldc(fbase + (o0 + (unsigned)(T * m) * es));
   }
   if constexpr (P > 1) {
     // x holds X_q; Y[i2] = sum_p w^(p i2) X_p with w^j = twy[j mod n2]
@@ -684,7 +689,7 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
     for (int m = 0; m < R1; ++m) {
       const int i3 = t + T * m;
       const int m3 = i3 > N / 2 ? i3 - N : i3;
-      const R scr = gop_scale<KIND, R>(D, f, m1, m2, m3, (size_t)i3 * es + col, invn);
+      const R scr = gop_scale<KIND, R>(D, f, m1, m2, m3, o0 + (unsigned)(T * m) * es, invn);
       x[m].x *= scr;
       x[m].y *= scr;
     }
@@ -692,7 +697,7 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
   fft2<N, +1>(x, lds + (size_t)c * LL, t, tw, active);
   if (active) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m) stc(base + (size_t)(t + T * m) * es, x[m]);
+    for (int m = 0; m < R1; ++m) stc(fbase + (o0 + (unsigned)(T * m) * es), x[m]);
   }
 }
 
