@@ -685,13 +685,33 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
     fft2<N, -1>(x, lds + (size_t)c * LL, t, tw, active);
   }
   if (active) {
+    constexpr int FVW = is_spin_kind(KIND) ? NF - 2 : NF - 1;   // first vW field
+    if ((f != 0 && f < FVW) || sizeof(R) == 4) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m) {
-      const int i3 = t + T * m;
-      const int m3 = i3 > N / 2 ? i3 - N : i3;
-      const R scr = gop_scale<KIND, R>(D, f, m1, m2, m3, o0 + (unsigned)(T * m) * es, invn);
-      x[m].x *= scr;
-      x[m].y *= scr;
+      for (int m = 0; m < R1; ++m) {
+        const int i3 = t + T * m;
+        const int m3 = i3 > N / 2 ? i3 - N : i3;
+        const R scr = gop_scale<KIND, R>(D, f, m1, m2, m3, o0 + (unsigned)(T * m) * es, invn);
+        x[m].x *= scr;
+        x[m].y *= scr;
+      }
+    } else {
+      // Hartree and vW in FP64: along the column |G|^2 = a0 + m3 (a1 + m3 a2)
+      const double gx = D.rb[0] * m1 + D.rb[1] * m2, gy = D.rb[3] * m1 + D.rb[4] * m2,
+                   gz = D.rb[6] * m1 + D.rb[7] * m2;
+      const double a0 = gx * gx + gy * gy + gz * gz;
+      const double a1 = 2.0 * (gx * D.rb[2] + gy * D.rb[5] + gz * D.rb[8]);
+      const double a2 = D.rb[2] * D.rb[2] + D.rb[5] * D.rb[5] + D.rb[8] * D.rb[8];
+      const double ch = 4.0 * kPi * invn;
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        const int i3 = t + T * m;
+        const double m3 = (double)(i3 > N / 2 ? i3 - N : i3);
+        const double q2 = fma(m3, fma(m3, a2, a1), a0);
+        const double scr = f == 0 ? (q2 > 0.0 ? ch / q2 : 0.0) : q2 * invn;
+        x[m].x *= (R)scr;
+        x[m].y *= (R)scr;
+      }
     }
   }
   fft2<N, +1>(x, lds + (size_t)c * LL, t, tw, active);
