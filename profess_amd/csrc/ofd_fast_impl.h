@@ -337,6 +337,20 @@ __host__ __device__ constexpr int inv_rpb(int nf, int t) {
 }
 
 // ------------------------------------------------------------ x forward (R2C)
+// LDS of the x forward pass per workgroup. For rows of 256 points and more
+// whose complex lines would take more than 32 KB per workgroup (FP64), the
+// lines exchange one real component at a time instead (a line of
+// col_stride<M>() reals): half the LDS, for about 40 more VGPRs, which at
+// these lengths lets four workgroups share a CU instead of three (shorter
+// rows would lose more occupancy to the registers than they gain in LDS).
+template <int M, int NF, typename C> constexpr bool fx_scalar() {
+  return M >= 128 && (size_t)inv_rpb(NF, P2<M>::R2) * NF * line_lds<M>() * sizeof(C) > 32768;
+}
+template <int M, int NF, typename C> constexpr size_t fx_fwd_lds() {
+  return fx_scalar<M, NF, C>() ? (size_t)inv_rpb(NF, P2<M>::R2) * NF * col_stride<M>() * (sizeof(C) / 2)
+                               : (size_t)inv_rpb(NF, P2<M>::R2) * NF * line_lds<M>() * sizeof(C);
+}
+
 // One workgroup = RPB rows x NF fields; thread (line = row*NF + field, lane t):
 // every line forms its field from rho, runs the half-length complex FFT in
 // registers and LDS, and writes its R2C-split half row.
@@ -409,28 +423,71 @@ __global__ void __launch_bounds__(256) k_fx_fwd(const ReqDesc* __restrict__ desc
 #pragma unroll
     for (int m = 0; m < R1; ++m) z[m] = field_pair<KIND, INMODE, R>(D, f, base + 2 * (t + T * m));
   }
-  C* mine = lds + (size_t)line * LL;
-  fft2<M, -1>(z, mine, t, tw, active);
-  if (active) {
+  if constexpr (fx_scalar<M, NF, C>()) {
+    // the split below reads Z[k] and Z[M-k] of other lanes: real parts first
+    // (giving er and oi), then imaginary parts, which finish each output
+    R* mr = reinterpret_cast<R*>(smem) + (size_t)line * col_stride<M>();
+    fft2s<M, -1>(z, mr, t, tw, active);
+    if (active) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m) mine[t + T * m] = z[m];
-  }
-  __syncthreads();
-  if (active) {
-    gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)row * (M + 1);
+      for (int m = 0; m < R1; ++m) mr[t + T * m] = z[m].x;
+    }
+    __syncthreads();
+    R er[R1 + 1], oi[R1 + 1];
+    if (active) {
 #pragma unroll
-    for (int m = 0; m <= R1; ++m) {
-      const int k = m < R1 ? t + T * m : M;
-      if (m == R1 && t != 0) continue;
-      const C zk = mine[k == M ? 0 : k];
-      const C zm = mine[k == 0 ? 0 : M - k];
-      const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
-      const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
-      const C w = tws[k];
-      C x;
-      x.x = er + (w.x * orr - w.y * oi);
-      x.y = ei + (w.x * oi + w.y * orr);
-      stc(out + k, x);
+      for (int m = 0; m <= R1; ++m) {
+        const int k = m < R1 ? t + T * m : M;
+        const R xk = mr[k == M ? 0 : k], xm = mr[k == 0 ? 0 : M - k];
+        er[m] = (R)0.5 * (xk + xm);
+        oi[m] = (R)-0.5 * (xk - xm);
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) mr[t + T * m] = z[m].y;
+    }
+    __syncthreads();
+    if (active) {
+      gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)row * (M + 1);
+#pragma unroll
+      for (int m = 0; m <= R1; ++m) {
+        const int k = m < R1 ? t + T * m : M;
+        if (m == R1 && t != 0) continue;
+        const R yk = mr[k == M ? 0 : k], ym = mr[k == 0 ? 0 : M - k];
+        const R ei = (R)0.5 * (yk - ym), orr = (R)0.5 * (yk + ym);
+        const C w = tws[k];
+        C x;
+        x.x = er[m] + (w.x * orr - w.y * oi[m]);
+        x.y = ei + (w.x * oi[m] + w.y * orr);
+        stc(out + k, x);
+      }
+    }
+  } else {
+    C* mine = lds + (size_t)line * LL;
+    fft2<M, -1>(z, mine, t, tw, active);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) mine[t + T * m] = z[m];
+    }
+    __syncthreads();
+    if (active) {
+      gptr<C> out = glb(reinterpret_cast<C*>(D.work)) + (size_t)f * d.nc + (size_t)row * (M + 1);
+#pragma unroll
+      for (int m = 0; m <= R1; ++m) {
+        const int k = m < R1 ? t + T * m : M;
+        if (m == R1 && t != 0) continue;
+        const C zk = mine[k == M ? 0 : k];
+        const C zm = mine[k == 0 ? 0 : M - k];
+        const R er = (R)0.5 * (zk.x + zm.x), ei = (R)0.5 * (zk.y - zm.y);
+        const R orr = (R)0.5 * (zk.y + zm.y), oi = (R)-0.5 * (zk.x - zm.x);
+        const C w = tws[k];
+        C x;
+        x.x = er + (w.x * orr - w.y * oi);
+        x.y = ei + (w.x * oi + w.y * orr);
+        stc(out + k, x);
+      }
     }
   }
 }
@@ -1567,7 +1624,7 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
     constexpr int RPB = fast::inv_rpb(NF, T);
     const long long nrows = (long long)sp.d.n2 * sp.d.n3;
     const int ntile = (int)((nrows + RPB - 1) / RPB);
-    size_t lds = (size_t)RPB * NF * line_lds<M>() * sizeof(C);
+    const size_t lds = fx_fwd_lds<M, NF, C>();
     allow_lds<k_fx_fwd<M, KIND, INMODE, R>>(lds);
     hipLaunchKernelGGL((k_fx_fwd<M, KIND, INMODE, R>), dim3(ntile, nb), dim3(RPB * NF * T), lds, st, dd, sp.d, tw,
                        tws);
