@@ -989,31 +989,31 @@ __device__ __forceinline__ double epi32(const ReqDesc& D, double rho_d, double v
 }
 
 // Epilogue of the x-inverse passes over the workgroup's RPB rows, whose NF
-// inverse-transformed fields sit in LDS (line rr*NF + j of stride LL complex;
-// real point i of a row is real component i of its line). Consecutive
-// threads take consecutive points, EPT points per thread, in chunks of PF:
-// the global inputs of a whole chunk are loaded before its first point is
-// computed, so they are in flight together instead of one point at a time,
-// and the first chunk's loads are issued before the workgroup barrier that
-// completes the lines in LDS (this function starts with that barrier).
-template <int M, int KIND, int INMODE, int OUTMODE, int RPB, int NT, int LL, typename R>
-__device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict__ lr, long long row0,
-                                           long long nrows, int blk) {
+// inverse-transformed fields sit in LDS: points IOFF .. IOFF + PPR - 1 of
+// each row (PPR = 2M: the whole row), point IOFF + ii of row rr, field j at
+// real ii of line rr*NF + j (LR reals per line). Consecutive threads take
+// consecutive points, EPT points per thread, in chunks of PF: the global
+// inputs of a whole chunk are loaded before its first point is computed, so
+// they are in flight together instead of one point at a time, and the first
+// chunk's loads are issued before the workgroup barrier that completes the
+// lines in LDS (this function starts with that barrier). The sums go to acc.
+template <int M, int KIND, int INMODE, int OUTMODE, int RPB, int NT, int LR, int PPR, typename R>
+__device__ __forceinline__ void x_epi_points(const ReqDesc& D, const R* __restrict__ lr, long long row0,
+                                             long long nrows, int ioff, double (&acc)[kNSum]) {
   constexpr int NF = NF_<KIND>::v;
-  constexpr int EPT = (RPB * 2 * M + NT - 1) / NT;
+  constexpr int EPT = (RPB * PPR + NT - 1) / NT;
   // global loads per point: u and vion, v (trial points and the optimizer
   // forms), b and w (Newton form); at most 16 doubles in flight per thread
   constexpr int NL = 2 + (INMODE == kInPhi || OUTMODE != kOutPot ? 1 : 0) + (OUTMODE == kOutNewton ? 2 : 0);
   constexpr int PF = EPT < 16 / NL ? EPT : 16 / NL;
   const int nrow = (int)min((long long)RPB, nrows - row0);
-  const int npts = nrow * 2 * M;
-  const size_t gbase = (size_t)row0 * 2 * M;
+  const int npts = nrow * PPR;
+  const size_t gbase = (size_t)row0 * 2 * M + ioff;
+  // global index of the workgroup's point e (row e / PPR, point e % PPR + ioff)
+  auto gidx = [&](int e) { return PPR == 2 * M ? gbase + e : gbase + (size_t)(e / PPR) * 2 * M + e % PPR; };
   // the second vector: the phi direction of a trial point (kInPhi), the
   // search direction of the signed form, the CG direction of the Newton form
   const bool want_v = D.v != nullptr && (INMODE == kInPhi || OUTMODE != kOutPot);
-  double acc[kNSum];
-#pragma unroll
-  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
 #pragma unroll
   for (int c0 = 0; c0 < EPT; c0 += PF) {
     double pu[PF], pv[PF], pion[PF], pb[PF], pw[PF];
@@ -1021,7 +1021,7 @@ __device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict
     for (int q = 0; q < PF; ++q) {
       // points past the workgroup's last read its last point (in bounds, no
       // branch between the loads) and are not computed
-      const size_t g = gbase + min((int)threadIdx.x + (c0 + q) * NT, npts - 1);
+      const size_t g = gidx(min((int)threadIdx.x + (c0 + q) * NT, npts - 1));
       pv[q] = pb[q] = pw[q] = 0.0;
       pu[q] = glb(D.u)[g];
       if (want_v) pv[q] = glb(D.v)[g];
@@ -1044,11 +1044,11 @@ __device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict
     for (int q = 0; q < PF; ++q) {
       const int e = threadIdx.x + (c0 + q) * NT;
       if (c0 + q >= EPT || e >= npts) break;
-      const int rr = e / (2 * M), i = e - rr * (2 * M);
-      const size_t g = gbase + e;
+      const int rr = e / PPR, ii = e - rr * PPR;
+      const size_t g = gidx(e);
       double o[NF];
 #pragma unroll
-      for (int j = 0; j < NF; ++j) o[j] = (double)lr[2 * (size_t)(rr * NF + j) * LL + i];
+      for (int j = 0; j < NF; ++j) o[j] = (double)lr[(size_t)(rr * NF + j) * LR + ii];
       if constexpr (is_spin_kind(KIND)) {
         // both channels: the second one's inputs are loaded here
         double t1 = 0.0, t2 = 0.0, r1, r2;
@@ -1124,13 +1124,28 @@ __device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict
       }
     }
   }
-  // partials laid out [slot][workgroup]; the finalize kernel sums each slot
-  // in a fixed order (no device-scope fence or atomic here: on a multi-XCD
-  // part a release fence per workgroup writes back the whole L2)
-  // the energy sums (slots 0-5), plus the optimizer sums of the signed form
-  // (6-10) or of the Newton form (6-8)
+}
+
+// The workgroup's partial sums, laid out [slot][workgroup]; the finalize
+// kernel sums each slot in a fixed order (no device-scope fence or atomic
+// here: on a multi-XCD part a release fence per workgroup writes back the
+// whole L2). The energy sums (slots 0-5), plus the optimizer sums of the
+// signed form (6-10) or of the Newton form (6-8).
+template <int OUTMODE>
+__device__ __forceinline__ void x_epi_sums(const ReqDesc& D, double (&acc)[kNSum], int blk) {
   constexpr int NU = OUTMODE == kOutPot ? 6 : (OUTMODE == kOutNewton ? 9 : kNSum);
   block_sum_strided<kNSum, NU>(acc, glb(D.sums) + blk, gridDim.x);
+}
+
+// The epilogue over whole rows held as complex lines of stride LL.
+template <int M, int KIND, int INMODE, int OUTMODE, int RPB, int NT, int LL, typename R>
+__device__ __forceinline__ void x_epilogue(const ReqDesc& D, const R* __restrict__ lr, long long row0,
+                                           long long nrows, int blk) {
+  double acc[kNSum];
+#pragma unroll
+  for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
+  x_epi_points<M, KIND, INMODE, OUTMODE, RPB, NT, 2 * LL, 2 * M>(D, lr, row0, nrows, 0, acc);
+  x_epi_sums<OUTMODE>(D, acc, blk);
 }
 
 // One workgroup = RPB rows x NF fields; thread (line = row*NF + field, lane t).
@@ -1172,12 +1187,40 @@ __global__ void __launch_bounds__(256) k_fx_inv(const ReqDesc* __restrict__ desc
       z[m].y = ei + orr;
     }
   }
-  fft2<M, +1>(z, mine, t, tw, active);
-  if (active) {
+  if constexpr (fx_scalar<M, NF, C>()) {
+    // one real component at a time through a line of LS reals, then the
+    // epilogue in two halves of each row: the complex outputs t + T m with
+    // m < R1/2 are the row's reals 0 .. M-1, the others M .. 2M-1
+    constexpr int LS = col_stride<M>(), H = R1 / 2;
+    static_assert(R1 % 2 == 0 && LS >= M, "a half row fits a line");
+    R* mr = reinterpret_cast<R*>(smem) + (size_t)line * LS;
+    fft2s<M, +1>(z, mr, t, tw, active);
+    double acc[kNSum];
 #pragma unroll
-    for (int m = 0; m < R1; ++m) mine[t + T * m] = z[m];
+    for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h) __syncthreads();   // the first half's points are read
+      if (active) {
+#pragma unroll
+        for (int m = 0; m < H; ++m) {
+          const int i = 2 * (t + T * m);   // the half's reals i, i + 1
+          mr[i] = z[h * H + m].x;
+          mr[i + 1] = z[h * H + m].y;
+        }
+      }
+      x_epi_points<M, KIND, INMODE, OUTMODE, RPB, RPB * NF * T, LS, M>(D, reinterpret_cast<const R*>(smem), row0,
+                                                                      nrows, h * M, acc);
+    }
+    x_epi_sums<OUTMODE>(D, acc, blk);
+  } else {
+    fft2<M, +1>(z, mine, t, tw, active);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) mine[t + T * m] = z[m];
+    }
+    x_epilogue<M, KIND, INMODE, OUTMODE, RPB, RPB * NF * T, LL>(D, reinterpret_cast<const R*>(lds), row0, nrows, blk);
   }
-  x_epilogue<M, KIND, INMODE, OUTMODE, RPB, RPB * NF * T, LL>(D, reinterpret_cast<const R*>(lds), row0, nrows, blk);
 }
 
 // ------------------------------------------------------------ general two-stage lengths
@@ -1632,7 +1675,7 @@ static void launch_x(const ShapePlan& sp, const FastTiles& ft, const ReqDesc* dd
     constexpr int RPB = fast::inv_rpb(NF, T);
     const long long nrows = (long long)sp.d.n2 * sp.d.n3;
     const int ntile = (int)((nrows + RPB - 1) / RPB);
-    size_t lds = (size_t)RPB * NF * line_lds<M>() * sizeof(C);
+    const size_t lds = fx_fwd_lds<M, NF, C>();
     allow_lds<k_fx_inv<M, KIND, INMODE, OUTMODE, R>>(lds);
     hipLaunchKernelGGL((k_fx_inv<M, KIND, INMODE, OUTMODE, R>), dim3(ntile, nb), dim3(RPB * NF * T), lds, st, dd,
                        sp.d, tw, tws);
