@@ -689,6 +689,11 @@ __device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, 
 // joins the classes' transforms) as the first z FFT stage reads its inputs
 // (fft2_join), then its members run as ordinary columns. The inverse z
 // transform leaves the spectra in the ordinary layout.
+// The z pass exchanges one real component at a time (a column line of
+// col_stride<N>() reals, half the LDS) for FP64 columns of 256 points and
+// more, where complex lines leave two or three workgroups per CU.
+template <int N, typename R, int P> constexpr bool fz_scalar() { return P == 1 && N >= 256 && sizeof(R) == 8; }
+
 template <int N, int KIND, typename R, int P>
 __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
                                                 const typename V2<R>::T* __restrict__ tw, int kb,
@@ -738,6 +743,8 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
 #pragma unroll
     for (int p = 0; p < P; ++p) w[p] = active ? twy[(size_t)p * i2 % d.n2] : C{};
     fft2_join<N, -1, P>(x, lds + (size_t)c * LL, lds + (size_t)cg * LL, kg * LL, t, tw, active, w);
+  } else if constexpr (fz_scalar<N, R, P>()) {
+    fft2s<N, -1>(x, reinterpret_cast<R*>(smem) + (size_t)c * LL, t, tw, active);
   } else {
     fft2<N, -1>(x, lds + (size_t)c * LL, t, tw, active);
   }
@@ -771,7 +778,8 @@ __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ desc
       }
     }
   }
-  fft2<N, +1>(x, lds + (size_t)c * LL, t, tw, active);
+  if constexpr (fz_scalar<N, R, P>()) fft2s<N, +1>(x, reinterpret_cast<R*>(smem) + (size_t)c * LL, t, tw, active);
+  else fft2<N, +1>(x, lds + (size_t)c * LL, t, tw, active);
   if (active) {
 #pragma unroll
     for (int m = 0; m < R1; ++m) stc(fbase + (o0 + (unsigned)(T * m) * es), x[m]);
@@ -1729,11 +1737,16 @@ static void launch_z(const ShapePlan& sp, int kb, const ReqDesc* dd, int nb, hip
   } else {
     // after the split plane pass: kb/P groups of P columns per workgroup (kb
     // rounded down to a multiple of P)
-    const int kbj = P > 1 ? std::max(P, kb / P * P) : kb;
+    // (the one-component z exchange: half the LDS per column, so a whole
+    // 256-thread workgroup of columns unless the tuner picked a tile)
+    const int kbj = P > 1 ? std::max(P, kb / P * P)
+                          : (fz_scalar<N, R, 1>() && sp.kb_z <= 0 ? 256 / P2<N>::R2 : kb);
     const long long ngrp = (long long)sp.d.K1 * (sp.d.n2 / P);
     const dim3 grid((unsigned)((ngrp + kbj / P - 1) / (kbj / P)), fn, nb), block(kbj * P2<N>::R2);
     const size_t lds = (size_t)kbj * col_stride<N>() * sizeof(C);
     if (P == 2) launch_zk<k_fz_gop<N, KIND, R, 2>>(grid, block, lds, st, dd, sp.d, tw, kbj, twy, fo);
+    else if (fz_scalar<N, R, 1>())
+      launch_zk<k_fz_gop<N, KIND, R, 1>>(grid, block, lds / 2, st, dd, sp.d, tw, kbj, twy, fo);
     else launch_zk<k_fz_gop<N, KIND, R, 1>>(grid, block, lds, st, dd, sp.d, tw, kbj, twy, fo);
   }
 }
