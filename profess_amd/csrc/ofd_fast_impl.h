@@ -681,6 +681,11 @@ __device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, 
   }
 }
 
+// The z pass exchanges one real component at a time (a column line of
+// col_stride<N>() reals, half the LDS) for FP64 columns of 256 points and
+// more, where complex lines leave two or three workgroups per CU.
+template <int N, typename R, int P> constexpr bool fz_scalar() { return P == 1 && N >= 256 && sizeof(R) == 8; }
+
 //
 // P > 1 (after the split plane pass with P row classes): the kb columns of a
 // workgroup are P members of kb/P groups; member q of a group holds
@@ -689,11 +694,6 @@ __device__ __forceinline__ R gop_scale(const ReqDesc& D, int f, int m1, int m2, 
 // joins the classes' transforms) as the first z FFT stage reads its inputs
 // (fft2_join), then its members run as ordinary columns. The inverse z
 // transform leaves the spectra in the ordinary layout.
-// The z pass exchanges one real component at a time (a column line of
-// col_stride<N>() reals, half the LDS) for FP64 columns of 256 points and
-// more, where complex lines leave two or three workgroups per CU.
-template <int N, typename R, int P> constexpr bool fz_scalar() { return P == 1 && N >= 256 && sizeof(R) == 8; }
-
 template <int N, int KIND, typename R, int P>
 __global__ void __launch_bounds__(256) k_fz_gop(const ReqDesc* __restrict__ descs, GridDims d,
                                                 const typename V2<R>::T* __restrict__ tw, int kb,
